@@ -1,0 +1,360 @@
+"""Algorithm objects (python/astra/algorithm.py:28-90).
+
+Types built here (include/astra/AlgorithmTypelist.h:62-90): FP_CUDA,
+BP_CUDA, FBP_CUDA, SIRT_CUDA, CGLS_CUDA, FP3D_CUDA, BP3D_CUDA, FDK_CUDA,
+SIRT3D_CUDA, CGLS3D_CUDA.  Each mirrors the configuration keys of the
+reference algorithm class and runs through libastra_amd on one GPU.
+"""
+import ctypes
+import math
+
+import numpy as np
+
+from . import _core, _lib
+from . import geometry as G
+from ._registry import algorithms as _reg
+from ._registry import data2d as _d2, data3d as _d3
+from ._registry import projectors2d as _p2, projectors3d as _p3
+from .log import AstraError
+
+try:
+    import torch
+except ImportError:  # pragma: no cover
+    torch = None
+
+
+def _opt(cfg, key, default=None):
+    for sect in ("option", "options"):
+        o = cfg.get(sect) or {}
+        if key in o:
+            return o[key]
+    return cfg.get(key, default)
+
+
+def _truthy(v):
+    if isinstance(v, str):
+        return v.lower() in ("yes", "true", "1", "on")
+    return bool(v)
+
+
+class _Algo:
+    is3d = False
+
+    def __init__(self, cfg):
+        self.cfg = cfg
+        reg = _d3 if self.is3d else _d2
+        self.sino = reg.get(cfg["ProjectionDataId"]) if "ProjectionDataId" in cfg else None
+        vid = cfg.get("ReconstructionDataId", cfg.get("VolumeDataId"))
+        self.vol = reg.get(vid) if vid is not None else None
+        if self.sino is None or self.vol is None:
+            raise AstraError("algorithm needs ProjectionDataId and a volume data id")
+        opts = {}
+        gi = _opt(cfg, "GPUindex", _opt(cfg, "GPUIndex"))
+        if gi is not None:
+            opts["GPUindex"] = gi
+        for k in ("DetectorSuperSampling", "VoxelSuperSampling", "PixelSuperSampling"):
+            v = _opt(cfg, k)
+            if v is not None:
+                opts["VoxelSuperSampling" if k == "PixelSuperSampling" else k] = v
+        pid = cfg.get("ProjectorId")
+        if pid is not None:
+            self.proj = (_p3 if self.is3d else _p2).get(pid)
+            if opts.get("GPUindex") is not None:
+                self.proj = _core.Projector(self.proj.type, self.proj.proj_geom, self.proj.vol_geom,
+                                            dict(self.proj.options, **opts))
+        else:
+            self.proj = _core.projector_for(self.sino.geom, self.vol.geom, opts)
+        if tuple(self.proj.proj_shape) != self.sino.shape or tuple(self.proj.vol_shape) != self.vol.shape:
+            raise AstraError("data objects do not match the projector geometry")
+
+    def run(self, iterations):
+        raise NotImplementedError
+
+    def get_res_norm(self):
+        raise AstraError("this algorithm has no residual norm")
+
+    def release(self):
+        pass
+
+
+# ---------------------------------------------------------- FP / BP -------
+class _FP(_Algo):
+    def run(self, iterations=1):
+        dev = self.proj.device()
+        v = self.vol.on_device(dev)
+        s = self.sino.on_device(dev, init=False)
+        _core.fp(self.proj, v, s, _lib.MODE_SET)
+        self.sino.write_back(s)
+        _core.sync(dev)
+
+
+class _BP(_Algo):
+    def run(self, iterations=1):
+        dev = self.proj.device()
+        s = self.sino.on_device(dev)
+        v = self.vol.on_device(dev, init=False)
+        _core.bp(self.proj, s, v, _lib.MODE_SET)
+        self.vol.write_back(v)
+        _core.sync(dev)
+
+
+class FP_CUDA(_FP):
+    pass
+
+
+class BP_CUDA(_BP):
+    pass
+
+
+class FP3D_CUDA(_FP):
+    is3d = True
+
+
+class BP3D_CUDA(_BP):
+    is3d = True
+
+
+# ------------------------------------------------------------- FBP / FDK --
+def _filter_handle(n_det, ftype, d=1.0):
+    """Half-spectrum filter handle for the reference's filter names
+    (src/Filters.cpp:473-560); only Ram-Lak and 'none' are built."""
+    t = (ftype or "ram-lak").lower()
+    if t in ("none",):
+        return None, True
+    if t not in ("ram-lak", "ramlak"):
+        raise AstraError("FilterType '%s' is not part of this build (Ram-Lak only)" % ftype)
+    n_fft = 1
+    while n_fft < 2 * n_det:
+        n_fft *= 2
+    nb = n_fft // 2 + 1
+    spec = np.zeros(nb, dtype=np.float32)
+    _lib.call("astra_amd_ramlak_spectrum", n_det, float(d), spec.ctypes.data_as(ctypes.c_void_p), nb)
+    h = ctypes.c_void_p()
+    _lib.call("astra_amd_filter_create", n_det, spec.ctypes.data_as(ctypes.c_void_p), nb, ctypes.byref(h))
+    return h, False
+
+
+class FBP_CUDA(_Algo):
+    """CCudaFilteredBackProjectionAlgorithm (src/CudaFilteredBackProjectionAlgorithm.cpp:174-239)."""
+
+    def __init__(self, cfg):
+        super().__init__(cfg)
+        if _truthy(_opt(cfg, "ShortScan", False)):
+            raise AstraError("FBP_CUDA: ShortScan (Parker weighting) is not part of this build")
+        self.ftype = cfg.get("FilterType", _opt(cfg, "FilterType", "ram-lak"))
+        self.fd = float(cfg.get("FilterD", _opt(cfg, "FilterD", 1.0)))
+
+    def run(self, iterations=1):
+        dev = self.proj.device()
+        nd = self.proj.proj_shape[1]
+        px, py = G.pixel_lengths(self.vol.geom)
+        scale = float(self.proj.out_scale) / float(px * py)
+        s = self.sino.on_device(dev).clone()
+        v = self.vol.on_device(dev, init=False)
+        h = ctypes.c_void_p(self.proj.handle(dev))
+        with torch.cuda.device(dev):
+            f, skip = _filter_handle(nd, self.ftype, self.fd)
+            try:
+                if skip:
+                    raise AstraError("FBP_CUDA with FilterType 'none' is not part of this build")
+                _lib.call("astra_amd_fbp2d", h, _core.ptr(s), _core.ptr(v), scale, f,
+                          _lib.MODE_SET, _core.stream_ptr(dev))
+                _core.sync(dev)
+            finally:
+                if f is not None:
+                    _lib.call("astra_amd_filter_destroy", f)
+        self.vol.write_back(v)
+        _core.sync(dev)
+
+
+class FDK_CUDA(_Algo):
+    """CCudaFDKAlgorithm3D (src/CudaFDKAlgorithm3D.cpp:183-198) via a
+    single-GPU, unchunked FDK job."""
+    is3d = True
+
+    def __init__(self, cfg):
+        super().__init__(cfg)
+        if _truthy(_opt(cfg, "ShortScan", False)):
+            raise AstraError("FDK_CUDA: ShortScan (Parker weighting) is not part of this build")
+        self.ftype = cfg.get("FilterType", _opt(cfg, "FilterType", "ram-lak"))
+        self.fd = float(cfg.get("FilterD", _opt(cfg, "FilterD", 1.0)))
+        if self.proj.kind != "cone":
+            raise AstraError("FDK_CUDA requires a cone geometry")
+
+    def run(self, iterations=1, mode=_lib.MODE_SET):
+        dev = self.proj.device()
+        p = self.sino.on_device(dev).clone()  # FDK filters in place
+        v = self.vol.on_device(dev, init=(mode == _lib.MODE_ADD))
+        fdk(self.proj, p, v, mode, self.ftype, self.fd)
+        self.vol.write_back(v)
+        _core.sync(dev)
+
+
+def fdk(proj, p, v, mode, ftype="ram-lak", d=1.0):
+    """FDK of device tensor p (overwritten by the filtered data) into v."""
+    dev = v.device
+    h = ctypes.c_void_p(proj.handle(dev))
+    nu = proj.proj_shape[2]
+    with torch.cuda.device(dev):
+        t = (ftype or "ram-lak").lower()
+        if t in ("ram-lak", "ramlak") and d == 1.0:
+            f = None  # the geometry's cached Ram-Lak filter
+        else:
+            f, skip = _filter_handle(nu, ftype, d)
+            if skip:
+                raise AstraError("FDK_CUDA with FilterType 'none' is not part of this build")
+        try:
+            _lib.call("astra_amd_fdk", h, _core.ptr(p), _core.ptr(v), 1.0, 0, f, mode,
+                      _core.stream_ptr(dev))
+            _core.sync(dev)
+        finally:
+            if f is not None:
+                _lib.call("astra_amd_filter_destroy", f)
+
+
+# ------------------------------------------------------- SIRT / CGLS -----
+class _Iterative(_Algo):
+    algo_type = _lib.ALGO_SIRT
+
+    def __init__(self, cfg):
+        super().__init__(cfg)
+        reg = _d3 if self.is3d else _d2
+        self.masks = {}
+        for key, name in (("ReconstructionMaskId", "ReconstructionMask"),
+                          ("SinogramMaskId", "SinogramMask"),
+                          ("MinMaskId", "MinMask"), ("MaxMaskId", "MaxMask")):
+            mid = _opt(cfg, key)
+            if mid is not None:
+                self.masks[name] = reg.get(mid)
+        self.options = {}
+        if self.algo_type == _lib.ALGO_SIRT:
+            self.options["Relaxation"] = float(_opt(cfg, "Relaxation", 1.0))
+        # MinConstraint / MaxConstraint, or the deprecated Use*/..Value pair
+        # (src/ReconstructionAlgorithm2D.cpp:105-127)
+        if _opt(cfg, "MinConstraint") is not None:
+            self.options["MinConstraint"] = float(_opt(cfg, "MinConstraint"))
+        elif _truthy(_opt(cfg, "UseMinConstraint", False)):
+            self.options["MinConstraint"] = float(_opt(cfg, "MinConstraintValue", 0.0))
+        if _opt(cfg, "MaxConstraint") is not None:
+            self.options["MaxConstraint"] = float(_opt(cfg, "MaxConstraint"))
+        elif _truthy(_opt(cfg, "UseMaxConstraint", False)):
+            self.options["MaxConstraint"] = float(_opt(cfg, "MaxConstraintValue", 255.0))
+        self._handles = {}
+        self._keep = {}
+
+    def _handle(self, dev):
+        h = self._handles.get(dev.index)
+        if h is not None:
+            return h
+        g = ctypes.c_void_p(self.proj.handle(dev))
+        out = ctypes.c_void_p()
+        with torch.cuda.device(dev):
+            if self.is3d:
+                p = self.proj.params3d()
+                _lib.call("astra_amd_algo3d_create", self.algo_type, g, ctypes.byref(p), ctypes.byref(out))
+            else:
+                p = self.proj.params2d()
+                _lib.call("astra_amd_algo2d_create", self.algo_type, g, ctypes.byref(p), ctypes.byref(out))
+        for k, v in self.options.items():
+            _lib.call("astra_amd_algo_set_option", out, k.encode(), float(v))
+        self._handles[dev.index] = out
+        return out
+
+    def _bind_masks(self, h, dev):
+        keep = {}
+        for name, d in self.masks.items():
+            t = d.on_device(dev)
+            keep[name] = t
+            _lib.call("astra_amd_algo_set_mask", h, name.encode(), _core.ptr(t))
+        self._keep[dev.index] = keep
+
+    def run(self, iterations=1):
+        dev = self.proj.device()
+        h = self._handle(dev)
+        with torch.cuda.device(dev):
+            self._bind_masks(h, dev)
+            b = self.sino.on_device(dev)
+            x = self.vol.on_device(dev)
+            _lib.call("astra_amd_algo_run", h, _core.ptr(b), _core.ptr(x), int(iterations),
+                      _core.stream_ptr(dev))
+            self.vol.write_back(x)
+            _core.sync(dev)
+
+    def get_res_norm(self):
+        dev = self.proj.device()
+        h = self._handle(dev)
+        with torch.cuda.device(dev):
+            self._bind_masks(h, dev)
+            b = self.sino.on_device(dev)
+            x = self.vol.on_device(dev)
+            r = ctypes.c_double()
+            _lib.call("astra_amd_algo_residual_norm", h, _core.ptr(b), _core.ptr(x), ctypes.byref(r),
+                      _core.stream_ptr(dev))
+        return float(r.value)
+
+    def release(self):
+        for dev, h in self._handles.items():
+            try:
+                with torch.cuda.device(dev):
+                    _lib.call("astra_amd_algo_destroy", h)
+            except Exception:
+                pass
+        self._handles.clear()
+
+
+class SIRT_CUDA(_Iterative):
+    algo_type = _lib.ALGO_SIRT
+
+
+class CGLS_CUDA(_Iterative):
+    algo_type = _lib.ALGO_CGLS
+
+
+class SIRT3D_CUDA(_Iterative):
+    algo_type = _lib.ALGO_SIRT
+    is3d = True
+
+
+class CGLS3D_CUDA(_Iterative):
+    algo_type = _lib.ALGO_CGLS
+    is3d = True
+
+
+TYPES = {c.__name__: c for c in (FP_CUDA, BP_CUDA, FBP_CUDA, SIRT_CUDA, CGLS_CUDA, FP3D_CUDA,
+                                  BP3D_CUDA, FDK_CUDA, SIRT3D_CUDA, CGLS3D_CUDA)}
+NOT_BUILT = ("FP", "BP", "FBP", "SIRT", "SART", "ART", "CGLS", "SART_CUDA", "EM_CUDA",
+             "DART_MASK_CUDA", "DART_SMOOTHING_CUDA", "DART_MASK_CUDA3D",
+             "DART_SMOOTHING_CUDA3D", "ROI_SELECT_CUDA", "DATA_OPERATION_CUDA")
+
+
+def create(config):
+    t = config.get("type")
+    if t in TYPES:
+        return _reg.add(TYPES[t](config))
+    if t in NOT_BUILT:
+        raise AstraError("algorithm type '%s' is not part of this build" % t)
+    raise AstraError("unknown algorithm type '%s'" % t)
+
+
+def run(i, iterations=1):
+    _reg.get(i).run(iterations)
+
+
+def get_res_norm(i):
+    return _reg.get(i).get_res_norm()
+
+
+def delete(ids):
+    _reg.delete(ids)
+
+
+def clear():
+    _reg.clear()
+
+
+def info():
+    return _reg.info()
+
+
+def get_plugin_object(i):
+    raise AstraError("plugins are not part of this build")

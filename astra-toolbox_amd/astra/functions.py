@@ -1,0 +1,67 @@
+"""Helper functions (python/astra/functions.py:47-300)."""
+import numpy as np
+
+from . import _registry, geometry as G
+from .log import AstraError
+
+
+def clear():
+    """Delete every object (functions.py:47-58)."""
+    for r in (_registry.algorithms, _registry.data2d, _registry.data3d,
+              _registry.projectors2d, _registry.projectors3d):
+        r.clear()
+
+
+def add_noise_to_sino(sinogram_in, I0, seed=None):
+    """Poisson noise for I0 incident photons (functions.py:83-121):
+    b = -max * log(Poisson(I0 exp(-p / max)) / I0)."""
+    if isinstance(sinogram_in, np.ndarray):
+        sino = np.array(sinogram_in, dtype=np.float64)
+    else:
+        from . import data2d
+        sino = np.array(data2d.get(sinogram_in), dtype=np.float64)
+    mx = sino.max()
+    if mx <= 0:
+        return sino.astype(np.float32)
+    rng = np.random.default_rng(seed)
+    counts = rng.poisson(I0 * np.exp(-sino / mx))
+    counts = np.maximum(counts, 1)
+    return (-mx * np.log(counts / I0)).astype(np.float32)
+
+
+def geom_2vec(proj_geom):
+    return G.geom_2vec(proj_geom)
+
+
+def geom_size(geom, dim=None):
+    """Shape of the data that belongs to a geometry (functions.py:156-165)."""
+    if "GridSliceCount" in geom or "GridRowCount" in geom:
+        s = G.vol_shape(geom)
+    else:
+        s = G.proj_shape(geom)
+    return s if dim is None else s[dim]
+
+
+def move_vol_geom(geom, pos, is_relative=False):
+    """Translate a volume geometry window (functions.py:123-154)."""
+    import copy
+    g = copy.deepcopy(geom)
+    o = g["option"]
+    axes = ["X", "Y", "Z"][: len(pos)]
+    for ax, p in zip(axes, pos):
+        w = o["WindowMax" + ax] - o["WindowMin" + ax]
+        if is_relative:
+            o["WindowMin" + ax] += p
+            o["WindowMax" + ax] += p
+        else:
+            o["WindowMin" + ax] = p - w / 2.0
+            o["WindowMax" + ax] = p + w / 2.0
+    return g
+
+
+def data_op(op, data, scalar, gpu_core, mask=None):
+    raise AstraError("data_op (DATA_OPERATION_CUDA) is not part of this build")
+
+
+def geom_postalignment(proj_geom, factor):
+    raise AstraError("geom_postalignment is not part of this build")

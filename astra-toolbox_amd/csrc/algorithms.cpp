@@ -1,0 +1,374 @@
+// Reconstruction algorithm objects over device data (SIRT, CGLS in 2D and
+// 3D) and the FDK / FBP drivers.
+//
+// The iteration state lives in projector-friendly layouts: residuals are
+// kept in the zero-padded angle-major layout that the back projector reads,
+// the forward projector writes them there directly with the residual and
+// line weights fused into its epilogue, and the back projector applies the
+// SIRT update (pixel weights, relaxation, clamping) in its epilogue.  One
+// SIRT iteration is therefore two projector launches plus the forward
+// projector's input copy; no host synchronisation happens inside run().
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <mutex>
+#include <string>
+
+#include "devutil.hpp"
+#include "internal.hpp"
+
+namespace amx {
+
+// ------------------------------------------------------------ operator ---
+// Uniform view of a 2D or 3D geometry for the algorithm loops.
+struct Op {
+    Geom2D *g2 = nullptr;
+    Geom3D *g3 = nullptr;
+    float out_scale = 1.f;
+    int kernel = ASTRA_AMD_KER3D_DEFAULT;
+    unsigned rays = 1;
+
+    size_t nvol() const {
+        return g3 ? (size_t)g3->nx * g3->ny * g3->nz : (size_t)g2->w * g2->h;
+    }
+    size_t nproj() const {
+        return g3 ? (size_t)g3->nv * g3->na * g3->nu : (size_t)g2->na * g2->nd;
+    }
+    size_t npad() const { return g3 ? padded_am_elems(*g3) : padded_sino_elems(*g2); }
+
+    // out_padded = mode(scale * A x) with the residual epilogue when mode is OUT_RESID
+    void fp(const float *x, float *out_padded, float scale, int mode, const float *b,
+            const float *w, hipStream_t s) const {
+        if (g3)
+            fp3d_launch(*g3, x, out_padded, out_scale * scale, mode, b, w, true, s);
+        else
+            fp2d_launch(*g2, x, out_padded, out_scale * scale, mode, b, w, true, rays, s);
+    }
+    void bp(const float *pp, float *x, float scale, int mode, const float *uw, float lo,
+            float hi, hipStream_t s) const {
+        if (g3) {
+            int k = (kernel == ASTRA_AMD_KER3D_2D_WEIGHTING) ? kernel : ASTRA_AMD_KER3D_DEFAULT;
+            bp3d_launch_padded(*g3, pp, x, out_scale * scale, k, mode, uw, lo, hi, s);
+        } else {
+            bp2d_launch_padded(*g2, pp, x, out_scale * scale, false, mode, uw, lo, hi, s);
+        }
+    }
+    void to_padded(const float *proj, float *pp, hipStream_t s) const {
+        if (g3)
+            proj_to_padded_am(*g3, proj, pp, s);
+        else
+            sino_to_padded(*g2, proj, pp, s);
+    }
+};
+
+struct AlgoBase {
+    Op op;
+    int type = ASTRA_AMD_ALGO_SIRT;
+    double relaxation = 1.0;
+    float lo = -std::numeric_limits<float>::infinity();
+    float hi = std::numeric_limits<float>::infinity();
+    const float *recon_mask = nullptr, *sino_mask = nullptr;
+    const float *min_mask = nullptr, *max_mask = nullptr;
+    bool weights_ready = false;
+    int device = 0;
+    DevArray line_w, pixel_w, r, w, p, z, t, scal;
+    std::mutex mu;
+
+    virtual ~AlgoBase() = default;
+    virtual void run(const float *b, float *x, int iters, hipStream_t s) = 0;
+
+    double residual(const float *b, const float *x, hipStream_t s) {
+        ensure_buffers();
+        const float *xin = x;
+        if (recon_mask) {
+            copy_launch(t.as<float>(), x, op.nvol(), s);
+            mul_launch(t.as<float>(), recon_mask, op.nvol(), s);
+            xin = t.as<float>();
+        }
+        op.fp(xin, r.as<float>(), -1.f, OUT_RESID, b, nullptr, s);
+        dot_launch(r.as<float>(), r.as<float>(), op.npad(), scal.as<double>(), s);
+        double h = 0;
+        AMX_HIP(hipMemcpyAsync(&h, scal.as<double>(), sizeof(double), hipMemcpyDeviceToHost, s));
+        AMX_HIP(hipStreamSynchronize(s));
+        return std::sqrt(std::max(h, 0.0));
+    }
+
+    void ensure_buffers() {
+        if (r.bytes()) return;
+        const size_t nv = op.nvol() * sizeof(float), np = op.npad() * sizeof(float);
+        r = DevArray(np);
+        t = DevArray(std::max(nv, op.nproj() * sizeof(float)));
+        scal = DevArray(4 * sizeof(double));
+        // padded buffers must have zero borders: they are only ever written
+        // in their interior
+        AMX_HIP(hipMemset(r.as<float>(), 0, np));
+        if (type == ASTRA_AMD_ALGO_SIRT) {
+            line_w = DevArray(np);
+            pixel_w = DevArray(nv);
+            AMX_HIP(hipMemset(line_w.as<float>(), 0, np));
+        } else {
+            w = DevArray(np);
+            p = DevArray(nv);
+            z = DevArray(nv);
+            AMX_HIP(hipMemset(w.as<float>(), 0, np));
+        }
+    }
+};
+
+// SIRT: CCudaSirtAlgorithm::precomputeWeights / run
+// (src/CudaSirtAlgorithm.cpp:187-222, 347-379) and
+// CCudaSirtAlgorithm3D (src/CudaSirtAlgorithm3D.cpp:261-376).
+struct Sirt : AlgoBase {
+    void precompute(hipStream_t s) {
+        const size_t nv = op.nvol(), np = op.nproj();
+        float *tv = t.as<float>();
+        // line weights: inv(FP(mask or 1)) [x sinogram mask]
+        if (recon_mask)
+            copy_launch(tv, recon_mask, nv, s);
+        else
+            fill_launch(tv, nv, 1.f, s);
+        op.fp(tv, line_w.as<float>(), 1.f, OUT_SET, nullptr, nullptr, s);
+        invert_launch(line_w.as<float>(), op.npad(), 1.f, s);
+        // pixel weights: lambda * inv(BP(sinogram mask or 1)) [x volume mask]
+        if (sino_mask)
+            copy_launch(tv, sino_mask, np, s);
+        else
+            fill_launch(tv, np, 1.f, s);
+        op.to_padded(tv, r.as<float>(), s);
+        if (sino_mask) {
+            // line weights x sinogram mask (padded layout)
+            mul_launch(line_w.as<float>(), r.as<float>(), op.npad(), s);
+        }
+        op.bp(r.as<float>(), pixel_w.as<float>(), 1.f, OUT_SET, nullptr, lo, hi, s);
+        invert_launch(pixel_w.as<float>(), nv, (float)relaxation, s);
+        if (recon_mask) mul_launch(pixel_w.as<float>(), recon_mask, nv, s);
+    }
+
+    void run(const float *b, float *x, int iters, hipStream_t s) override {
+        ensure_buffers();
+        if (!weights_ready || recon_mask || sino_mask) {
+            precompute(s);
+            weights_ready = true;
+        }
+        const size_t nv = op.nvol();
+        for (int it = 0; it < iters; ++it) {
+            const float *xin = x;
+            if (recon_mask) {
+                copy_launch(t.as<float>(), x, nv, s);
+                mul_launch(t.as<float>(), recon_mask, nv, s);
+                xin = t.as<float>();
+            }
+            // r = (b - A x) * lineW, written in the padded BP layout
+            op.fp(xin, r.as<float>(), -1.f, OUT_RESID, b, line_w.as<float>(), s);
+            // x = clamp(x + pixelW * A^T r)
+            op.bp(r.as<float>(), x, 1.f, OUT_UPDATE, pixel_w.as<float>(), lo, hi, s);
+            if (min_mask || max_mask)
+                clamp_launch(x, nv, -INFINITY, INFINITY, min_mask, max_mask, s);
+        }
+    }
+};
+
+// CGLS: CCudaCglsAlgorithm::run (src/CudaCglsAlgorithm.cpp:166-266) and
+// CCudaCglsAlgorithm3D (src/CudaCglsAlgorithm3D.cpp:264-360); restarts on
+// every run().  The scalars gamma, <w,w> stay in device memory (float64).
+struct Cgls : AlgoBase {
+    void run(const float *b, float *x, int iters, hipStream_t s) override {
+        ensure_buffers();
+        const size_t nv = op.nvol(), np = op.npad();
+        double *g0 = scal.as<double>(), *g1 = g0 + 1, *ww = g0 + 2;
+        const float *xin = x;
+        if (recon_mask) {
+            copy_launch(t.as<float>(), x, nv, s);
+            mul_launch(t.as<float>(), recon_mask, nv, s);
+            xin = t.as<float>();
+        }
+        // r = b - A x ; p = A^T r ; gamma = <p, p>
+        op.fp(xin, r.as<float>(), -1.f, OUT_RESID, b, nullptr, s);
+        op.bp(r.as<float>(), p.as<float>(), 1.f, OUT_SET, nullptr, lo, hi, s);
+        if (recon_mask) mul_launch(p.as<float>(), recon_mask, nv, s);
+        dot_launch(p.as<float>(), p.as<float>(), nv, g0, s);
+        for (int it = 0; it < iters; ++it) {
+            // w = A p ; alpha = gamma / <w, w>
+            op.fp(p.as<float>(), w.as<float>(), 1.f, OUT_SET, nullptr, nullptr, s);
+            dot_launch(w.as<float>(), w.as<float>(), np, ww, s);
+            axpy_ratio_launch(x, p.as<float>(), nv, g0, ww, 1.0, s);
+            axpy_ratio_launch(r.as<float>(), w.as<float>(), np, g0, ww, -1.0, s);
+            // z = A^T r ; beta = <z,z> / gamma ; p = z + beta p
+            op.bp(r.as<float>(), z.as<float>(), 1.f, OUT_SET, nullptr, lo, hi, s);
+            if (recon_mask) mul_launch(z.as<float>(), recon_mask, nv, s);
+            dot_launch(z.as<float>(), z.as<float>(), nv, g1, s);
+            xpby_ratio_launch(p.as<float>(), z.as<float>(), nv, g1, g0, s);
+            std::swap(g0, g1);
+        }
+    }
+};
+
+AlgoBase *algo_build(int type, Geom2D *g2, Geom3D *g3, float out_scale, int kernel,
+                     unsigned rays) {
+    AlgoBase *a = nullptr;
+    if (type == ASTRA_AMD_ALGO_SIRT)
+        a = new Sirt();
+    else if (type == ASTRA_AMD_ALGO_CGLS)
+        a = new Cgls();
+    else
+        fail(ASTRA_AMD_ERR_ARG, "algorithm: unknown type %d", type);
+    a->type = type;
+    a->op.g2 = g2;
+    a->op.g3 = g3;
+    a->op.out_scale = out_scale;
+    a->op.kernel = kernel;
+    a->op.rays = rays;
+    AMX_HIP(hipGetDevice(&a->device));
+    return a;
+}
+
+void algo_destroy(AlgoBase *a) { delete a; }
+
+void algo_set_option(AlgoBase *a, const char *key, double v) {
+    std::string k(key ? key : "");
+    if (k == "Relaxation") {
+        a->relaxation = v;
+        a->weights_ready = false;
+    } else if (k == "MinConstraint") {
+        a->lo = (float)v;
+    } else if (k == "MaxConstraint") {
+        a->hi = (float)v;
+    } else {
+        fail(ASTRA_AMD_ERR_ARG, "algorithm: unknown option '%s'", k.c_str());
+    }
+}
+
+void algo_set_mask(AlgoBase *a, const char *key, const float *m) {
+    std::string k(key ? key : "");
+    if (k == "ReconstructionMask") a->recon_mask = m;
+    else if (k == "SinogramMask") a->sino_mask = m;
+    else if (k == "MinMask") a->min_mask = m;
+    else if (k == "MaxMask") a->max_mask = m;
+    else fail(ASTRA_AMD_ERR_ARG, "algorithm: unknown mask '%s'", k.c_str());
+    a->weights_ready = false;
+}
+
+void algo_run(AlgoBase *a, const float *b, float *x, int iters, hipStream_t s) {
+    std::lock_guard<std::mutex> lk(a->mu);
+    a->run(b, x, iters, s);
+}
+
+double algo_residual(AlgoBase *a, const float *b, const float *x, hipStream_t s) {
+    std::lock_guard<std::mutex> lk(a->mu);
+    return a->residual(b, x, s);
+}
+
+// ----------------------------------------------------------------- FDK ---
+Filter *filter_build(unsigned n_det, const float *half, unsigned n_bins) {
+    AMX_REQUIRE(n_det > 0, "filter: n_det must be positive");
+    unsigned n = next_pow2(2 * n_det);
+    AMX_REQUIRE(n_bins == n / 2 + 1, "filter: expected %u spectrum bins for %u detectors, got %u",
+                n / 2 + 1, n_det, n_bins);
+    AMX_REQUIRE(half != nullptr, "filter: spectrum is NULL");
+    auto *f = new Filter();
+    f->n_det = n_det;
+    f->n_fft = n;
+    AMX_HIP(hipGetDevice(&f->device));
+    std::vector<float> full(n);
+    for (unsigned k = 0; k < n; ++k) full[k] = half[k <= n / 2 ? k : n - k];
+    std::vector<float> tw(n);  // n/2 complex
+    for (unsigned k = 0; k < n / 2; ++k) {
+        double ang = -2.0 * M_PI * (double)k / (double)n;
+        tw[2 * k] = (float)std::cos(ang);
+        tw[2 * k + 1] = (float)std::sin(ang);
+    }
+    try {
+        f->spec = DevArray(sizeof(float) * n);
+        f->spec.upload(full.data(), sizeof(float) * n);
+        f->twiddle = DevArray(sizeof(float) * n);
+        f->twiddle.upload(tw.data(), sizeof(float) * n);
+    } catch (...) {
+        delete f;
+        throw;
+    }
+    return f;
+}
+
+Filter *ramlak_filter(unsigned n_det) {
+    std::vector<double> h;
+    ramlak_half_spectrum(n_det, 1.0, h);
+    std::vector<float> hf(h.begin(), h.end());
+    return filter_build(n_det, hf.data(), (unsigned)hf.size());
+}
+
+// FDK pre-weights and default filter of a geometry, built once
+static void fdk_prepare(const Geom3D &g) {
+    std::lock_guard<std::mutex> lk(g.fdk_mu);
+    if (g.fdk_filter) return;
+    std::vector<float> w;
+    fdk_weights(g, w);
+    DevArray dw(sizeof(float) * w.size());
+    dw.upload(w.data(), sizeof(float) * w.size());
+    g.fdk_w = std::move(dw);
+    g.fdk_filter.reset(ramlak_filter(g.nu));
+}
+
+// astraCUDA3d::FDK (cuda/3d/fdk.cu:415-478): pre-weight, filter, FDK BP
+void fdk_run(const Geom3D &g, float *proj, float *vol, float out_scale, const Filter *f,
+             int mode, hipStream_t s) {
+    AMX_REQUIRE(g.kind == ASTRA_AMD_GEOM_CONE, "FDK requires a cone geometry");
+    fdk_prepare(g);
+    const Filter *flt = f ? f : g.fdk_filter.get();
+    AMX_REQUIRE(flt->n_det == g.nu, "FDK: filter built for %u detectors, geometry has %u",
+                flt->n_det, g.nu);
+    // fused cosine pre-weight + ramp filter over all nv * na rows
+    filter_rows_launch(*flt, proj, (size_t)g.nv * g.na, g.fdk_w.as<float>(), g.na, s);
+    bp3d_launch(g, proj, vol, out_scale, ASTRA_AMD_KER3D_FDK_WEIGHTING, mode, nullptr,
+                -INFINITY, INFINITY, s);
+}
+
+// FDK_PreWeight alone (cuda/3d/fdk.cu:168-264, full scan)
+void fdk_preweight_only(const Geom3D &g, float *proj, hipStream_t s) {
+    AMX_REQUIRE(g.kind == ASTRA_AMD_GEOM_CONE, "FDK pre-weighting requires a cone geometry");
+    fdk_prepare(g);
+    rowscale_launch(proj, (size_t)g.nv * g.na, g.nu, g.fdk_w.as<float>(), g.na, s);
+}
+
+// ----------------------------------------------------------------- FBP ---
+// CCudaFilteredBackProjectionAlgorithm / astraCUDA::FBP
+// (src/CudaFilteredBackProjectionAlgorithm.cpp:174-239, cuda/2d/fbp.cu:77-190)
+void fbp2d_run(const Geom2D &g, float *sino, float *vol, float out_scale, const Filter *f,
+               int mode, hipStream_t s) {
+    std::unique_ptr<Filter> own;
+    if (!f) {
+        own.reset(ramlak_filter(g.nd));
+        f = own.get();
+    }
+    AMX_REQUIRE(f->n_det == g.nd, "FBP: filter built for %u detectors, geometry has %u",
+                f->n_det, g.nd);
+    if (g.kind == ASTRA_AMD_GEOM_FAN) {
+        // one-row cone pre-weighting with getFanParameters of the last angle
+        // (cuda/2d/fbp.cu:112-148, src/GeometryUtil2D.cpp:138-177)
+        const float *p = &g.vecs[6 * (size_t)(g.na - 1)];
+        double sx = p[0], sy = p[1], dsx = p[2], dsy = p[3], ux = p[4], uy = p[5];
+        double sod = std::sqrt(sx * sx + sy * sy);
+        double du = std::sqrt(ux * ux + uy * uy);
+        double t = (-dsx * ux - dsy * uy) / (ux * ux + uy * uy);
+        double odd = std::sqrt((dsx + t * ux) * (dsx + t * ux) + (dsy + t * uy) * (dsy + t * uy));
+        double D = sod + odd;
+        double W = D * (D / (du * sod)) * (M_PI / 2.0) / (double)g.na;
+        std::vector<float> w(g.nd);
+        for (unsigned i = 0; i < g.nd; ++i) {
+            double U = (i - 0.5 * g.nd + 0.5) * du;
+            w[i] = (float)(W / std::sqrt(D * D + U * U));
+        }
+        DevArray dw(sizeof(float) * g.nd);
+        dw.upload(w.data(), sizeof(float) * g.nd);
+        filter_rows_launch(*f, sino, g.na, dw.as<float>(), g.na, s);
+        bp2d_launch(g, sino, vol, out_scale, true, mode, nullptr, -INFINITY, INFINITY, s);
+        AMX_HIP(hipStreamSynchronize(s));  // dw and own are released on return
+    } else {
+        filter_rows_launch(*f, sino, g.na, nullptr, 1, s);
+        bp2d_launch(g, sino, vol, out_scale * (float)((M_PI / 2.0) / (double)g.na), false, mode,
+                    nullptr, -INFINITY, INFINITY, s);
+        if (own) AMX_HIP(hipStreamSynchronize(s));
+    }
+}
+
+}  // namespace amx

@@ -1,0 +1,398 @@
+// extern "C" entry points of libastra_amd (declared in include/astra_amd.h).
+// Every entry point converts exceptions into a status code and a
+// thread-local message, the role CLogger::getLastErrMsg plays in the
+// reference (include/astra/Logging.h:169).
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "devutil.hpp"
+#include "internal.hpp"
+
+namespace amx {
+struct AlgoBase;
+AlgoBase *algo_build(int type, Geom2D *g2, Geom3D *g3, float out_scale, int kernel,
+                     unsigned rays);
+void algo_destroy(AlgoBase *a);
+void algo_set_option(AlgoBase *a, const char *key, double v);
+void algo_set_mask(AlgoBase *a, const char *key, const float *m);
+void algo_run(AlgoBase *a, const float *b, float *x, int iters, hipStream_t s);
+double algo_residual(AlgoBase *a, const float *b, const float *x, hipStream_t s);
+Filter *filter_build(unsigned n_det, const float *half, unsigned n_bins);
+void fdk_run(const Geom3D &g, float *proj, float *vol, float out_scale, const Filter *f,
+             int mode, hipStream_t s);
+void fdk_preweight_only(const Geom3D &g, float *proj, hipStream_t s);
+void fbp2d_run(const Geom2D &g, float *sino, float *vol, float out_scale, const Filter *f,
+               int mode, hipStream_t s);
+}  // namespace amx
+
+struct astra_amd_geom2d_s {
+    amx::Geom2D *g;
+};
+struct astra_amd_geom3d_s {
+    amx::Geom3D *g;
+};
+struct astra_amd_filter_s {
+    amx::Filter *f;
+};
+struct astra_amd_algo_s {
+    amx::AlgoBase *a;
+    int device;
+};
+
+namespace {
+
+thread_local std::string t_err;
+
+template <class F> int guarded(F &&fn) {
+    try {
+        fn();
+        return ASTRA_AMD_OK;
+    } catch (const amx::Error &e) {
+        t_err = e.what();
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        t_err = "host allocation failed";
+        return ASTRA_AMD_ERR_ALLOC;
+    } catch (const std::exception &e) {
+        t_err = e.what();
+        return ASTRA_AMD_ERR_INTERNAL;
+    } catch (...) {
+        t_err = "unknown error";
+        return ASTRA_AMD_ERR_INTERNAL;
+    }
+}
+
+void check_device(int dev) {
+    int cur = -1;
+    AMX_HIP(hipGetDevice(&cur));
+    AMX_REQUIRE(cur == dev, "handle belongs to device %d but device %d is current", dev, cur);
+}
+
+int out_mode(int mode) {
+    AMX_REQUIRE(mode == ASTRA_AMD_MODE_ADD || mode == ASTRA_AMD_MODE_SET,
+                "mode must be ASTRA_AMD_MODE_ADD or ASTRA_AMD_MODE_SET (got %d)", mode);
+    return mode == ASTRA_AMD_MODE_ADD ? amx::OUT_ADD : amx::OUT_SET;
+}
+
+hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+}  // namespace
+
+extern "C" {
+
+const char *astra_amd_last_error(void) { return t_err.c_str(); }
+
+const char *astra_amd_version(void) { return "astra_amd 0.1 (gfx950)"; }
+
+int astra_amd_set_device(int index) {
+    return guarded([&] { AMX_HIP(hipSetDevice(index)); });
+}
+
+int astra_amd_stream_sync(void *stream) {
+    return guarded([&] { AMX_HIP(hipStreamSynchronize(as_stream(stream))); });
+}
+
+// ------------------------------------------------------------- 2D -------
+int astra_amd_geom2d_create(int kind, unsigned int vol_width, unsigned int vol_height,
+                            unsigned int n_angles, unsigned int n_dets, const float *vecs,
+                            astra_amd_geom2d *out) {
+    return guarded([&] {
+        AMX_REQUIRE(out != nullptr, "geom2d_create: out is NULL");
+        *out = nullptr;
+        amx::Geom2D *g = amx::geom2d_build(kind, vol_width, vol_height, n_angles, n_dets, vecs);
+        *out = new astra_amd_geom2d_s{g};
+    });
+}
+
+int astra_amd_geom2d_destroy(astra_amd_geom2d g) {
+    return guarded([&] {
+        if (!g) return;
+        delete g->g;
+        delete g;
+    });
+}
+
+static void check2d(astra_amd_geom2d g, const astra_amd_params2d *p) {
+    AMX_REQUIRE(g && g->g, "geometry handle is NULL");
+    AMX_REQUIRE(p != nullptr, "params is NULL");
+    check_device(g->g->device);
+}
+
+int astra_amd_fp2d(astra_amd_geom2d g, const float *vol, float *sino,
+                   const astra_amd_params2d *p, int mode, void *stream) {
+    return guarded([&] {
+        check2d(g, p);
+        AMX_REQUIRE(vol && sino, "fp2d: NULL data pointer");
+        unsigned rays = p->rays_per_det ? p->rays_per_det : 1;
+        amx::fp2d_launch(*g->g, vol, sino, p->output_scale, out_mode(mode), nullptr, nullptr,
+                         false, rays, as_stream(stream));
+    });
+}
+
+int astra_amd_bp2d(astra_amd_geom2d g, const float *sino, float *vol,
+                   const astra_amd_params2d *p, int mode, void *stream) {
+    return guarded([&] {
+        check2d(g, p);
+        AMX_REQUIRE(vol && sino, "bp2d: NULL data pointer");
+        if (p->rays_per_pixel_dim > 1)
+            amx::fail(ASTRA_AMD_ERR_UNSUPPORTED, "bp2d: VoxelSuperSampling > 1 is not built");
+        amx::bp2d_launch(*g->g, sino, vol, p->output_scale, false, out_mode(mode), nullptr,
+                         -INFINITY, INFINITY, as_stream(stream));
+    });
+}
+
+int astra_amd_bp2d_fbp_weighted(astra_amd_geom2d g, const float *sino, float *vol,
+                                const astra_amd_params2d *p, int mode, void *stream) {
+    return guarded([&] {
+        check2d(g, p);
+        AMX_REQUIRE(vol && sino, "bp2d_fbp_weighted: NULL data pointer");
+        AMX_REQUIRE(g->g->kind == ASTRA_AMD_GEOM_FAN, "bp2d_fbp_weighted: FAN geometry required");
+        amx::bp2d_launch(*g->g, sino, vol, p->output_scale, true, out_mode(mode), nullptr,
+                         -INFINITY, INFINITY, as_stream(stream));
+    });
+}
+
+int astra_amd_bp2d_matched(astra_amd_geom2d g, const float *sino, float *vol,
+                           const astra_amd_params2d *p, int mode, void *stream) {
+    return guarded([&] {
+        check2d(g, p);
+        AMX_REQUIRE(vol && sino, "bp2d_matched: NULL data pointer");
+        amx::bp2d_matched_launch(*g->g, sino, vol, p->output_scale, out_mode(mode),
+                                 as_stream(stream));
+    });
+}
+
+// ------------------------------------------------------------- 3D -------
+int astra_amd_geom3d_create(int kind, unsigned int vol_x, unsigned int vol_y,
+                            unsigned int vol_z, unsigned int n_angles, unsigned int det_u,
+                            unsigned int det_v, const double *vecs, const float *vol_scale,
+                            astra_amd_geom3d *out) {
+    return guarded([&] {
+        AMX_REQUIRE(out != nullptr, "geom3d_create: out is NULL");
+        *out = nullptr;
+        amx::Geom3D *g = amx::geom3d_build(kind, vol_x, vol_y, vol_z, n_angles, det_u, det_v,
+                                           vecs, vol_scale);
+        *out = new astra_amd_geom3d_s{g};
+    });
+}
+
+int astra_amd_geom3d_destroy(astra_amd_geom3d g) {
+    return guarded([&] {
+        if (!g) return;
+        delete g->g;
+        delete g;
+    });
+}
+
+static void check3d(astra_amd_geom3d g, const astra_amd_params3d *p) {
+    AMX_REQUIRE(g && g->g, "geometry handle is NULL");
+    AMX_REQUIRE(p != nullptr, "params is NULL");
+    check_device(g->g->device);
+    if (p->rays_per_det_dim > 1 || p->rays_per_voxel_dim > 1)
+        amx::fail(ASTRA_AMD_ERR_UNSUPPORTED, "3D supersampling > 1 is not built");
+}
+
+int astra_amd_fp3d(astra_amd_geom3d g, const float *vol, float *proj,
+                   const astra_amd_params3d *p, int mode, void *stream) {
+    return guarded([&] {
+        check3d(g, p);
+        AMX_REQUIRE(vol && proj, "fp3d: NULL data pointer");
+        if (p->kernel == ASTRA_AMD_KER3D_SUM_SQUARE_WEIGHTS)
+            amx::fail(ASTRA_AMD_ERR_UNSUPPORTED, "fp3d: sum_square_weights kernel is not built");
+        AMX_REQUIRE(p->kernel >= 0 && p->kernel <= ASTRA_AMD_KER3D_MATCHED_BP,
+                    "fp3d: unknown kernel %d", p->kernel);
+        amx::fp3d_launch(*g->g, vol, proj, p->output_scale, out_mode(mode), nullptr, nullptr,
+                         false, as_stream(stream));
+    });
+}
+
+int astra_amd_bp3d(astra_amd_geom3d g, const float *proj, float *vol,
+                   const astra_amd_params3d *p, int mode, void *stream) {
+    return guarded([&] {
+        check3d(g, p);
+        AMX_REQUIRE(vol && proj, "bp3d: NULL data pointer");
+        const int m = out_mode(mode);
+        switch (p->kernel) {
+            case ASTRA_AMD_KER3D_DEFAULT:
+            case ASTRA_AMD_KER3D_2D_WEIGHTING:
+                amx::bp3d_launch(*g->g, proj, vol, p->output_scale, p->kernel, m, nullptr,
+                                 -INFINITY, INFINITY, as_stream(stream));
+                break;
+            case ASTRA_AMD_KER3D_FDK_WEIGHTING:
+                AMX_REQUIRE(g->g->kind == ASTRA_AMD_GEOM_CONE,
+                            "bp3d: fdk_weighting requires a CONE geometry");
+                amx::bp3d_launch(*g->g, proj, vol, p->output_scale, p->kernel, m, nullptr,
+                                 -INFINITY, INFINITY, as_stream(stream));
+                break;
+            case ASTRA_AMD_KER3D_MATCHED_BP:
+                amx::bp3d_matched_launch(*g->g, proj, vol, p->output_scale, m, as_stream(stream));
+                break;
+            case ASTRA_AMD_KER3D_SUM_SQUARE_WEIGHTS:
+                amx::fail(ASTRA_AMD_ERR_UNSUPPORTED, "bp3d: sum_square_weights is FP-only");
+            default:
+                amx::fail(ASTRA_AMD_ERR_ARG, "bp3d: unknown kernel %d", p->kernel);
+        }
+    });
+}
+
+// ---------------------------------------------------------- filters -----
+int astra_amd_ramlak_spectrum(unsigned int n_det, float d, float *out, unsigned int n_out) {
+    return guarded([&] {
+        AMX_REQUIRE(n_det > 0 && out != nullptr, "ramlak_spectrum: invalid arguments");
+        std::vector<double> h;
+        amx::ramlak_half_spectrum(n_det, d, h);
+        AMX_REQUIRE(n_out == h.size(), "ramlak_spectrum: expected n_out = %zu", h.size());
+        for (size_t k = 0; k < h.size(); ++k) out[k] = (float)h[k];
+    });
+}
+
+int astra_amd_filter_create(unsigned int n_det, const float *half_spectrum, unsigned int n_bins,
+                            astra_amd_filter *out) {
+    return guarded([&] {
+        AMX_REQUIRE(out != nullptr, "filter_create: out is NULL");
+        *out = nullptr;
+        amx::Filter *f = amx::filter_build(n_det, half_spectrum, n_bins);
+        *out = new astra_amd_filter_s{f};
+    });
+}
+
+int astra_amd_filter_destroy(astra_amd_filter f) {
+    return guarded([&] {
+        if (!f) return;
+        delete f->f;
+        delete f;
+    });
+}
+
+int astra_amd_filter_rows(astra_amd_filter f, float *rows, size_t n_rows, void *stream) {
+    return guarded([&] {
+        AMX_REQUIRE(f && f->f, "filter handle is NULL");
+        AMX_REQUIRE(rows != nullptr || n_rows == 0, "filter_rows: NULL data pointer");
+        check_device(f->f->device);
+        amx::filter_rows_launch(*f->f, rows, n_rows, nullptr, 1, as_stream(stream));
+    });
+}
+
+int astra_amd_fdk_preweight(astra_amd_geom3d g, float *proj, void *stream) {
+    return guarded([&] {
+        AMX_REQUIRE(g && g->g, "geometry handle is NULL");
+        AMX_REQUIRE(proj != nullptr, "fdk_preweight: NULL data pointer");
+        check_device(g->g->device);
+        amx::fdk_preweight_only(*g->g, proj, as_stream(stream));
+    });
+}
+
+int astra_amd_fdk(astra_amd_geom3d g, float *proj, float *vol, float output_scale,
+                  int short_scan, astra_amd_filter f, int mode, void *stream) {
+    return guarded([&] {
+        AMX_REQUIRE(g && g->g, "geometry handle is NULL");
+        AMX_REQUIRE(proj && vol, "fdk: NULL data pointer");
+        check_device(g->g->device);
+        if (short_scan)
+            amx::fail(ASTRA_AMD_ERR_UNSUPPORTED, "fdk: short-scan (Parker) weighting is not built");
+        amx::fdk_run(*g->g, proj, vol, output_scale, f ? f->f : nullptr, out_mode(mode),
+                     as_stream(stream));
+    });
+}
+
+int astra_amd_fbp2d(astra_amd_geom2d g, float *sino, float *vol, float output_scale,
+                    astra_amd_filter f, int mode, void *stream) {
+    return guarded([&] {
+        AMX_REQUIRE(g && g->g, "geometry handle is NULL");
+        AMX_REQUIRE(sino && vol, "fbp2d: NULL data pointer");
+        check_device(g->g->device);
+        amx::fbp2d_run(*g->g, sino, vol, output_scale, f ? f->f : nullptr, out_mode(mode),
+                       as_stream(stream));
+    });
+}
+
+// -------------------------------------------------------- elementwise ---
+int astra_amd_dot(const float *a, const float *b, size_t n, double *out, void *stream) {
+    return guarded([&] {
+        AMX_REQUIRE(out != nullptr, "dot: out is NULL");
+        AMX_REQUIRE((a && b) || n == 0, "dot: NULL data pointer");
+        hipStream_t s = as_stream(stream);
+        amx::Scratch r(sizeof(double), s);
+        if (n == 0) {
+            *out = 0.0;
+            return;
+        }
+        amx::dot_launch(a, b, n, r.as<double>(), s);
+        AMX_HIP(hipMemcpyAsync(out, r.as<double>(), sizeof(double), hipMemcpyDeviceToHost, s));
+        AMX_HIP(hipStreamSynchronize(s));
+    });
+}
+
+// -------------------------------------------------------- algorithms ----
+int astra_amd_algo2d_create(int type, astra_amd_geom2d g, const astra_amd_params2d *p,
+                            astra_amd_algo *out) {
+    return guarded([&] {
+        AMX_REQUIRE(out != nullptr, "algo2d_create: out is NULL");
+        *out = nullptr;
+        check2d(g, p);
+        if (p->rays_per_pixel_dim > 1)
+            amx::fail(ASTRA_AMD_ERR_UNSUPPORTED, "VoxelSuperSampling > 1 is not built");
+        unsigned rays = p->rays_per_det ? p->rays_per_det : 1;
+        amx::AlgoBase *a = amx::algo_build(type, g->g, nullptr, p->output_scale, 0, rays);
+        *out = new astra_amd_algo_s{a, g->g->device};
+    });
+}
+
+int astra_amd_algo3d_create(int type, astra_amd_geom3d g, const astra_amd_params3d *p,
+                            astra_amd_algo *out) {
+    return guarded([&] {
+        AMX_REQUIRE(out != nullptr, "algo3d_create: out is NULL");
+        *out = nullptr;
+        check3d(g, p);
+        AMX_REQUIRE(p->kernel == ASTRA_AMD_KER3D_DEFAULT || p->kernel == ASTRA_AMD_KER3D_2D_WEIGHTING,
+                    "algo3d: projection kernel %d not usable for SIRT/CGLS", p->kernel);
+        amx::AlgoBase *a = amx::algo_build(type, nullptr, g->g, p->output_scale, p->kernel, 1);
+        *out = new astra_amd_algo_s{a, g->g->device};
+    });
+}
+
+int astra_amd_algo_set_option(astra_amd_algo a, const char *key, double value) {
+    return guarded([&] {
+        AMX_REQUIRE(a && a->a, "algorithm handle is NULL");
+        amx::algo_set_option(a->a, key, value);
+    });
+}
+
+int astra_amd_algo_set_mask(astra_amd_algo a, const char *key, const float *mask) {
+    return guarded([&] {
+        AMX_REQUIRE(a && a->a, "algorithm handle is NULL");
+        amx::algo_set_mask(a->a, key, mask);
+    });
+}
+
+int astra_amd_algo_run(astra_amd_algo a, const float *sino, float *vol, int iterations,
+                       void *stream) {
+    return guarded([&] {
+        AMX_REQUIRE(a && a->a, "algorithm handle is NULL");
+        AMX_REQUIRE(sino && vol, "algo_run: NULL data pointer");
+        AMX_REQUIRE(iterations >= 0, "algo_run: negative iteration count");
+        check_device(a->device);
+        amx::algo_run(a->a, sino, vol, iterations, as_stream(stream));
+    });
+}
+
+int astra_amd_algo_residual_norm(astra_amd_algo a, const float *sino, const float *vol,
+                                 double *out, void *stream) {
+    return guarded([&] {
+        AMX_REQUIRE(a && a->a && out, "algo_residual_norm: NULL argument");
+        AMX_REQUIRE(sino && vol, "algo_residual_norm: NULL data pointer");
+        check_device(a->device);
+        *out = amx::algo_residual(a->a, sino, vol, as_stream(stream));
+    });
+}
+
+int astra_amd_algo_destroy(astra_amd_algo a) {
+    return guarded([&] {
+        if (!a) return;
+        amx::algo_destroy(a->a);
+        delete a;
+    });
+}
+
+}  // extern "C"

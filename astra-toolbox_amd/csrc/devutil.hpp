@@ -1,0 +1,78 @@
+// Device helpers shared by the HIP kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace amx {
+
+// Zero border (in texels) around every padded copy.  Kernels work in padded
+// texel coordinates p = t + 2 (t = texture coordinate - 0.5) clamped to
+// [0.5, n + 2.5]: the two taps of a linear fetch then land in [0, n + 3],
+// inside the array plus its 2-texel zero border.  The clamp never changes a
+// value (both taps outside the data read 0, as with border addressing) and
+// every access stays in bounds.  v_med3_f32 returns a finite value or NaN;
+// a NaN coordinate converts to offset 0, which is also in bounds.
+constexpr int kPad = 2;
+
+__device__ __forceinline__ float clamp_pad(float p, float hi) {
+    return __builtin_amdgcn_fmed3f(p, 0.5f, hi);
+}
+
+// Pair of floats at base + byte offset (one global_load_dwordx2 with a
+// scalar base and a 32-bit vector offset).
+__device__ __forceinline__ float2 ld2b(const float *base, unsigned byte_off) {
+    typedef float f2u __attribute__((ext_vector_type(2), aligned(4)));
+    f2u v = *reinterpret_cast<const f2u *>(reinterpret_cast<const char *>(base) + byte_off);
+    return make_float2(v.x, v.y);
+}
+
+// byte offset of padded texel (fl1, fl2) in a plane of row pitch `pitch`
+__device__ __forceinline__ unsigned texel_off(float fl1, float fl2, float pitch) {
+    return (unsigned)fmaf(fl2, pitch, fl1) << 2;
+}
+
+// Pair of adjacent floats at a 4-byte aligned address (one dwordx2 load).
+typedef float f2a4 __attribute__((ext_vector_type(2), aligned(4)));
+
+__device__ __forceinline__ float clampc(float t, float hi) {
+    // fmaxf first: a NaN coordinate becomes -1.5 (stays in bounds)
+    return fminf(fmaxf(t, -1.5f), hi);
+}
+
+__device__ __forceinline__ f2a4 ld2(const float *p) { return *reinterpret_cast<const f2a4 *>(p); }
+
+__device__ __forceinline__ float lerp(float a, float b, float w) { return fmaf(w, b - a, a); }
+
+__device__ __forceinline__ int wave_min(int v) {
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ int wave_max(int v) {
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Slice range [lo, hi] of s in [0, n0) for which -1 < a*s + c < n can hold
+// (widened by one slice on each side; exactness is not required because
+// samples outside the array read the zero border).
+__device__ __forceinline__ void clip_range(double a, double c, double n, int n0, int &lo, int &hi) {
+    if (fabs(a) < 1e-12) {
+        if (!(c > -1.0 && c < n)) { lo = 1; hi = 0; }
+        return;
+    }
+    double s1 = (-1.0 - c) / a, s2 = (n - c) / a;
+    double smin = fmin(s1, s2), smax = fmax(s1, s2);
+    double l = floor(smin) - 1.0, h = ceil(smax) + 1.0;
+    if (l > (double)lo) lo = (l > (double)n0) ? n0 : (int)l;
+    if (h < (double)hi) hi = (h < -1.0) ? -1 : (int)h;
+}
+
+// Output epilogue modes (internal)
+// OUT_ADD    out += v
+// OUT_SET    out  = v
+// OUT_RESID  out  = (b + v) * w            (w optional)
+// OUT_UPDATE out  = clamp(out + w * v)     (w optional)
+// OUT_ADDMUL out  = (out + v) * w          (w optional)
+enum { OUT_ADD = 0, OUT_SET = 1, OUT_RESID = 2, OUT_UPDATE = 3, OUT_ADDMUL = 4 };
+
+}  // namespace amx

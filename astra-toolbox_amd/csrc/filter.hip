@@ -1,0 +1,155 @@
+// Row filtering for FBP / FDK on gfx950: a hand-written batched radix-2 FFT
+// in LDS.  Two real rows share one complex transform (row a in the real
+// part, row a+1 in the imaginary part): with a real, even filter h the
+// filtered rows are Re and Im of IFFT(H . FFT(x + i y)).  Forward pass is
+// decimation-in-time (bit-reversed load, natural-order spectrum), the
+// inverse is decimation-in-frequency (natural input, bit-reversed output),
+// so no explicit reordering pass is needed.  The optional FDK cosine
+// pre-weighting is applied while loading (one HBM read + one HBM write per
+// sample for pre-weight + filter together).
+//
+// Semantics: zero-pad each row of n_det samples to n_fft, multiply the
+// spectrum by the real filter, inverse transform, divide by n_fft and keep
+// the first n_det samples (cuda/3d/fdk.cu:351-398, cuda/2d/fft.cu:52-131).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "devutil.hpp"
+#include "internal.hpp"
+
+namespace amx {
+
+struct FilterArgs {
+    float *rows;
+    size_t n_rows;
+    unsigned n_det, n_fft, log2n;
+    const float *spec;    // [n_fft] real spectrum
+    const float2 *tw;     // [n_fft/2] exp(-2 pi i k / n_fft)
+    const float *weights; // optional [n_rows / rows_per_weight][n_det]
+    size_t rows_per_weight;
+};
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+__global__ __launch_bounds__(256) void filter_rows_kernel(FilterArgs A) {
+    extern __shared__ float2 X[];
+    const unsigned N = A.n_fft;
+    const unsigned half_n = N >> 1;
+    const size_t r0 = (size_t)blockIdx.x * 2;
+    const bool has1 = r0 + 1 < A.n_rows;
+    float *row0 = A.rows + r0 * A.n_det;
+    float *row1 = row0 + A.n_det;
+    const float *w0 = A.weights ? A.weights + (r0 / A.rows_per_weight) * A.n_det : nullptr;
+    const float *w1 = (A.weights && has1) ? A.weights + ((r0 + 1) / A.rows_per_weight) * A.n_det : nullptr;
+    const unsigned shift = 32 - A.log2n;
+
+    // load (bit-reversed positions), zero padding beyond n_det
+    for (unsigned j = threadIdx.x; j < N; j += blockDim.x) {
+        float a = 0.f, b = 0.f;
+        if (j < A.n_det) {
+            a = row0[j];
+            if (w0) a *= w0[j];
+            if (has1) {
+                b = row1[j];
+                if (w1) b *= w1[j];
+            }
+        }
+        X[__brev(j) >> shift] = make_float2(a, b);
+    }
+    __syncthreads();
+
+    // forward DIT: natural-order spectrum
+    for (unsigned len = 2, step = half_n; len <= N; len <<= 1, step >>= 1) {
+        const unsigned h = len >> 1;
+        for (unsigned b = threadIdx.x; b < half_n; b += blockDim.x) {
+            unsigned k = b & (h - 1);
+            unsigned i = (b - k) * 2 + k;
+            float2 t = cmul(X[i + h], A.tw[k * step]);
+            float2 u = X[i];
+            X[i] = make_float2(u.x + t.x, u.y + t.y);
+            X[i + h] = make_float2(u.x - t.x, u.y - t.y);
+        }
+        __syncthreads();
+    }
+
+    // x filter, conjugate (inverse via conj(FFT(conj(.))))
+    for (unsigned k = threadIdx.x; k < N; k += blockDim.x) {
+        float hk = A.spec[k];
+        float2 v = X[k];
+        X[k] = make_float2(v.x * hk, -v.y * hk);
+    }
+    __syncthreads();
+
+    // DIF: bit-reversed output
+    for (unsigned len = N, step = 1; len >= 2; len >>= 1, step <<= 1) {
+        const unsigned h = len >> 1;
+        for (unsigned b = threadIdx.x; b < half_n; b += blockDim.x) {
+            unsigned k = b & (h - 1);
+            unsigned i = (b - k) * 2 + k;
+            float2 u = X[i], v = X[i + h];
+            X[i] = make_float2(u.x + v.x, u.y + v.y);
+            X[i + h] = cmul(make_float2(u.x - v.x, u.y - v.y), A.tw[k * step]);
+        }
+        __syncthreads();
+    }
+
+    // y = conj(F) / N: row0 = Re F / N, row1 = -Im F / N
+    const float inv = 1.0f / (float)N;
+    for (unsigned j = threadIdx.x; j < A.n_det; j += blockDim.x) {
+        float2 v = X[__brev(j) >> shift];
+        row0[j] = v.x * inv;
+        if (has1) row1[j] = -v.y * inv;
+    }
+}
+
+void filter_rows_launch(const Filter &f, float *rows, size_t n_rows, const float *weights,
+                        size_t rows_per_weight, hipStream_t s) {
+    if (n_rows == 0) return;
+    FilterArgs A{};
+    A.rows = rows;
+    A.n_rows = n_rows;
+    A.n_det = f.n_det;
+    A.n_fft = f.n_fft;
+    unsigned l = 0;
+    while ((1u << l) < f.n_fft) ++l;
+    A.log2n = l;
+    A.spec = f.spec.as<float>();
+    A.tw = f.twiddle.as<float2>();
+    A.weights = weights;
+    A.rows_per_weight = rows_per_weight ? rows_per_weight : 1;
+    const size_t lds = (size_t)f.n_fft * sizeof(float2);
+    AMX_REQUIRE(lds <= 65536, "filter: n_fft %u too large for one workgroup", f.n_fft);
+    const size_t groups = (n_rows + 1) / 2;
+    AMX_REQUIRE(groups < (1ull << 31), "filter: too many rows");
+    filter_rows_kernel<<<(unsigned)groups, 256, lds, s>>>(A);
+    AMX_LAUNCHED();
+}
+
+void fdk_weights(const Geom3D &g, std::vector<float> &w) {
+    // devFDK_preweight (cuda/3d/fdk.cu:70-113) with the scalar geometry that
+    // FDK() extracts from angle 0 (cuda/3d/fdk.cu:427-445), evaluated in FP64.
+    const double *p = g.vecs.data();
+    double dcx = p[3] + 0.5 * g.nu * p[6];
+    double dcy = p[4] + 0.5 * g.nu * p[7];
+    double dcz = p[5] + 0.5 * g.nv * p[11];
+    double sod = std::sqrt(p[0] * p[0] + p[1] * p[1]);
+    double odd = std::sqrt(dcx * dcx + dcy * dcy);
+    double du = std::sqrt(p[6] * p[6] + p[7] * p[7]);
+    double dv = std::fabs(p[11]);
+    double zshift = dcz - p[2];
+    double D = sod + odd;
+    double W = D * (D / (du * sod)) * (M_PI / 2.0) / (double)g.na;
+    w.resize((size_t)g.nv * g.nu);
+    for (unsigned iv = 0; iv < g.nv; ++iv) {
+        double V = (iv - 0.5 * g.nv + 0.5) * dv + zshift;
+        for (unsigned iu = 0; iu < g.nu; ++iu) {
+            double U = (iu - 0.5 * g.nu + 0.5) * du;
+            w[(size_t)iv * g.nu + iu] = (float)(W / std::sqrt(D * D + U * U + V * V));
+        }
+    }
+}
+
+}  // namespace amx

@@ -1,0 +1,371 @@
+// Host-side geometry precomputation (FP64) for the device kernels, plus the
+// small runtime helpers (errors, scratch allocation).
+//
+// The per-angle coefficients follow the reference's transferConstants
+// routines; they are computed once per geometry handle (not once per call as
+// in the reference) and kept in device memory.
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+
+#include "internal.hpp"
+
+namespace amx {
+
+void fail(int code, const char *fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    throw Error(code, buf);
+}
+
+// ------------------------------------------------------------- memory ----
+void configure_pool_once() {
+    static std::mutex m;
+    static std::vector<int> done;
+    int dev = 0;
+    AMX_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(m);
+    for (int d : done)
+        if (d == dev) return;
+    hipMemPool_t pool;
+    if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+        uint64_t thr = UINT64_MAX;
+        (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
+    }
+    (void)hipGetLastError();
+    done.push_back(dev);
+}
+
+Scratch::Scratch(size_t bytes, hipStream_t s) : s_(s) {
+    configure_pool_once();
+    if (bytes == 0) bytes = 256;
+    hipError_t e = hipMallocAsync(&p_, bytes, s);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        fail(ASTRA_AMD_ERR_ALLOC, "device scratch allocation of %zu bytes failed: %s", bytes,
+             hipGetErrorString(e));
+    }
+}
+
+Scratch::~Scratch() {
+    if (p_) (void)hipFreeAsync(p_, s_);
+}
+
+DevArray::DevArray(size_t bytes) : bytes_(bytes) {
+    if (bytes == 0) return;
+    hipError_t e = hipMalloc(&p_, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        p_ = nullptr;
+        fail(ASTRA_AMD_ERR_ALLOC, "device allocation of %zu bytes failed: %s", bytes,
+             hipGetErrorString(e));
+    }
+}
+
+DevArray::~DevArray() {
+    if (p_) (void)hipFree(p_);
+}
+
+DevArray &DevArray::operator=(DevArray &&o) noexcept {
+    if (this != &o) {
+        if (p_) (void)hipFree(p_);
+        p_ = o.p_;
+        bytes_ = o.bytes_;
+        o.p_ = nullptr;
+        o.bytes_ = 0;
+    }
+    return *this;
+}
+
+void DevArray::upload(const void *host, size_t bytes) {
+    AMX_REQUIRE(bytes <= bytes_, "internal: upload larger than buffer");
+    AMX_HIP(hipMemcpy(p_, host, bytes, hipMemcpyHostToDevice));
+}
+
+unsigned next_pow2(unsigned n) {
+    // calcNextPowerOfTwo (src/Filters.cpp:662-669)
+    unsigned x = 1;
+    while (x < n) x *= 2;
+    return x;
+}
+
+// ----------------------------------------------------------- 3D ----------
+namespace {
+
+struct V3 {
+    double x, y, z;
+};
+inline V3 sub(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline double det3x(V3 b, V3 c) { return b.y * c.z - b.z * c.y; }
+inline double det3y(V3 b, V3 c) { return -(b.x * c.z - b.z * c.x); }
+inline double det3z(V3 b, V3 c) { return b.x * c.y - b.y * c.x; }
+inline double det3(V3 a, V3 b, V3 c) {
+    return a.x * det3x(b, c) + a.y * det3y(b, c) + a.z * det3z(b, c);
+}
+// norm of cross(a, b) with components scaled by (sy sz, sx sz, sx sy)
+inline double scaled_cross_norm(V3 a, V3 b, const float *sc) {
+    double cx = det3x(a, b) * sc[1] * sc[2];
+    double cy = det3y(a, b) * sc[0] * sc[2];
+    double cz = det3z(a, b) * sc[0] * sc[1];
+    return std::sqrt(cx * cx + cy * cy + cz * cz);
+}
+
+const int kPerm[3][3] = {{0, 1, 2}, {1, 0, 2}, {2, 0, 1}};
+
+int principal_axis(float dX, float dY, float dZ) {
+    // ties resolve to the lower axis (cuda/3d/cone_fp.cu:383-388)
+    if (dX >= dY && dX >= dZ) return 0;
+    if (dY >= dX && dY >= dZ) return 1;
+    return 2;
+}
+
+void cone_bp_coeffs(const double *p, int kernel, const float *vs, BP3DAngle &o) {
+    // DevConeParams of cuda/3d/cone_bp.cu:221-302, with the texel-centre
+    // offset (-0.5) and the 2-texel padding (+2) folded into the numerators.
+    V3 s{p[0], p[1], p[2]}, d{p[3], p[4], p[5]}, u{p[6], p[7], p[8]}, v{p[9], p[10], p[11]};
+    V3 smd = sub(s, d);
+    double sc;
+    if (kernel == ASTRA_AMD_KER3D_FDK_WEIGHTING)
+        sc = 1.0 / det3(u, v, s);
+    else if (kernel == ASTRA_AMD_KER3D_2D_WEIGHTING)
+        sc = scaled_cross_norm(u, V3{0, 0, 1}, vs) / det3(u, v, smd);
+    else
+        sc = std::sqrt(scaled_cross_norm(u, v, vs)) / det3(u, v, smd);
+    double nu[4] = {sc * det3x(v, smd), sc * det3y(v, smd), sc * det3z(v, smd), sc * det3(s, v, d)};
+    double nv[4] = {-sc * det3x(u, smd), -sc * det3y(u, smd), -sc * det3z(u, smd),
+                    -sc * det3(s, u, d)};
+    double dn[4] = {-sc * det3x(u, v), -sc * det3y(u, v), -sc * det3z(u, v), sc * det3(u, v, s)};
+    if (kernel == ASTRA_AMD_KER3D_FDK_WEIGHTING) dn[3] = 1.0;  // cone_bp.cu:101
+    for (int k = 0; k < 4; ++k) {
+        o.nu[k] = (float)(nu[k] + 1.5 * dn[k]);
+        o.nv[k] = (float)(nv[k] + 1.5 * dn[k]);
+        o.dn[k] = (float)dn[k];
+    }
+}
+
+void par3d_bp_coeffs(const double *p, int kernel, const float *vs, BP3DAngle &o, float &w) {
+    // DevPar3DParams + scale of cuda/3d/par3d_bp.cu:202-244
+    V3 r{p[0], p[1], p[2]}, d{p[3], p[4], p[5]}, u{p[6], p[7], p[8]}, v{p[9], p[10], p[11]};
+    double den = det3(r, u, v);
+    double nu[4] = {-det3x(r, v) / den, -det3y(r, v) / den, -det3z(r, v) / den,
+                    -det3(r, d, v) / den};
+    double nv[4] = {det3x(r, u) / den, det3y(r, u) / den, det3z(r, u) / den, det3(r, d, u) / den};
+    for (int k = 0; k < 4; ++k) {
+        o.nu[k] = (float)(nu[k] + (k == 3 ? 1.5 : 0.0));
+        o.nv[k] = (float)(nv[k] + (k == 3 ? 1.5 : 0.0));
+        o.dn[k] = (k == 3) ? 1.0f : 0.0f;
+    }
+    if (kernel == ASTRA_AMD_KER3D_2D_WEIGHTING)
+        w = (float)(1.0 / scaled_cross_norm(u, V3{0, 0, 1}, vs));
+    else
+        w = (float)(1.0 / scaled_cross_norm(u, v, vs));
+}
+
+}  // namespace
+
+Geom3D *geom3d_build(int kind, unsigned nx, unsigned ny, unsigned nz, unsigned na, unsigned nu,
+                     unsigned nv, const double *vecs, const float *vol_scale) {
+    AMX_REQUIRE(kind == ASTRA_AMD_GEOM_CONE || kind == ASTRA_AMD_GEOM_PAR3D,
+                "geom3d: kind must be CONE or PAR3D (got %d)", kind);
+    AMX_REQUIRE(nx > 0 && ny > 0 && nz > 0 && na > 0 && nu > 0 && nv > 0,
+                "geom3d: all dimensions must be positive");
+    AMX_REQUIRE(vecs != nullptr, "geom3d: vecs is NULL");
+    // padded planes must stay below 2^24 texels (offsets are formed in fp32)
+    AMX_REQUIRE(nx <= 4000 && ny <= 4000 && nz <= 4000 && nu <= 4000 && nv <= 4000,
+                "geom3d: dimensions must be <= 4000");
+    const float one[3] = {1.f, 1.f, 1.f};
+    const float *vs = vol_scale ? vol_scale : one;
+    AMX_REQUIRE(vs[0] > 0 && vs[1] > 0 && vs[2] > 0, "geom3d: voxel sizes must be positive");
+
+    auto *g = new Geom3D();
+    g->kind = kind;
+    g->nx = nx; g->ny = ny; g->nz = nz; g->na = na; g->nu = nu; g->nv = nv;
+    for (int k = 0; k < 3; ++k) g->vol_scale[k] = vs[k];
+    AMX_HIP(hipGetDevice(&g->device));
+    g->vecs.assign(vecs, vecs + 12 * (size_t)na);
+    g->fp_host.resize(na);
+    g->dir_count.assign(3, 0);
+
+    std::vector<BP3DAngle> bd(na), bf(na), b2(na);
+    std::vector<float> wd(na, 0.f), w2(na, 0.f);
+    for (unsigned a = 0; a < na; ++a) {
+        const double *p = vecs + 12 * (size_t)a;
+        for (int k = 0; k < 12; ++k)
+            AMX_REQUIRE(std::isfinite(p[k]), "geom3d: non-finite vector component (angle %u)", a);
+        int dir;
+        if (kind == ASTRA_AMD_GEOM_CONE) {
+            // cuda/3d/cone_fp.cu:378-387: |src - detector centre| per axis,
+            // evaluated in double and compared as float
+            float d3[3];
+            for (int k = 0; k < 3; ++k)
+                d3[k] = std::fabs((float)(p[k] - (p[3 + k] + nu * p[6 + k] * 0.5f +
+                                                   nv * p[9 + k] * 0.5f)));
+            dir = principal_axis(d3[0], d3[1], d3[2]);
+        } else {
+            // cuda/3d/par3d_fp.cu:478-487
+            dir = principal_axis(std::fabs((float)p[0]), std::fabs((float)p[1]),
+                                 std::fabs((float)p[2]));
+        }
+        FP3DAngle &f = g->fp_host[a];
+        f.dir = dir;
+        f.pad_ = 0;
+        g->dir_count[dir]++;
+        const int *pm = kPerm[dir];
+        for (int k = 0; k < 3; ++k) {
+            int c = pm[k];
+            f.s[k] = p[c];
+            f.p0[k] = p[3 + c] + 0.5 * p[6 + c] + 0.5 * p[9 + c];
+            f.u[k] = p[6 + c];
+            f.v[k] = p[9 + c];
+        }
+        if (kind == ASTRA_AMD_GEOM_CONE) {
+            cone_bp_coeffs(p, ASTRA_AMD_KER3D_DEFAULT, vs, bd[a]);
+            cone_bp_coeffs(p, ASTRA_AMD_KER3D_FDK_WEIGHTING, vs, bf[a]);
+            cone_bp_coeffs(p, ASTRA_AMD_KER3D_2D_WEIGHTING, vs, b2[a]);
+        } else {
+            par3d_bp_coeffs(p, ASTRA_AMD_KER3D_DEFAULT, vs, bd[a], wd[a]);
+            par3d_bp_coeffs(p, ASTRA_AMD_KER3D_2D_WEIGHTING, vs, b2[a], w2[a]);
+            bf[a] = bd[a];
+        }
+    }
+    try {
+        g->fp = DevArray(sizeof(FP3DAngle) * na);
+        g->fp.upload(g->fp_host.data(), sizeof(FP3DAngle) * na);
+        g->bp_default = DevArray(sizeof(BP3DAngle) * na);
+        g->bp_default.upload(bd.data(), sizeof(BP3DAngle) * na);
+        g->bp_fdk = DevArray(sizeof(BP3DAngle) * na);
+        g->bp_fdk.upload(bf.data(), sizeof(BP3DAngle) * na);
+        g->bp_2d = DevArray(sizeof(BP3DAngle) * na);
+        g->bp_2d.upload(b2.data(), sizeof(BP3DAngle) * na);
+        g->bp_w_default = DevArray(sizeof(float) * na);
+        g->bp_w_default.upload(wd.data(), sizeof(float) * na);
+        g->bp_w_2d = DevArray(sizeof(float) * na);
+        g->bp_w_2d.upload(w2.data(), sizeof(float) * na);
+        for (int k = 0; k < 3; ++k) {
+            std::vector<int> idx;
+            for (unsigned a = 0; a < na; ++a)
+                if (g->fp_host[a].dir == k) idx.push_back((int)a);
+            if (!idx.empty()) {
+                g->group_idx[k] = DevArray(sizeof(int) * idx.size());
+                g->group_idx[k].upload(idx.data(), sizeof(int) * idx.size());
+            }
+        }
+    } catch (...) {
+        delete g;
+        throw;
+    }
+    return g;
+}
+
+// ----------------------------------------------------------- 2D ----------
+Geom2D *geom2d_build(int kind, unsigned w, unsigned h, unsigned na, unsigned nd,
+                     const float *vecs) {
+    AMX_REQUIRE(kind == ASTRA_AMD_GEOM_PAR || kind == ASTRA_AMD_GEOM_FAN,
+                "geom2d: kind must be PAR or FAN (got %d)", kind);
+    AMX_REQUIRE(w > 0 && h > 0 && na > 0 && nd > 0, "geom2d: all dimensions must be positive");
+    AMX_REQUIRE(w <= 16000 && h <= 16000 && nd <= 1000000, "geom2d: dimensions too large");
+    AMX_REQUIRE(vecs != nullptr, "geom2d: vecs is NULL");
+    auto *g = new Geom2D();
+    g->kind = kind;
+    g->w = w; g->h = h; g->na = na; g->nd = nd;
+    AMX_HIP(hipGetDevice(&g->device));
+    g->vecs.assign(vecs, vecs + 6 * (size_t)na);
+
+    std::vector<FP2DAngle> fp(na);
+    std::vector<BP2DAngle> bp(na), bpf(na);
+    for (unsigned a = 0; a < na; ++a) {
+        const float *p = vecs + 6 * (size_t)a;
+        for (int k = 0; k < 6; ++k)
+            AMX_REQUIRE(std::isfinite(p[k]), "geom2d: non-finite vector component (angle %u)", a);
+        FP2DAngle &f = fp[a];
+        f.p0[0] = (double)p[2] + 0.5 * (double)p[4];
+        f.p0[1] = (double)p[3] + 0.5 * (double)p[5];
+        f.u[0] = p[4];
+        f.u[1] = p[5];
+        f.r[0] = p[0];
+        f.r[1] = p[1];
+        for (int k = 0; k < 6; ++k) f.raw[k] = p[k];
+        // cuda/2d/par_fp.cu:272 (par only; fan decides per ray)
+        f.vertical = std::fabs(p[0]) <= std::fabs(p[1]) ? 1 : 0;
+        f.pad_ = 0;
+        if (kind == ASTRA_AMD_GEOM_PAR) {
+            // cuda/2d/par_bp.cu:172-178: T = x c - y_row s + off with
+            // y_row = -y_world, weight |r| / |d|
+            double rx = p[0], ry = p[1], dsx = p[2], dsy = p[3], ux = p[4], uy = p[5];
+            double d = ux * ry - uy * rx;
+            AMX_REQUIRE(d != 0.0, "geom2d: detector parallel to rays (angle %u)", a);
+            double c = ry / d, s = -rx / d, off = (dsy * rx - dsx * ry) / d;
+            BP2DAngle &b = bp[a];
+            b.num[0] = (float)(off + 1.5);  // -0.5 texel centre, +2 padding
+            b.num[1] = (float)c;
+            b.num[2] = (float)s;
+            b.den[0] = 1.f; b.den[1] = 0.f; b.den[2] = 0.f;
+            b.w = (float)(std::sqrt(rx * rx + ry * ry) / std::fabs(d));
+            b.pad_ = 0.f;
+            bpf[a] = b;
+        } else {
+            // DevFanParams, cuda/2d/fan_bp.cu:240-271
+            double sx = p[0], sy = p[1], dsx = p[2], dsy = p[3], ux = p[4], uy = p[5];
+            double smdx = sx - dsx, smdy = sy - dsy;
+            auto det2 = [](double ax, double ay, double bx, double by) { return ax * by - ay * bx; };
+            for (int fbp = 0; fbp < 2; ++fbp) {
+                double sc = fbp ? 1.0 / det2(ux, uy, sx, sy)
+                                : std::sqrt(ux * ux + uy * uy) / det2(ux, uy, smdx, smdy);
+                double nc = sc * det2(sx, sy, dsx, dsy), nxc = sc * smdy, nyc = -sc * smdx;
+                double dc = fbp ? 1.0 : sc * det2(ux, uy, sx, sy), dxc = sc * uy, dyc = -sc * ux;
+                BP2DAngle &b = fbp ? bpf[a] : bp[a];
+                b.num[0] = (float)(nc + 1.5 * dc);  // -0.5 texel centre, +2 padding
+                b.num[1] = (float)(nxc + 1.5 * dxc);
+                b.num[2] = (float)(nyc + 1.5 * dyc);
+                b.den[0] = (float)dc;
+                b.den[1] = (float)dxc;
+                b.den[2] = (float)dyc;
+                b.w = 1.f;
+                b.pad_ = 0.f;
+            }
+        }
+    }
+    try {
+        g->fp = DevArray(sizeof(FP2DAngle) * na);
+        g->fp.upload(fp.data(), sizeof(FP2DAngle) * na);
+        g->bp = DevArray(sizeof(BP2DAngle) * na);
+        g->bp.upload(bp.data(), sizeof(BP2DAngle) * na);
+        g->bp_fbp = DevArray(sizeof(BP2DAngle) * na);
+        g->bp_fbp.upload(bpf.data(), sizeof(BP2DAngle) * na);
+    } catch (...) {
+        delete g;
+        throw;
+    }
+    return g;
+}
+
+// ------------------------------------------------------- filters ---------
+void ramlak_half_spectrum(unsigned n_det, double d, std::vector<double> &out) {
+    // genFilter(FILTER_RAMLAK) (src/Filters.cpp:133-160, 473-485) with the
+    // spatial kernel of generateRampFilter (src/Filters.cpp:41-68):
+    // h[0] = 1/4, h[j odd] = -1/(pi min(j, N-j))^2.  h is real and even, so
+    // its DFT is real: H[k] = 1/4 + sum_{odd j} h[j] cos(2 pi j k / N).
+    // Filter = 2 H[k].
+    unsigned n = next_pow2(2 * n_det);
+    unsigned nb = n / 2 + 1;
+    out.assign(nb, 0.0);
+    std::vector<double> cs(n);
+    for (unsigned j = 0; j < n; ++j) cs[j] = std::cos(2.0 * M_PI * (double)j / (double)n);
+    for (unsigned k = 0; k < nb; ++k) {
+        double acc = 0.25;
+        for (unsigned j = 1; j < n; j += 2) {
+            unsigned jj = (2 * j > n) ? n - j : j;
+            double h = -1.0 / ((M_PI * jj) * (M_PI * jj));
+            acc += h * cs[(size_t)((uint64_t)j * k % n)];
+        }
+        double wv = 2.0 * M_PI * (double)k / (double)n;
+        out[k] = (wv > M_PI * d) ? 0.0 : 2.0 * acc;
+    }
+}
+
+}  // namespace amx

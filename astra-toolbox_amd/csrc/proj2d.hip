@@ -1,0 +1,412 @@
+// 2D projectors for gfx950: parallel / fan Joseph forward projection,
+// voxel-driven back projection (plain and FBP-weighted), and the exact
+// transpose of the forward projection.
+//
+// Data layout in HBM
+//   user volume     [rows][cols] (row axis is world -y)
+//   user sinogram   [angle][det]
+//   FP input copies rows-as-slices  [h][w+4]   (vertical rays)
+//                   cols-as-slices  [w][h+4]   (horizontal rays, transposed)
+//   BP input copy   [angle][det+4]
+// Copies carry a 2-texel zero border so that clamped coordinates read
+// exactly the border-0 texture values without bounds checks.
+//
+// Sampling semantics follow cuda/2d/par_fp.cu:53-200, cuda/2d/fan_fp.cu:55-193,
+// cuda/2d/par_bp.cu:52-87 and cuda/2d/fan_bp.cu:57-103.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "devutil.hpp"
+#include "internal.hpp"
+
+namespace amx {
+
+// ------------------------------------------------------------ prep -------
+// vol [h][w] -> rows copy [h][w+4] and cols copy [w][h+4] in one pass
+// through a 64 x 64 LDS tile.  grid = (ceil((w+4)/64), ceil((h+4)/64)).
+__global__ __launch_bounds__(256) void vol2d_copies(const float *__restrict__ vol,
+                                                    float *__restrict__ rows,
+                                                    float *__restrict__ cols, unsigned w,
+                                                    unsigned h) {
+    __shared__ float tile[64][65];
+    const int pw = w + 2 * kPad, ph = h + 2 * kPad;
+    const int c0 = blockIdx.x * 64, r0 = blockIdx.y * 64;  // padded coordinates
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    for (int i = ty; i < 64; i += 4) {
+        int pr = r0 + i, pc = c0 + tx;
+        int r = pr - kPad, c = pc - kPad;
+        float v = 0.f;
+        if (r >= 0 && r < (int)h && c >= 0 && c < (int)w) v = vol[(size_t)r * w + c];
+        tile[i][tx] = v;
+        // rows copy keeps only real rows, padded columns
+        if (r >= 0 && r < (int)h && pc < pw) rows[(size_t)r * pw + pc] = v;
+    }
+    __syncthreads();
+    for (int i = ty; i < 64; i += 4) {
+        int pc = c0 + i, pr = r0 + tx;
+        int c = pc - kPad;
+        if (c >= 0 && c < (int)w && pr < ph) cols[(size_t)c * ph + pr] = tile[tx][i];
+    }
+}
+
+// sino [a][d] -> [a][d+4]; one workgroup per angle row
+__global__ __launch_bounds__(256) void sino_pad(const float *__restrict__ sino,
+                                                float *__restrict__ sp, unsigned nd) {
+    const unsigned pd = nd + 2 * kPad;
+    const float *src = sino + (size_t)blockIdx.x * nd;
+    float *dst = sp + (size_t)blockIdx.x * pd;
+    for (unsigned i = threadIdx.x; i < pd; i += blockDim.x) {
+        int d = (int)i - kPad;
+        dst[i] = (d >= 0 && d < (int)nd) ? src[d] : 0.f;
+    }
+}
+
+// ------------------------------------------------------------- FP --------
+struct FP2DArgs {
+    const float *rows;  // [h][w+4]
+    const float *cols;  // [w][h+4]
+    unsigned w, h, na, nd;
+    const FP2DAngle *ang;
+    float osc;
+    unsigned rays;  // rays per detector (DetectorSuperSampling)
+    float *out;
+    int mode;
+    const float *rb, *rw;
+    int padded_out;  // write [a][d+4]
+};
+
+struct Ray2D {
+    float a, c, scale;
+    int lo, hi;
+    bool vertical;
+};
+
+// FP64 setup of one (sub-)ray through detector position `fdet`.
+template <bool FAN>
+__device__ __forceinline__ Ray2D ray2d(const FP2DAngle &G, double fdet, bool vertical,
+                                      unsigned w, unsigned h) {
+    double P0 = G.p0[0] + (fdet - 0.5) * G.u[0];
+    double P1 = G.p0[1] + (fdet - 0.5) * G.u[1];
+    double rx = FAN ? G.r[0] - P0 : G.r[0];
+    double ry = FAN ? G.r[1] - P1 : G.r[1];
+    Ray2D R;
+    R.vertical = vertical;
+    R.lo = 0;
+    double a, c, len = sqrt(rx * rx + ry * ry);
+    if (vertical) {
+        // rows are slices: t(r) = a r + c along columns
+        double k = rx / ry;
+        a = -k;
+        c = a * (-0.5 * h + 0.5) + P0 - P1 * k + 0.5 * w - 0.5;
+        R.hi = (int)h - 1;
+        clip_range(a, c, (double)w, (int)h, R.lo, R.hi);
+        R.scale = (float)(len / fabs(ry));
+    } else {
+        // columns are slices: t(c) = a c + c0 along rows
+        double k = ry / rx;
+        a = -k;
+        c = a * (-0.5 * w + 0.5) + 0.5 * h - 0.5 - P1 + P0 * k;
+        R.hi = (int)w - 1;
+        clip_range(a, c, (double)h, (int)w, R.lo, R.hi);
+        R.scale = (float)(len / fabs(rx));
+    }
+    R.a = (float)a;
+    R.c = (float)(c + kPad);  // padded texel coordinate
+    return R;
+}
+
+__device__ __forceinline__ float march2d(const float *__restrict__ base, unsigned pitch, float hi,
+                                         const Ray2D &R) {
+    float acc0 = 0.f, acc1 = 0.f;
+    const float *p = base + (size_t)(R.lo > 0 ? R.lo : 0) * pitch;
+    int s = R.lo;
+    for (; s + 1 <= R.hi; s += 2) {
+        const float sf = (float)s;
+        float t0 = clamp_pad(fmaf(R.a, sf, R.c), hi);
+        float t1 = clamp_pad(fmaf(R.a, sf + 1.f, R.c), hi);
+        float f0 = floorf(t0), f1 = floorf(t1);
+        float2 q0 = ld2b(p, (unsigned)f0 << 2), q1 = ld2b(p + pitch, (unsigned)f1 << 2);
+        acc0 += lerp(q0.x, q0.y, t0 - f0);
+        acc1 += lerp(q1.x, q1.y, t1 - f1);
+        p += 2 * pitch;
+    }
+    if (s <= R.hi) {
+        float t0 = clamp_pad(fmaf(R.a, (float)s, R.c), hi);
+        float f0 = floorf(t0);
+        float2 q0 = ld2b(p, (unsigned)f0 << 2);
+        acc0 += lerp(q0.x, q0.y, t0 - f0);
+    }
+    return acc0 + acc1;
+}
+
+// One thread per (detector, angle); 64 detectors x 4 angles per group.
+template <bool FAN>
+__global__ __launch_bounds__(256) void fp2d_kernel(FP2DArgs A) {
+    const int det = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int angle = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (det >= (int)A.nd || angle >= (int)A.na) return;
+    const FP2DAngle &G = A.ang[angle];
+    bool vertical;
+    if (FAN) {
+        // per-ray orientation, in float32 with the reference's +0.5 in world
+        // units (cuda/2d/fan_fp.cu:79-83)
+        const float sx = G.raw[0], sy = G.raw[1], dsx = G.raw[2], dsy = G.raw[3];
+        const float ux = G.raw[4], uy = G.raw[5];
+        float tx = fmaf((float)det, ux, dsx);
+        tx = tx + 0.5f;
+        tx = tx - sx;
+        float ty = fmaf((float)det, uy, dsy);
+        ty = ty + 0.5f;
+        ty = ty - sy;
+        vertical = fabsf(ty) > fabsf(tx);
+    } else {
+        vertical = G.vertical != 0;
+    }
+    float sum = 0.f;
+    for (unsigned k = 0; k < A.rays; ++k) {
+        double fdet = det + (0.5 + k) / (double)A.rays;
+        Ray2D R = ray2d<FAN>(G, fdet, vertical, A.w, A.h);
+        float v = vertical ? march2d(A.rows, A.w + 2 * kPad, (float)A.w + 2.5f, R)
+                           : march2d(A.cols, A.h + 2 * kPad, (float)A.h + 2.5f, R);
+        sum += v * R.scale;
+    }
+    const float val = sum * (A.osc / (float)A.rays);
+    const size_t idx = (size_t)angle * A.nd + det;
+    const size_t oidx = A.padded_out ? (size_t)angle * (A.nd + 2 * kPad) + det + kPad : idx;
+    if (A.mode == OUT_ADD) {
+        A.out[oidx] += val;
+    } else if (A.mode == OUT_SET) {
+        A.out[oidx] = val;
+    } else if (A.mode == OUT_RESID) {
+        float r = A.rb[idx] + val;
+        if (A.rw) r *= A.rw[oidx];
+        A.out[oidx] = r;
+    } else {
+        float r = A.out[oidx] + val;
+        if (A.rw) r *= A.rw[oidx];
+        A.out[oidx] = r;
+    }
+}
+
+// ------------------------------------------------------------- BP --------
+struct BP2DArgs {
+    const float *sp;  // [a][nd+4]
+    unsigned w, h, na, nd;
+    const BP2DAngle *ang;
+    float osc;
+    float *vol;
+    int mode;
+    const float *uw;
+    float lo, hi;
+};
+
+// KIND: 0 parallel (constant per-angle weight), 1 fan (weight fr),
+// 2 fan FBP (weight fr^2).  One thread per column x and 4 rows.
+template <int KIND>
+__global__ __launch_bounds__(256) void bp2d_kernel(BP2DArgs A) {
+    constexpr int PY = 4;
+    const int X = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int Y0 = (blockIdx.y * 4 + (threadIdx.x >> 6)) * PY;
+    if (X >= (int)A.w || Y0 >= (int)A.h) return;
+    const float fx = X - 0.5f * A.w + 0.5f;
+    const float fy0 = -(Y0 - 0.5f * A.h + 0.5f);  // world y of row Y0
+    const unsigned pitch = A.nd + 2 * kPad;
+    const float hi = (float)A.nd + 2.5f;
+    float acc[PY];
+#pragma unroll
+    for (int i = 0; i < PY; ++i) acc[i] = 0.f;
+    const float *row = A.sp;
+    for (unsigned a = 0; a < A.na; ++a, row += pitch) {
+        const BP2DAngle c = A.ang[a];
+        float num = fmaf(fy0, c.num[2], fmaf(fx, c.num[1], c.num[0]));
+        float den = fmaf(fy0, c.den[2], fmaf(fx, c.den[1], c.den[0]));
+#pragma unroll
+        for (int i = 0; i < PY; ++i) {
+            float fr = (KIND == 0) ? 1.f : __builtin_amdgcn_rcpf(den);
+            float t = clamp_pad(num * fr, hi);
+            float f = floorf(t);
+            float2 q = ld2b(row, (unsigned)f << 2);
+            float val = lerp(q.x, q.y, t - f);
+            float wgt = (KIND == 0) ? c.w : (KIND == 1 ? fr : fr * fr);
+            acc[i] = fmaf(wgt, val, acc[i]);
+            num -= c.num[2];  // next row: world y decreases by 1
+            den -= c.den[2];
+        }
+    }
+    for (int i = 0; i < PY; ++i) {
+        int Y = Y0 + i;
+        if (Y >= (int)A.h) break;
+        size_t idx = (size_t)Y * A.w + X;
+        float val = acc[i] * A.osc;
+        if (A.mode == OUT_ADD) {
+            A.vol[idx] += val;
+        } else if (A.mode == OUT_SET) {
+            A.vol[idx] = val;
+        } else {
+            float wv = A.uw ? A.uw[idx] : 1.f;
+            A.vol[idx] = fminf(fmaxf(fmaf(wv, val, A.vol[idx]), A.lo), A.hi);
+        }
+    }
+}
+
+// --------------------------------------------------- matched BP ----------
+// Exact transpose of fp2d_kernel: every sample scatters its two linear
+// weights into the padded slice copies (fp32 atomics).
+template <bool FAN>
+__global__ __launch_bounds__(256) void fp2d_transpose_kernel(FP2DArgs A, const float *__restrict__ sino) {
+    const int det = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int angle = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (det >= (int)A.nd || angle >= (int)A.na) return;
+    const FP2DAngle &G = A.ang[angle];
+    bool vertical;
+    if (FAN) {
+        const float sx = G.raw[0], sy = G.raw[1], dsx = G.raw[2], dsy = G.raw[3];
+        const float ux = G.raw[4], uy = G.raw[5];
+        float tx = fmaf((float)det, ux, dsx);
+        tx = tx + 0.5f;
+        tx = tx - sx;
+        float ty = fmaf((float)det, uy, dsy);
+        ty = ty + 0.5f;
+        ty = ty - sy;
+        vertical = fabsf(ty) > fabsf(tx);
+    } else {
+        vertical = G.vertical != 0;
+    }
+    const float y0 = sino[(size_t)angle * A.nd + det] * (A.osc / (float)A.rays);
+    if (y0 == 0.f) return;
+    float *rows = const_cast<float *>(A.rows);
+    float *cols = const_cast<float *>(A.cols);
+    for (unsigned k = 0; k < A.rays; ++k) {
+        double fdet = det + (0.5 + k) / (double)A.rays;
+        Ray2D R = ray2d<FAN>(G, fdet, vertical, A.w, A.h);
+        const float y = y0 * R.scale;
+        float *base = vertical ? rows : cols;
+        const unsigned pitch = vertical ? A.w + 2 * kPad : A.h + 2 * kPad;
+        const float hi = vertical ? (float)A.w + 2.5f : (float)A.h + 2.5f;
+        for (int s = R.lo; s <= R.hi; ++s) {
+            float t = clamp_pad(fmaf(R.a, (float)s, R.c), hi);
+            float f = floorf(t);
+            float wt = t - f;
+            float *p = base + (size_t)s * pitch + ((unsigned)f);
+            atomicAdd(p, y * (1.f - wt));
+            atomicAdd(p + 1, y * wt);
+        }
+    }
+}
+
+// vol[r][c] (+)= rows[r][c+2] + cols[c][r+2]
+__global__ __launch_bounds__(256) void unpad2d_add(const float *__restrict__ rows,
+                                                   const float *__restrict__ cols,
+                                                   float *__restrict__ vol, unsigned w,
+                                                   unsigned h) {
+    const unsigned c = blockIdx.x * 64 + (threadIdx.x & 63);
+    const unsigned r = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (c >= w || r >= h) return;
+    vol[(size_t)r * w + c] += rows[(size_t)r * (w + 2 * kPad) + c + kPad] +
+                              cols[(size_t)c * (h + 2 * kPad) + r + kPad];
+}
+
+// ------------------------------------------------------------ launch -----
+namespace {
+size_t rows_elems(const Geom2D &g) { return (size_t)g.h * (g.w + 2 * kPad); }
+size_t cols_elems(const Geom2D &g) { return (size_t)g.w * (g.h + 2 * kPad); }
+}  // namespace
+
+size_t padded_sino_elems(const Geom2D &g) { return (size_t)g.na * (g.nd + 2 * kPad); }
+
+void sino_to_padded(const Geom2D &g, const float *sino, float *sp, hipStream_t s) {
+    sino_pad<<<g.na, 256, 0, s>>>(sino, sp, g.nd);
+    AMX_LAUNCHED();
+}
+
+void fp2d_launch(const Geom2D &g, const float *vol, float *sino, float out_scale, int mode,
+                 const float *resid_b, const float *resid_w, bool padded_out, unsigned rays,
+                 hipStream_t s) {
+    AMX_REQUIRE(rays >= 1 && rays <= 64, "fp2d: rays per detector must be in [1, 64]");
+    Scratch copies((rows_elems(g) + cols_elems(g)) * sizeof(float), s);
+    float *rows = copies.as<float>();
+    float *cols = rows + rows_elems(g);
+    dim3 pg((g.w + 2 * kPad + 63) / 64, (g.h + 2 * kPad + 63) / 64);
+    vol2d_copies<<<pg, 256, 0, s>>>(vol, rows, cols, g.w, g.h);
+    AMX_LAUNCHED();
+    FP2DArgs A{};
+    A.rows = rows;
+    A.cols = cols;
+    A.w = g.w; A.h = g.h; A.na = g.na; A.nd = g.nd;
+    A.ang = g.fp.as<FP2DAngle>();
+    A.osc = out_scale;
+    A.rays = rays;
+    A.out = sino;
+    A.mode = mode;
+    A.rb = resid_b;
+    A.rw = resid_w;
+    A.padded_out = padded_out ? 1 : 0;
+    dim3 grid((g.nd + 63) / 64, (g.na + 3) / 4);
+    if (g.kind == ASTRA_AMD_GEOM_FAN)
+        fp2d_kernel<true><<<grid, 256, 0, s>>>(A);
+    else
+        fp2d_kernel<false><<<grid, 256, 0, s>>>(A);
+    AMX_LAUNCHED();
+}
+
+void bp2d_launch_padded(const Geom2D &g, const float *sp, float *vol, float out_scale, bool fbp,
+                        int mode, const float *upd_w, float lo, float hi, hipStream_t s) {
+    BP2DArgs A{};
+    A.sp = sp;
+    A.w = g.w; A.h = g.h; A.na = g.na; A.nd = g.nd;
+    A.osc = out_scale;
+    A.vol = vol;
+    A.mode = mode;
+    A.uw = upd_w;
+    A.lo = lo;
+    A.hi = hi;
+    dim3 grid((g.w + 63) / 64, (g.h + 15) / 16);
+    if (g.kind == ASTRA_AMD_GEOM_PAR) {
+        A.ang = g.bp.as<BP2DAngle>();
+        bp2d_kernel<0><<<grid, 256, 0, s>>>(A);
+    } else if (!fbp) {
+        A.ang = g.bp.as<BP2DAngle>();
+        bp2d_kernel<1><<<grid, 256, 0, s>>>(A);
+    } else {
+        A.ang = g.bp_fbp.as<BP2DAngle>();
+        bp2d_kernel<2><<<grid, 256, 0, s>>>(A);
+    }
+    AMX_LAUNCHED();
+}
+
+void bp2d_launch(const Geom2D &g, const float *sino, float *vol, float out_scale, bool fbp,
+                 int mode, const float *upd_w, float lo, float hi, hipStream_t s) {
+    Scratch sp(padded_sino_elems(g) * sizeof(float), s);
+    sino_to_padded(g, sino, sp.as<float>(), s);
+    bp2d_launch_padded(g, sp.as<float>(), vol, out_scale, fbp, mode, upd_w, lo, hi, s);
+}
+
+void bp2d_matched_launch(const Geom2D &g, const float *sino, float *vol, float out_scale,
+                         int mode, hipStream_t s) {
+    const size_t npix = (size_t)g.w * g.h;
+    if (mode == OUT_SET) fill_launch(vol, npix, 0.f, s);
+    Scratch copies((rows_elems(g) + cols_elems(g)) * sizeof(float), s);
+    float *rows = copies.as<float>();
+    float *cols = rows + rows_elems(g);
+    fill_launch(rows, rows_elems(g) + cols_elems(g), 0.f, s);
+    FP2DArgs A{};
+    A.rows = rows;
+    A.cols = cols;
+    A.w = g.w; A.h = g.h; A.na = g.na; A.nd = g.nd;
+    A.ang = g.fp.as<FP2DAngle>();
+    A.osc = out_scale;
+    A.rays = 1;
+    dim3 grid((g.nd + 63) / 64, (g.na + 3) / 4);
+    if (g.kind == ASTRA_AMD_GEOM_FAN)
+        fp2d_transpose_kernel<true><<<grid, 256, 0, s>>>(A, sino);
+    else
+        fp2d_transpose_kernel<false><<<grid, 256, 0, s>>>(A, sino);
+    AMX_LAUNCHED();
+    dim3 ug((g.w + 63) / 64, (g.h + 3) / 4);
+    unpad2d_add<<<ug, 256, 0, s>>>(rows, cols, vol, g.w, g.h);
+    AMX_LAUNCHED();
+}
+
+}  // namespace amx

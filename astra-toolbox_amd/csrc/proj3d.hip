@@ -1,0 +1,542 @@
+// 3D projectors for gfx950: cone / par3d ray-driven forward projection and
+// voxel-driven back projection (default, FDK and 2D weighting), plus the
+// exact-transpose (matched) back projection.
+//
+// Data layout in HBM
+//   user volume       [z][y][x]
+//   user projections  [v][angle][u]
+//   FP input copies   slice-major and zero-padded per direction group:
+//                       x-group  [x][z+4][y+4]   (c0 = x, c2 = z, c1 = y)
+//                       y-group  [y][z+4][x+4]   (c0 = y, c2 = z, c1 = x)
+//                       z-group  [z][y+4][x+4]   (c0 = z, c2 = y, c1 = x)
+//                     so that the in-slice fast axis is contiguous for the
+//                     rays of every group and one slice is one contiguous
+//                     plane (32-bit offsets inside a plane).
+//   BP input copy     angle-major and zero-padded: [angle][v+4][u+4]
+//
+// Sampling semantics follow cuda/3d/cone_fp.cu:154-231,
+// cuda/3d/par3d_fp.cu:152-230, cuda/3d/cone_bp.cu:55-133 and
+// cuda/3d/par3d_bp.cu:55-120 with ideal fp32 (bi)linear interpolation in
+// place of the 8-bit-weight texture unit.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "devutil.hpp"
+#include "internal.hpp"
+
+namespace amx {
+
+// ----------------------------------------------------------- prep --------
+// dst[c0][c2 + 2][c1 + 2] = vol[z][y][x] for the z- and y-groups (the fast
+// axis stays x).  One workgroup per destination row; borders written 0.
+__global__ __launch_bounds__(256) void pad_copy_xfast(const float *__restrict__ vol,
+                                                      float *__restrict__ dst, unsigned nx,
+                                                      unsigned ny, unsigned nz, int group) {
+    const unsigned px = nx + 2 * kPad;
+    const unsigned row = blockIdx.x;  // c0 * n2p + padded c2
+    const unsigned n2p = (group == 2) ? ny + 2 * kPad : nz + 2 * kPad;
+    const unsigned n2 = (group == 2) ? ny : nz;
+    const unsigned c0 = row / n2p;
+    const int c2 = (int)(row % n2p) - kPad;
+    float *out = dst + (size_t)row * px;
+    const bool inside = c2 >= 0 && c2 < (int)n2;
+    const float *src = (group == 2) ? vol + ((size_t)c0 * ny + (inside ? c2 : 0)) * nx   // [z=c0][y=c2]
+                                    : vol + ((size_t)(inside ? c2 : 0) * ny + c0) * nx;  // [z=c2][y=c0]
+    for (unsigned i = threadIdx.x; i < px; i += blockDim.x) {
+        int x = (int)i - kPad;
+        out[i] = (inside && x >= 0 && x < (int)nx) ? src[x] : 0.f;
+    }
+}
+
+// x-group: dst[x][z + 2][y + 2] = vol[z][y][x]; per z-plane an (x, y)
+// transpose through a 64 x 64 LDS tile.  grid = (y tiles, x tiles, nz + 4).
+__global__ __launch_bounds__(256) void pad_transpose_x(const float *__restrict__ vol,
+                                                       float *__restrict__ dst, unsigned nx,
+                                                       unsigned ny, unsigned nz) {
+    __shared__ float tile[64][65];
+    const unsigned pyn = ny + 2 * kPad, pzn = nz + 2 * kPad;
+    const int zr = blockIdx.z;  // padded z row
+    const int z = zr - kPad;
+    const int x0 = blockIdx.y * 64;
+    const int yp0 = blockIdx.x * 64;  // padded y tile origin
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    for (int r = ty; r < 64; r += 4) {
+        int y = yp0 + r - kPad;
+        int x = x0 + tx;
+        float v = 0.f;
+        if (z >= 0 && z < (int)nz && y >= 0 && y < (int)ny && x < (int)nx)
+            v = vol[((size_t)z * ny + y) * nx + x];
+        tile[r][tx] = v;
+    }
+    __syncthreads();
+    for (int r = ty; r < 64; r += 4) {
+        int x = x0 + r;
+        int yp = yp0 + tx;
+        if (x < (int)nx && yp < (int)pyn) dst[((size_t)x * pzn + zr) * pyn + yp] = tile[tx][r];
+    }
+}
+
+// proj [v][a][u] -> padded angle-major [a][v + 2][u + 2] (borders 0).
+// One workgroup per destination row.
+__global__ __launch_bounds__(256) void proj_to_am(const float *__restrict__ proj,
+                                                  float *__restrict__ pp, unsigned nu,
+                                                  unsigned na, unsigned nv) {
+    const unsigned pu = nu + 2 * kPad, pv = nv + 2 * kPad;
+    const unsigned row = blockIdx.x;  // a * pv + padded v
+    const unsigned a = row / pv;
+    const int v = (int)(row % pv) - kPad;
+    float *out = pp + (size_t)row * pu;
+    const bool inside = v >= 0 && v < (int)nv;
+    const float *src = proj + ((size_t)(inside ? v : 0) * na + a) * nu;
+    for (unsigned i = threadIdx.x; i < pu; i += blockDim.x) {
+        int u = (int)i - kPad;
+        out[i] = (inside && u >= 0 && u < (int)nu) ? src[u] : 0.f;
+    }
+}
+
+// ------------------------------------------------------------- FP --------
+struct FPArgs {
+    const float *volp;  // padded slice-major copy of this group
+    unsigned n0, n1, n2;
+    const FP3DAngle *ang;
+    const int *alist;  // angle indices of this direction group
+    unsigned nu, nv, na;
+    float fs1, fs2, osc;
+    float *out;
+    int mode;
+    const float *rb;  // OUT_RESID: b in the output layout
+    const float *rw;  // OUT_RESID: weights in the output layout (may be NULL)
+    int padded_out;   // write [a][v+2][u+2] instead of [v][a][u]
+};
+
+// One thread per ray; a 256-thread workgroup covers a 16 x 16 detector tile
+// of one angle, each wave a 16 u x 4 v sub-tile.  Ray setup runs in FP64;
+// the slice loop is wave-uniform over the union of the lanes' clipped slice
+// ranges, so the slice base address is scalar and every fetch is one
+// dwordx2 load with a 32-bit vector offset.
+template <bool CONE>
+__device__ __forceinline__ void fp3d_ray(const FPArgs &A, const FP3DAngle &G, int u, int v,
+                                         float &a1f, float &a2f, float &c1f, float &c2f,
+                                         float &scale, int &lo, int &hi) {
+    double P[3], d[3];
+    for (int k = 0; k < 3; ++k) P[k] = G.p0[k] + (double)u * G.u[k] + (double)v * G.v[k];
+    for (int k = 0; k < 3; ++k) d[k] = CONE ? (G.s[k] - P[k]) : G.s[k];
+    const double *B = CONE ? G.s : P;
+    double a1 = d[1] / d[0], a2 = d[2] / d[0];
+    // texel coordinate - 0.5 at slice s: a_k (s - n0/2 + 0.5 - B0) + B_k + n_k/2 - 0.5
+    double base = -0.5 * A.n0 + 0.5 - B[0];
+    double c1 = a1 * base + B[1] + 0.5 * A.n1 - 0.5;
+    double c2 = a2 * base + B[2] + 0.5 * A.n2 - 0.5;
+    lo = 0;
+    hi = (int)A.n0 - 1;
+    clip_range(a1, c1, (double)A.n1, (int)A.n0, lo, hi);
+    clip_range(a2, c2, (double)A.n2, (int)A.n0, lo, hi);
+    a1f = (float)a1;
+    a2f = (float)a2;
+    c1f = (float)(c1 + kPad);  // padded texel coordinates
+    c2f = (float)(c2 + kPad);
+    scale = (float)(sqrt(a1 * a1 * A.fs1 + a2 * a2 * A.fs2 + 1.0) * A.osc);
+}
+
+template <bool CONE>
+__global__ __launch_bounds__(256) void fp3d_kernel(FPArgs A) {
+    const unsigned tiles_u = (A.nu + 15) / 16;
+    const unsigned tu = blockIdx.x % tiles_u, tv = blockIdx.x / tiles_u;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int u = tu * 16 + (lane & 15);
+    const int v = tv * 16 + wave * 4 + (lane >> 4);
+    const int angle = A.alist[blockIdx.y];
+    const FP3DAngle &G = A.ang[angle];
+    const bool active = u < (int)A.nu && v < (int)A.nv;
+
+    float a1f = 0.f, a2f = 0.f, c1f = 0.5f, c2f = 0.5f, scale = 0.f;
+    int lo = (int)A.n0, hi = -1;
+    if (active) fp3d_ray<CONE>(A, G, u, v, a1f, a2f, c1f, c2f, scale, lo, hi);
+    const int slo = __builtin_amdgcn_readfirstlane(wave_min(lo));
+    const int shi = __builtin_amdgcn_readfirstlane(wave_max(hi));
+
+    const unsigned pitch = A.n1 + 2 * kPad;
+    const size_t plane = (size_t)(A.n2 + 2 * kPad) * pitch;
+    const float pitchf = (float)pitch;
+    const unsigned rowb = pitch * 4u;
+    const float hi1 = (float)A.n1 + 2.5f, hi2 = (float)A.n2 + 2.5f;
+    float acc0 = 0.f, acc1 = 0.f;
+    int s = slo;
+    const float *pl = A.volp + (size_t)(slo > 0 ? slo : 0) * plane;
+    for (; s + 1 <= shi; s += 2) {
+        const float sf = (float)s;
+        float p1 = clamp_pad(fmaf(a1f, sf, c1f), hi1);
+        float p2 = clamp_pad(fmaf(a2f, sf, c2f), hi2);
+        float q1 = clamp_pad(fmaf(a1f, sf + 1.f, c1f), hi1);
+        float q2 = clamp_pad(fmaf(a2f, sf + 1.f, c2f), hi2);
+        float fp1 = floorf(p1), fp2 = floorf(p2), fq1 = floorf(q1), fq2 = floorf(q2);
+        unsigned op = texel_off(fp1, fp2, pitchf), oq = texel_off(fq1, fq2, pitchf);
+        const float *pl2 = pl + plane;
+        float2 r0 = ld2b(pl, op), r1 = ld2b(pl, op + rowb);
+        float2 t0 = ld2b(pl2, oq), t1 = ld2b(pl2, oq + rowb);
+        float w1 = p1 - fp1, w2 = p2 - fp2, x1 = q1 - fq1, x2 = q2 - fq2;
+        acc0 += lerp(lerp(r0.x, r0.y, w1), lerp(r1.x, r1.y, w1), w2);
+        acc1 += lerp(lerp(t0.x, t0.y, x1), lerp(t1.x, t1.y, x1), x2);
+        pl += 2 * plane;
+    }
+    if (s <= shi) {
+        const float sf = (float)s;
+        float p1 = clamp_pad(fmaf(a1f, sf, c1f), hi1);
+        float p2 = clamp_pad(fmaf(a2f, sf, c2f), hi2);
+        float fp1 = floorf(p1), fp2 = floorf(p2);
+        unsigned op = texel_off(fp1, fp2, pitchf);
+        float2 r0 = ld2b(pl, op), r1 = ld2b(pl, op + rowb);
+        acc0 += lerp(lerp(r0.x, r0.y, p1 - fp1), lerp(r1.x, r1.y, p1 - fp1), p2 - fp2);
+    }
+    if (!active) return;
+    const float val = (acc0 + acc1) * scale;
+    size_t idx = ((size_t)v * A.na + angle) * A.nu + u;
+    size_t oidx = A.padded_out
+                      ? ((size_t)angle * (A.nv + 2 * kPad) + (v + kPad)) * (A.nu + 2 * kPad) + u + kPad
+                      : idx;
+    if (A.mode == OUT_ADD) {
+        A.out[oidx] += val;
+    } else if (A.mode == OUT_SET) {
+        A.out[oidx] = val;
+    } else if (A.mode == OUT_RESID) {  // (b + val) * w
+        float r = A.rb[idx] + val;
+        if (A.rw) r *= A.rw[oidx];
+        A.out[oidx] = r;
+    } else {  // OUT_ADDMUL: (out + val) * w
+        float r = A.out[oidx] + val;
+        if (A.rw) r *= A.rw[oidx];
+        A.out[oidx] = r;
+    }
+}
+
+// ------------------------------------------------------------- BP --------
+struct BPArgs {
+    const float *pp;  // padded angle-major projections
+    unsigned nu, nv, na;
+    const BP3DAngle *ang;
+    const float *aw;  // par3d per-angle weights
+    unsigned nx, ny, nz;
+    float osc;
+    float *vol;
+    int mode;
+    const float *uw;  // OUT_UPDATE: per-voxel weights (may be NULL)
+    float lo, hi;     // OUT_UPDATE: clamp
+};
+
+// KIND: 0 cone fr^2 (default / FDK), 1 cone fr (2D weighting), 2 par3d.
+// One thread per (x, y) column of ZS voxels; 16 x 16 threads per group.
+// The angle loop is uniform, so the per-angle coefficients are scalar loads
+// and the angle's projection plane base is a scalar address.
+template <int KIND, int ZS>
+__global__ __launch_bounds__(256) void bp3d_kernel(BPArgs A) {
+    const unsigned tiles_x = (A.nx + 15) / 16;
+    const int X = (blockIdx.x % tiles_x) * 16 + (threadIdx.x & 15);
+    const int Y = (blockIdx.x / tiles_x) * 16 + (threadIdx.x >> 4);
+    const int Z0 = blockIdx.y * ZS;
+    if (X >= (int)A.nx || Y >= (int)A.ny) return;
+    const float fx = X - 0.5f * A.nx + 0.5f;
+    const float fy = Y - 0.5f * A.ny + 0.5f;
+    const float fz = Z0 - 0.5f * A.nz + 0.5f;
+    const unsigned pitch = A.nu + 2 * kPad;
+    const size_t plane = (size_t)(A.nv + 2 * kPad) * pitch;
+    const float pitchf = (float)pitch;
+    const unsigned rowb = pitch * 4u;
+    const float hiu = (float)A.nu + 2.5f, hiv = (float)A.nv + 2.5f;
+    float acc[ZS];
+#pragma unroll
+    for (int i = 0; i < ZS; ++i) acc[i] = 0.f;
+
+    const float *pl = A.pp;
+    for (unsigned a = 0; a < A.na; ++a, pl += plane) {
+        const BP3DAngle c = A.ang[a];
+        float un = fmaf(fz, c.nu[2], fmaf(fy, c.nu[1], fmaf(fx, c.nu[0], c.nu[3])));
+        float vn = fmaf(fz, c.nv[2], fmaf(fy, c.nv[1], fmaf(fx, c.nv[0], c.nv[3])));
+        float dn = fmaf(fz, c.dn[2], fmaf(fy, c.dn[1], fmaf(fx, c.dn[0], c.dn[3])));
+        const float wa = (KIND == 2) ? A.aw[a] : 1.f;
+#pragma unroll
+        for (int i = 0; i < ZS; ++i) {
+            float fr = (KIND == 2) ? 1.f : __builtin_amdgcn_rcpf(dn);
+            float pu = clamp_pad(un * fr, hiu);
+            float pv = clamp_pad(vn * fr, hiv);
+            float fu = floorf(pu), fv = floorf(pv);
+            unsigned o = texel_off(fu, fv, pitchf);
+            float2 r0 = ld2b(pl, o), r1 = ld2b(pl, o + rowb);
+            float val = lerp(lerp(r0.x, r0.y, pu - fu), lerp(r1.x, r1.y, pu - fu), pv - fv);
+            float wgt = (KIND == 0) ? fr * fr : (KIND == 1 ? fr : wa);
+            acc[i] = fmaf(wgt, val, acc[i]);
+            un += c.nu[2];
+            vn += c.nv[2];
+            dn += c.dn[2];
+        }
+    }
+    const int nzr = min(ZS, (int)A.nz - Z0);
+    for (int i = 0; i < ZS; ++i) {
+        if (i >= nzr) break;
+        size_t idx = ((size_t)(Z0 + i) * A.ny + Y) * A.nx + X;
+        float val = acc[i] * A.osc;
+        if (A.mode == OUT_ADD) {
+            A.vol[idx] += val;
+        } else if (A.mode == OUT_SET) {
+            A.vol[idx] = val;
+        } else {  // OUT_UPDATE: clamp(x + w * val)
+            float w = A.uw ? A.uw[idx] : 1.f;
+            A.vol[idx] = fminf(fmaxf(fmaf(w, val, A.vol[idx]), A.lo), A.hi);
+        }
+    }
+}
+
+// --------------------------------------------------- matched BP ----------
+// Exact transpose of fp3d_kernel: every (ray, slice) sample scatters its
+// four bilinear weights into the padded slice-major accumulation copy of
+// the angle's direction group (fp32 atomics).
+template <bool CONE>
+__global__ __launch_bounds__(256) void fp3d_transpose_kernel(FPArgs A, const float *__restrict__ proj) {
+    const unsigned tiles_u = (A.nu + 15) / 16;
+    const unsigned tu = blockIdx.x % tiles_u, tv = blockIdx.x / tiles_u;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int u = tu * 16 + (lane & 15);
+    const int v = tv * 16 + wave * 4 + (lane >> 4);
+    const int angle = A.alist[blockIdx.y];
+    const FP3DAngle &G = A.ang[angle];
+    if (u >= (int)A.nu || v >= (int)A.nv) return;
+    float a1f, a2f, c1f, c2f, scale;
+    int lo, hi;
+    fp3d_ray<CONE>(A, G, u, v, a1f, a2f, c1f, c2f, scale, lo, hi);
+    const float y = proj[((size_t)v * A.na + angle) * A.nu + u] * scale;
+    if (y == 0.f) return;
+    const unsigned pitch = A.n1 + 2 * kPad;
+    const size_t plane = (size_t)(A.n2 + 2 * kPad) * pitch;
+    const float pitchf = (float)pitch;
+    const float hi1 = (float)A.n1 + 2.5f, hi2 = (float)A.n2 + 2.5f;
+    float *acc = const_cast<float *>(A.volp);
+    for (int s = lo; s <= hi; ++s) {
+        const float sf = (float)s;
+        float p1 = clamp_pad(fmaf(a1f, sf, c1f), hi1);
+        float p2 = clamp_pad(fmaf(a2f, sf, c2f), hi2);
+        float f1 = floorf(p1), f2 = floorf(p2);
+        float w1 = p1 - f1, w2 = p2 - f2;
+        float *q = acc + (size_t)s * plane + (texel_off(f1, f2, pitchf) >> 2);
+        atomicAdd(q, y * (1.f - w1) * (1.f - w2));
+        atomicAdd(q + 1, y * w1 * (1.f - w2));
+        atomicAdd(q + pitch, y * (1.f - w1) * w2);
+        atomicAdd(q + pitch + 1, y * w1 * w2);
+    }
+}
+
+// vol[z][y][x] (+)= interior of a padded slice-major group copy
+__global__ __launch_bounds__(256) void unpad_group_add(const float *__restrict__ src,
+                                                       float *__restrict__ vol, unsigned nx,
+                                                       unsigned ny, unsigned nz, int group) {
+    size_t n = (size_t)nx * ny * nz;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        unsigned x = i % nx;
+        unsigned y = (i / nx) % ny;
+        unsigned z = i / ((size_t)nx * ny);
+        size_t s;
+        if (group == 0)
+            s = ((size_t)x * (nz + 2 * kPad) + z + kPad) * (ny + 2 * kPad) + y + kPad;
+        else if (group == 1)
+            s = ((size_t)y * (nz + 2 * kPad) + z + kPad) * (nx + 2 * kPad) + x + kPad;
+        else
+            s = ((size_t)z * (ny + 2 * kPad) + y + kPad) * (nx + 2 * kPad) + x + kPad;
+        vol[i] += src[s];
+    }
+}
+
+// ------------------------------------------------------------ launch -----
+namespace {
+
+// dims of direction group g along (c0, c1, c2)
+void group_dims(const Geom3D &g, int grp, unsigned &n0, unsigned &n1, unsigned &n2) {
+    if (grp == 0) { n0 = g.nx; n1 = g.ny; n2 = g.nz; }
+    else if (grp == 1) { n0 = g.ny; n1 = g.nx; n2 = g.nz; }
+    else { n0 = g.nz; n1 = g.nx; n2 = g.ny; }
+}
+
+size_t group_elems(const Geom3D &g, int grp) {
+    unsigned n0, n1, n2;
+    group_dims(g, grp, n0, n1, n2);
+    return (size_t)n0 * (n1 + 2 * kPad) * (n2 + 2 * kPad);
+}
+
+void make_group_copy(const Geom3D &g, int grp, const float *vol, float *dst, hipStream_t s) {
+    if (grp == 0) {
+        dim3 grid((g.ny + 2 * kPad + 63) / 64, (g.nx + 63) / 64, g.nz + 2 * kPad);
+        pad_transpose_x<<<grid, 256, 0, s>>>(vol, dst, g.nx, g.ny, g.nz);
+    } else {
+        unsigned rows = (grp == 2) ? g.nz * (g.ny + 2 * kPad) : g.ny * (g.nz + 2 * kPad);
+        pad_copy_xfast<<<rows, 256, 0, s>>>(vol, dst, g.nx, g.ny, g.nz, grp);
+    }
+    AMX_LAUNCHED();
+}
+
+// Cube test and per-direction scale factors (cuda/3d/cone_fp.cu:339-370,
+// cuda/3d/par3d_fp.cu:438-472).
+void fp_scales(const Geom3D &g, int grp, float out_scale, float &fs1, float &fs2, float &osc) {
+    const float vx = g.vol_scale[0], vy = g.vol_scale[1], vz = g.vol_scale[2];
+    bool cube = !(std::fabs(vx / vy - 1.0) > 0.00001) && !(std::fabs(vx / vz - 1.0) > 0.00001);
+    if (cube) {
+        fs1 = fs2 = 1.f;
+        osc = out_scale * vx;
+        return;
+    }
+    float s1, s2;
+    if (grp == 0) {
+        s1 = vy / vx; s2 = vz / vx; osc = out_scale * vx;
+    } else if (grp == 1) {
+        s1 = vx / vy;
+        // par3d keeps volScale.fY / volScale.fY here (cuda/3d/par3d_fp.cu:459)
+        s2 = (g.kind == ASTRA_AMD_GEOM_PAR3D) ? vy / vy : vz / vy;
+        osc = out_scale * vy;
+    } else {
+        s1 = vx / vz; s2 = vy / vz; osc = out_scale * vz;
+    }
+    fs1 = s1 * s1;
+    fs2 = s2 * s2;
+}
+
+FPArgs fp_args(const Geom3D &g, int grp, float out_scale) {
+    FPArgs A{};
+    group_dims(g, grp, A.n0, A.n1, A.n2);
+    A.ang = g.fp.as<FP3DAngle>();
+    A.alist = g.group_idx[grp].as<int>();
+    A.nu = g.nu; A.nv = g.nv; A.na = g.na;
+    fp_scales(g, grp, out_scale, A.fs1, A.fs2, A.osc);
+    return A;
+}
+
+}  // namespace
+
+void fp3d_launch(const Geom3D &g, const float *vol, float *proj, float out_scale, int mode,
+                 const float *resid_b, const float *resid_w, bool padded_out, hipStream_t s) {
+    size_t need = 0;
+    int groups[3], ng = 0;
+    for (int k = 0; k < 3; ++k)
+        if (g.dir_count[k]) {
+            need = std::max(need, group_elems(g, k));
+            groups[ng++] = k;
+        }
+    Scratch copy(need * sizeof(float), s);
+    const unsigned tiles = ((g.nu + 15) / 16) * ((g.nv + 15) / 16);
+    for (int i = 0; i < ng; ++i) {
+        const int k = groups[i];
+        make_group_copy(g, k, vol, copy.as<float>(), s);
+        FPArgs A = fp_args(g, k, out_scale);
+        A.volp = copy.as<float>();
+        A.out = proj;
+        A.rb = resid_b;
+        A.rw = nullptr;
+        A.padded_out = padded_out ? 1 : 0;
+        // Groups run one after another on the stream.  The first applies
+        // the requested mode; the others accumulate, and for a weighted
+        // residual the last one applies the weights.
+        const bool last = (i == ng - 1);
+        if (i == 0) {
+            A.mode = mode;
+            if (mode == OUT_RESID && last) A.rw = resid_w;
+        } else if (mode == OUT_RESID && last && resid_w) {
+            A.mode = OUT_ADDMUL;
+            A.rw = resid_w;
+        } else {
+            A.mode = OUT_ADD;
+        }
+        dim3 grid(tiles, g.dir_count[k]);
+        if (g.kind == ASTRA_AMD_GEOM_CONE)
+            fp3d_kernel<true><<<grid, 256, 0, s>>>(A);
+        else
+            fp3d_kernel<false><<<grid, 256, 0, s>>>(A);
+        AMX_LAUNCHED();
+    }
+}
+
+void proj_to_padded_am(const Geom3D &g, const float *proj, float *pp, hipStream_t s) {
+    const unsigned pv = g.nv + 2 * kPad;
+    proj_to_am<<<g.na * pv, 256, 0, s>>>(proj, pp, g.nu, g.na, g.nv);
+    AMX_LAUNCHED();
+}
+
+size_t padded_am_elems(const Geom3D &g) {
+    return (size_t)g.na * (g.nv + 2 * kPad) * (g.nu + 2 * kPad);
+}
+
+void bp3d_launch_padded(const Geom3D &g, const float *pp, float *vol, float out_scale, int kernel,
+                        int mode, const float *upd_w, float lo, float hi, hipStream_t s) {
+    BPArgs A{};
+    A.pp = pp;
+    A.nu = g.nu; A.nv = g.nv; A.na = g.na;
+    A.nx = g.nx; A.ny = g.ny; A.nz = g.nz;
+    A.vol = vol;
+    A.mode = mode;
+    A.uw = upd_w;
+    A.lo = lo;
+    A.hi = hi;
+    const float vx = g.vol_scale[0], vy = g.vol_scale[1], vz = g.vol_scale[2];
+    // ConeBP_Array / Par3DBP_Array output scale (cuda/3d/cone_bp.cu:322-328,
+    // cuda/3d/par3d_bp.cu:263)
+    if (g.kind == ASTRA_AMD_GEOM_CONE && kernel == ASTRA_AMD_KER3D_FDK_WEIGHTING)
+        A.osc = out_scale / vx;
+    else
+        A.osc = out_scale * (vx * vy * vz);
+    constexpr int ZS = 8;
+    dim3 grid(((g.nx + 15) / 16) * ((g.ny + 15) / 16), (g.nz + ZS - 1) / ZS);
+    if (g.kind == ASTRA_AMD_GEOM_CONE) {
+        if (kernel == ASTRA_AMD_KER3D_FDK_WEIGHTING) {
+            A.ang = g.bp_fdk.as<BP3DAngle>();
+            bp3d_kernel<0, ZS><<<grid, 256, 0, s>>>(A);
+        } else if (kernel == ASTRA_AMD_KER3D_2D_WEIGHTING) {
+            A.ang = g.bp_2d.as<BP3DAngle>();
+            bp3d_kernel<1, ZS><<<grid, 256, 0, s>>>(A);
+        } else {
+            A.ang = g.bp_default.as<BP3DAngle>();
+            bp3d_kernel<0, ZS><<<grid, 256, 0, s>>>(A);
+        }
+    } else {
+        if (kernel == ASTRA_AMD_KER3D_2D_WEIGHTING) {
+            A.ang = g.bp_2d.as<BP3DAngle>();
+            A.aw = g.bp_w_2d.as<float>();
+        } else {
+            A.ang = g.bp_default.as<BP3DAngle>();
+            A.aw = g.bp_w_default.as<float>();
+        }
+        bp3d_kernel<2, ZS><<<grid, 256, 0, s>>>(A);
+    }
+    AMX_LAUNCHED();
+}
+
+void bp3d_launch(const Geom3D &g, const float *proj, float *vol, float out_scale, int kernel,
+                 int mode, const float *upd_w, float lo, float hi, hipStream_t s) {
+    Scratch pp(padded_am_elems(g) * sizeof(float), s);
+    proj_to_padded_am(g, proj, pp.as<float>(), s);
+    bp3d_launch_padded(g, pp.as<float>(), vol, out_scale, kernel, mode, upd_w, lo, hi, s);
+}
+
+void bp3d_matched_launch(const Geom3D &g, const float *proj, float *vol, float out_scale,
+                         int mode, hipStream_t s) {
+    const size_t nvox = (size_t)g.nx * g.ny * g.nz;
+    if (mode == OUT_SET) fill_launch(vol, nvox, 0.f, s);
+    const unsigned tiles = ((g.nu + 15) / 16) * ((g.nv + 15) / 16);
+    size_t need = 0;
+    for (int k = 0; k < 3; ++k)
+        if (g.dir_count[k]) need = std::max(need, group_elems(g, k));
+    Scratch acc(need * sizeof(float), s);
+    for (int k = 0; k < 3; ++k) {
+        if (!g.dir_count[k]) continue;
+        fill_launch(acc.as<float>(), group_elems(g, k), 0.f, s);
+        FPArgs A = fp_args(g, k, out_scale);
+        A.volp = acc.as<float>();
+        dim3 grid(tiles, g.dir_count[k]);
+        if (g.kind == ASTRA_AMD_GEOM_CONE)
+            fp3d_transpose_kernel<true><<<grid, 256, 0, s>>>(A, proj);
+        else
+            fp3d_transpose_kernel<false><<<grid, 256, 0, s>>>(A, proj);
+        AMX_LAUNCHED();
+        unsigned blocks = (unsigned)std::min<size_t>((nvox + 255) / 256, 1u << 20);
+        unpad_group_add<<<blocks, 256, 0, s>>>(acc.as<float>(), vol, g.nx, g.ny, g.nz, k);
+        AMX_LAUNCHED();
+    }
+}
+
+}  // namespace amx

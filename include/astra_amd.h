@@ -1,0 +1,214 @@
+/*
+ * astra_amd.h - C ABI of the MI355X-native ASTRA projection hot path.
+ *
+ * Plain C: pointers, sizes and opaque handles only.  Every entry point
+ * returns an int status (ASTRA_AMD_OK = 0); on failure the message of the
+ * calling thread is available from astra_amd_last_error().  All data
+ * pointers passed to projector / algorithm calls are DEVICE pointers to
+ * C-contiguous float32 arrays; host-side arrays only appear in geometry and
+ * filter construction.  Calls are asynchronous on the given hipStream_t
+ * (NULL = the null stream), except where stated.
+ *
+ * Layouts (reference: cuda/3d/cone_fp.cu:229, cuda/3d/util3d.cu:60-75,
+ * cuda/2d/par_fp.cu:124):
+ *   2D volume   [rows][cols]          (row axis is world -y)
+ *   2D sinogram [angles][dets]
+ *   3D volume   [z][y][x]
+ *   3D proj     [v][angles][u]
+ * Projector outputs are accumulated (MODE_ADD) or overwritten (MODE_SET),
+ * always multiplied by the output scale, like the reference kernels' "+="
+ * (e.g. cuda/3d/cone_bp.cu:132) preceded by CompositeGeometryManager's
+ * zeroing for MODE_SET (src/CompositeGeometryManager.cpp:1123,1157).
+ */
+#ifndef ASTRA_AMD_H
+#define ASTRA_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+#define ASTRA_AMD_OK 0
+#define ASTRA_AMD_ERR_ARG 1         /* invalid argument / shape mismatch   */
+#define ASTRA_AMD_ERR_HIP 2         /* HIP runtime error                   */
+#define ASTRA_AMD_ERR_UNSUPPORTED 3 /* valid in ASTRA, not built here      */
+#define ASTRA_AMD_ERR_ALLOC 4       /* device allocation failed            */
+#define ASTRA_AMD_ERR_INTERNAL 5
+
+/* Write modes: CCompositeGeometryManager::EJobMode
+ * (include/astra/CompositeGeometryManager.h:127-130). */
+#define ASTRA_AMD_MODE_ADD 0
+#define ASTRA_AMD_MODE_SET 1
+
+/* Geometry kinds (include/astra/GeometryUtil2D.h:53-116,
+ * include/astra/GeometryUtil3D.h:47-123). */
+#define ASTRA_AMD_GEOM_PAR 0   /* 2D SParProjection                       */
+#define ASTRA_AMD_GEOM_FAN 1   /* 2D SFanProjection                       */
+#define ASTRA_AMD_GEOM_PAR3D 2 /* 3D SPar3DProjection                     */
+#define ASTRA_AMD_GEOM_CONE 3  /* 3D SConeProjection                      */
+
+/* 3D projection kernels: Cuda3DProjectionKernel
+ * (include/astra/cuda/3d/dims3d.h:45-51). */
+#define ASTRA_AMD_KER3D_DEFAULT 0
+#define ASTRA_AMD_KER3D_SUM_SQUARE_WEIGHTS 1 /* not built: UNSUPPORTED     */
+#define ASTRA_AMD_KER3D_FDK_WEIGHTING 2
+#define ASTRA_AMD_KER3D_2D_WEIGHTING 3
+#define ASTRA_AMD_KER3D_MATCHED_BP 4 /* BP = exact transpose of the FP     */
+
+/* SProjectorParams2D (include/astra/cuda/2d/dims.h:40-59). */
+typedef struct {
+    unsigned int rays_per_det;       /* iRaysPerDet (FP supersampling)     */
+    unsigned int rays_per_pixel_dim; /* iRaysPerPixelDim (BP)              */
+    float output_scale;              /* fOutputScale                       */
+} astra_amd_params2d;
+
+/* SProjectorParams3D (include/astra/cuda/3d/dims3d.h:53-66); volScale
+ * lives in the geometry handle. */
+typedef struct {
+    unsigned int rays_per_det_dim;   /* iRaysPerDetDim                     */
+    unsigned int rays_per_voxel_dim; /* iRaysPerVoxelDim                   */
+    float output_scale;              /* fOutputScale                       */
+    int kernel;                      /* ASTRA_AMD_KER3D_*                  */
+} astra_amd_params3d;
+
+typedef struct astra_amd_geom2d_s *astra_amd_geom2d;
+typedef struct astra_amd_geom3d_s *astra_amd_geom3d;
+typedef struct astra_amd_filter_s *astra_amd_filter;
+typedef struct astra_amd_algo_s *astra_amd_algo;
+
+/* ------------------------------------------------------------- library */
+/* Thread-local message of the last failing call on this thread
+ * (CLogger::getLastErrMsg, include/astra/Logging.h:169). */
+const char *astra_amd_last_error(void);
+/* Library version string. */
+const char *astra_amd_version(void);
+/* Make device `index` current for this host thread
+ * (astraCUDA::setGPUIndex, cuda/2d/astra.cu:94-105). */
+int astra_amd_set_device(int index);
+/* Block until all work on `stream` is complete. */
+int astra_amd_stream_sync(void *stream);
+
+/* ----------------------------------------------------- 2D geometry ---- */
+/* Build a device-resident 2D geometry from NORMALISED projection vectors
+ * (voxel units, volume centred at 0): 6 floats per angle,
+ * [rayX,rayY,dSX,dSY,uX,uY] (PAR) or [srcX,srcY,dSX,dSY,uX,uY] (FAN).
+ * Replaces Geometry2DParameters (include/astra/GeometryUtil2D.h:119-164)
+ * plus the per-call transferConstants of cuda/2d/{par,fan}_{fp,bp}.cu.
+ * Synchronous; the handle belongs to the current device. */
+int astra_amd_geom2d_create(int kind, unsigned int vol_width, unsigned int vol_height,
+                            unsigned int n_angles, unsigned int n_dets, const float *vecs,
+                            astra_amd_geom2d *out);
+int astra_amd_geom2d_destroy(astra_amd_geom2d g);
+
+/* ----------------------------------------------------- 3D geometry ---- */
+/* Build a device-resident 3D geometry from NORMALISED vectors: 12 doubles
+ * per angle [s, dS, u, v] (s = source for CONE, ray for PAR3D).
+ * Replaces Geometry3DParameters (include/astra/GeometryUtil3D.h:160-224)
+ * and the transferConstants of cuda/3d/{cone,par3d}_{fp,bp}.cu.
+ * vol_scale = SVolScale3D (voxel size x, y, z).  Synchronous. */
+int astra_amd_geom3d_create(int kind, unsigned int vol_x, unsigned int vol_y,
+                            unsigned int vol_z, unsigned int n_angles, unsigned int det_u,
+                            unsigned int det_v, const double *vecs, const float *vol_scale,
+                            astra_amd_geom3d *out);
+int astra_amd_geom3d_destroy(astra_amd_geom3d g);
+
+/* ------------------------------------------------------ projectors ---- */
+/* 2D forward projection: astraCUDA::FP (include/astra/cuda/2d/mem2d.h:77-80,
+ * cuda/2d/mem2d.cu:224-261). */
+int astra_amd_fp2d(astra_amd_geom2d g, const float *vol, float *sino,
+                   const astra_amd_params2d *p, int mode, void *stream);
+/* 2D back projection: astraCUDA::BP (include/astra/cuda/2d/mem2d.h:81-83,
+ * cuda/2d/mem2d.cu:278-315). */
+int astra_amd_bp2d(astra_amd_geom2d g, const float *sino, float *vol,
+                   const astra_amd_params2d *p, int mode, void *stream);
+/* Fan-beam FBP-weighted back projection: FanBP_FBPWeighted
+ * (cuda/2d/fan_bp.cu:311-336). FAN geometries only. */
+int astra_amd_bp2d_fbp_weighted(astra_amd_geom2d g, const float *sino, float *vol,
+                                const astra_amd_params2d *p, int mode, void *stream);
+/* 3D forward projection: astraCUDA3d::FP (include/astra/cuda/3d/mem3d.h:107-108,
+ * cuda/3d/mem3d.cu:334-398). */
+int astra_amd_fp3d(astra_amd_geom3d g, const float *vol, float *proj,
+                   const astra_amd_params3d *p, int mode, void *stream);
+/* 3D back projection: astraCUDA3d::BP (include/astra/cuda/3d/mem3d.h:109-110,
+ * cuda/3d/mem3d.cu:416-485).  kernel DEFAULT / FDK_WEIGHTING / 2D_WEIGHTING
+ * follow dev_cone_BP / dev_par3D_BP; MATCHED_BP is the exact transpose of
+ * astra_amd_fp3d (no reference counterpart for cone/par3d). */
+int astra_amd_bp3d(astra_amd_geom3d g, const float *proj, float *vol,
+                   const astra_amd_params3d *p, int mode, void *stream);
+/* Exact transpose of astra_amd_fp2d (no reference counterpart). */
+int astra_amd_bp2d_matched(astra_amd_geom2d g, const float *sino, float *vol,
+                           const astra_amd_params2d *p, int mode, void *stream);
+
+/* ------------------------------------------------------ FDK / filter -- */
+/* Ram-Lak half spectrum of genFilter (src/Filters.cpp:133-160,473-485):
+ * n_fft = nextpow2(2*n_det) (src/Filters.cpp:662-669), out has
+ * n_fft/2+1 floats.  Host-only, synchronous. */
+int astra_amd_ramlak_spectrum(unsigned int n_det, float d, float *out, unsigned int n_out);
+/* Upload a real half spectrum (n_fft/2+1 floats, host) for rows of
+ * n_det samples.  Replaces prepareCuFFTFilter (cuda/2d/fft.cu:375-604). */
+int astra_amd_filter_create(unsigned int n_det, const float *half_spectrum,
+                            unsigned int n_bins, astra_amd_filter *out);
+int astra_amd_filter_destroy(astra_amd_filter f);
+/* Filter n_rows contiguous rows of n_det floats in place:
+ * zero-pad to n_fft, x spectrum, inverse, /n_fft, keep n_det
+ * (cuda/3d/fdk.cu:351-398, cuda/2d/fft.cu:52-131). */
+int astra_amd_filter_rows(astra_amd_filter f, float *rows, size_t n_rows,
+                          void *stream);
+/* FDK cosine pre-weighting in place: devFDK_preweight (cuda/3d/fdk.cu:70-113)
+ * with the scalar geometry FDK() extracts from angle 0 (fdk.cu:427-445).
+ * CONE geometries only. */
+int astra_amd_fdk_preweight(astra_amd_geom3d g, float *proj, void *stream);
+/* Full FDK: pre-weight + Ram-Lak filter (filter handle, NULL = Ram-Lak
+ * built internally) + FDK-weighted cone BP, astraCUDA3d::FDK
+ * (cuda/3d/fdk.cu:415-478).  `proj` is used as scratch and overwritten
+ * with the filtered projections.  Short scan is UNSUPPORTED. */
+int astra_amd_fdk(astra_amd_geom3d g, float *proj, float *vol, float output_scale,
+                  int short_scan, astra_amd_filter f, int mode, void *stream);
+/* 2D FBP: astraCUDA::FBP (cuda/2d/fbp.cu:77-190).  FAN: one-row cone
+ * pre-weighting (fbp.cu:106-148), filter, FBP-weighted fan BP; PAR: filter,
+ * BP with output_scale * (pi/2)/n_angles.  The caller folds the
+ * 1/pixel-area factor of CCudaFilteredBackProjectionAlgorithm::run
+ * (src/CudaFilteredBackProjectionAlgorithm.cpp:210-212) into output_scale.
+ * `sino` is overwritten with the filtered sinogram.  f = NULL: Ram-Lak. */
+int astra_amd_fbp2d(astra_amd_geom2d g, float *sino, float *vol, float output_scale,
+                    astra_amd_filter f, int mode, void *stream);
+
+/* ---------------------------------------------------- elementwise ---- */
+/* Sum of a[i]*b[i] accumulated in float64 (dotProduct2D / dotProduct3D,
+ * cuda/2d/util.cu:304-357, cuda/3d/mem3d.cu:581-590). Synchronous. */
+int astra_amd_dot(const float *a, const float *b, size_t n, double *out, void *stream);
+
+/* ------------------------------------------------------ algorithms ---- */
+/* Iterative algorithms over device data, the GPU algorithm objects
+ * CCudaSirtAlgorithm / CCudaCglsAlgorithm / CCudaSirtAlgorithm3D /
+ * CCudaCglsAlgorithm3D (src/CudaSirtAlgorithm.cpp:284-397,
+ * src/CudaCglsAlgorithm.cpp:166-266, src/CudaSirtAlgorithm3D.cpp:301-381,
+ * src/CudaCglsAlgorithm3D.cpp:264-360).  The geometry handle must outlive
+ * the algorithm. */
+#define ASTRA_AMD_ALGO_SIRT 0
+#define ASTRA_AMD_ALGO_CGLS 1
+int astra_amd_algo2d_create(int type, astra_amd_geom2d g, const astra_amd_params2d *p,
+                            astra_amd_algo *out);
+int astra_amd_algo3d_create(int type, astra_amd_geom3d g, const astra_amd_params3d *p,
+                            astra_amd_algo *out);
+/* Options: "Relaxation", "MinConstraint", "MaxConstraint"
+ * (src/CudaSirtAlgorithm.cpp:80-91, src/ReconstructionAlgorithm2D.cpp). */
+int astra_amd_algo_set_option(astra_amd_algo a, const char *key, double value);
+/* Optional masks (device pointers, lifetime >= algorithm): "ReconstructionMask"
+ * (volume shape), "SinogramMask" (projection shape), "MinMask", "MaxMask". */
+int astra_amd_algo_set_mask(astra_amd_algo a, const char *key, const float *mask);
+/* run(iterations): vol is read as the initial iterate and overwritten. */
+int astra_amd_algo_run(astra_amd_algo a, const float *sino, float *vol, int iterations,
+                       void *stream);
+/* ||b - A x|| (getResidualNorm). Synchronous. */
+int astra_amd_algo_residual_norm(astra_amd_algo a, const float *sino, const float *vol,
+                                 double *out, void *stream);
+int astra_amd_algo_destroy(astra_amd_algo a);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ASTRA_AMD_H */

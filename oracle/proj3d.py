@@ -1,0 +1,250 @@
+"""Oracle 3D projectors (float64), restating the reference CUDA sampling.
+
+Test infrastructure only (see ``oracle/__init__.py``).
+
+Layouts: volume ``[z][y][x]``, projections ``[v][angle][u]``
+(cuda/3d/cone_fp.cu:229, cuda/3d/cone_bp.cu:132).  Vectors are the
+normalised rows from ``oracle.geometry.normalise_3d``; ``vol_scale`` is the
+voxel size triple (``SVolScale3D``) and ``out_scale`` is
+``SProjectorParams3D::fOutputScale``.
+"""
+import math
+
+import numpy as np
+
+from .proj2d import tex2d
+
+f32 = np.float32
+
+# principal-axis coordinate permutations of DIR_X / DIR_Y / DIR_Z
+# (cuda/3d/cone_fp.cu:61-98): (c0, c1, c2) = axes of (slice, dim1, dim2)
+_DIRS = {0: (0, 1, 2), 1: (1, 0, 2), 2: (2, 0, 1)}
+
+
+def _slice_plane(vol, d, s):
+    """2D plane of slice s along axis d, indexed [dim2][dim1]."""
+    if d == 0:
+        return vol[:, :, s]          # [z][y]
+    if d == 1:
+        return vol[:, s, :]          # [z][x]
+    return vol[s, :, :]              # [y][x]
+
+
+def _shape_xyz(vol):
+    nz, ny, nx = vol.shape
+    return (nx, ny, nz)
+
+
+def cone_direction(vec, n_u, n_v):
+    """Per-angle principal axis of ``ConeFP_Array_internal``
+    (cuda/3d/cone_fp.cu:378-387): largest |src - detector centre| component,
+    compared as float32 with ties going to the lower axis."""
+    s = vec[0:3]
+    c = vec[3:6] + n_u * vec[6:9] * 0.5 + n_v * vec[9:12] * 0.5
+    dX, dY, dZ = (abs(f32(s[k] - c[k])) for k in range(3))
+    if dX >= dY and dX >= dZ:
+        return 0
+    if dY >= dX and dY >= dZ:
+        return 1
+    return 2
+
+
+def par3d_direction(vec):
+    """``Par3DFP_Array_internal`` (cuda/3d/par3d_fp.cu:474-488)."""
+    dX, dY, dZ = (abs(f32(vec[k])) for k in range(3))
+    if dX >= dY and dX >= dZ:
+        return 0
+    if dY >= dX and dY >= dZ:
+        return 1
+    return 2
+
+
+def _fp_scale(d, vol_scale, out_scale, par3d=False):
+    """SCALE_CUBE / SCALE_NONCUBE factors (cuda/3d/cone_fp.cu:100-112,
+    342-370; cuda/3d/par3d_fp.cu:442-472).  Returns (s1, s2, outscale)."""
+    vx, vy, vz = (f32(x) for x in vol_scale)
+    cube = abs(float(vx) / float(vy) - 1.0) <= 1e-5 and abs(float(vx) / float(vz) - 1.0) <= 1e-5
+    if cube:
+        return 1.0, 1.0, float(f32(out_scale) * vx)
+    if d == 0:
+        s1, s2, o = vy / vx, vz / vx, vx
+    elif d == 1:
+        # par3d keeps the reference's fScale2 = volScale.fY / volScale.fY
+        # (cuda/3d/par3d_fp.cu:459); cone uses fZ / fY (cone_fp.cu:363)
+        s1, s2, o = vx / vy, (vy / vy if par3d else vz / vy), vy
+    else:
+        s1, s2, o = vx / vz, vy / vz, vz
+    return float(s1 * s1), float(s2 * s2), float(f32(out_scale) * o)
+
+
+def _ray_sum(vol, d, a1, a2, b1, b2):
+    """Sum over every slice of the bilinear sample at (f1, f2)."""
+    n = _shape_xyz(vol)
+    c0, c1, c2 = _DIRS[d]
+    n0, n1, n2 = n[c0], n[c1], n[c2]
+    acc = np.zeros_like(a1)
+    for s in range(n0):
+        p = s - 0.5 * n0 + 0.5
+        f1 = a1 * p + b1 + 0.5 * n1
+        f2 = a2 * p + b2 + 0.5 * n2
+        acc += tex2d(_slice_plane(vol, d, s), f1, f2)
+    return acc
+
+
+def cone_fp(vol, vecs, n_u, n_v, vol_scale=(1.0, 1.0, 1.0), out_scale=1.0):
+    """Cone-beam ray-driven FP: ``cone_FP_t`` (cuda/3d/cone_fp.cu:154-231).
+
+    Ray to the centre of detector pixel (u, v) (DetS + (u+.5)U + (v+.5)V),
+    written as (c1, c2) = a * c0 + b along the principal axis, sampled at
+    every slice centre; the sum is scaled by sqrt(a1^2 s1 + a2^2 s2 + 1).
+    """
+    vol = np.asarray(vol, dtype=np.float64)
+    nA = vecs.shape[0]
+    proj = np.zeros((n_v, nA, n_u))
+    uu, vv = np.meshgrid(np.arange(n_u) + 0.5, np.arange(n_v) + 0.5)
+    for a in range(nA):
+        vec = vecs[a]
+        d = cone_direction(vec, n_u, n_v)
+        c0, c1, c2 = _DIRS[d]
+        src = vec[0:3]
+        det = [vec[3 + k] + uu * vec[6 + k] + vv * vec[9 + k] for k in range(3)]
+        den = src[c0] - det[c0]
+        a1 = (src[c1] - det[c1]) / den
+        a2 = (src[c2] - det[c2]) / den
+        b1 = src[c1] - a1 * src[c0]
+        b2 = src[c2] - a2 * src[c0]
+        s1, s2, osc = _fp_scale(d, vol_scale, out_scale)
+        proj[:, a, :] = _ray_sum(vol, d, a1, a2, b1, b2) * np.sqrt(a1 * a1 * s1 + a2 * a2 * s2 + 1.0) * osc
+    return proj
+
+
+def par3d_fp(vol, vecs, n_u, n_v, vol_scale=(1.0, 1.0, 1.0), out_scale=1.0):
+    """Parallel-beam 3D FP: ``par3D_FP_t`` (cuda/3d/par3d_fp.cu:152-230)."""
+    vol = np.asarray(vol, dtype=np.float64)
+    nA = vecs.shape[0]
+    proj = np.zeros((n_v, nA, n_u))
+    uu, vv = np.meshgrid(np.arange(n_u) + 0.5, np.arange(n_v) + 0.5)
+    for a in range(nA):
+        vec = vecs[a]
+        d = par3d_direction(vec)
+        c0, c1, c2 = _DIRS[d]
+        ray = vec[0:3]
+        det = [vec[3 + k] + uu * vec[6 + k] + vv * vec[9 + k] for k in range(3)]
+        a1 = ray[c1] / ray[c0]
+        a2 = ray[c2] / ray[c0]
+        b1 = det[c1] - a1 * det[c0]
+        b2 = det[c2] - a2 * det[c0]
+        s1, s2, osc = _fp_scale(d, vol_scale, out_scale, par3d=True)
+        a1f = np.full_like(b1, a1)
+        a2f = np.full_like(b2, a2)
+        proj[:, a, :] = _ray_sum(vol, d, a1f, a2f, b1, b2) * math.sqrt(a1 * a1 * s1 + a2 * a2 * s2 + 1.0) * osc
+    return proj
+
+
+# ---------------------------------------------------------------- BP ----
+
+def _det3x(b, c):
+    return b[1] * c[2] - b[2] * c[1]
+
+
+def _det3y(b, c):
+    return -(b[0] * c[2] - b[2] * c[0])
+
+
+def _det3z(b, c):
+    return b[0] * c[1] - b[1] * c[0]
+
+
+def _det3(a, b, c):
+    return a[0] * _det3x(b, c) + a[1] * _det3y(b, c) + a[2] * _det3z(b, c)
+
+
+def _scaled_cross_norm(a, b, sc):
+    """``scaled_cross3(a, b, sc).norm()`` (include/astra/cuda/3d/util3d.h:82-88)."""
+    cx = _det3x(a, b) * sc[1] * sc[2]
+    cy = _det3y(a, b) * sc[0] * sc[2]
+    cz = _det3z(a, b) * sc[0] * sc[1]
+    return math.sqrt(cx * cx + cy * cy + cz * cz)
+
+
+def cone_bp_params(vec, kernel, vol_scale):
+    """``DevConeParams`` (cuda/3d/cone_bp.cu:221-302): NumU, NumV, Den as
+    (x, y, z, w) coefficient quadruples, float64."""
+    s = np.asarray(vec[0:3], dtype=np.float64)
+    d = np.asarray(vec[3:6], dtype=np.float64)
+    u = np.asarray(vec[6:9], dtype=np.float64)
+    v = np.asarray(vec[9:12], dtype=np.float64)
+    smd = s - d
+    if kernel == "fdk":
+        scale = 1.0 / _det3(u, v, s)
+    elif kernel == "2d":
+        scale = _scaled_cross_norm(u, (0.0, 0.0, 1.0), vol_scale) / _det3(u, v, smd)
+    else:
+        scale = math.sqrt(_scaled_cross_norm(u, v, vol_scale)) / _det3(u, v, smd)
+    num_u = (scale * _det3x(v, smd), scale * _det3y(v, smd), scale * _det3z(v, smd), scale * _det3(s, v, d))
+    num_v = (-scale * _det3x(u, smd), -scale * _det3y(u, smd), -scale * _det3z(u, smd), -scale * _det3(s, u, d))
+    den = (-scale * _det3x(u, v), -scale * _det3y(u, v), -scale * _det3z(u, v), scale * _det3(u, v, s))
+    return num_u, num_v, den
+
+
+def _voxel_grid(shape):
+    nz, ny, nx = shape
+    X = np.arange(nx) - 0.5 * nx + 0.5
+    Y = np.arange(ny) - 0.5 * ny + 0.5
+    Z = np.arange(nz) - 0.5 * nz + 0.5
+    return np.meshgrid(Z, Y, X, indexing="ij")
+
+
+def cone_bp(proj, vecs, shape, vol_scale=(1.0, 1.0, 1.0), out_scale=1.0, kernel="default"):
+    """Cone-beam voxel-driven BP: ``dev_cone_BP<KERNEL>``
+    (cuda/3d/cone_bp.cu:55-133) and its output scale (``ConeBP_Array``,
+    :322-328).  kernel: 'default' (weight fr^2), 'fdk' (Den.w = 1, fr^2,
+    output / voxel x), '2d' (weight fr)."""
+    proj = np.asarray(proj, dtype=np.float64)
+    fz, fy, fx = _voxel_grid(shape)
+    vol = np.zeros(shape)
+    for a in range(vecs.shape[0]):
+        nu, nv, dn = cone_bp_params(vecs[a], kernel, vol_scale)
+        U = nu[3] + fx * nu[0] + fy * nu[1] + fz * nu[2]
+        V = nv[3] + fx * nv[0] + fy * nv[1] + fz * nv[2]
+        D = (1.0 if kernel == "fdk" else dn[3]) + fx * dn[0] + fy * dn[1] + fz * dn[2]
+        fr = 1.0 / D
+        val = tex2d(proj[:, a, :], U * fr, V * fr)
+        vol += (fr if kernel == "2d" else fr * fr) * val
+    vx, vy, vz = (f32(x) for x in vol_scale)
+    if kernel == "fdk":
+        osc = float(f32(out_scale) / vx)
+    else:
+        osc = float(f32(out_scale) * (vx * vy * vz))
+    return vol * osc
+
+
+def par3d_bp_params(vec, kernel, vol_scale):
+    """``DevPar3DParams`` + scale (cuda/3d/par3d_bp.cu:202-244)."""
+    r = np.asarray(vec[0:3], dtype=np.float64)
+    d = np.asarray(vec[3:6], dtype=np.float64)
+    u = np.asarray(vec[6:9], dtype=np.float64)
+    v = np.asarray(vec[9:12], dtype=np.float64)
+    den = _det3(r, u, v)
+    num_u = (-_det3x(r, v) / den, -_det3y(r, v) / den, -_det3z(r, v) / den, -_det3(r, d, v) / den)
+    num_v = (_det3x(r, u) / den, _det3y(r, u) / den, _det3z(r, u) / den, _det3(r, d, u) / den)
+    if kernel == "2d":
+        s = 1.0 / _scaled_cross_norm(u, (0.0, 0.0, 1.0), vol_scale)
+    else:
+        s = 1.0 / _scaled_cross_norm(u, v, vol_scale)
+    return num_u, num_v, s
+
+
+def par3d_bp(proj, vecs, shape, vol_scale=(1.0, 1.0, 1.0), out_scale=1.0, kernel="default"):
+    """Parallel-beam 3D BP: ``dev_par3D_BP`` (cuda/3d/par3d_bp.cu:55-120),
+    output scale volScale x*y*z (:263)."""
+    proj = np.asarray(proj, dtype=np.float64)
+    fz, fy, fx = _voxel_grid(shape)
+    vol = np.zeros(shape)
+    for a in range(vecs.shape[0]):
+        nu, nv, s = par3d_bp_params(vecs[a], kernel, vol_scale)
+        U = nu[3] + fx * nu[0] + fy * nu[1] + fz * nu[2]
+        V = nv[3] + fx * nv[0] + fy * nv[1] + fz * nv[2]
+        vol += tex2d(proj[:, a, :], U, V) * s
+    vx, vy, vz = (f32(x) for x in vol_scale)
+    return vol * float(f32(out_scale) * (vx * vy * vz))
