@@ -68,6 +68,9 @@ SIGNATURES = {
     "astra_amd_algo_run": [_vp, _vp, _vp, _i, _vp],
     "astra_amd_algo_residual_norm": [_vp, _vp, _vp, ctypes.POINTER(_d), _vp],
     "astra_amd_algo_destroy": [_vp],
+    "astra_amd_profile_enable": [_i],
+    "astra_amd_profile_read": [ctypes.c_char_p, ctypes.POINTER(_d), ctypes.POINTER(_d),
+                               ctypes.POINTER(ctypes.c_ulonglong)],
 }
 STRINGS = ["astra_amd_last_error", "astra_amd_version"]
 
@@ -119,3 +122,16 @@ def call(name, *args):
 
 def version():
     return load().astra_amd_version().decode()
+
+
+def profile_enable(on=True):
+    """Start (clearing previous records) or stop per-kernel event timing."""
+    call("astra_amd_profile_enable", 1 if on else 0)
+
+
+def profile_read(kernel):
+    """(total_ms, work_units, launches) recorded for a kernel name."""
+    ms, units, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_ulonglong()
+    call("astra_amd_profile_read", kernel.encode(), ctypes.byref(ms), ctypes.byref(units),
+         ctypes.byref(n))
+    return ms.value, units.value, n.value

@@ -29,8 +29,7 @@ def ellipses_2d(n_rows, n_cols=None, count=10, seed=0):
     return np.clip(img, 0.0, 1.0).astype(np.float32)
 
 
-def ellipsoids_3d(n, count=20, seed=0, chunk=64):
-    """n x n x n ellipsoid phantom, [z][y][x]; built slab-wise to bound memory."""
+def _ellipsoid_params(n, count, seed):
     rng = np.random.default_rng(seed)
     params = []
     for _ in range(count):
@@ -41,6 +40,29 @@ def ellipsoids_3d(n, count=20, seed=0, chunk=64):
         phi = np.pi * rng.random()
         rho = rng.random()
         params.append((c, ax, phi, rho))
+    return params
+
+
+def ellipsoids_3d_device(n, count=20, seed=0, device="cuda"):
+    """Same phantom as ellipsoids_3d, rasterised on a torch device."""
+    import torch
+
+    ar = torch.arange(n, device=device, dtype=torch.float32) - (n - 1) / 2.0
+    zz, yy, xx = ar.view(n, 1, 1), ar.view(1, n, 1), ar.view(1, 1, n)
+    vol = torch.zeros((n, n, n), dtype=torch.float32, device=device)
+    for c, ax, phi, rho in _ellipsoid_params(n, count, seed):
+        cs, sn = float(np.cos(phi)), float(np.sin(phi))
+        u = (xx - float(c[0])) * cs + (yy - float(c[1])) * sn
+        v = -(xx - float(c[0])) * sn + (yy - float(c[1])) * cs
+        w = zz - float(c[2])
+        inside = (u / float(ax[0])) ** 2 + (v / float(ax[1])) ** 2 + (w / float(ax[2])) ** 2 <= 1.0
+        vol += inside.to(torch.float32) * float(rho)
+    return vol.clamp_(0.0, 1.0)
+
+
+def ellipsoids_3d(n, count=20, seed=0, chunk=64):
+    """n x n x n ellipsoid phantom, [z][y][x]; built slab-wise to bound memory."""
+    params = _ellipsoid_params(n, count, seed)
     vol = np.zeros((n, n, n), dtype=np.float32)
     yy, xx = np.mgrid[0:n, 0:n].astype(np.float64)
     xx = xx - (n - 1) / 2.0

@@ -307,6 +307,19 @@ int astra_amd_fbp2d(astra_amd_geom2d g, float *sino, float *vol, float output_sc
     });
 }
 
+// ---------------------------------------------------------- profiling ---
+int astra_amd_profile_enable(int on) {
+    return guarded([&] { amx::prof_enable(on != 0); });
+}
+
+int astra_amd_profile_read(const char *kernel, double *total_ms, double *units,
+                           unsigned long long *launches) {
+    return guarded([&] {
+        AMX_REQUIRE(kernel && total_ms && units && launches, "profile_read: NULL argument");
+        amx::prof_read(kernel, *total_ms, *units, *launches);
+    });
+}
+
 // -------------------------------------------------------- elementwise ---
 int astra_amd_dot(const float *a, const float *b, size_t n, double *out, void *stream) {
     return guarded([&] {

@@ -124,8 +124,10 @@ void filter_rows_launch(const Filter &f, float *rows, size_t n_rows, const float
     AMX_REQUIRE(lds <= 65536, "filter: n_fft %u too large for one workgroup", f.n_fft);
     const size_t groups = (n_rows + 1) / 2;
     AMX_REQUIRE(groups < (1ull << 31), "filter: too many rows");
+    ProfToken pt = prof_begin("filter_rows", (double)n_rows, s);
     filter_rows_kernel<<<(unsigned)groups, 256, lds, s>>>(A);
     AMX_LAUNCHED();
+    prof_end(pt, s);
 }
 
 void fdk_weights(const Geom3D &g, std::vector<float> &w) {

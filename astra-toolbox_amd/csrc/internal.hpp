@@ -152,6 +152,17 @@ struct Geom2D {
     DevArray bp, bp_fbp;      // BP2DAngle[na]
 };
 
+// ------------------------------------------------------ profiling --------
+struct ProfToken {
+    long index = -1;
+    bool active = false;
+};
+bool prof_on();
+ProfToken prof_begin(const char *name, double units, hipStream_t s);
+void prof_end(const ProfToken &t, hipStream_t s);
+void prof_enable(bool on);
+bool prof_read(const char *name, double &ms, double &units, unsigned long long &launches);
+
 // ------------------------------------------------------ launchers --------
 // 3D
 // mode: OUT_ADD / OUT_SET / OUT_RESID (out = (b + scale*Ax) * w); padded_out

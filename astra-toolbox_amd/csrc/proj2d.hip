@@ -344,11 +344,14 @@ void fp2d_launch(const Geom2D &g, const float *vol, float *sino, float out_scale
     A.rw = resid_w;
     A.padded_out = padded_out ? 1 : 0;
     dim3 grid((g.nd + 63) / 64, (g.na + 3) / 4);
+    ProfToken pt = prof_begin(g.kind == ASTRA_AMD_GEOM_FAN ? "fp2d_fan" : "fp2d_par",
+                              (double)g.na * g.nd * rays * std::max(g.w, g.h), s);
     if (g.kind == ASTRA_AMD_GEOM_FAN)
         fp2d_kernel<true><<<grid, 256, 0, s>>>(A);
     else
         fp2d_kernel<false><<<grid, 256, 0, s>>>(A);
     AMX_LAUNCHED();
+    prof_end(pt, s);
 }
 
 void bp2d_launch_padded(const Geom2D &g, const float *sp, float *vol, float out_scale, bool fbp,
@@ -363,6 +366,8 @@ void bp2d_launch_padded(const Geom2D &g, const float *sp, float *vol, float out_
     A.lo = lo;
     A.hi = hi;
     dim3 grid((g.w + 63) / 64, (g.h + 15) / 16);
+    ProfToken pt = prof_begin(g.kind == ASTRA_AMD_GEOM_PAR ? "bp2d_par" : (fbp ? "bp2d_fbp" : "bp2d_fan"),
+                              (double)g.w * g.h * g.na, s);
     if (g.kind == ASTRA_AMD_GEOM_PAR) {
         A.ang = g.bp.as<BP2DAngle>();
         bp2d_kernel<0><<<grid, 256, 0, s>>>(A);
@@ -374,6 +379,7 @@ void bp2d_launch_padded(const Geom2D &g, const float *sp, float *vol, float out_
         bp2d_kernel<2><<<grid, 256, 0, s>>>(A);
     }
     AMX_LAUNCHED();
+    prof_end(pt, s);
 }
 
 void bp2d_launch(const Geom2D &g, const float *sino, float *vol, float out_scale, bool fbp,

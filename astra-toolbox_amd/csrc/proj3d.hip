@@ -428,27 +428,20 @@ void fp3d_launch(const Geom3D &g, const float *vol, float *proj, float out_scale
         A.volp = copy.as<float>();
         A.out = proj;
         A.rb = resid_b;
-        A.rw = nullptr;
+        A.rw = resid_w;
         A.padded_out = padded_out ? 1 : 0;
-        // Groups run one after another on the stream.  The first applies
-        // the requested mode; the others accumulate, and for a weighted
-        // residual the last one applies the weights.
-        const bool last = (i == ng - 1);
-        if (i == 0) {
-            A.mode = mode;
-            if (mode == OUT_RESID && last) A.rw = resid_w;
-        } else if (mode == OUT_RESID && last && resid_w) {
-            A.mode = OUT_ADDMUL;
-            A.rw = resid_w;
-        } else {
-            A.mode = OUT_ADD;
-        }
+        // The direction groups partition the angles: every output element
+        // is written by exactly one group, which applies the full mode.
+        A.mode = mode;
         dim3 grid(tiles, g.dir_count[k]);
+        ProfToken pt = prof_begin(g.kind == ASTRA_AMD_GEOM_CONE ? "fp3d_cone" : "fp3d_par3d",
+                                  (double)g.dir_count[k] * g.nu * g.nv * A.n0, s);
         if (g.kind == ASTRA_AMD_GEOM_CONE)
             fp3d_kernel<true><<<grid, 256, 0, s>>>(A);
         else
             fp3d_kernel<false><<<grid, 256, 0, s>>>(A);
         AMX_LAUNCHED();
+        prof_end(pt, s);
     }
 }
 
@@ -482,6 +475,9 @@ void bp3d_launch_padded(const Geom3D &g, const float *pp, float *vol, float out_
         A.osc = out_scale * (vx * vy * vz);
     constexpr int ZS = 8;
     dim3 grid(((g.nx + 15) / 16) * ((g.ny + 15) / 16), (g.nz + ZS - 1) / ZS);
+    const char *pname = g.kind != ASTRA_AMD_GEOM_CONE ? "bp3d_par3d"
+                        : (kernel == ASTRA_AMD_KER3D_FDK_WEIGHTING ? "bp3d_fdk" : "bp3d_cone");
+    ProfToken pt = prof_begin(pname, (double)g.nx * g.ny * g.nz * g.na, s);
     if (g.kind == ASTRA_AMD_GEOM_CONE) {
         if (kernel == ASTRA_AMD_KER3D_FDK_WEIGHTING) {
             A.ang = g.bp_fdk.as<BP3DAngle>();
@@ -504,6 +500,7 @@ void bp3d_launch_padded(const Geom3D &g, const float *pp, float *vol, float out_
         bp3d_kernel<2, ZS><<<grid, 256, 0, s>>>(A);
     }
     AMX_LAUNCHED();
+    prof_end(pt, s);
 }
 
 void bp3d_launch(const Geom3D &g, const float *proj, float *vol, float out_scale, int kernel,

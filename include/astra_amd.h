@@ -175,6 +175,19 @@ int astra_amd_fdk(astra_amd_geom3d g, float *proj, float *vol, float output_scal
 int astra_amd_fbp2d(astra_amd_geom2d g, float *sino, float *vol, float output_scale,
                     astra_amd_filter f, int mode, void *stream);
 
+/* ------------------------------------------------------ profiling ---- */
+/* Per-kernel event timing (no counterpart in the reference, which has no
+ * tracing): while enabled, every hot kernel launch is bracketed by a
+ * hipEvent pair on its own stream.  enable(1) clears previous records. */
+int astra_amd_profile_enable(int on);
+/* Totals for one kernel name ("fp3d_cone", "fp3d_par3d", "bp3d_cone",
+ * "bp3d_fdk", "bp3d_par3d", "fp2d_par", "fp2d_fan", "bp2d_par", "bp2d_fan",
+ * "bp2d_fbp", "filter_rows"): summed duration, algorithmic work units
+ * (ray-slice samples, voxel-angle updates or filtered rows) and launches.
+ * Synchronises the recorded events. */
+int astra_amd_profile_read(const char *kernel, double *total_ms, double *units,
+                           unsigned long long *launches);
+
 /* ---------------------------------------------------- elementwise ---- */
 /* Sum of a[i]*b[i] accumulated in float64 (dotProduct2D / dotProduct3D,
  * cuda/2d/util.cu:304-357, cuda/3d/mem3d.cu:581-590). Synchronous. */
