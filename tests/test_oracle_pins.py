@@ -1,0 +1,296 @@
+"""Pins of the FP64 oracle (CPU only).
+
+Each test checks an oracle function against something it does not restate:
+a closed form, a different formula for the same quantity (2D vs 3D, cone vs
+parallel limit), the reference's own known answers, or an invariant.  A
+plausible transcription slip (sign, index, transposed operand, dropped
+factor) fails at least one of them.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import algorithms as OA, fdk as OF, geometry as OG, proj2d as O2, proj3d as O3
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(b))
+
+
+# ------------------------------------------------------------- geometry ---
+def test_cone_vectors_closed_form():
+    """genConeProjections at angle 0 and pi/2 (src/GeometryUtil3D.cpp:42-83)."""
+    v = OG.gen_cone_projections([0.0, np.pi / 2], 4, 2, 100.0, 50.0, 1.5, 2.0)
+    assert np.allclose(v[0], [0, -100, 0, -3.0, 50, -2.0, 1.5, 0, 0, 0, 0, 2.0])
+    # rotation by +90 degrees: (x, y) -> (-y, x)
+    assert np.allclose(v[1], [100, 0, 0, -50, -3.0, -2.0, 0, 1.5, 0, 0, 0, 2.0], atol=1e-5)
+
+
+def test_normalisation_translate_scale():
+    vg = {"GridRowCount": 10, "GridColCount": 20, "GridSliceCount": 5,
+          "option": {"WindowMinX": -5, "WindowMaxX": 15, "WindowMinY": 0, "WindowMaxY": 20,
+                     "WindowMinZ": -2.5, "WindowMaxZ": 2.5}}
+    v = np.array([[1.0, 2, 3, 4, 5, 6, 1, 0, 0, 0, 0, 1]])
+    out, vs = OG.normalise_3d(v, "cone", vg)
+    # voxel (1, 2, 1); centre (5, 10, 0): src (1-5, (2-10)/2, 3), corner (4-5, (5-10)/2, 6)
+    assert vs == (1.0, 2.0, 1.0)
+    assert np.allclose(out[0], [-4, -4, 3, -1, -2.5, 6, 1, 0, 0, 0, 0, 1])
+
+
+# ------------------------------------------------------------ 2D FP/BP ---
+def test_par_fp_angle0_column_sums():
+    """python/astra/tests.py:27-48: at angle 0 a parallel FP is the column sum."""
+    rng = np.random.default_rng(0)
+    vol = rng.random((24, 24))
+    vg = OG.vol_geom_2d(24, 24)
+    vec, osc = OG.normalise_2d(OG.gen_par_projections([0.0], 24, 1.0), "par", vg)
+    assert np.allclose(O2.par_fp(vol, vec, 24, osc)[0], vol.sum(axis=0), atol=1e-12)
+
+
+def test_par2d_equals_par3d_single_slice():
+    """2D parallel FP/BP (angle/offset trig form, row axis -y) against the 3D
+    parallel projector (vector form, no row flip) on a one-slice volume."""
+    rng = np.random.default_rng(1)
+    n, nd = 20, 28
+    angles = np.array([0.1, 0.5, 1.1, 2.0, 2.9])  # away from 45 degree ties
+    vol = rng.random((n, n))
+    v2, osc = OG.normalise_2d(OG.gen_par_projections(angles, nd, 0.9), "par", OG.vol_geom_2d(n, n))
+    s2 = O2.par_fp(vol, v2, nd, osc)
+    v3, vs = OG.normalise_3d(OG.gen_par3d_projections(angles, nd, 1, 0.9, 1.0), "par3d",
+                             OG.vol_geom_3d(n, n, 1))
+    vol3 = vol[::-1][None]  # 2D row r is world y = -(r - n/2 + 0.5)
+    s3 = O3.par3d_fp(vol3, v3, nd, 1, vs)[0]
+    assert rel(s2, s3) < 1e-6
+    y = rng.random((len(angles), nd))
+    b2 = O2.par_bp(y, v2, (n, n), osc)
+    b3 = O3.par3d_bp(y[None], v3, (1, n, n), vs)[0][::-1]
+    # the 2D BP weight is |r|/|d| = 1/w; the 3D one is 1/|u x v| = 1/(w * 1)
+    assert rel(b2, b3) < 1e-6
+
+
+def test_fan2d_equals_cone_single_slice():
+    """2D fan FP (per-ray orientation) against the cone FP (per-angle
+    orientation) on a one-slice volume, for a narrow fan whose rays all share
+    the central ray's orientation."""
+    rng = np.random.default_rng(2)
+    n, nd = 16, 12
+    angles = np.array([0.2, 1.0, 2.4, 3.6, 5.0])
+    vol = rng.random((n, n))
+    v2, osc = OG.normalise_2d(OG.gen_fan_projections(angles, nd, 300.0, 200.0, 1.0), "fan",
+                              OG.vol_geom_2d(n, n))
+    for a in range(len(angles)):
+        hor = {O2.fan_ray_is_horizontal(v2[a], t) for t in range(nd)}
+        assert len(hor) == 1
+    s2 = O2.fan_fp(vol, v2, nd, osc)
+    v3, vs = OG.normalise_3d(OG.gen_cone_projections(angles, nd, 1, 300.0, 200.0, 1.0, 1.0),
+                             "cone", OG.vol_geom_3d(n, n, 1))
+    s3 = O3.cone_fp(vol[::-1][None], v3, nd, 1, vs)[0]
+    assert rel(s2, s3) < 1e-6
+
+
+def test_fan_bp_matches_cone_2d_weighting():
+    """The cone '2d_weighting' BP is built to approximate the fan BP on a
+    single slice (cuda/3d/cone_bp.cu:263-270); with a one-row detector of
+    unit height they coincide."""
+    rng = np.random.default_rng(3)
+    n, nd = 16, 24
+    angles = np.linspace(0, 2 * np.pi, 7, endpoint=False)
+    v2, osc = OG.normalise_2d(OG.gen_fan_projections(angles, nd, 100.0, 60.0, 1.0), "fan",
+                              OG.vol_geom_2d(n, n))
+    v3, vs = OG.normalise_3d(OG.gen_cone_projections(angles, nd, 1, 100.0, 60.0, 1.0, 1.0),
+                             "cone", OG.vol_geom_3d(n, n, 1))
+    y = rng.random((len(angles), nd))
+    b2 = O2.fan_bp(y, v2, (n, n), osc)
+    b3 = O3.cone_bp(y[None], v3, (1, n, n), vs, kernel="2d")[0][::-1]
+    assert rel(b2, b3) < 1e-6
+
+
+@pytest.mark.parametrize("geom", ["par", "fan"])
+def test_2d_bp_is_approximate_adjoint(geom):
+    """<A x, y> ~ <x, A^T y> for the unmatched BP (tests/python/functional/
+    test_line2d.py:591-640 accepts 1e-1 for the GPU kernels)."""
+    rng = np.random.default_rng(4)
+    n, nd, na = 32, 48, 40
+    vg = OG.vol_geom_2d(n, n)
+    if geom == "par":
+        ang = np.linspace(0, np.pi, na, endpoint=False)
+        v, osc = OG.normalise_2d(OG.gen_par_projections(ang, nd, 1.0), "par", vg)
+        A = lambda x: O2.par_fp(x, v, nd, osc)
+        AT = lambda y: O2.par_bp(y, v, (n, n), osc)
+    else:
+        ang = np.linspace(0, 2 * np.pi, na, endpoint=False)
+        v, osc = OG.normalise_2d(OG.gen_fan_projections(ang, nd, 200.0, 100.0, 1.0), "fan", vg)
+        A = lambda x: O2.fan_fp(x, v, nd, osc)
+        AT = lambda y: O2.fan_bp(y, v, (n, n), osc)
+    x = rng.random((n, n))
+    y = rng.random((na, nd))
+    lhs, rhs = np.sum(A(x) * y), np.sum(x * AT(y))
+    assert abs(lhs - rhs) / abs(lhs) < 5e-2
+
+
+# ------------------------------------------------------------ 3D FP/BP ---
+def test_cone_fp_ones_is_chord_length():
+    """For a constant volume every sample inside reads 1: the ray sum is the
+    exact chord length n0 * |d| / |d_principal| (rays that stay inside)."""
+    n = 24
+    vec, vs = OG.normalise_3d(OG.gen_cone_projections([0.3, 2.0], 6, 6, 400.0, 200.0, 1.0, 1.0),
+                              "cone", OG.vol_geom_3d(n, n, n))
+    p = O3.cone_fp(np.ones((n, n, n)), vec, 6, 6, vs)
+    for a in range(2):
+        d0 = O3.cone_direction(vec[a], 6, 6)
+        for iv in range(6):
+            for iu in range(6):
+                det = vec[a, 3:6] + (iu + .5) * vec[a, 6:9] + (iv + .5) * vec[a, 9:12]
+                dvec = det - vec[a, 0:3]
+                assert abs(p[iv, a, iu] - n * np.linalg.norm(dvec) / abs(dvec[d0])) < 1e-9
+
+
+def test_cone_tends_to_par3d():
+    """Cone with the source at 1e7 and the detector through the origin is
+    the parallel geometry up to O(1e-6): FP, and BP including weights."""
+    rng = np.random.default_rng(5)
+    ang = [0.4, 1.3, 2.5]
+    vc, _ = OG.normalise_3d(OG.gen_cone_projections(ang, 12, 10, 1e7, 0.0, 1.0, 1.0), "cone",
+                            OG.vol_geom_3d(10, 12, 8))
+    vp, _ = OG.normalise_3d(OG.gen_par3d_projections(ang, 12, 10, 1.0, 1.0), "par3d",
+                            OG.vol_geom_3d(10, 12, 8))
+    x = rng.random((8, 10, 12))
+    assert rel(O3.cone_fp(x, vc, 12, 10), O3.par3d_fp(x, vp, 12, 10)) < 1e-5
+    y = rng.random((10, 3, 12))
+    assert rel(O3.cone_bp(y, vc, (8, 10, 12)), O3.par3d_bp(y, vp, (8, 10, 12))) < 1e-5
+
+
+def test_par3d_fp_angle0_sums():
+    rng = np.random.default_rng(6)
+    x = rng.random((6, 8, 10))
+    v, vs = OG.normalise_3d(OG.gen_par3d_projections([0.0], 10, 6, 1.0, 1.0), "par3d",
+                            OG.vol_geom_3d(8, 10, 6))
+    assert np.allclose(O3.par3d_fp(x, v, 10, 6, vs)[:, 0, :], x.sum(axis=1), atol=1e-12)
+
+
+def test_cone_bp_is_approximate_adjoint():
+    """3D adjoint check (tests/python/functional/test_rec_scaling.py:147-174
+    accepts 1e-1)."""
+    rng = np.random.default_rng(7)
+    n, na, nd = 16, 24, 24
+    v, vs = OG.normalise_3d(OG.gen_cone_projections(np.linspace(0, 2 * np.pi, na, False),
+                                                    nd, nd, 80.0, 40.0, 1.0, 1.0),
+                            "cone", OG.vol_geom_3d(n, n, n))
+    x = rng.random((n, n, n))
+    y = rng.random((nd, na, nd))
+    lhs = np.sum(O3.cone_fp(x, v, nd, nd, vs) * y)
+    rhs = np.sum(x * O3.cone_bp(y, v, (n, n, n), vs))
+    assert abs(lhs - rhs) / abs(lhs) < 5e-2
+
+
+def test_noncube_fp_scale_matches_physical_length():
+    """SCALE_NONCUBE: for a constant volume the FP is the physical chord
+    length through the box, whatever the voxel aspect ratio."""
+    vg = {"GridRowCount": 8, "GridColCount": 8, "GridSliceCount": 8,
+          "option": {"WindowMinX": -8, "WindowMaxX": 8, "WindowMinY": -4, "WindowMaxY": 4,
+                     "WindowMinZ": -2, "WindowMaxZ": 2}}
+    ang = [0.25]
+    vec, vs = OG.normalise_3d(OG.gen_par3d_projections(ang, 3, 3, 0.5, 0.5), "par3d", vg)
+    p = O3.par3d_fp(np.ones((8, 8, 8)), vec, 3, 3, vs)
+    # ray along (-sin a, cos a, 0); principal axis y (|cos| > |sin|): the box
+    # is 8 units tall in y, so the chord is 8 / cos a for rays that stay inside
+    assert abs(p[1, 0, 1] - 8.0 / math.cos(0.25)) < 1e-9
+
+
+# --------------------------------------------------------------- filter ---
+def test_ramp_filter_real_space_kernel():
+    """The inverse DFT of the Ram-Lak spectrum is the reference test's
+    real-space kernel 2h: 1/2 at 0, -2/(pi j)^2 at odd j
+    (tests/python/unit/test_algorithms_2d.py:67-76)."""
+    nd = 24
+    H = OF.ramp_filter(nd)
+    n = 2 * (len(H) - 1)
+    k = np.fft.irfft(H, n=n)
+    assert abs(k[0] - 0.5) < 1e-12
+    for j in range(1, n // 2):
+        want = -2.0 / (math.pi * j) ** 2 if j % 2 else 0.0
+        assert abs(k[j] - want) < 1e-12
+        assert abs(k[n - j] - want) < 1e-12
+
+
+def test_filter_rows_is_zero_padded_convolution():
+    """FFT filtering of zero-padded rows = direct circular convolution with
+    the real-space kernel, truncated to the row (cuda/2d/fft.cu:98-131)."""
+    rng = np.random.default_rng(8)
+    nd = 20
+    H = OF.ramp_filter(nd)
+    n = 2 * (len(H) - 1)
+    kern = np.fft.irfft(H, n=n)
+    rows = rng.random((3, nd))
+    got = OF.filter_rows(rows, H)
+    want = np.zeros_like(rows)
+    for r in range(3):
+        for i in range(nd):
+            want[r, i] = sum(rows[r, j] * kern[(i - j) % n] for j in range(nd))
+    assert np.allclose(got, want, atol=1e-12)
+
+
+def test_fdk_preweight_centre_pixel():
+    """devFDK_preweight closed form at the detector centre: W / D."""
+    na, nu, nv = 10, 4, 4
+    vec, _ = OG.normalise_3d(OG.gen_cone_projections(np.linspace(0, 2 * np.pi, na, False),
+                                                     nu, nv, 300.0, 100.0, 1.0, 1.0),
+                             "cone", OG.vol_geom_3d(8, 8, 8))
+    w = OF.fdk_preweight(np.ones((nv, na, nu)), vec)
+    D = 400.0
+    W = D * (D / (1.0 * 300.0)) * (math.pi / 2) / na
+    # pixels (1,1),(1,2),(2,1),(2,2) sit at u, v = +-0.5
+    assert abs(w[1, 0, 1] - W / math.sqrt(D * D + 0.5)) < 1e-12
+
+
+@pytest.mark.slow
+def test_fdk_cube_scaling():
+    """FDK of a unit cube recovers density 1 +- 5e-2 in its centre
+    (tests/python/functional/test_rec_scaling.py:89-145)."""
+    n, na, nd = 32, 90, 48
+    ang = np.linspace(0, 2 * np.pi, na, endpoint=False)
+    vec, vs = OG.normalise_3d(OG.gen_cone_projections(ang, nd, nd, 200.0, 100.0, 1.0, 1.0),
+                              "cone", OG.vol_geom_3d(n, n, n))
+    vol = np.zeros((n, n, n))
+    vol[12:20, 12:20, 12:20] = 1.0
+    proj = O3.cone_fp(vol, vec, nd, nd, vs)
+    rec = OF.fdk(proj, vec, (n, n, n), vs)
+    assert abs(rec[14:18, 14:18, 14:18].mean() - 1.0) < 5e-2
+
+
+def test_fbp_square_scaling():
+    """Parallel and fan FBP of a square recover 1 +- 5e-2 in its centre."""
+    n = 64
+    vol = np.zeros((n, n))
+    vol[26:38, 26:38] = 1.0
+    vg = OG.vol_geom_2d(n, n)
+    ang = np.linspace(0, np.pi, 120, endpoint=False)
+    v, osc = OG.normalise_2d(OG.gen_par_projections(ang, 96, 1.0), "par", vg)
+    rec = OF.fbp_par(O2.par_fp(vol, v, 96, osc), v, (n, n), osc, 1.0)
+    assert abs(rec[29:35, 29:35].mean() - 1.0) < 5e-2
+    ang = np.linspace(0, 2 * np.pi, 180, endpoint=False)
+    v, osc = OG.normalise_2d(OG.gen_fan_projections(ang, 128, 300.0, 200.0, 1.0), "fan", vg)
+    rec = OF.fbp_fan(O2.fan_fp(vol, v, 128, osc), v, (n, n), osc, 1.0)
+    assert abs(rec[29:35, 29:35].mean() - 1.0) < 5e-2
+
+
+# ------------------------------------------------------------ algorithms -
+def test_sirt_and_cgls_reduce_the_residual():
+    rng = np.random.default_rng(9)
+    n, nd = 24, 36
+    ang = np.linspace(0, np.pi, 30, endpoint=False)
+    v, osc = OG.normalise_2d(OG.gen_par_projections(ang, nd, 1.0), "par", OG.vol_geom_2d(n, n))
+    A = lambda x: O2.par_fp(x, v, nd, osc)
+    AT = lambda y: O2.par_bp(y, v, (n, n), osc)
+    x_true = rng.random((n, n))
+    b = A(x_true)
+    x0 = np.zeros((n, n))
+    r0 = OA.residual_norm(A, b, x0)
+    r_s = [OA.residual_norm(A, b, OA.sirt(A, AT, b, x0, k)) for k in (2, 8)]
+    r_c = [OA.residual_norm(A, b, OA.cgls(A, AT, b, x0, k)) for k in (2, 8)]
+    assert r0 > r_s[0] > r_s[1]
+    assert r0 > r_c[0] > r_c[1]
+    # SIRT with a non-negativity / upper bound keeps the iterate inside
+    xc = OA.sirt(A, AT, b, x0, 5, min_c=0.2, max_c=0.7)
+    assert xc.min() >= 0.2 and xc.max() <= 0.7
