@@ -68,6 +68,11 @@ SIGNATURES = {
     "astra_amd_algo_run": [_vp, _vp, _vp, _i, _vp],
     "astra_amd_algo_residual_norm": [_vp, _vp, _vp, ctypes.POINTER(_d), _vp],
     "astra_amd_algo_destroy": [_vp],
+    "astra_amd_batch2d_create": [_u, _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(_vp)],
+    "astra_amd_batch2d_offsets": [_vp, _vp, _vp],
+    "astra_amd_batch2d_fp": [_vp, _vp, _vp, _vp],
+    "astra_amd_batch2d_sirt": [_vp, _vp, _vp, _i, _f, _f, _f, _i, _vp],
+    "astra_amd_batch2d_destroy": [_vp],
     "astra_amd_profile_enable": [_i],
     "astra_amd_profile_read": [ctypes.c_char_p, ctypes.POINTER(_d), ctypes.POINTER(_d),
                                ctypes.POINTER(ctypes.c_ulonglong)],

@@ -29,6 +29,28 @@ def ellipses_2d(n_rows, n_cols=None, count=10, seed=0):
     return np.clip(img, 0.0, 1.0).astype(np.float32)
 
 
+def ellipses_2d_device(n, count=10, seed=0, device="cuda"):
+    """Square n x n version of ellipses_2d rasterised on a torch device."""
+    import torch
+
+    rng = np.random.default_rng(seed)
+    ar = torch.arange(n, device=device, dtype=torch.float32) - (n - 1) / 2.0
+    y, x = ar.view(n, 1), ar.view(1, n)
+    img = torch.zeros((n, n), dtype=torch.float32, device=device)
+    for _ in range(count):
+        r = 0.8 * (n / 2.0) * np.sqrt(rng.random())
+        t = 2 * np.pi * rng.random()
+        cx, cy = r * np.cos(t), r * np.sin(t)
+        a, b = rng.uniform(0.05, 0.4, 2) * n
+        phi = np.pi * rng.random()
+        rho = rng.random()
+        c, s = float(np.cos(phi)), float(np.sin(phi))
+        u = (x - cx) * c + (y - cy) * s
+        v = -(x - cx) * s + (y - cy) * c
+        img += ((u / a) ** 2 + (v / b) ** 2 <= 1.0).to(torch.float32) * float(rho)
+    return img.clamp_(0.0, 1.0)
+
+
 def _ellipsoid_params(n, count, seed):
     rng = np.random.default_rng(seed)
     params = []

@@ -24,7 +24,21 @@ void fdk_run(const Geom3D &g, float *proj, float *vol, float out_scale, const Fi
 void fdk_preweight_only(const Geom3D &g, float *proj, hipStream_t s);
 void fbp2d_run(const Geom2D &g, float *sino, float *vol, float out_scale, const Filter *f,
                int mode, hipStream_t s);
+struct Batch2D;
+Batch2D *batch2d_build(unsigned n, const int *kinds, const unsigned *ws, const unsigned *hs,
+                       const unsigned *nas, const unsigned *nds, const float *vecs,
+                       const float *oscs);
+void batch2d_destroy(Batch2D *b);
+void batch2d_offsets(const Batch2D &b, size_t *vol_off, size_t *sino_off);
+void batch2d_sirt(Batch2D &b, const float *sinos, float *vols, int iters, double relaxation,
+                  float lo, float hi, bool use_graph, hipStream_t s);
+void batch2d_fp(Batch2D &b, const float *vols, float *sinos, hipStream_t s);
 }  // namespace amx
+
+struct astra_amd_batch2d_s {
+    amx::Batch2D *b;
+    int device;
+};
 
 struct astra_amd_geom2d_s {
     amx::Geom2D *g;
@@ -304,6 +318,56 @@ int astra_amd_fbp2d(astra_amd_geom2d g, float *sino, float *vol, float output_sc
         check_device(g->g->device);
         amx::fbp2d_run(*g->g, sino, vol, output_scale, f ? f->f : nullptr, out_mode(mode),
                        as_stream(stream));
+    });
+}
+
+// ------------------------------------------------------ batched 2D -----
+int astra_amd_batch2d_create(unsigned int n_slices, const int *kinds, const unsigned int *widths,
+                             const unsigned int *heights, const unsigned int *n_angles,
+                             const unsigned int *n_dets, const float *vecs,
+                             const float *output_scales, astra_amd_batch2d *out) {
+    return guarded([&] {
+        AMX_REQUIRE(out != nullptr, "batch2d_create: out is NULL");
+        *out = nullptr;
+        amx::Batch2D *b = amx::batch2d_build(n_slices, kinds, widths, heights, n_angles, n_dets,
+                                             vecs, output_scales);
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        *out = new astra_amd_batch2d_s{b, dev};
+    });
+}
+
+int astra_amd_batch2d_offsets(astra_amd_batch2d b, size_t *vol_offsets, size_t *sino_offsets) {
+    return guarded([&] {
+        AMX_REQUIRE(b && b->b, "batch handle is NULL");
+        amx::batch2d_offsets(*b->b, vol_offsets, sino_offsets);
+    });
+}
+
+int astra_amd_batch2d_fp(astra_amd_batch2d b, const float *vols, float *sinos, void *stream) {
+    return guarded([&] {
+        AMX_REQUIRE(b && b->b, "batch handle is NULL");
+        check_device(b->device);
+        amx::batch2d_fp(*b->b, vols, sinos, as_stream(stream));
+    });
+}
+
+int astra_amd_batch2d_sirt(astra_amd_batch2d b, const float *sinos, float *vols, int iterations,
+                           float relaxation, float min_constraint, float max_constraint,
+                           int use_graph, void *stream) {
+    return guarded([&] {
+        AMX_REQUIRE(b && b->b, "batch handle is NULL");
+        check_device(b->device);
+        amx::batch2d_sirt(*b->b, sinos, vols, iterations, relaxation, min_constraint,
+                          max_constraint, use_graph != 0, as_stream(stream));
+    });
+}
+
+int astra_amd_batch2d_destroy(astra_amd_batch2d b) {
+    return guarded([&] {
+        if (!b) return;
+        amx::batch2d_destroy(b->b);
+        delete b;
     });
 }
 

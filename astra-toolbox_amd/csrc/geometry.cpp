@@ -181,6 +181,8 @@ Geom3D *geom3d_build(int kind, unsigned nx, unsigned ny, unsigned nz, unsigned n
     const float one[3] = {1.f, 1.f, 1.f};
     const float *vs = vol_scale ? vol_scale : one;
     AMX_REQUIRE(vs[0] > 0 && vs[1] > 0 && vs[2] > 0, "geom3d: voxel sizes must be positive");
+    for (size_t k = 0; k < 12 * (size_t)na; ++k)
+        AMX_REQUIRE(std::isfinite(vecs[k]), "geom3d: non-finite vector component (angle %zu)", k / 12);
 
     auto *g = new Geom3D();
     g->kind = kind;
@@ -195,8 +197,6 @@ Geom3D *geom3d_build(int kind, unsigned nx, unsigned ny, unsigned nz, unsigned n
     std::vector<float> wd(na, 0.f), w2(na, 0.f);
     for (unsigned a = 0; a < na; ++a) {
         const double *p = vecs + 12 * (size_t)a;
-        for (int k = 0; k < 12; ++k)
-            AMX_REQUIRE(std::isfinite(p[k]), "geom3d: non-finite vector component (angle %u)", a);
         int dir;
         if (kind == ASTRA_AMD_GEOM_CONE) {
             // cuda/3d/cone_fp.cu:378-387: |src - detector centre| per axis,
@@ -270,19 +270,37 @@ Geom2D *geom2d_build(int kind, unsigned w, unsigned h, unsigned na, unsigned nd,
     AMX_REQUIRE(w > 0 && h > 0 && na > 0 && nd > 0, "geom2d: all dimensions must be positive");
     AMX_REQUIRE(w <= 16000 && h <= 16000 && nd <= 1000000, "geom2d: dimensions too large");
     AMX_REQUIRE(vecs != nullptr, "geom2d: vecs is NULL");
+    std::vector<FP2DAngle> fp(na);
+    std::vector<BP2DAngle> bp(na), bpf(na);
+    for (unsigned a = 0; a < na; ++a)
+        angle2d_params(kind, vecs + 6 * (size_t)a, a, fp[a], bp[a], bpf[a]);
+
     auto *g = new Geom2D();
     g->kind = kind;
     g->w = w; g->h = h; g->na = na; g->nd = nd;
-    AMX_HIP(hipGetDevice(&g->device));
     g->vecs.assign(vecs, vecs + 6 * (size_t)na);
+    try {
+        AMX_HIP(hipGetDevice(&g->device));
+        g->fp = DevArray(sizeof(FP2DAngle) * na);
+        g->fp.upload(fp.data(), sizeof(FP2DAngle) * na);
+        g->bp = DevArray(sizeof(BP2DAngle) * na);
+        g->bp.upload(bp.data(), sizeof(BP2DAngle) * na);
+        g->bp_fbp = DevArray(sizeof(BP2DAngle) * na);
+        g->bp_fbp.upload(bpf.data(), sizeof(BP2DAngle) * na);
+    } catch (...) {
+        delete g;
+        throw;
+    }
+    return g;
+}
 
-    std::vector<FP2DAngle> fp(na);
-    std::vector<BP2DAngle> bp(na), bpf(na);
-    for (unsigned a = 0; a < na; ++a) {
-        const float *p = vecs + 6 * (size_t)a;
+// Per-angle FP and BP coefficients of one 2D projection vector (normalised
+// [rayX, rayY, dSX, dSY, uX, uY] or [srcX, srcY, ...]).
+void angle2d_params(int kind, const float *p, unsigned a, FP2DAngle &f, BP2DAngle &bpa,
+                    BP2DAngle &bpfa) {
+    {
         for (int k = 0; k < 6; ++k)
             AMX_REQUIRE(std::isfinite(p[k]), "geom2d: non-finite vector component (angle %u)", a);
-        FP2DAngle &f = fp[a];
         f.p0[0] = (double)p[2] + 0.5 * (double)p[4];
         f.p0[1] = (double)p[3] + 0.5 * (double)p[5];
         f.u[0] = p[4];
@@ -300,14 +318,14 @@ Geom2D *geom2d_build(int kind, unsigned w, unsigned h, unsigned na, unsigned nd,
             double d = ux * ry - uy * rx;
             AMX_REQUIRE(d != 0.0, "geom2d: detector parallel to rays (angle %u)", a);
             double c = ry / d, s = -rx / d, off = (dsy * rx - dsx * ry) / d;
-            BP2DAngle &b = bp[a];
+            BP2DAngle &b = bpa;
             b.num[0] = (float)(off + 1.5);  // -0.5 texel centre, +2 padding
             b.num[1] = (float)c;
             b.num[2] = (float)s;
             b.den[0] = 1.f; b.den[1] = 0.f; b.den[2] = 0.f;
             b.w = (float)(std::sqrt(rx * rx + ry * ry) / std::fabs(d));
             b.pad_ = 0.f;
-            bpf[a] = b;
+            bpfa = b;
         } else {
             // DevFanParams, cuda/2d/fan_bp.cu:240-271
             double sx = p[0], sy = p[1], dsx = p[2], dsy = p[3], ux = p[4], uy = p[5];
@@ -318,7 +336,7 @@ Geom2D *geom2d_build(int kind, unsigned w, unsigned h, unsigned na, unsigned nd,
                                 : std::sqrt(ux * ux + uy * uy) / det2(ux, uy, smdx, smdy);
                 double nc = sc * det2(sx, sy, dsx, dsy), nxc = sc * smdy, nyc = -sc * smdx;
                 double dc = fbp ? 1.0 : sc * det2(ux, uy, sx, sy), dxc = sc * uy, dyc = -sc * ux;
-                BP2DAngle &b = fbp ? bpf[a] : bp[a];
+                BP2DAngle &b = fbp ? bpfa : bpa;
                 b.num[0] = (float)(nc + 1.5 * dc);  // -0.5 texel centre, +2 padding
                 b.num[1] = (float)(nxc + 1.5 * dxc);
                 b.num[2] = (float)(nyc + 1.5 * dyc);
@@ -330,18 +348,6 @@ Geom2D *geom2d_build(int kind, unsigned w, unsigned h, unsigned na, unsigned nd,
             }
         }
     }
-    try {
-        g->fp = DevArray(sizeof(FP2DAngle) * na);
-        g->fp.upload(fp.data(), sizeof(FP2DAngle) * na);
-        g->bp = DevArray(sizeof(BP2DAngle) * na);
-        g->bp.upload(bp.data(), sizeof(BP2DAngle) * na);
-        g->bp_fbp = DevArray(sizeof(BP2DAngle) * na);
-        g->bp_fbp.upload(bpf.data(), sizeof(BP2DAngle) * na);
-    } catch (...) {
-        delete g;
-        throw;
-    }
-    return g;
 }
 
 // ------------------------------------------------------- filters ---------

@@ -217,6 +217,8 @@ Geom3D *geom3d_build(int kind, unsigned nx, unsigned ny, unsigned nz, unsigned n
                      unsigned nv, const double *vecs, const float *vol_scale);
 Geom2D *geom2d_build(int kind, unsigned w, unsigned h, unsigned na, unsigned nd,
                      const float *vecs);
+void angle2d_params(int kind, const float *p, unsigned a, FP2DAngle &f, BP2DAngle &bp,
+                    BP2DAngle &bp_fbp);
 void ramlak_half_spectrum(unsigned n_det, double d, std::vector<double> &out);
 unsigned next_pow2(unsigned n);
 

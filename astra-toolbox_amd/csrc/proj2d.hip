@@ -10,45 +10,24 @@
 //   BP input copy   [angle][det+4]
 // Copies carry a 2-texel zero border so that clamped coordinates read
 // exactly the border-0 texture values without bounds checks.
-//
-// Sampling semantics follow cuda/2d/par_fp.cu:53-200, cuda/2d/fan_fp.cu:55-193,
-// cuda/2d/par_bp.cu:52-87 and cuda/2d/fan_bp.cu:57-103.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
 
-#include "devutil.hpp"
-#include "internal.hpp"
+#include "proj2d_dev.hpp"
 
 namespace amx {
 
 // ------------------------------------------------------------ prep -------
-// vol [h][w] -> rows copy [h][w+4] and cols copy [w][h+4] in one pass
-// through a 64 x 64 LDS tile.  grid = (ceil((w+4)/64), ceil((h+4)/64)).
+// vol [h][w] -> rows copy [h][w+4] and cols copy [w][h+4] in one pass.
+// grid = (ceil((w+4)/64), ceil((h+4)/64)).
 __global__ __launch_bounds__(256) void vol2d_copies(const float *__restrict__ vol,
                                                     float *__restrict__ rows,
                                                     float *__restrict__ cols, unsigned w,
                                                     unsigned h) {
     __shared__ float tile[64][65];
-    const int pw = w + 2 * kPad, ph = h + 2 * kPad;
-    const int c0 = blockIdx.x * 64, r0 = blockIdx.y * 64;  // padded coordinates
-    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-    for (int i = ty; i < 64; i += 4) {
-        int pr = r0 + i, pc = c0 + tx;
-        int r = pr - kPad, c = pc - kPad;
-        float v = 0.f;
-        if (r >= 0 && r < (int)h && c >= 0 && c < (int)w) v = vol[(size_t)r * w + c];
-        tile[i][tx] = v;
-        // rows copy keeps only real rows, padded columns
-        if (r >= 0 && r < (int)h && pc < pw) rows[(size_t)r * pw + pc] = v;
-    }
-    __syncthreads();
-    for (int i = ty; i < 64; i += 4) {
-        int pc = c0 + i, pr = r0 + tx;
-        int c = pc - kPad;
-        if (c >= 0 && c < (int)w && pr < ph) cols[(size_t)c * ph + pr] = tile[tx][i];
-    }
+    vol2d_copy_tile(vol, rows, cols, w, h, blockIdx.x * 64, blockIdx.y * 64, tile);
 }
 
 // sino [a][d] -> [a][d+4]; one workgroup per angle row
@@ -77,117 +56,17 @@ struct FP2DArgs {
     int padded_out;  // write [a][d+4]
 };
 
-struct Ray2D {
-    float a, c, scale;
-    int lo, hi;
-    bool vertical;
-};
-
-// FP64 setup of one (sub-)ray through detector position `fdet`.
-template <bool FAN>
-__device__ __forceinline__ Ray2D ray2d(const FP2DAngle &G, double fdet, bool vertical,
-                                      unsigned w, unsigned h) {
-    double P0 = G.p0[0] + (fdet - 0.5) * G.u[0];
-    double P1 = G.p0[1] + (fdet - 0.5) * G.u[1];
-    double rx = FAN ? G.r[0] - P0 : G.r[0];
-    double ry = FAN ? G.r[1] - P1 : G.r[1];
-    Ray2D R;
-    R.vertical = vertical;
-    R.lo = 0;
-    double a, c, len = sqrt(rx * rx + ry * ry);
-    if (vertical) {
-        // rows are slices: t(r) = a r + c along columns
-        double k = rx / ry;
-        a = -k;
-        c = a * (-0.5 * h + 0.5) + P0 - P1 * k + 0.5 * w - 0.5;
-        R.hi = (int)h - 1;
-        clip_range(a, c, (double)w, (int)h, R.lo, R.hi);
-        R.scale = (float)(len / fabs(ry));
-    } else {
-        // columns are slices: t(c) = a c + c0 along rows
-        double k = ry / rx;
-        a = -k;
-        c = a * (-0.5 * w + 0.5) + 0.5 * h - 0.5 - P1 + P0 * k;
-        R.hi = (int)w - 1;
-        clip_range(a, c, (double)h, (int)w, R.lo, R.hi);
-        R.scale = (float)(len / fabs(rx));
-    }
-    R.a = (float)a;
-    R.c = (float)(c + kPad);  // padded texel coordinate
-    return R;
-}
-
-__device__ __forceinline__ float march2d(const float *__restrict__ base, unsigned pitch, float hi,
-                                         const Ray2D &R) {
-    float acc0 = 0.f, acc1 = 0.f;
-    const float *p = base + (size_t)(R.lo > 0 ? R.lo : 0) * pitch;
-    int s = R.lo;
-    for (; s + 1 <= R.hi; s += 2) {
-        const float sf = (float)s;
-        float t0 = clamp_pad(fmaf(R.a, sf, R.c), hi);
-        float t1 = clamp_pad(fmaf(R.a, sf + 1.f, R.c), hi);
-        float f0 = floorf(t0), f1 = floorf(t1);
-        float2 q0 = ld2b(p, (unsigned)f0 << 2), q1 = ld2b(p + pitch, (unsigned)f1 << 2);
-        acc0 += lerp(q0.x, q0.y, t0 - f0);
-        acc1 += lerp(q1.x, q1.y, t1 - f1);
-        p += 2 * pitch;
-    }
-    if (s <= R.hi) {
-        float t0 = clamp_pad(fmaf(R.a, (float)s, R.c), hi);
-        float f0 = floorf(t0);
-        float2 q0 = ld2b(p, (unsigned)f0 << 2);
-        acc0 += lerp(q0.x, q0.y, t0 - f0);
-    }
-    return acc0 + acc1;
-}
-
 // One thread per (detector, angle); 64 detectors x 4 angles per group.
 template <bool FAN>
 __global__ __launch_bounds__(256) void fp2d_kernel(FP2DArgs A) {
     const int det = blockIdx.x * 64 + (threadIdx.x & 63);
     const int angle = blockIdx.y * 4 + (threadIdx.x >> 6);
     if (det >= (int)A.nd || angle >= (int)A.na) return;
-    const FP2DAngle &G = A.ang[angle];
-    bool vertical;
-    if (FAN) {
-        // per-ray orientation, in float32 with the reference's +0.5 in world
-        // units (cuda/2d/fan_fp.cu:79-83)
-        const float sx = G.raw[0], sy = G.raw[1], dsx = G.raw[2], dsy = G.raw[3];
-        const float ux = G.raw[4], uy = G.raw[5];
-        float tx = fmaf((float)det, ux, dsx);
-        tx = tx + 0.5f;
-        tx = tx - sx;
-        float ty = fmaf((float)det, uy, dsy);
-        ty = ty + 0.5f;
-        ty = ty - sy;
-        vertical = fabsf(ty) > fabsf(tx);
-    } else {
-        vertical = G.vertical != 0;
-    }
-    float sum = 0.f;
-    for (unsigned k = 0; k < A.rays; ++k) {
-        double fdet = det + (0.5 + k) / (double)A.rays;
-        Ray2D R = ray2d<FAN>(G, fdet, vertical, A.w, A.h);
-        float v = vertical ? march2d(A.rows, A.w + 2 * kPad, (float)A.w + 2.5f, R)
-                           : march2d(A.cols, A.h + 2 * kPad, (float)A.h + 2.5f, R);
-        sum += v * R.scale;
-    }
-    const float val = sum * (A.osc / (float)A.rays);
+    const float val =
+        fp2d_value<FAN>(A.ang[angle], det, A.rays, A.w, A.h, A.rows, A.cols) * A.osc;
     const size_t idx = (size_t)angle * A.nd + det;
     const size_t oidx = A.padded_out ? (size_t)angle * (A.nd + 2 * kPad) + det + kPad : idx;
-    if (A.mode == OUT_ADD) {
-        A.out[oidx] += val;
-    } else if (A.mode == OUT_SET) {
-        A.out[oidx] = val;
-    } else if (A.mode == OUT_RESID) {
-        float r = A.rb[idx] + val;
-        if (A.rw) r *= A.rw[oidx];
-        A.out[oidx] = r;
-    } else {
-        float r = A.out[oidx] + val;
-        if (A.rw) r *= A.rw[oidx];
-        A.out[oidx] = r;
-    }
+    store_fp(A.out, oidx, val, A.mode, A.rb, idx, A.rw);
 }
 
 // ------------------------------------------------------------- BP --------
@@ -202,52 +81,19 @@ struct BP2DArgs {
     float lo, hi;
 };
 
-// KIND: 0 parallel (constant per-angle weight), 1 fan (weight fr),
-// 2 fan FBP (weight fr^2).  One thread per column x and 4 rows.
+// One thread per column x and 4 consecutive rows; 64 x 16 pixels per group.
 template <int KIND>
 __global__ __launch_bounds__(256) void bp2d_kernel(BP2DArgs A) {
     constexpr int PY = 4;
     const int X = blockIdx.x * 64 + (threadIdx.x & 63);
     const int Y0 = (blockIdx.y * 4 + (threadIdx.x >> 6)) * PY;
     if (X >= (int)A.w || Y0 >= (int)A.h) return;
-    const float fx = X - 0.5f * A.w + 0.5f;
-    const float fy0 = -(Y0 - 0.5f * A.h + 0.5f);  // world y of row Y0
-    const unsigned pitch = A.nd + 2 * kPad;
-    const float hi = (float)A.nd + 2.5f;
     float acc[PY];
-#pragma unroll
-    for (int i = 0; i < PY; ++i) acc[i] = 0.f;
-    const float *row = A.sp;
-    for (unsigned a = 0; a < A.na; ++a, row += pitch) {
-        const BP2DAngle c = A.ang[a];
-        float num = fmaf(fy0, c.num[2], fmaf(fx, c.num[1], c.num[0]));
-        float den = fmaf(fy0, c.den[2], fmaf(fx, c.den[1], c.den[0]));
-#pragma unroll
-        for (int i = 0; i < PY; ++i) {
-            float fr = (KIND == 0) ? 1.f : __builtin_amdgcn_rcpf(den);
-            float t = clamp_pad(num * fr, hi);
-            float f = floorf(t);
-            float2 q = ld2b(row, (unsigned)f << 2);
-            float val = lerp(q.x, q.y, t - f);
-            float wgt = (KIND == 0) ? c.w : (KIND == 1 ? fr : fr * fr);
-            acc[i] = fmaf(wgt, val, acc[i]);
-            num -= c.num[2];  // next row: world y decreases by 1
-            den -= c.den[2];
-        }
-    }
+    bp2d_column<KIND, PY>(X, Y0, A.w, A.h, A.na, A.nd, A.sp, A.ang, acc);
     for (int i = 0; i < PY; ++i) {
-        int Y = Y0 + i;
+        const int Y = Y0 + i;
         if (Y >= (int)A.h) break;
-        size_t idx = (size_t)Y * A.w + X;
-        float val = acc[i] * A.osc;
-        if (A.mode == OUT_ADD) {
-            A.vol[idx] += val;
-        } else if (A.mode == OUT_SET) {
-            A.vol[idx] = val;
-        } else {
-            float wv = A.uw ? A.uw[idx] : 1.f;
-            A.vol[idx] = fminf(fmaxf(fmaf(wv, val, A.vol[idx]), A.lo), A.hi);
-        }
+        store_bp(A.vol, (size_t)Y * A.w + X, acc[i] * A.osc, A.mode, A.uw, A.lo, A.hi);
     }
 }
 
@@ -260,26 +106,13 @@ __global__ __launch_bounds__(256) void fp2d_transpose_kernel(FP2DArgs A, const f
     const int angle = blockIdx.y * 4 + (threadIdx.x >> 6);
     if (det >= (int)A.nd || angle >= (int)A.na) return;
     const FP2DAngle &G = A.ang[angle];
-    bool vertical;
-    if (FAN) {
-        const float sx = G.raw[0], sy = G.raw[1], dsx = G.raw[2], dsy = G.raw[3];
-        const float ux = G.raw[4], uy = G.raw[5];
-        float tx = fmaf((float)det, ux, dsx);
-        tx = tx + 0.5f;
-        tx = tx - sx;
-        float ty = fmaf((float)det, uy, dsy);
-        ty = ty + 0.5f;
-        ty = ty - sy;
-        vertical = fabsf(ty) > fabsf(tx);
-    } else {
-        vertical = G.vertical != 0;
-    }
+    const bool vertical = ray_vertical<FAN>(G, det);
     const float y0 = sino[(size_t)angle * A.nd + det] * (A.osc / (float)A.rays);
     if (y0 == 0.f) return;
     float *rows = const_cast<float *>(A.rows);
     float *cols = const_cast<float *>(A.cols);
     for (unsigned k = 0; k < A.rays; ++k) {
-        double fdet = det + (0.5 + k) / (double)A.rays;
+        const double fdet = det + (0.5 + k) / (double)A.rays;
         Ray2D R = ray2d<FAN>(G, fdet, vertical, A.w, A.h);
         const float y = y0 * R.scale;
         float *base = vertical ? rows : cols;
@@ -289,7 +122,7 @@ __global__ __launch_bounds__(256) void fp2d_transpose_kernel(FP2DArgs A, const f
             float t = clamp_pad(fmaf(R.a, (float)s, R.c), hi);
             float f = floorf(t);
             float wt = t - f;
-            float *p = base + (size_t)s * pitch + ((unsigned)f);
+            float *p = base + (size_t)s * pitch + (unsigned)f;
             atomicAdd(p, y * (1.f - wt));
             atomicAdd(p + 1, y * wt);
         }

@@ -175,6 +175,35 @@ int astra_amd_fdk(astra_amd_geom3d g, float *proj, float *vol, float output_scal
 int astra_amd_fbp2d(astra_amd_geom2d g, float *sino, float *vol, float output_scale,
                     astra_amd_filter f, int mode, void *stream);
 
+/* ------------------------------------------------- batched 2D SIRT --- */
+/* Many independent 2D slices (parallel or fan, each with its own volume
+ * size, angle count and detector count) reconstructed together.  No
+ * reference counterpart: replaces running one CCudaSirtAlgorithm per slice
+ * (src/CudaSirtAlgorithm.cpp:284-397).  Per slice i: kinds[i] (GEOM_PAR /
+ * GEOM_FAN), widths/heights (volume cols/rows), n_angles, n_dets,
+ * output_scales (the 2D fOutputScale, i.e. the pixel length); vecs holds
+ * the normalised 6-float vectors of all slices' angles, slice after slice.
+ * Volumes are packed [h][w] slice after slice, sinograms [n_angles][n_dets]
+ * slice after slice (offsets from astra_amd_batch2d_offsets). */
+typedef struct astra_amd_batch2d_s *astra_amd_batch2d;
+int astra_amd_batch2d_create(unsigned int n_slices, const int *kinds, const unsigned int *widths,
+                             const unsigned int *heights, const unsigned int *n_angles,
+                             const unsigned int *n_dets, const float *vecs,
+                             const float *output_scales, astra_amd_batch2d *out);
+/* Element offsets of every slice in the packed volume / sinogram buffers
+ * (n_slices + 1 entries each; the last is the total).  Either may be NULL. */
+int astra_amd_batch2d_offsets(astra_amd_batch2d b, size_t *vol_offsets, size_t *sino_offsets);
+/* sinos = A vols for every slice (MODE_SET). */
+int astra_amd_batch2d_fp(astra_amd_batch2d b, const float *vols, float *sinos, void *stream);
+/* `iterations` SIRT iterations on every slice (vols = initial iterates,
+ * overwritten), with relaxation and min/max constraints (+-INFINITY = off).
+ * use_graph != 0: the iterations are captured once into a hipGraph and
+ * replayed as one graph launch (re-captured when arguments change). */
+int astra_amd_batch2d_sirt(astra_amd_batch2d b, const float *sinos, float *vols, int iterations,
+                           float relaxation, float min_constraint, float max_constraint,
+                           int use_graph, void *stream);
+int astra_amd_batch2d_destroy(astra_amd_batch2d b);
+
 /* ------------------------------------------------------ profiling ---- */
 /* Per-kernel event timing (no counterpart in the reference, which has no
  * tracing): while enabled, every hot kernel launch is bracketed by a
