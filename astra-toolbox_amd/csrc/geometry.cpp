@@ -233,6 +233,14 @@ Geom3D *geom3d_build(int kind, unsigned nx, unsigned ny, unsigned nz, unsigned n
             bf[a] = bd[a];
         }
     }
+    auto zconst = [&](const std::vector<BP3DAngle> &b) {
+        for (unsigned a = 0; a < na; ++a)
+            if (b[a].nu[2] != 0.f || b[a].dn[2] != 0.f) return false;
+        return true;
+    };
+    g->zc_default = zconst(bd);
+    g->zc_fdk = zconst(bf);
+    g->zc_2d = zconst(b2);
     try {
         g->fp = DevArray(sizeof(FP3DAngle) * na);
         g->fp.upload(g->fp_host.data(), sizeof(FP3DAngle) * na);

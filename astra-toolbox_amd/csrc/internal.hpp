@@ -136,6 +136,10 @@ struct Geom3D {
     DevArray fp;                 // FP3DAngle[na]
     DevArray bp_default, bp_fdk, bp_2d;  // BP3DAngle[na]
     DevArray bp_w_default, bp_w_2d;      // float[na] par3d weights
+    // per BP coefficient set: U and the denominator do not depend on z for
+    // any angle (nu[2] == dn[2] == 0, e.g. circular cone with a vertical
+    // detector), so the BP hoists them out of its z loop
+    bool zc_default = false, zc_fdk = false, zc_2d = false;
     DevArray group_idx[3];               // int[dir_count[k]] angle lists
     // FDK pre-weights [nv][nu] and default Ram-Lak filter, built on first use
     mutable std::mutex fdk_mu;

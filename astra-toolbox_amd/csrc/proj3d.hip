@@ -22,6 +22,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 
 #include "devutil.hpp"
 #include "internal.hpp"
@@ -109,7 +110,32 @@ struct FPArgs {
     const float *rb;  // OUT_RESID: b in the output layout
     const float *rw;  // OUT_RESID: weights in the output layout (may be NULL)
     int padded_out;   // write [a][v+2][u+2] instead of [v][a][u]
+    // workgroup map (fp_map): detector tiles, angles per block, and the
+    // range [lfirst, lfirst + lcount) of alist covered by this launch
+    unsigned tiles, ablk, lfirst, lcount;
 };
+
+// Workgroup -> (detector tile, angle) map.  Workgroups are handed out
+// round-robin over the 8 XCDs (workgroup L runs on XCD L % 8), and each XCD
+// has its own L2.  The grid is a multiple of 8; XCD x takes the contiguous
+// logical range [x * per, (x + 1) * per), in the order (angle block, tile,
+// angle within block).  The workgroups resident on one XCD at a time are
+// then a few neighbouring tiles, each over `ablk` consecutive angles, which
+// start together and march through the same slices at the same time: rays
+// of neighbouring angles sample nearly the same voxels, so they share the
+// XCD's L2 instead of each fetching the volume from HBM.
+__device__ __forceinline__ bool fp_map(const FPArgs &A, unsigned &tile, int &angle) {
+    const unsigned L = blockIdx.x;
+    const unsigned per = gridDim.x >> 3;
+    const unsigned logical = (L & 7u) * per + (L >> 3);
+    const unsigned a_in = logical % A.ablk;
+    const unsigned rest = logical / A.ablk;
+    tile = rest % A.tiles;
+    const unsigned li = (rest / A.tiles) * A.ablk + a_in;
+    if (li >= A.lcount) return false;
+    angle = A.alist[A.lfirst + li];
+    return true;
+}
 
 // One thread per ray; a 256-thread workgroup covers a 16 x 16 detector tile
 // of one angle, each wave a 16 u x 4 v sub-tile.  Ray setup runs in FP64;
@@ -142,12 +168,14 @@ __device__ __forceinline__ void fp3d_ray(const FPArgs &A, const FP3DAngle &G, in
 
 template <bool CONE>
 __global__ __launch_bounds__(256) void fp3d_kernel(FPArgs A) {
+    unsigned tile;
+    int angle;
+    if (!fp_map(A, tile, angle)) return;  // uniform over the workgroup
     const unsigned tiles_u = (A.nu + 15) / 16;
-    const unsigned tu = blockIdx.x % tiles_u, tv = blockIdx.x / tiles_u;
+    const unsigned tu = tile % tiles_u, tv = tile / tiles_u;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int u = tu * 16 + (lane & 15);
     const int v = tv * 16 + wave * 4 + (lane >> 4);
-    const int angle = A.alist[blockIdx.y];
     const FP3DAngle &G = A.ang[angle];
     const bool active = u < (int)A.nu && v < (int)A.nv;
 
@@ -229,7 +257,12 @@ struct BPArgs {
 // One thread per (x, y) column of ZS voxels; 16 x 16 threads per group.
 // The angle loop is uniform, so the per-angle coefficients are scalar loads
 // and the angle's projection plane base is a scalar address.
-template <int KIND, int ZS>
+// ZC: U and the denominator are constant along z for every angle (checked
+// on the host: nu[2] == dn[2] == 0); the reciprocal, the U coordinate, its
+// tap offset and weight and the voxel weight are then computed once per
+// column and angle.  Both forms take the same fp32 operations on the same
+// values (adding a zero increment is exact), so they give identical results.
+template <int KIND, int ZS, bool ZC>
 __global__ __launch_bounds__(256) void bp3d_kernel(BPArgs A) {
     const unsigned tiles_x = (A.nx + 15) / 16;
     const int X = (blockIdx.x % tiles_x) * 16 + (threadIdx.x & 15);
@@ -255,20 +288,41 @@ __global__ __launch_bounds__(256) void bp3d_kernel(BPArgs A) {
         float vn = fmaf(fz, c.nv[2], fmaf(fy, c.nv[1], fmaf(fx, c.nv[0], c.nv[3])));
         float dn = fmaf(fz, c.dn[2], fmaf(fy, c.dn[1], fmaf(fx, c.dn[0], c.dn[3])));
         const float wa = (KIND == 2) ? A.aw[a] : 1.f;
+        if (ZC) {
+            const float fr = (KIND == 2) ? 1.f : __builtin_amdgcn_rcpf(dn);
+            const float pu = clamp_pad(un * fr, hiu);
+            const float fu = floorf(pu);
+            const float wu = pu - fu;
+            const unsigned ou = (unsigned)fu << 2;
+            const float wgt = (KIND == 0) ? fr * fr : (KIND == 1 ? fr : wa);
 #pragma unroll
-        for (int i = 0; i < ZS; ++i) {
-            float fr = (KIND == 2) ? 1.f : __builtin_amdgcn_rcpf(dn);
-            float pu = clamp_pad(un * fr, hiu);
-            float pv = clamp_pad(vn * fr, hiv);
-            float fu = floorf(pu), fv = floorf(pv);
-            unsigned o = texel_off(fu, fv, pitchf);
-            float2 r0 = ld2b(pl, o), r1 = ld2b(pl, o + rowb);
-            float val = lerp(lerp(r0.x, r0.y, pu - fu), lerp(r1.x, r1.y, pu - fu), pv - fv);
-            float wgt = (KIND == 0) ? fr * fr : (KIND == 1 ? fr : wa);
-            acc[i] = fmaf(wgt, val, acc[i]);
-            un += c.nu[2];
-            vn += c.nv[2];
-            dn += c.dn[2];
+            for (int i = 0; i < ZS; ++i) {
+                float pv = clamp_pad(vn * fr, hiv);
+                // pv in [0.5, nv + 2.5]: floor is exact and fract = pv - floor(pv)
+                unsigned iv = (unsigned)floorf(pv);
+                float wv = __builtin_amdgcn_fractf(pv);
+                unsigned o = __umul24(iv, rowb) + ou;
+                float2 r0 = ld2b(pl, o), r1 = ld2b(pl, o + rowb);
+                float val = lerp(lerp(r0.x, r0.y, wu), lerp(r1.x, r1.y, wu), wv);
+                acc[i] = fmaf(wgt, val, acc[i]);
+                vn += c.nv[2];
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < ZS; ++i) {
+                float fr = (KIND == 2) ? 1.f : __builtin_amdgcn_rcpf(dn);
+                float pu = clamp_pad(un * fr, hiu);
+                float pv = clamp_pad(vn * fr, hiv);
+                float fu = floorf(pu), fv = floorf(pv);
+                unsigned o = texel_off(fu, fv, pitchf);
+                float2 r0 = ld2b(pl, o), r1 = ld2b(pl, o + rowb);
+                float val = lerp(lerp(r0.x, r0.y, pu - fu), lerp(r1.x, r1.y, pu - fu), pv - fv);
+                float wgt = (KIND == 0) ? fr * fr : (KIND == 1 ? fr : wa);
+                acc[i] = fmaf(wgt, val, acc[i]);
+                un += c.nu[2];
+                vn += c.nv[2];
+                dn += c.dn[2];
+            }
         }
     }
     const int nzr = min(ZS, (int)A.nz - Z0);
@@ -293,12 +347,14 @@ __global__ __launch_bounds__(256) void bp3d_kernel(BPArgs A) {
 // the angle's direction group (fp32 atomics).
 template <bool CONE>
 __global__ __launch_bounds__(256) void fp3d_transpose_kernel(FPArgs A, const float *__restrict__ proj) {
+    unsigned tile;
+    int angle;
+    if (!fp_map(A, tile, angle)) return;
     const unsigned tiles_u = (A.nu + 15) / 16;
-    const unsigned tu = blockIdx.x % tiles_u, tv = blockIdx.x / tiles_u;
+    const unsigned tu = tile % tiles_u, tv = tile / tiles_u;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int u = tu * 16 + (lane & 15);
     const int v = tv * 16 + wave * 4 + (lane >> 4);
-    const int angle = A.alist[blockIdx.y];
     const FP3DAngle &G = A.ang[angle];
     if (u >= (int)A.nu || v >= (int)A.nv) return;
     float a1f, a2f, c1f, c2f, scale;
@@ -398,6 +454,47 @@ void fp_scales(const Geom3D &g, int grp, float out_scale, float &fs1, float &fs2
     fs2 = s2 * s2;
 }
 
+// ASTRA_AMD_BP3D_GENERAL=1 forces the general (z-dependent) BP form even
+// where the z-constant form applies (for comparing the two).
+bool zc_disabled() {
+    static const bool v = [] {
+        const char *e = std::getenv("ASTRA_AMD_BP3D_GENERAL");
+        return e && e[0] == '1';
+    }();
+    return v;
+}
+
+// Angles per block of the FP workgroup map (fp_map); ASTRA_AMD_FP3D_ANGLE_BLOCK
+// overrides the default (1..64).
+unsigned fp_angle_block() {
+    static const unsigned v = [] {
+        const char *e = std::getenv("ASTRA_AMD_FP3D_ANGLE_BLOCK");
+        long x = e ? std::strtol(e, nullptr, 10) : 0;
+        return (x >= 1 && x <= 64) ? (unsigned)x : 16u;
+    }();
+    return v;
+}
+
+// Launch `kern` over the angles of one direction group, in as many
+// launches as the 2^32 work-item limit of one grid requires.
+template <typename K>
+void fp_group_launches(FPArgs A, unsigned count, K kern, hipStream_t s) {
+    A.tiles = ((A.nu + 15) / 16) * ((A.nv + 15) / 16);
+    A.ablk = std::min(fp_angle_block(), count);
+    const size_t per_block = (size_t)A.tiles * A.ablk;
+    const size_t max_groups = (1u << 24) - 8;  // 256 threads each
+    AMX_REQUIRE(per_block <= max_groups, "fp3d: detector too large for one launch");
+    const unsigned blocks_per_launch = (unsigned)(max_groups / per_block);
+    for (unsigned first = 0; first < count; first += blocks_per_launch * A.ablk) {
+        A.lfirst = first;
+        A.lcount = std::min(count - first, blocks_per_launch * A.ablk);
+        const unsigned nblk = (A.lcount + A.ablk - 1) / A.ablk;
+        const unsigned groups = (unsigned)(((size_t)nblk * per_block + 7) & ~(size_t)7);
+        kern(A, groups);
+        AMX_LAUNCHED();
+    }
+}
+
 FPArgs fp_args(const Geom3D &g, int grp, float out_scale) {
     FPArgs A{};
     group_dims(g, grp, A.n0, A.n1, A.n2);
@@ -420,7 +517,6 @@ void fp3d_launch(const Geom3D &g, const float *vol, float *proj, float out_scale
             groups[ng++] = k;
         }
     Scratch copy(need * sizeof(float), s);
-    const unsigned tiles = ((g.nu + 15) / 16) * ((g.nv + 15) / 16);
     for (int i = 0; i < ng; ++i) {
         const int k = groups[i];
         make_group_copy(g, k, vol, copy.as<float>(), s);
@@ -433,14 +529,15 @@ void fp3d_launch(const Geom3D &g, const float *vol, float *proj, float out_scale
         // The direction groups partition the angles: every output element
         // is written by exactly one group, which applies the full mode.
         A.mode = mode;
-        dim3 grid(tiles, g.dir_count[k]);
         ProfToken pt = prof_begin(g.kind == ASTRA_AMD_GEOM_CONE ? "fp3d_cone" : "fp3d_par3d",
                                   (double)g.dir_count[k] * g.nu * g.nv * A.n0, s);
-        if (g.kind == ASTRA_AMD_GEOM_CONE)
-            fp3d_kernel<true><<<grid, 256, 0, s>>>(A);
-        else
-            fp3d_kernel<false><<<grid, 256, 0, s>>>(A);
-        AMX_LAUNCHED();
+        const bool cone = g.kind == ASTRA_AMD_GEOM_CONE;
+        fp_group_launches(A, g.dir_count[k], [&](const FPArgs &a, unsigned groups) {
+            if (cone)
+                fp3d_kernel<true><<<groups, 256, 0, s>>>(a);
+            else
+                fp3d_kernel<false><<<groups, 256, 0, s>>>(a);
+        }, s);
         prof_end(pt, s);
     }
 }
@@ -478,27 +575,41 @@ void bp3d_launch_padded(const Geom3D &g, const float *pp, float *vol, float out_
     const char *pname = g.kind != ASTRA_AMD_GEOM_CONE ? "bp3d_par3d"
                         : (kernel == ASTRA_AMD_KER3D_FDK_WEIGHTING ? "bp3d_fdk" : "bp3d_cone");
     ProfToken pt = prof_begin(pname, (double)g.nx * g.ny * g.nz * g.na, s);
+    int kind;
+    bool zc;
     if (g.kind == ASTRA_AMD_GEOM_CONE) {
         if (kernel == ASTRA_AMD_KER3D_FDK_WEIGHTING) {
             A.ang = g.bp_fdk.as<BP3DAngle>();
-            bp3d_kernel<0, ZS><<<grid, 256, 0, s>>>(A);
+            kind = 0;
+            zc = g.zc_fdk;
         } else if (kernel == ASTRA_AMD_KER3D_2D_WEIGHTING) {
             A.ang = g.bp_2d.as<BP3DAngle>();
-            bp3d_kernel<1, ZS><<<grid, 256, 0, s>>>(A);
+            kind = 1;
+            zc = g.zc_2d;
         } else {
             A.ang = g.bp_default.as<BP3DAngle>();
-            bp3d_kernel<0, ZS><<<grid, 256, 0, s>>>(A);
+            kind = 0;
+            zc = g.zc_default;
         }
     } else {
         if (kernel == ASTRA_AMD_KER3D_2D_WEIGHTING) {
             A.ang = g.bp_2d.as<BP3DAngle>();
             A.aw = g.bp_w_2d.as<float>();
+            zc = g.zc_2d;
         } else {
             A.ang = g.bp_default.as<BP3DAngle>();
             A.aw = g.bp_w_default.as<float>();
+            zc = g.zc_default;
         }
-        bp3d_kernel<2, ZS><<<grid, 256, 0, s>>>(A);
+        kind = 2;
     }
+    if (zc_disabled()) zc = false;
+    if (kind == 0)
+        zc ? bp3d_kernel<0, ZS, true><<<grid, 256, 0, s>>>(A) : bp3d_kernel<0, ZS, false><<<grid, 256, 0, s>>>(A);
+    else if (kind == 1)
+        zc ? bp3d_kernel<1, ZS, true><<<grid, 256, 0, s>>>(A) : bp3d_kernel<1, ZS, false><<<grid, 256, 0, s>>>(A);
+    else
+        zc ? bp3d_kernel<2, ZS, true><<<grid, 256, 0, s>>>(A) : bp3d_kernel<2, ZS, false><<<grid, 256, 0, s>>>(A);
     AMX_LAUNCHED();
     prof_end(pt, s);
 }
@@ -514,7 +625,6 @@ void bp3d_matched_launch(const Geom3D &g, const float *proj, float *vol, float o
                          int mode, hipStream_t s) {
     const size_t nvox = (size_t)g.nx * g.ny * g.nz;
     if (mode == OUT_SET) fill_launch(vol, nvox, 0.f, s);
-    const unsigned tiles = ((g.nu + 15) / 16) * ((g.nv + 15) / 16);
     size_t need = 0;
     for (int k = 0; k < 3; ++k)
         if (g.dir_count[k]) need = std::max(need, group_elems(g, k));
@@ -524,12 +634,13 @@ void bp3d_matched_launch(const Geom3D &g, const float *proj, float *vol, float o
         fill_launch(acc.as<float>(), group_elems(g, k), 0.f, s);
         FPArgs A = fp_args(g, k, out_scale);
         A.volp = acc.as<float>();
-        dim3 grid(tiles, g.dir_count[k]);
-        if (g.kind == ASTRA_AMD_GEOM_CONE)
-            fp3d_transpose_kernel<true><<<grid, 256, 0, s>>>(A, proj);
-        else
-            fp3d_transpose_kernel<false><<<grid, 256, 0, s>>>(A, proj);
-        AMX_LAUNCHED();
+        const bool cone = g.kind == ASTRA_AMD_GEOM_CONE;
+        fp_group_launches(A, g.dir_count[k], [&](const FPArgs &a, unsigned groups) {
+            if (cone)
+                fp3d_transpose_kernel<true><<<groups, 256, 0, s>>>(a, proj);
+            else
+                fp3d_transpose_kernel<false><<<groups, 256, 0, s>>>(a, proj);
+        }, s);
         unsigned blocks = (unsigned)std::min<size_t>((nvox + 255) / 256, 1u << 20);
         unpad_group_add<<<blocks, 256, 0, s>>>(acc.as<float>(), vol, g.nx, g.ny, g.nz, k);
         AMX_LAUNCHED();

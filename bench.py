@@ -63,6 +63,22 @@ def cpu_baseline(iters):
     }
 
 
+def copy_calibration(dev, nbytes=1 << 30):
+    """One plain device-to-device copy of `nbytes` (a kernel of known HBM
+    traffic: nbytes read + nbytes written) used to scale the profiler's
+    FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device=dev)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize(dev)
+    b.copy_(a)
+    torch.cuda.synchronize(dev)
+    log("bench: copy calibration of %d bytes done" % nbytes)
+    del a, b
+    torch.cuda.empty_cache()
+
+
 def batched_sirt_bench(n_slices, n_plain, iters):
     """Batched 2D SIRT slices/s (one hipGraph launch for all iterations of all
     slices) vs the plain per-slice SIRT_CUDA drop-in (one algorithm object
@@ -140,6 +156,8 @@ def main():
     ap.add_argument("--plain-slices", type=int, default=64)
     ap.add_argument("--batched-iters", type=int, default=50)
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run one 1 GiB device copy first (profiler counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the projection path is measured on a single "
@@ -153,6 +171,8 @@ def main():
     from astra.algorithm import fdk
 
     dev = torch.device("cuda", 0)
+    if args.copy_calibration:
+        copy_calibration(dev)
     n, na, nd = args.size, args.angles, args.det
     sod, odd = 1000.0, 500.0
     angles = np.linspace(0, 2 * np.pi, na, endpoint=False)
