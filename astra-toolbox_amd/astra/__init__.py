@@ -8,7 +8,8 @@ runs in libastra_amd (hand-written HIP kernels for gfx950).
 """
 import os
 
-from . import algorithm, data2d, data3d, experimental, functions, log, projector, projector3d
+from . import (algorithm, autograd, batched, data2d, data3d, experimental, functions, log, projector,
+               projector3d)
 from ._core import set_gpu_index
 from .creators import (astra_dict, create_backprojection, create_backprojection3d_gpu,
                        create_proj_geom, create_projector, create_reconstruction, create_sino,

@@ -141,6 +141,65 @@ def batched_sirt_bench(n_slices, n_plain, iters):
             "first_run_seconds": setup_s}
 
 
+def differentiable_bench(n=256, na=360, nd=384, iters=20):
+    """Differentiable cone projector: one forward projection + matched-adjoint
+    gradient of 1/2 ||A x - y||^2, and a 20-iteration unrolled SIRT
+    (forward + backward w.r.t. the projections) with recomputed iterates vs
+    autograd through stored iterates (time and peak torch memory)."""
+    import torch
+
+    import astra
+    from astra import autograd as AG, phantom
+
+    dev = torch.device("cuda", 0)
+    ang = np.linspace(0, 2 * np.pi, na, endpoint=False)
+    vg = astra.create_vol_geom(n, n, n)
+    pg = astra.create_proj_geom("cone", 1.0, 1.0, nd, nd, ang, 1000.0, 500.0)
+    pid = astra.create_projector("cuda3d", pg, vg)
+    truth = phantom.ellipsoids_3d_device(n, count=20, seed=0, device=dev)
+    y = AG.forward_project(truth, pid).detach()
+    target = truth.clone()
+
+    def grad_step():
+        x = torch.zeros_like(truth).requires_grad_(True)
+        loss = 0.5 * ((AG.forward_project(x, pid) - y) ** 2).sum()
+        loss.backward()
+        return x.grad
+
+    grad_step()
+    torch.cuda.synchronize(dev)
+    t = time.perf_counter()
+    for _ in range(3):
+        grad_step()
+    torch.cuda.synchronize(dev)
+    grad_s = (time.perf_counter() - t) / 3
+
+    W = AG.sirt_weights(pid)
+    res = {}
+    for name, fn in (("recompute", AG.sirt_unrolled), ("stored", AG.sirt_unrolled_stored)):
+        b = y.clone().requires_grad_(True)
+        torch.cuda.synchronize(dev)
+        torch.cuda.reset_peak_memory_stats(dev)
+        base = torch.cuda.memory_allocated(dev)
+        t = time.perf_counter()
+        x = fn(b, pid, iters, weights=W)
+        loss = ((x - target) ** 2).sum()
+        loss.backward()
+        torch.cuda.synchronize(dev)
+        res[name] = {"seconds": time.perf_counter() - t,
+                     "peak_extra_MiB": (torch.cuda.max_memory_allocated(dev) - base) / 2 ** 20}
+        del b, x, loss
+    log("bench: differentiable %d^3: grad step %.3f s; unrolled SIRT %d its fwd+bwd recompute %.2f s "
+        "(+%.0f MiB) vs stored %.2f s (+%.0f MiB)" % (n, grad_s, iters, res["recompute"]["seconds"],
+                                                      res["recompute"]["peak_extra_MiB"],
+                                                      res["stored"]["seconds"],
+                                                      res["stored"]["peak_extra_MiB"]))
+    astra.projector3d.delete(pid)
+    return {"workload": "cone %d^3, %d angles, %dx%d detector, SOD 1000, ODD 500" % (n, na, nd, nd),
+            "fp_plus_matched_grad_seconds": grad_s, "unrolled_sirt_iterations": iters,
+            "unrolled_sirt": res}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -156,6 +215,7 @@ def main():
     ap.add_argument("--plain-slices", type=int, default=64)
     ap.add_argument("--batched-iters", type=int, default=50)
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-diff", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 1 GiB device copy first (profiler counter scaling)")
     args = ap.parse_args()
@@ -248,6 +308,10 @@ def main():
     if not args.no_batched:
         batched_res = batched_sirt_bench(args.batched_slices, args.plain_slices, args.batched_iters)
 
+    diff_res = None
+    if not args.no_diff:
+        diff_res = differentiable_bench()
+
     cpu = None
     if not args.no_cpu:
         log("bench: CPU baseline (%d SIRT iterations)" % args.cpu_iters)
@@ -276,6 +340,7 @@ def main():
                      "avg_launch_ms": kd["avg_launch_ms"]},
         "fdk_seconds": fdk_s,
         "batched_sirt": batched_res,
+        "differentiable": diff_res,
         "kernels": kernels,
         "cpu_baseline": cpu,
     }

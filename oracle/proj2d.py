@@ -53,6 +53,47 @@ def tex2d(img, x, y):
             + wy * ((1 - wx) * at(y0 + 1, x0) + wx * at(y0 + 1, x0 + 1)))
 
 
+def _scatter(n, idx, vals):
+    out = np.bincount(idx.ravel(), weights=vals.ravel(), minlength=n + 1)
+    return out[:n]
+
+
+def tex1d_T(n, c, vals):
+    """Transpose of ``tex1d``: a line of length n receiving every sample's
+    value times its two linear weights (taps outside the line dropped)."""
+    t = np.asarray(c, dtype=np.float64) - 0.5
+    i0 = np.floor(t)
+    w = t - i0
+    i0 = i0.astype(np.int64)
+    vals = np.broadcast_to(np.asarray(vals, dtype=np.float64), t.shape)
+    out = np.zeros(n)
+    for i, wt in ((i0, 1.0 - w), (i0 + 1, w)):
+        ok = (i >= 0) & (i < n)
+        out += _scatter(n, np.where(ok, i, n), np.where(ok, wt * vals, 0.0))
+    return out
+
+
+def tex2d_T(shape, x, y, vals):
+    """Transpose of ``tex2d``: an image ``[h][w]`` receiving every sample's
+    value times its four bilinear weights."""
+    h, w_ = shape
+    tx = np.asarray(x, dtype=np.float64) - 0.5
+    ty = np.asarray(y, dtype=np.float64) - 0.5
+    x0 = np.floor(tx)
+    y0 = np.floor(ty)
+    wx = tx - x0
+    wy = ty - y0
+    x0 = x0.astype(np.int64)
+    y0 = y0.astype(np.int64)
+    vals = np.broadcast_to(np.asarray(vals, dtype=np.float64), tx.shape)
+    out = np.zeros(h * w_)
+    for yy, xx, wt in ((y0, x0, (1 - wy) * (1 - wx)), (y0, x0 + 1, (1 - wy) * wx),
+                       (y0 + 1, x0, wy * (1 - wx)), (y0 + 1, x0 + 1, wy * wx)):
+        ok = (xx >= 0) & (xx < w_) & (yy >= 0) & (yy < h)
+        out += _scatter(h * w_, np.where(ok, yy * w_ + xx, h * w_), np.where(ok, wt * vals, 0.0))
+    return out.reshape(h, w_)
+
+
 def par_parameters(vec, n_det):
     """``getParParameters`` (src/GeometryUtil2D.cpp:111-134), in float64.
 
@@ -105,6 +146,31 @@ def par_fp(vol, vecs, n_det, out_scale=1.0):
                 acc += tex1d(vol[:, col], fp)
         sino[a] = acc * dist
     return sino
+
+
+def par_fp_T(sino, vecs, shape, out_scale=1.0):
+    """Exact transpose of ``par_fp`` (the matched adjoint: the same samples
+    and weights, scattered)."""
+    sino = np.asarray(sino, dtype=np.float64)
+    H, W = shape
+    n_det = sino.shape[1]
+    vol = np.zeros((H, W))
+    det = np.arange(n_det, dtype=np.float64)
+    for a in range(vecs.shape[0]):
+        theta, dsize, off = par_parameters(vecs[a], n_det)
+        c, s = math.cos(theta), math.sin(theta)
+        vertical = abs(f32(vecs[a][0])) <= abs(f32(vecs[a][1]))
+        if vertical:
+            vals = sino[a] * (out_scale / abs(c))
+            p0 = (det - 0.5 * n_det + 0.5 - off) * (dsize / c) + 0.5 * W
+            for r in range(H):
+                vol[r] += tex1d_T(W, p0 + (r - 0.5 * H + 0.5) * (s / c), vals)
+        else:
+            vals = sino[a] * (out_scale / abs(s))
+            p0 = (det - 0.5 * n_det + 0.5 - off) * (-dsize / s) + 0.5 * H
+            for col in range(W):
+                vol[:, col] += tex1d_T(H, p0 + (col - 0.5 * W + 0.5) * (c / s), vals)
+    return vol
 
 
 def par_bp(sino, vecs, shape, out_scale=1.0):
@@ -187,6 +253,40 @@ def fan_fp(vol, vecs, n_det, out_scale=1.0):
             v = tex2d(vol, fxx, np.broadcast_to(rows[None, :] + 0.5, fxx.shape)).sum(axis=1)
             sino[a, g] = v * dist
     return sino
+
+
+def fan_fp_T(sino, vecs, shape, out_scale=1.0):
+    """Exact transpose of ``fan_fp`` (matched adjoint)."""
+    sino = np.asarray(sino, dtype=np.float64)
+    H, W = shape
+    n_det = sino.shape[1]
+    vol = np.zeros((H, W))
+    cols = np.arange(W, dtype=np.float64)
+    rows = np.arange(H, dtype=np.float64)
+    tdet = np.arange(n_det)
+    for a in range(vecs.shape[0]):
+        sx, sy, dsx, dsy, ux, uy = (float(x) for x in vecs[a])
+        hor = np.array([fan_ray_is_horizontal(vecs[a], t) for t in tdet])
+        fdet = tdet + 0.5
+        dx = dsx + fdet * ux
+        dy = dsy + fdet * uy
+        if hor.any():
+            h = np.nonzero(hor)[0]
+            alpha = (sy - dy[h]) / (sx - dx[h])
+            beta = sy - alpha * sx
+            vals = sino[a, h] * np.sqrt(alpha * alpha + 1.0) * out_scale
+            fy = -alpha[:, None] * (cols[None, :] - 0.5 * W + 0.5) - beta[:, None] + 0.5 * H
+            vol += tex2d_T((H, W), np.broadcast_to(cols[None, :] + 0.5, fy.shape), fy,
+                           np.broadcast_to(vals[:, None], fy.shape))
+        if (~hor).any():
+            g = np.nonzero(~hor)[0]
+            alpha = (sx - dx[g]) / (sy - dy[g])
+            beta = sx - alpha * sy
+            vals = sino[a, g] * np.sqrt(alpha * alpha + 1.0) * out_scale
+            fxx = -alpha[:, None] * (rows[None, :] - 0.5 * H + 0.5) + beta[:, None] + 0.5 * W
+            vol += tex2d_T((H, W), fxx, np.broadcast_to(rows[None, :] + 0.5, fxx.shape),
+                           np.broadcast_to(vals[:, None], fxx.shape))
+    return vol
 
 
 def fan_bp_params(vec, fbp=False):

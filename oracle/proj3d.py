@@ -12,7 +12,7 @@ import math
 
 import numpy as np
 
-from .proj2d import tex2d
+from .proj2d import tex2d, tex2d_T
 
 f32 = np.float32
 
@@ -139,6 +139,66 @@ def par3d_fp(vol, vecs, n_u, n_v, vol_scale=(1.0, 1.0, 1.0), out_scale=1.0):
         a2f = np.full_like(b2, a2)
         proj[:, a, :] = _ray_sum(vol, d, a1f, a2f, b1, b2) * math.sqrt(a1 * a1 * s1 + a2 * a2 * s2 + 1.0) * osc
     return proj
+
+
+def _ray_scatter(vol, d, a1, a2, b1, b2, vals):
+    """Transpose of ``_ray_sum``: add vals x bilinear weights of every slice
+    sample into vol (in place)."""
+    n = _shape_xyz(vol)
+    c0, c1, c2 = _DIRS[d]
+    n0, n1, n2 = n[c0], n[c1], n[c2]
+    for s in range(n0):
+        p = s - 0.5 * n0 + 0.5
+        f1 = a1 * p + b1 + 0.5 * n1
+        f2 = a2 * p + b2 + 0.5 * n2
+        plane = _slice_plane(vol, d, s)
+        plane += tex2d_T(plane.shape, f1, f2, vals)
+
+
+def cone_fp_T(proj, vecs, shape, vol_scale=(1.0, 1.0, 1.0), out_scale=1.0):
+    """Exact transpose of ``cone_fp`` (the matched adjoint: the same rays,
+    samples and weights as the forward projection, scattered)."""
+    proj = np.asarray(proj, dtype=np.float64)
+    n_v, nA, n_u = proj.shape
+    vol = np.zeros(shape)
+    uu, vv = np.meshgrid(np.arange(n_u) + 0.5, np.arange(n_v) + 0.5)
+    for a in range(nA):
+        vec = vecs[a]
+        d = cone_direction(vec, n_u, n_v)
+        c0, c1, c2 = _DIRS[d]
+        src = vec[0:3]
+        det = [vec[3 + k] + uu * vec[6 + k] + vv * vec[9 + k] for k in range(3)]
+        den = src[c0] - det[c0]
+        a1 = (src[c1] - det[c1]) / den
+        a2 = (src[c2] - det[c2]) / den
+        b1 = src[c1] - a1 * src[c0]
+        b2 = src[c2] - a2 * src[c0]
+        s1, s2, osc = _fp_scale(d, vol_scale, out_scale)
+        vals = proj[:, a, :] * np.sqrt(a1 * a1 * s1 + a2 * a2 * s2 + 1.0) * osc
+        _ray_scatter(vol, d, a1, a2, b1, b2, vals)
+    return vol
+
+
+def par3d_fp_T(proj, vecs, shape, vol_scale=(1.0, 1.0, 1.0), out_scale=1.0):
+    """Exact transpose of ``par3d_fp``."""
+    proj = np.asarray(proj, dtype=np.float64)
+    n_v, nA, n_u = proj.shape
+    vol = np.zeros(shape)
+    uu, vv = np.meshgrid(np.arange(n_u) + 0.5, np.arange(n_v) + 0.5)
+    for a in range(nA):
+        vec = vecs[a]
+        d = par3d_direction(vec)
+        c0, c1, c2 = _DIRS[d]
+        ray = vec[0:3]
+        det = [vec[3 + k] + uu * vec[6 + k] + vv * vec[9 + k] for k in range(3)]
+        a1 = ray[c1] / ray[c0]
+        a2 = ray[c2] / ray[c0]
+        b1 = det[c1] - a1 * det[c0]
+        b2 = det[c2] - a2 * det[c0]
+        s1, s2, osc = _fp_scale(d, vol_scale, out_scale, par3d=True)
+        vals = proj[:, a, :] * math.sqrt(a1 * a1 * s1 + a2 * a2 * s2 + 1.0) * osc
+        _ray_scatter(vol, d, np.full_like(b1, a1), np.full_like(b2, a2), b1, b2, vals)
+    return vol
 
 
 # ---------------------------------------------------------------- BP ----

@@ -294,3 +294,91 @@ def test_sirt_and_cgls_reduce_the_residual():
     # SIRT with a non-negativity / upper bound keeps the iterate inside
     xc = OA.sirt(A, AT, b, x0, 5, min_c=0.2, max_c=0.7)
     assert xc.min() >= 0.2 and xc.max() <= 0.7
+
+
+# ---------------------------------------------------------------- matched adjoints
+# The oracle's exact transposes (cone_fp_T, par3d_fp_T, par_fp_T, fan_fp_T)
+# are pinned in FP64 by the dot-product identity <A x, y> = <x, A^T y> on
+# random data (a wrong index, weight or scale breaks it far above 1e-12),
+# and on a tiny case against the explicit matrix of the forward projector.
+
+def _dot_rel(fp, fpT, xshape, yshape, seed):
+    rng = np.random.default_rng(seed)
+    x = rng.standard_normal(xshape)
+    y = rng.standard_normal(yshape)
+    lhs = float(np.sum(fp(x) * y))
+    rhs = float(np.sum(x * fpT(y)))
+    return abs(lhs - rhs) / abs(lhs)
+
+
+def test_cone_transpose_dot_product_fp64():
+    n, na, nu, nv = 12, 9, 14, 13
+    v, vs = OG.normalise_3d(OG.gen_cone_projections(np.linspace(0, 2 * np.pi, na, False),
+                                                    nu, nv, 30.0, 20.0, 1.5, 1.2),
+                            "cone", OG.vol_geom_3d(n, n, n))
+    rel = _dot_rel(lambda x: O3.cone_fp(x, v, nu, nv, vs), lambda y: O3.cone_fp_T(y, v, (n, n, n), vs),
+                   (n, n, n), (nv, na, nu), 1)
+    assert rel < 1e-12
+
+
+def test_cone_vec_transpose_all_directions_fp64():
+    # steep rays put angles in all three direction groups
+    n, nu, nv = 10, 12, 12
+    ang = np.linspace(0, np.pi, 6, endpoint=False)
+    vecs = []
+    for a in ang:
+        s = np.array([np.sin(a) * 20, -np.cos(a) * 20, 0.0])
+        vecs.append(np.concatenate([s, -s / 2.0, [np.cos(a), np.sin(a), 0.0], [0, 0, 1.0]]))
+    vecs.append(np.array([0, 0, 25.0, 0, 0, -10.0, 1.0, 0, 0, 0, 1.0, 0]))  # along z
+    v, vs = OG.normalise_3d(OG.conevec_from_astra(np.array(vecs), nu, nv), "cone", OG.vol_geom_3d(n, n, n))
+    assert {O3.cone_direction(x, nu, nv) for x in v} == {0, 1, 2}
+    rel = _dot_rel(lambda x: O3.cone_fp(x, v, nu, nv, vs), lambda y: O3.cone_fp_T(y, v, (n, n, n), vs),
+                   (n, n, n), (nv, len(v), nu), 2)
+    assert rel < 1e-12
+
+
+def test_par3d_transpose_dot_product_fp64():
+    n, na, nu, nv = 10, 8, 13, 12
+    v, vs = OG.normalise_3d(OG.gen_par3d_projections(np.linspace(0, np.pi, na, False), nu, nv, 1.1, 0.9),
+                            "parallel3d", OG.vol_geom_3d(n, n, n))
+    rel = _dot_rel(lambda x: O3.par3d_fp(x, v, nu, nv, vs), lambda y: O3.par3d_fp_T(y, v, (n, n, n), vs),
+                   (n, n, n), (nv, na, nu), 3)
+    assert rel < 1e-12
+
+
+def test_2d_transposes_dot_product_fp64():
+    n, na, nd = 17, 11, 25
+    ang = np.linspace(0, np.pi, na, False)
+    vg = OG.vol_geom_2d(n, n)
+    v, osc = OG.normalise_2d(OG.gen_par_projections(ang, nd, 1.3), "parallel", vg)
+    rel = _dot_rel(lambda x: O2.par_fp(x, v, nd, osc), lambda y: O2.par_fp_T(y, v, (n, n), osc),
+                   (n, n), (na, nd), 4)
+    assert rel < 1e-12
+    v, osc = OG.normalise_2d(OG.gen_fan_projections(np.linspace(0, 2 * np.pi, na, False), nd, 40.0, 20.0, 1.4),
+                             "fan", vg)
+    rel = _dot_rel(lambda x: O2.fan_fp(x, v, nd, osc), lambda y: O2.fan_fp_T(y, v, (n, n), osc),
+                   (n, n), (na, nd), 5)
+    assert rel < 1e-12
+
+
+def test_cone_transpose_equals_explicit_matrix():
+    """Brute force on a tiny case: the matrix of cone_fp (one column per
+    unit voxel) transposed equals cone_fp_T applied to unit projections."""
+    n, na, nu, nv = 4, 3, 5, 4
+    v, vs = OG.normalise_3d(OG.gen_cone_projections(np.linspace(0, 2 * np.pi, na, False),
+                                                    nu, nv, 12.0, 8.0, 1.5, 1.5),
+                            "cone", OG.vol_geom_3d(n, n, n))
+    cols = []
+    for i in range(n ** 3):
+        e = np.zeros(n ** 3)
+        e[i] = 1.0
+        cols.append(O3.cone_fp(e.reshape(n, n, n), v, nu, nv, vs).ravel())
+    A = np.stack(cols, axis=1)
+    rows = []
+    for j in range(nv * na * nu):
+        e = np.zeros(nv * na * nu)
+        e[j] = 1.0
+        rows.append(O3.cone_fp_T(e.reshape(nv, na, nu), v, (n, n, n), vs).ravel())
+    AT = np.stack(rows, axis=0)
+    assert np.abs(A).max() > 0
+    np.testing.assert_allclose(AT, A, rtol=0, atol=1e-12 * np.abs(A).max())
