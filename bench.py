@@ -211,7 +211,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fdk", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=100)
-    ap.add_argument("--batched-slices", type=int, default=512)
+    ap.add_argument("--batched-slices", type=int, default=4096)
     ap.add_argument("--plain-slices", type=int, default=64)
     ap.add_argument("--batched-iters", type=int, default=50)
     ap.add_argument("--no-batched", action="store_true")
