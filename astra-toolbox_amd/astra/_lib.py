@@ -54,7 +54,9 @@ SIGNATURES = {
     "astra_amd_fp3d": [_vp, _vp, _vp, ctypes.POINTER(Params3D), _i, _vp],
     "astra_amd_bp3d": [_vp, _vp, _vp, ctypes.POINTER(Params3D), _i, _vp],
     "astra_amd_ramlak_spectrum": [_u, _f, _vp, _u],
+    "astra_amd_filter_spectrum": [_i, _u, _f, _f, _vp, _u],
     "astra_amd_filter_create": [_u, _vp, _u, ctypes.POINTER(_vp)],
+    "astra_amd_filter_create_custom": [_u, _u, _i, _vp, _u, ctypes.POINTER(_vp)],
     "astra_amd_filter_destroy": [_vp],
     "astra_amd_filter_rows": [_vp, _vp, _sz, _vp],
     "astra_amd_fdk_preweight": [_vp, _vp, _vp],

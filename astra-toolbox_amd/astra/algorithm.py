@@ -115,23 +115,66 @@ class BP3D_CUDA(_BP):
 
 
 # ------------------------------------------------------------- FBP / FDK --
-def _filter_handle(n_det, ftype, d=1.0):
-    """Half-spectrum filter handle for the reference's filter names
-    (src/Filters.cpp:473-560); only Ram-Lak and 'none' are built."""
-    t = (ftype or "ram-lak").lower()
-    if t in ("none",):
-        return None, True
-    if t not in ("ram-lak", "ramlak"):
-        raise AstraError("FilterType '%s' is not part of this build (Ram-Lak only)" % ftype)
+# filter names of convertStringToFilter (src/Filters.cpp:509-545) -> window
+_WINDOWS = {"ram-lak": 0, "ramlak": 0, "shepp-logan": 1, "cosine": 2, "hamming": 3, "hann": 4,
+            "tukey": 5, "lanczos": 6, "triangular": 7, "gaussian": 8, "barlett-hann": 9,
+            "blackman": 10, "nuttall": 11, "blackman-harris": 12, "blackman-nuttall": 13,
+            "flat-top": 14, "kaiser": 15, "parzen": 16}
+# user filters: name -> (real space, one row per angle)
+_CUSTOM = {"projection": (0, False), "sinogram": (0, True), "rprojection": (1, False),
+           "rsinogram": (1, True)}
+
+
+def filter_config(cfg):
+    """FilterType / FilterD / FilterParameter / FilterSinogramId, read from
+    cfg['option'] or cfg itself (getFilterConfigForAlgorithm,
+    src/Filters.cpp:548-640).  Returns (type, d, parameter, custom array)."""
+    ftype = str(_opt(cfg, "FilterType", "ram-lak")).lower()
+    if ftype not in _WINDOWS and ftype not in _CUSTOM and ftype != "none":
+        raise AstraError("Failed to convert \"%s\" into a filter." % ftype)
+    d = float(_opt(cfg, "FilterD", 1.0))
+    param = float(_opt(cfg, "FilterParameter", -1.0))
+    data = None
+    if ftype in _CUSTOM:
+        fid = _opt(cfg, "FilterSinogramId", None)
+        if fid is None:
+            raise AstraError("FilterType '%s' needs FilterSinogramId" % ftype)
+        from . import data2d
+        data = np.ascontiguousarray(data2d.get(int(fid)), dtype=np.float32)
+        if data.ndim == 1:
+            data = data.reshape(1, -1)
+    return ftype, d, param, data
+
+
+def _filter_handle(n_det, n_angles, fc):
+    """Native filter handle for a filter_config tuple (the caller destroys
+    it); None means the built-in Ram-Lak (d = 1)."""
+    ftype, d, param, data = fc
+    if ftype in ("ram-lak", "ramlak") and d == 1.0:
+        return None
+    h = ctypes.c_void_p()
+    if ftype == "none":
+        _lib.call("astra_amd_filter_create", n_det, None, 0, ctypes.byref(h))
+        return h
+    if ftype in _CUSTOM:
+        real, per_angle = _CUSTOM[ftype]
+        rows = data.shape[0]
+        if rows != (n_angles if per_angle else 1):
+            raise AstraError("FilterSinogramId for '%s' must have %d row(s), got %d"
+                             % (ftype, n_angles if per_angle else 1, rows))
+        _lib.call("astra_amd_filter_create_custom", n_det, rows, real,
+                  data.ctypes.data_as(ctypes.c_void_p), data.shape[1], ctypes.byref(h))
+        return h
     n_fft = 1
     while n_fft < 2 * n_det:
         n_fft *= 2
     nb = n_fft // 2 + 1
     spec = np.zeros(nb, dtype=np.float32)
-    _lib.call("astra_amd_ramlak_spectrum", n_det, float(d), spec.ctypes.data_as(ctypes.c_void_p), nb)
-    h = ctypes.c_void_p()
-    _lib.call("astra_amd_filter_create", n_det, spec.ctypes.data_as(ctypes.c_void_p), nb, ctypes.byref(h))
-    return h, False
+    _lib.call("astra_amd_filter_spectrum", _WINDOWS[ftype], n_det, d, param,
+              spec.ctypes.data_as(ctypes.c_void_p), nb)
+    _lib.call("astra_amd_filter_create", n_det, spec.ctypes.data_as(ctypes.c_void_p), nb,
+              ctypes.byref(h))
+    return h
 
 
 class FBP_CUDA(_Algo):
@@ -141,8 +184,7 @@ class FBP_CUDA(_Algo):
         super().__init__(cfg)
         if _truthy(_opt(cfg, "ShortScan", False)):
             raise AstraError("FBP_CUDA: ShortScan (Parker weighting) is not part of this build")
-        self.ftype = cfg.get("FilterType", _opt(cfg, "FilterType", "ram-lak"))
-        self.fd = float(cfg.get("FilterD", _opt(cfg, "FilterD", 1.0)))
+        self.filter = filter_config(cfg)
 
     def run(self, iterations=1):
         dev = self.proj.device()
@@ -153,10 +195,8 @@ class FBP_CUDA(_Algo):
         v = self.vol.on_device(dev, init=False)
         h = ctypes.c_void_p(self.proj.handle(dev))
         with torch.cuda.device(dev):
-            f, skip = _filter_handle(nd, self.ftype, self.fd)
+            f = _filter_handle(nd, self.proj.proj_shape[0], self.filter)
             try:
-                if skip:
-                    raise AstraError("FBP_CUDA with FilterType 'none' is not part of this build")
                 _lib.call("astra_amd_fbp2d", h, _core.ptr(s), _core.ptr(v), scale, f,
                           _lib.MODE_SET, _core.stream_ptr(dev))
                 _core.sync(dev)
@@ -176,8 +216,7 @@ class FDK_CUDA(_Algo):
         super().__init__(cfg)
         if _truthy(_opt(cfg, "ShortScan", False)):
             raise AstraError("FDK_CUDA: ShortScan (Parker weighting) is not part of this build")
-        self.ftype = cfg.get("FilterType", _opt(cfg, "FilterType", "ram-lak"))
-        self.fd = float(cfg.get("FilterD", _opt(cfg, "FilterD", 1.0)))
+        self.filter = filter_config(cfg)
         if self.proj.kind != "cone":
             raise AstraError("FDK_CUDA requires a cone geometry")
 
@@ -185,24 +224,19 @@ class FDK_CUDA(_Algo):
         dev = self.proj.device()
         p = self.sino.on_device(dev).clone()  # FDK filters in place
         v = self.vol.on_device(dev, init=(mode == _lib.MODE_ADD))
-        fdk(self.proj, p, v, mode, self.ftype, self.fd)
+        fdk(self.proj, p, v, mode, self.filter)
         self.vol.write_back(v)
         _core.sync(dev)
 
 
-def fdk(proj, p, v, mode, ftype="ram-lak", d=1.0):
-    """FDK of device tensor p (overwritten by the filtered data) into v."""
+def fdk(proj, p, v, mode, filter=None):
+    """FDK of device tensor p (overwritten by the filtered data) into v;
+    `filter` is a filter_config tuple (None: Ram-Lak)."""
     dev = v.device
     h = ctypes.c_void_p(proj.handle(dev))
-    nu = proj.proj_shape[2]
+    nv, na, nu = proj.proj_shape
     with torch.cuda.device(dev):
-        t = (ftype or "ram-lak").lower()
-        if t in ("ram-lak", "ramlak") and d == 1.0:
-            f = None  # the geometry's cached Ram-Lak filter
-        else:
-            f, skip = _filter_handle(nu, ftype, d)
-            if skip:
-                raise AstraError("FDK_CUDA with FilterType 'none' is not part of this build")
+        f = None if filter is None else _filter_handle(nu, na, filter)
         try:
             _lib.call("astra_amd_fdk", h, _core.ptr(p), _core.ptr(v), 1.0, 0, f, mode,
                       _core.stream_ptr(dev))

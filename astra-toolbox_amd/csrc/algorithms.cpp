@@ -260,34 +260,58 @@ double algo_residual(AlgoBase *a, const float *b, const float *x, hipStream_t s)
 }
 
 // ----------------------------------------------------------------- FDK ---
-Filter *filter_build(unsigned n_det, const float *half, unsigned n_bins) {
-    AMX_REQUIRE(n_det > 0, "filter: n_det must be positive");
-    unsigned n = next_pow2(2 * n_det);
-    AMX_REQUIRE(n_bins == n / 2 + 1, "filter: expected %u spectrum bins for %u detectors, got %u",
-                n / 2 + 1, n_det, n_bins);
-    AMX_REQUIRE(half != nullptr, "filter: spectrum is NULL");
-    auto *f = new Filter();
-    f->n_det = n_det;
-    f->n_fft = n;
-    AMX_HIP(hipGetDevice(&f->device));
-    std::vector<float> full(n);
-    for (unsigned k = 0; k < n; ++k) full[k] = half[k <= n / 2 ? k : n - k];
+namespace {
+void upload_twiddles(Filter &f) {
+    const unsigned n = f.n_fft;
     std::vector<float> tw(n);  // n/2 complex
     for (unsigned k = 0; k < n / 2; ++k) {
         double ang = -2.0 * M_PI * (double)k / (double)n;
         tw[2 * k] = (float)std::cos(ang);
         tw[2 * k + 1] = (float)std::sin(ang);
     }
-    try {
-        f->spec = DevArray(sizeof(float) * n);
-        f->spec.upload(full.data(), sizeof(float) * n);
-        f->twiddle = DevArray(sizeof(float) * n);
-        f->twiddle.upload(tw.data(), sizeof(float) * n);
-    } catch (...) {
-        delete f;
-        throw;
+    f.twiddle = DevArray(sizeof(float) * n);
+    f.twiddle.upload(tw.data(), sizeof(float) * n);
+}
+}  // namespace
+
+Filter *filter_build(unsigned n_det, const float *half, unsigned n_bins) {
+    AMX_REQUIRE(n_det > 0, "filter: n_det must be positive");
+    unsigned n = next_pow2(2 * n_det);
+    std::unique_ptr<Filter> f(new Filter());
+    f->n_det = n_det;
+    f->n_fft = n;
+    AMX_HIP(hipGetDevice(&f->device));
+    if (half == nullptr && n_bins == 0) {  // identity ('none')
+        f->identity = true;
+        return f.release();
     }
-    return f;
+    AMX_REQUIRE(n_bins == n / 2 + 1, "filter: expected %u spectrum bins for %u detectors, got %u",
+                n / 2 + 1, n_det, n_bins);
+    AMX_REQUIRE(half != nullptr, "filter: spectrum is NULL");
+    std::vector<float> full(n);
+    for (unsigned k = 0; k < n; ++k) full[k] = half[k <= n / 2 ? k : n - k];
+    f->spec = DevArray(sizeof(float) * n);
+    f->spec.upload(full.data(), sizeof(float) * n);
+    upload_twiddles(*f);
+    return f.release();
+}
+
+Filter *filter_build_custom(unsigned n_det, unsigned n_rows, int real_space, const float *data,
+                            unsigned width) {
+    AMX_REQUIRE(n_det > 0 && n_rows > 0, "filter: n_det and n_rows must be positive");
+    AMX_REQUIRE(data != nullptr, "filter: data is NULL");
+    std::vector<float> spec;
+    custom_filter_spectra(n_det, n_rows, real_space, data, width, spec);
+    std::unique_ptr<Filter> f(new Filter());
+    f->n_det = n_det;
+    f->n_fft = next_pow2(2 * n_det);
+    f->general = true;
+    f->n_spec = n_rows;
+    AMX_HIP(hipGetDevice(&f->device));
+    f->spec = DevArray(sizeof(float) * spec.size());
+    f->spec.upload(spec.data(), sizeof(float) * spec.size());
+    upload_twiddles(*f);
+    return f.release();
 }
 
 Filter *ramlak_filter(unsigned n_det) {
@@ -317,6 +341,8 @@ void fdk_run(const Geom3D &g, float *proj, float *vol, float out_scale, const Fi
     const Filter *flt = f ? f : g.fdk_filter.get();
     AMX_REQUIRE(flt->n_det == g.nu, "FDK: filter built for %u detectors, geometry has %u",
                 flt->n_det, g.nu);
+    AMX_REQUIRE(flt->n_spec == 1 || flt->n_spec == g.na,
+                "FDK: a per-angle filter needs %u rows, got %u", g.na, flt->n_spec);
     // fused cosine pre-weight + ramp filter over all nv * na rows
     filter_rows_launch(*flt, proj, (size_t)g.nv * g.na, g.fdk_w.as<float>(), g.na, s);
     bp3d_launch(g, proj, vol, out_scale, ASTRA_AMD_KER3D_FDK_WEIGHTING, mode, nullptr,
@@ -342,6 +368,8 @@ void fbp2d_run(const Geom2D &g, float *sino, float *vol, float out_scale, const 
     }
     AMX_REQUIRE(f->n_det == g.nd, "FBP: filter built for %u detectors, geometry has %u",
                 f->n_det, g.nd);
+    AMX_REQUIRE(f->n_spec == 1 || f->n_spec == g.na,
+                "FBP: a per-angle filter needs %u rows, got %u", g.na, f->n_spec);
     if (g.kind == ASTRA_AMD_GEOM_FAN) {
         // one-row cone pre-weighting with getFanParameters of the last angle
         // (cuda/2d/fbp.cu:112-148, src/GeometryUtil2D.cpp:138-177)

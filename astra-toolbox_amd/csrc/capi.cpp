@@ -261,6 +261,28 @@ int astra_amd_ramlak_spectrum(unsigned int n_det, float d, float *out, unsigned 
     });
 }
 
+int astra_amd_filter_spectrum(int type, unsigned int n_det, float d, float param, float *out,
+                              unsigned int n_out) {
+    return guarded([&] {
+        AMX_REQUIRE(n_det > 0 && out != nullptr, "filter_spectrum: invalid arguments");
+        std::vector<double> h;
+        amx::filter_half_spectrum(type, n_det, d, param, h);
+        AMX_REQUIRE(n_out == h.size(), "filter_spectrum: expected n_out = %zu", h.size());
+        for (size_t k = 0; k < h.size(); ++k) out[k] = (float)h[k];
+    });
+}
+
+int astra_amd_filter_create_custom(unsigned int n_det, unsigned int n_rows, int real_space,
+                                   const float *data, unsigned int width,
+                                   astra_amd_filter *out) {
+    return guarded([&] {
+        AMX_REQUIRE(out != nullptr, "filter_create_custom: out is NULL");
+        *out = nullptr;
+        amx::Filter *f = amx::filter_build_custom(n_det, n_rows, real_space, data, width);
+        *out = new astra_amd_filter_s{f};
+    });
+}
+
 int astra_amd_filter_create(unsigned int n_det, const float *half_spectrum, unsigned int n_bins,
                             astra_amd_filter *out) {
     return guarded([&] {

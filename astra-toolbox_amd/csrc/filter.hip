@@ -24,7 +24,9 @@ struct FilterArgs {
     float *rows;
     size_t n_rows;
     unsigned n_det, n_fft, log2n;
-    const float *spec;    // [n_fft] real spectrum
+    const float *spec;    // [n_fft] real spectrum (shared by every row)
+    const float2 *cspec;  // general: [n_spec][n_fft] complex spectra
+    unsigned n_spec;      // general: row r uses spectrum r % n_spec
     const float2 *tw;     // [n_fft/2] exp(-2 pi i k / n_fft)
     const float *weights; // optional [n_rows / rows_per_weight][n_det]
     size_t rows_per_weight;
@@ -34,6 +36,16 @@ __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
+__device__ __forceinline__ float2 conjf2(float2 a) { return make_float2(a.x, -a.y); }
+
+// GENERAL = false: one real, even spectrum for both rows (the pair's
+// transform is simply scaled).  GENERAL = true: each row has its own
+// (complex, Hermitian) spectrum; the two rows' spectra are separated from
+// the shared transform Z as X[k] = (Z[k] + conj Z[N-k]) / 2 and
+// Y[k] = (Z[k] - conj Z[N-k]) / 2i, filtered, and recombined as
+// W = Ha X + i Hb Y, whose inverse holds the filtered rows in its real and
+// imaginary parts.
+template <bool GENERAL>
 __global__ __launch_bounds__(256) void filter_rows_kernel(FilterArgs A) {
     extern __shared__ float2 X[];
     const unsigned N = A.n_fft;
@@ -76,10 +88,29 @@ __global__ __launch_bounds__(256) void filter_rows_kernel(FilterArgs A) {
     }
 
     // x filter, conjugate (inverse via conj(FFT(conj(.))))
-    for (unsigned k = threadIdx.x; k < N; k += blockDim.x) {
-        float hk = A.spec[k];
-        float2 v = X[k];
-        X[k] = make_float2(v.x * hk, -v.y * hk);
+    if (!GENERAL) {
+        for (unsigned k = threadIdx.x; k < N; k += blockDim.x) {
+            float hk = A.spec[k];
+            float2 v = X[k];
+            X[k] = make_float2(v.x * hk, -v.y * hk);
+        }
+    } else {
+        const float2 *Ha = A.cspec + (size_t)(r0 % A.n_spec) * N;
+        const float2 *Hb = has1 ? A.cspec + (size_t)((r0 + 1) % A.n_spec) * N : Ha;
+        // pairs (k, N - k) are disjoint for k = 0 .. N/2: one thread each
+        for (unsigned k = threadIdx.x; k <= half_n; k += blockDim.x) {
+            const unsigned m = (N - k) & (N - 1);
+            const float2 z1 = X[k], z2 = X[m];
+            const float2 xk = make_float2(0.5f * (z1.x + z2.x), 0.5f * (z1.y - z2.y));
+            const float2 d = make_float2(0.5f * (z1.x - z2.x), 0.5f * (z1.y + z2.y));
+            const float2 yk = make_float2(d.y, -d.x);  // d / i
+            const float2 a = cmul(Ha[k], xk), b = cmul(Hb[k], yk);
+            X[k] = conjf2(make_float2(a.x - b.y, a.y + b.x));  // conj(a + i b)
+            if (m != k) {
+                const float2 a2 = cmul(Ha[m], conjf2(xk)), b2 = cmul(Hb[m], conjf2(yk));
+                X[m] = conjf2(make_float2(a2.x - b2.y, a2.y + b2.x));
+            }
+        }
     }
     __syncthreads();
 
@@ -108,6 +139,10 @@ __global__ __launch_bounds__(256) void filter_rows_kernel(FilterArgs A) {
 void filter_rows_launch(const Filter &f, float *rows, size_t n_rows, const float *weights,
                         size_t rows_per_weight, hipStream_t s) {
     if (n_rows == 0) return;
+    if (f.identity) {  // no filtering; the pre-weights still apply
+        if (weights) rowscale_launch(rows, n_rows, f.n_det, weights, rows_per_weight, s);
+        return;
+    }
     FilterArgs A{};
     A.rows = rows;
     A.n_rows = n_rows;
@@ -116,7 +151,9 @@ void filter_rows_launch(const Filter &f, float *rows, size_t n_rows, const float
     unsigned l = 0;
     while ((1u << l) < f.n_fft) ++l;
     A.log2n = l;
-    A.spec = f.spec.as<float>();
+    A.spec = f.general ? nullptr : f.spec.as<float>();
+    A.cspec = f.general ? f.spec.as<float2>() : nullptr;
+    A.n_spec = f.n_spec;
     A.tw = f.twiddle.as<float2>();
     A.weights = weights;
     A.rows_per_weight = rows_per_weight ? rows_per_weight : 1;
@@ -125,7 +162,10 @@ void filter_rows_launch(const Filter &f, float *rows, size_t n_rows, const float
     const size_t groups = (n_rows + 1) / 2;
     AMX_REQUIRE(groups < (1ull << 31), "filter: too many rows");
     ProfToken pt = prof_begin("filter_rows", (double)n_rows, s);
-    filter_rows_kernel<<<(unsigned)groups, 256, lds, s>>>(A);
+    if (f.general)
+        filter_rows_kernel<true><<<(unsigned)groups, 256, lds, s>>>(A);
+    else
+        filter_rows_kernel<false><<<(unsigned)groups, 256, lds, s>>>(A);
     AMX_LAUNCHED();
     prof_end(pt, s);
 }

@@ -359,6 +359,148 @@ void angle2d_params(int kind, const float *p, unsigned a, FP2DAngle &f, BP2DAngl
 }
 
 // ------------------------------------------------------- filters ---------
+namespace {
+// Window of the reference's genFilter for bin k >= 1 of nb = n_fft/2 + 1
+// bins (src/Filters.cpp:151-472); w = 2 pi k / n_fft.  Evaluated in double.
+double filter_window(int type, unsigned k, unsigned nb, double w, double d, double param) {
+    const double fk = (double)k, nm1 = (double)nb - 1.0;
+    switch (type) {
+    case ASTRA_AMD_FILTER_SHEPPLOGAN: {
+        const double x = w / 2.0 / d;
+        return std::sin(x) / x;
+    }
+    case ASTRA_AMD_FILTER_COSINE:
+        return std::cos(w / 2.0 / d);
+    case ASTRA_AMD_FILTER_HAMMING:
+        return 0.54 + 0.46 * std::cos(w / d);
+    case ASTRA_AMD_FILTER_HANN:
+        return (1.0 + std::cos(w / d)) / 2.0;
+    case ASTRA_AMD_FILTER_TUKEY: {
+        const double alpha = param < 0 ? 0.5 : param;
+        const double half_n = (double)nb / 2.0;
+        const double enum_term = alpha * half_n, denom = (1.0 - alpha) * half_n;
+        if (half_n - enum_term <= fk && fk <= half_n + enum_term) return 1.0;
+        return 0.5 * (1.0 + std::cos(M_PI * (fk - enum_term) / denom));
+    }
+    case ASTRA_AMD_FILTER_LANCZOS: {
+        const double x = M_PI * (2.0 * fk / nm1 - 1.0);
+        return std::fabs(x) > 0.001 ? std::sin(x) / x : 1.0;
+    }
+    case ASTRA_AMD_FILTER_TRIANGULAR:
+        return 2.0 / nm1 * (nm1 / 2.0 - std::fabs(fk - nm1 / 2.0));
+    case ASTRA_AMD_FILTER_GAUSSIAN: {
+        const double sigma = param < 0 ? 0.4 : param;
+        const double q = nm1 / 2.0, t = (fk - q) / (sigma * q);
+        return std::exp(-0.5 * t * t);
+    }
+    case ASTRA_AMD_FILTER_BARTLETTHANN:
+        return 0.62 - 0.48 * std::fabs(fk / nm1 - 0.5) - 0.38 * std::cos(2.0 * M_PI * fk / nm1);
+    case ASTRA_AMD_FILTER_BLACKMAN: {
+        const double alpha = param < 0 ? 0.16 : param;
+        return (1.0 - alpha) / 2.0 - 0.5 * std::cos(M_PI * fk / nm1) +
+               alpha / 2.0 * std::cos(2.0 * M_PI * fk / nm1);
+    }
+    case ASTRA_AMD_FILTER_NUTTALL:
+    case ASTRA_AMD_FILTER_BLACKMANHARRIS:
+    case ASTRA_AMD_FILTER_BLACKMANNUTTALL:
+    case ASTRA_AMD_FILTER_FLATTOP: {
+        static const double a[4][5] = {{0.355768, 0.487396, 0.144232, 0.012604, 0.0},
+                                       {0.35875, 0.48829, 0.14128, 0.01168, 0.0},
+                                       {0.3635819, 0.4891775, 0.1365995, 0.0106411, 0.0},
+                                       {1.0, 1.93, 1.29, 0.388, 0.032}};
+        const double *c = a[type - ASTRA_AMD_FILTER_NUTTALL];
+        const double b = M_PI * fk / nm1;
+        return c[0] - c[1] * std::cos(2 * b) + c[2] * std::cos(4 * b) - c[3] * std::cos(6 * b) +
+               c[4] * std::cos(8 * b);
+    }
+    case ASTRA_AMD_FILTER_KAISER: {
+        // the reference evaluates the Bessel function J0 here (not I0)
+        const double alpha = param < 0 ? 3.0 : param;
+        const double x = 2.0 * fk / nm1 - 1.0;
+        return ::j0(M_PI * alpha * std::sqrt(1.0 - x * x)) / ::j0(M_PI * alpha);
+    }
+    case ASTRA_AMD_FILTER_PARZEN: {
+        const double q = fk / nm1;
+        return q <= 0.5 ? 1.0 - 6.0 * q * q * (1.0 - q) : 2.0 * (1.0 - q) * (1.0 - q) * (1.0 - q);
+    }
+    default:
+        return 1.0;  // Ram-Lak
+    }
+}
+}  // namespace
+
+void filter_half_spectrum(int type, unsigned n_det, double d, double param,
+                          std::vector<double> &out) {
+    AMX_REQUIRE(type >= ASTRA_AMD_FILTER_RAMLAK && type <= ASTRA_AMD_FILTER_PARZEN,
+                "filter: unknown window type %d", type);
+    AMX_REQUIRE(d > 0, "filter: FilterD must be positive");
+    ramlak_half_spectrum(n_det, d, out);
+    const unsigned n = next_pow2(2 * n_det), nb = n / 2 + 1;
+    for (unsigned k = 1; k < nb; ++k) {
+        const double w = 2.0 * M_PI * (double)k / (double)n;
+        out[k] *= filter_window(type, k, nb, w, d, param);
+    }
+}
+
+// In-place iterative radix-2 complex FFT (forward, unscaled) in double.
+static void fft_double(std::vector<double> &re, std::vector<double> &im) {
+    const size_t n = re.size();
+    for (size_t i = 1, j = 0; i < n; ++i) {
+        size_t bit = n >> 1;
+        for (; j & bit; bit >>= 1) j ^= bit;
+        j ^= bit;
+        if (i < j) {
+            std::swap(re[i], re[j]);
+            std::swap(im[i], im[j]);
+        }
+    }
+    for (size_t len = 2; len <= n; len <<= 1) {
+        const double ang = -2.0 * M_PI / (double)len;
+        for (size_t i = 0; i < n; i += len)
+            for (size_t k = 0; k < len / 2; ++k) {
+                const double c = std::cos(ang * (double)k), sn = std::sin(ang * (double)k);
+                const double xr = re[i + k + len / 2], xi = im[i + k + len / 2];
+                const double tr = xr * c - xi * sn, ti = xr * sn + xi * c;
+                re[i + k + len / 2] = re[i + k] - tr;
+                im[i + k + len / 2] = im[i + k] - ti;
+                re[i + k] += tr;
+                im[i + k] += ti;
+            }
+    }
+}
+
+void custom_filter_spectra(unsigned n_det, unsigned n_rows, int real_space, const float *data,
+                           unsigned width, std::vector<float> &out) {
+    const unsigned n = next_pow2(2 * n_det), nb = n / 2 + 1;
+    out.assign((size_t)n_rows * n * 2, 0.f);
+    if (!real_space) {
+        // Fourier-domain filter: nb real bins per row (cuda/2d/fft.cu:447-490);
+        // the full spectrum is its even extension
+        AMX_REQUIRE(width == nb, "filter: a Fourier-domain filter needs %u bins per row, got %u",
+                    nb, width);
+        for (unsigned r = 0; r < n_rows; ++r)
+            for (unsigned k = 0; k < n; ++k)
+                out[((size_t)r * n + k) * 2] = data[(size_t)r * width + (k <= n / 2 ? k : n - k)];
+        return;
+    }
+    // real-space kernel centred at width/2, wrapped into the padded length
+    // and transformed (cuda/2d/fft.cu:491-595)
+    AMX_REQUIRE(width > 0, "filter: empty real-space filter");
+    const unsigned used = std::min(width, n), start = (width - used) / 2, shift = width / 2;
+    std::vector<double> re(n), im(n);
+    for (unsigned r = 0; r < n_rows; ++r) {
+        std::fill(re.begin(), re.end(), 0.0);
+        std::fill(im.begin(), im.end(), 0.0);
+        for (unsigned j = start; j < start + used; ++j)
+            re[(j + n - shift) % n] = data[(size_t)r * width + j];
+        fft_double(re, im);
+        for (unsigned k = 0; k < n; ++k) {
+            out[((size_t)r * n + k) * 2] = (float)re[k];
+            out[((size_t)r * n + k) * 2 + 1] = (float)im[k];
+        }
+    }
+}
+
 void ramlak_half_spectrum(unsigned n_det, double d, std::vector<double> &out) {
     // genFilter(FILTER_RAMLAK) (src/Filters.cpp:133-160, 473-485) with the
     // spatial kernel of generateRampFilter (src/Filters.cpp:41-68):

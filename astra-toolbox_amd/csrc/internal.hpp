@@ -121,7 +121,13 @@ struct BP2DAngle {
 struct Filter {
     unsigned n_det, n_fft;
     int device;
-    DevArray spec;     // float[n_fft]: full real spectrum (symmetric)
+    bool identity = false;  // FilterType 'none': rows are left unfiltered
+    // general = false: one real, even spectrum for every row, float[n_fft].
+    // general = true: n_spec complex spectra float2[n_spec][n_fft] (row r
+    // uses spectrum r % n_spec).
+    bool general = false;
+    unsigned n_spec = 1;
+    DevArray spec;
     DevArray twiddle;  // float2[n_fft/2]: exp(-2 pi i k / n_fft)
 };
 
@@ -224,6 +230,13 @@ Geom2D *geom2d_build(int kind, unsigned w, unsigned h, unsigned na, unsigned nd,
 void angle2d_params(int kind, const float *p, unsigned a, FP2DAngle &f, BP2DAngle &bp,
                     BP2DAngle &bp_fbp);
 void ramlak_half_spectrum(unsigned n_det, double d, std::vector<double> &out);
+void filter_half_spectrum(int type, unsigned n_det, double d, double param,
+                          std::vector<double> &out);
+// complex spectra float2[n_rows][n_fft] (interleaved) of user filters
+void custom_filter_spectra(unsigned n_det, unsigned n_rows, int real_space, const float *data,
+                           unsigned width, std::vector<float> &out);
+Filter *filter_build_custom(unsigned n_det, unsigned n_rows, int real_space, const float *data,
+                            unsigned width);
 unsigned next_pow2(unsigned n);
 
 }  // namespace amx

@@ -57,6 +57,26 @@ extern "C" {
 #define ASTRA_AMD_KER3D_2D_WEIGHTING 3
 #define ASTRA_AMD_KER3D_MATCHED_BP 4 /* BP = exact transpose of the FP     */
 
+/* Filter windows of genFilter (E_FBPFILTER, include/astra/Filters.h:42-66;
+ * src/Filters.cpp:151-472), applied to the Ram-Lak spectrum. */
+#define ASTRA_AMD_FILTER_RAMLAK 0
+#define ASTRA_AMD_FILTER_SHEPPLOGAN 1
+#define ASTRA_AMD_FILTER_COSINE 2
+#define ASTRA_AMD_FILTER_HAMMING 3
+#define ASTRA_AMD_FILTER_HANN 4
+#define ASTRA_AMD_FILTER_TUKEY 5
+#define ASTRA_AMD_FILTER_LANCZOS 6
+#define ASTRA_AMD_FILTER_TRIANGULAR 7
+#define ASTRA_AMD_FILTER_GAUSSIAN 8
+#define ASTRA_AMD_FILTER_BARTLETTHANN 9
+#define ASTRA_AMD_FILTER_BLACKMAN 10
+#define ASTRA_AMD_FILTER_NUTTALL 11
+#define ASTRA_AMD_FILTER_BLACKMANHARRIS 12
+#define ASTRA_AMD_FILTER_BLACKMANNUTTALL 13
+#define ASTRA_AMD_FILTER_FLATTOP 14
+#define ASTRA_AMD_FILTER_KAISER 15
+#define ASTRA_AMD_FILTER_PARZEN 16
+
 /* SProjectorParams2D (include/astra/cuda/2d/dims.h:40-59). */
 typedef struct {
     unsigned int rays_per_det;       /* iRaysPerDet (FP supersampling)     */
@@ -146,10 +166,27 @@ int astra_amd_bp2d_matched(astra_amd_geom2d g, const float *sino, float *vol,
  * n_fft = nextpow2(2*n_det) (src/Filters.cpp:662-669), out has
  * n_fft/2+1 floats.  Host-only, synchronous. */
 int astra_amd_ramlak_spectrum(unsigned int n_det, float d, float *out, unsigned int n_out);
+/* Half spectrum (n_fft/2+1 floats) of a windowed ramp filter, genFilter
+ * (src/Filters.cpp:133-485): type ASTRA_AMD_FILTER_*, d = FilterD (bins
+ * above pi*d are zeroed), param = FilterParameter (< 0: the reference's
+ * default for TUKEY / GAUSSIAN / BLACKMAN / KAISER).  Host-only. */
+int astra_amd_filter_spectrum(int type, unsigned int n_det, float d, float param, float *out,
+                              unsigned int n_out);
 /* Upload a real half spectrum (n_fft/2+1 floats, host) for rows of
- * n_det samples.  Replaces prepareCuFFTFilter (cuda/2d/fft.cu:375-604). */
+ * n_det samples.  Replaces prepareCuFFTFilter (cuda/2d/fft.cu:375-604).
+ * half_spectrum = NULL with n_bins = 0 makes the identity filter
+ * (FilterType 'none': no filtering). */
 int astra_amd_filter_create(unsigned int n_det, const float *half_spectrum,
                             unsigned int n_bins, astra_amd_filter *out);
+/* User-supplied filters, prepareCuFFTFilter FILTER_PROJECTION / SINOGRAM /
+ * RPROJECTION / RSINOGRAM (cuda/2d/fft.cu:447-595).  n_rows = 1 (one filter
+ * for every row) or the angle count (row r of the data uses filter
+ * r % n_rows).  real_space = 0: each row holds n_fft/2+1 real Fourier bins;
+ * real_space = 1: each row is a real-space kernel of `width` taps centred
+ * at width/2, wrapped into n_fft and transformed (complex spectrum). */
+int astra_amd_filter_create_custom(unsigned int n_det, unsigned int n_rows, int real_space,
+                                   const float *data, unsigned int width,
+                                   astra_amd_filter *out);
 int astra_amd_filter_destroy(astra_amd_filter f);
 /* Filter n_rows contiguous rows of n_det floats in place:
  * zero-pad to n_fft, x spectrum, inverse, /n_fft, keep n_det

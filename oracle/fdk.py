@@ -42,16 +42,120 @@ def ramp_filter(n_det, d=1.0):
 
 
 def filter_rows(rows, H):
-    """Zero-pad each row to N, multiply the spectrum by the real filter and
-    keep the first len(row) samples (cuda/3d/fdk.cu:351-398,
-    cuda/2d/fft.cu:52-131): R2C, x filter, C2R, / N."""
+    """Zero-pad each row to N, multiply the spectrum by the filter and keep
+    the first len(row) samples (cuda/3d/fdk.cu:351-398,
+    cuda/2d/fft.cu:52-131): R2C, x filter, C2R, / N.
+
+    H is a real half spectrum (N/2 + 1 bins, every row), or an array
+    [n_spec, N/2 + 1] (possibly complex) of per-row half spectra: row r of
+    the flattened rows uses H[r % n_spec]."""
     rows = np.asarray(rows, dtype=np.float64)
-    n = 2 * (len(H) - 1)
+    H = np.asarray(H)
+    n = 2 * (H.shape[-1] - 1)
     m = rows.shape[-1]
     pad = np.zeros(rows.shape[:-1] + (n,))
     pad[..., :m] = rows
-    spec = np.fft.rfft(pad, axis=-1) * H
+    spec = np.fft.rfft(pad, axis=-1)
+    if H.ndim == 1:
+        spec = spec * H
+    else:
+        flat = spec.reshape(-1, spec.shape[-1])
+        flat = flat * H[np.arange(flat.shape[0]) % H.shape[0]]
+        spec = flat.reshape(spec.shape)
     return np.fft.irfft(spec, n=n, axis=-1)[..., :m]
+
+
+# windows of genFilter (src/Filters.cpp:151-472) for bins k >= 1 of
+# nb = N/2 + 1 bins, w = 2 pi k / N
+FILTER_TYPES = ("ram-lak", "shepp-logan", "cosine", "hamming", "hann", "tukey", "lanczos",
+                "triangular", "gaussian", "barlett-hann", "blackman", "nuttall", "blackman-harris",
+                "blackman-nuttall", "flat-top", "kaiser", "parzen")
+_COSINE_SUMS = {"nuttall": (0.355768, 0.487396, 0.144232, 0.012604, 0.0),
+                "blackman-harris": (0.35875, 0.48829, 0.14128, 0.01168, 0.0),
+                "blackman-nuttall": (0.3635819, 0.4891775, 0.1365995, 0.0106411, 0.0),
+                "flat-top": (1.0, 1.93, 1.29, 0.388, 0.032)}
+
+
+def filter_window(name, nb, n, d=1.0, param=-1.0):
+    """Window values for bins k = 1 .. nb-1 (float64)."""
+    k = np.arange(1, nb, dtype=np.float64)
+    w = 2.0 * math.pi * k / n
+    nm1 = nb - 1.0
+    if name == "ram-lak":
+        return np.ones_like(k)
+    if name == "shepp-logan":
+        x = w / 2.0 / d
+        return np.sin(x) / x
+    if name == "cosine":
+        return np.cos(w / 2.0 / d)
+    if name == "hamming":
+        return 0.54 + 0.46 * np.cos(w / d)
+    if name == "hann":
+        return (1.0 + np.cos(w / d)) / 2.0
+    if name == "tukey":
+        alpha = 0.5 if param < 0 else param
+        half = nb / 2.0
+        et, den = alpha * half, (1.0 - alpha) * half
+        inside = (half - et <= k) & (k <= half + et)
+        return np.where(inside, 1.0, 0.5 * (1.0 + np.cos(math.pi * (k - et) / den)))
+    if name == "lanczos":
+        x = math.pi * (2.0 * k / nm1 - 1.0)
+        return np.where(np.abs(x) > 0.001, np.sin(x) / np.where(x == 0, 1.0, x), 1.0)
+    if name == "triangular":
+        return 2.0 / nm1 * (nm1 / 2.0 - np.abs(k - nm1 / 2.0))
+    if name == "gaussian":
+        sigma = 0.4 if param < 0 else param
+        q = nm1 / 2.0
+        return np.exp(-0.5 * ((k - q) / (sigma * q)) ** 2)
+    if name == "barlett-hann":
+        return 0.62 - 0.48 * np.abs(k / nm1 - 0.5) - 0.38 * np.cos(2.0 * math.pi * k / nm1)
+    if name == "blackman":
+        alpha = 0.16 if param < 0 else param
+        return ((1.0 - alpha) / 2.0 - 0.5 * np.cos(math.pi * k / nm1)
+                + alpha / 2.0 * np.cos(2.0 * math.pi * k / nm1))
+    if name in _COSINE_SUMS:
+        a = _COSINE_SUMS[name]
+        b = math.pi * k / nm1
+        return a[0] - a[1] * np.cos(2 * b) + a[2] * np.cos(4 * b) - a[3] * np.cos(6 * b) + a[4] * np.cos(8 * b)
+    if name == "kaiser":
+        from scipy.special import j0   # the reference evaluates J0 (not I0)
+        alpha = 3.0 if param < 0 else param
+        x = 2.0 * k / nm1 - 1.0
+        return j0(math.pi * alpha * np.sqrt(1.0 - x * x)) / j0(math.pi * alpha)
+    if name == "parzen":
+        q = k / nm1
+        return np.where(q <= 0.5, 1.0 - 6.0 * q * q * (1.0 - q), 2.0 * (1.0 - q) ** 3)
+    raise ValueError("unknown filter %r" % name)
+
+
+def windowed_filter(name, n_det, d=1.0, param=-1.0):
+    """``genFilter`` half spectrum: Ram-Lak x window (bins >= 1), zeroed
+    where 2 pi k / N > pi d."""
+    n = next_pow2(2 * n_det)
+    nb = n // 2 + 1
+    H = 2.0 * np.real(np.fft.fft(ramp_kernel(n)))[:nb]
+    H[1:] *= filter_window(name, nb, n, d, param)
+    w = 2.0 * math.pi * np.arange(nb) / n
+    H[w > math.pi * d] = 0.0
+    return H
+
+
+def custom_filter(data, n_det, real_space):
+    """Per-row half spectra of user filters (cuda/2d/fft.cu:447-595):
+    Fourier-domain rows are used as given; a real-space kernel of width W
+    is centred at W/2, wrapped into N samples and transformed."""
+    data = np.atleast_2d(np.asarray(data, dtype=np.float64))
+    n = next_pow2(2 * n_det)
+    if not real_space:
+        return data
+    width = data.shape[1]
+    used = min(width, n)
+    start = (width - used) // 2
+    shift = width // 2
+    out = np.zeros((data.shape[0], n))
+    j = np.arange(start, start + used)
+    out[:, (j + n - shift) % n] = data[:, j]
+    return np.fft.rfft(out, axis=-1)
 
 
 def fdk_geometry(vecs, n_u, n_v):
@@ -84,27 +188,30 @@ def fdk_preweight(proj, vecs):
     return proj * weight[:, None, :]
 
 
-def fdk(proj, vecs, shape, vol_scale=(1.0, 1.0, 1.0), out_scale=1.0, d=1.0):
-    """Full FDK (cuda/3d/fdk.cu:415-478): pre-weight, Ram-Lak row filter,
-    FDK-weighted cone BP."""
+def fdk(proj, vecs, shape, vol_scale=(1.0, 1.0, 1.0), out_scale=1.0, d=1.0, H=None):
+    """Full FDK (cuda/3d/fdk.cu:415-478): pre-weight, row filter (Ram-Lak
+    unless H is given; per-angle spectra index the angle), FDK-weighted
+    cone BP."""
     n_v, n_a, n_u = proj.shape
     p = fdk_preweight(proj, vecs)
-    p = filter_rows(p, ramp_filter(n_u, d))
+    p = filter_rows(p, ramp_filter(n_u, d) if H is None else H)
     return proj3d.cone_bp(p, vecs, shape, vol_scale, out_scale, kernel="fdk")
 
 
-def fbp_fan(sino, vecs, shape, out_scale, pixel_area):
+def fbp_fan(sino, vecs, shape, out_scale, pixel_area, H=None, filt=None):
     """Fan-beam FBP (cuda/2d/fbp.cu:106-148, src/CudaFilteredBackProjection
     Algorithm.cpp:174-239): the fan sinogram is pre-weighted as a one-row
     cone, filtered, and back-projected with the FBP (fr^2) weighting;
-    output scale / pixel area."""
+    output scale / pixel area.  H: filter half spectra (default Ram-Lak);
+    filt = 'none' skips the filter."""
     n_a, n_det = sino.shape
     sod, odd, du = fan_parameters(vecs[-1], n_det)
     D = sod + odd
     w = D * (D / (du * sod)) * (math.pi / 2.0) / n_a
     U = (np.arange(n_det) - 0.5 * n_det + 0.5) * du
     p = np.asarray(sino, dtype=np.float64) * (w / np.sqrt(D * D + U * U))[None, :]
-    p = filter_rows(p, ramp_filter(n_det))
+    if H is not None or filt is None:
+        p = filter_rows(p, ramp_filter(n_det) if H is None else H)
     return proj2d.fan_bp(p, vecs, shape, out_scale / pixel_area, fbp_weighted=True)
 
 
@@ -119,10 +226,12 @@ def fan_parameters(vec, n_det):
     return sod, odd, du
 
 
-def fbp_par(sino, vecs, shape, out_scale, pixel_area):
+def fbp_par(sino, vecs, shape, out_scale, pixel_area, H=None, filt=None):
     """Parallel-beam FBP (cuda/2d/fbp.cu:150-189): filter, then BP with
-    output scale (pi/2)/nA / pixel area."""
+    output scale (pi/2)/nA / pixel area.  H / filt as for ``fbp_fan``."""
     n_a, n_det = sino.shape
-    p = filter_rows(sino, ramp_filter(n_det))
+    p = np.asarray(sino, dtype=np.float64)
+    if H is not None or filt is None:
+        p = filter_rows(p, ramp_filter(n_det) if H is None else H)
     return proj2d.par_bp(p, vecs, shape, out_scale * (math.pi / 2) / n_a / pixel_area)
 

@@ -48,3 +48,23 @@ def test_argument_errors_are_reported():
     assert b"NULL" in lib.astra_amd_last_error()
     with pytest.raises(Exception):
         _lib.call("astra_amd_algo_run", None, None, None, 1, None)
+
+
+def test_filter_spectra_match_the_oracle():
+    """astra_amd_filter_spectrum (host-only) for every window against the
+    FP64 oracle's genFilter restatement, float32 rounding apart."""
+    import numpy as np
+    from astra import _lib
+    from oracle import fdk as OF
+    for n_det, d, param in ((50, 1.0, -1.0), (97, 0.7, -1.0), (64, 1.0, 0.3)):
+        n = OF.next_pow2(2 * n_det)
+        nb = n // 2 + 1
+        for t, name in enumerate(OF.FILTER_TYPES):
+            out = np.zeros(nb, dtype=np.float32)
+            _lib.call("astra_amd_filter_spectrum", t, n_det, d, param,
+                      out.ctypes.data_as(ctypes.c_void_p), nb)
+            ref = OF.windowed_filter(name, n_det, d, param)
+            np.testing.assert_allclose(out, ref, rtol=1e-6, atol=1e-7 * np.abs(ref).max(),
+                                       err_msg=name)
+    with pytest.raises(Exception):
+        _lib.call("astra_amd_filter_spectrum", 99, 10, 1.0, -1.0, None, 0)

@@ -1,0 +1,156 @@
+"""FBP / FDK with every filter of the reference (FilterType windows with
+FilterD / FilterParameter, 'none', and the user filters projection /
+sinogram / rprojection / rsinogram given by FilterSinogramId) against the
+FP64 oracle.  Filter inputs follow the reference's own tests
+(tests/python/unit/test_algorithms_2d.py:76-110, 271-309)."""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+N, NA, ND = 64, 90, 96
+
+
+@pytest.fixture(scope="module")
+def par():
+    import astra
+    from oracle import geometry as OG
+    angles = np.linspace(0, np.pi, NA, endpoint=False)
+    vg = astra.create_vol_geom(N, N)
+    pg = astra.create_proj_geom("parallel", 1.0, ND, angles)
+    ovec, osc = OG.normalise_2d(OG.gen_par_projections(angles, ND, 1.0), "par", vg)
+    sino = np.random.default_rng(31).random((NA, ND), dtype=np.float32)
+    return dict(astra=astra, vg=vg, pg=pg, ovec=ovec, osc=osc, sino=sino)
+
+
+def _fbp(c, **opts):
+    astra = c["astra"]
+    pid = astra.create_projector("cuda", c["pg"], c["vg"])
+    sid = astra.data2d.create("-sino", c["pg"], c["sino"])
+    rid = astra.data2d.create("-vol", c["vg"], 0)
+    cfg = astra.astra_dict("FBP_CUDA")
+    cfg.update(ProjectorId=pid, ProjectionDataId=sid, ReconstructionDataId=rid, option=opts)
+    aid = astra.algorithm.create(cfg)
+    astra.algorithm.run(aid)
+    out = astra.data2d.get(rid)
+    astra.algorithm.delete(aid)
+    astra.data2d.delete([sid, rid])
+    astra.projector.delete(pid)
+    return out
+
+
+def _fbp_ref(c, **kw):
+    from oracle import fdk as OF
+    return OF.fbp_par(c["sino"], c["ovec"], (N, N), c["osc"], 1.0, **kw)
+
+
+FILTERS = ["ram-lak", "shepp-logan", "cosine", "hamming", "hann", "tukey", "lanczos", "triangular",
+           "gaussian", "barlett-hann", "blackman", "nuttall", "blackman-harris", "blackman-nuttall",
+           "flat-top", "kaiser", "parzen"]
+
+
+@pytest.mark.parametrize("name", FILTERS)
+def test_fbp_windows(par, name):
+    from oracle import fdk as OF
+    got = _fbp(par, FilterType=name)
+    ref = _fbp_ref(par, H=OF.windowed_filter(name, ND))
+    assert rel_l2(got, ref) <= 1e-5
+
+
+def test_fbp_filter_d_and_parameter(par):
+    from oracle import fdk as OF
+    for name, d, p in (("hann", 0.6, -1.0), ("shepp-logan", 0.8, -1.0), ("gaussian", 1.0, 0.25),
+                       ("tukey", 1.0, 0.3), ("blackman", 1.0, 0.2), ("kaiser", 1.0, 2.0)):
+        got = _fbp(par, FilterType=name, FilterD=d, FilterParameter=p)
+        ref = _fbp_ref(par, H=OF.windowed_filter(name, ND, d, p))
+        assert rel_l2(got, ref) <= 1e-5, name
+
+
+def test_fbp_no_filter(par):
+    got = _fbp(par, FilterType="none")
+    assert rel_l2(got, _fbp_ref(par, filt="none")) <= 1e-5
+
+
+def _fourier_filter():
+    n = 1
+    while n < 2 * ND:
+        n *= 2
+    return np.linspace(0, 1, n // 2 + 1).reshape(1, -1)
+
+
+def _real_filter():
+    k = np.zeros((1, ND))
+    for i in range(ND // 4):
+        f = np.pi * (2 * i + 1)
+        k[0, ND // 2 + 2 * i + 1] = k[0, ND // 2 - (2 * i + 1)] = -2.0 / (f * f)
+    k[0, ND // 2] = 0.5
+    return k
+
+
+@pytest.mark.parametrize("kind", ["projection", "sinogram", "rprojection", "rsinogram"])
+def test_fbp_custom_filters(par, kind):
+    from oracle import fdk as OF
+    astra = par["astra"]
+    base = _fourier_filter() if kind in ("projection", "sinogram") else _real_filter()
+    if kind in ("sinogram", "rsinogram"):
+        base = np.outer(np.random.default_rng(32).random(NA), base)
+    fg = astra.create_proj_geom("parallel", 1.0, base.shape[1], np.zeros(base.shape[0]))
+    fid = astra.data2d.create("-sino", fg, base.astype(np.float32))
+    got = _fbp(par, FilterType=kind, FilterSinogramId=fid)
+    H = OF.custom_filter(base.astype(np.float32), ND, kind.startswith("r"))
+    assert rel_l2(got, _fbp_ref(par, H=H)) <= 1e-5
+    astra.data2d.delete(fid)
+
+
+def test_fan_fbp_window():
+    import astra
+    from oracle import fdk as OF, geometry as OG, proj2d as O2
+    n, na, nd = 64, 90, 96
+    angles = np.linspace(0, 2 * np.pi, na, endpoint=False)
+    vg = astra.create_vol_geom(n, n)
+    pg = astra.create_proj_geom("fanflat", 1.0, nd, angles, 200.0, 100.0)
+    ovec, osc = OG.normalise_2d(OG.gen_fan_projections(angles, nd, 200.0, 100.0, 1.0), "fan", vg)
+    from astra import phantom
+    sino = O2.fan_fp(phantom.ellipses_2d(n, seed=33), ovec, nd, osc).astype(np.float32)
+    pid = astra.create_projector("cuda", pg, vg)
+    sid = astra.data2d.create("-sino", pg, sino)
+    rid = astra.data2d.create("-vol", vg, 0)
+    cfg = astra.astra_dict("FBP_CUDA")
+    cfg.update(ProjectorId=pid, ProjectionDataId=sid, ReconstructionDataId=rid,
+               option={"FilterType": "hamming"})
+    astra.algorithm.run(astra.algorithm.create(cfg))
+    ref = OF.fbp_fan(sino, ovec, (n, n), osc, 1.0, H=OF.windowed_filter("hamming", nd))
+    assert rel_l2(astra.data2d.get(rid), ref) <= 1e-5
+
+
+@pytest.mark.parametrize("kind", ["shepp-logan", "rsinogram"])
+def test_fdk_filters(kind):
+    import astra
+    from oracle import fdk as OF, geometry as OG, proj3d as O3
+    from astra import phantom
+    n, na, nd = 32, 45, 48
+    angles = np.linspace(0, 2 * np.pi, na, endpoint=False)
+    vg = astra.create_vol_geom(n, n, n)
+    pg = astra.create_proj_geom("cone", 1.0, 1.0, nd, nd, angles, 150.0, 75.0)
+    ovec, vs = OG.normalise_3d(OG.gen_cone_projections(angles, nd, nd, 150.0, 75.0, 1.0, 1.0),
+                               "cone", OG.vol_geom_3d(n, n, n))
+    proj = O3.cone_fp(phantom.ellipsoids_3d(n, seed=34), ovec, nd, nd, vs).astype(np.float32)
+    opts = {"FilterType": kind}
+    if kind == "rsinogram":
+        base = np.outer(np.random.default_rng(35).random(na) + 0.5,
+                        np.array([-0.1, -0.2, 1.0, -0.2, -0.1]))
+        fg = astra.create_proj_geom("parallel", 1.0, base.shape[1], np.zeros(na))
+        opts["FilterSinogramId"] = astra.data2d.create("-sino", fg, base.astype(np.float32))
+        H = OF.custom_filter(base.astype(np.float32), nd, True)
+    else:
+        H = OF.windowed_filter(kind, nd)
+    pid = astra.create_projector("cuda3d", pg, vg)
+    sid = astra.data3d.create("-sino", pg, proj)
+    rid = astra.data3d.create("-vol", vg, 0)
+    cfg = astra.astra_dict("FDK_CUDA")
+    cfg.update(ProjectorId=pid, ProjectionDataId=sid, ReconstructionDataId=rid, option=opts)
+    astra.algorithm.run(astra.algorithm.create(cfg))
+    ref = OF.fdk(proj, ovec, (n, n, n), vs, H=H)
+    assert rel_l2(astra.data3d.get(rid), ref) <= 1e-5

@@ -382,3 +382,90 @@ def test_cone_transpose_equals_explicit_matrix():
     AT = np.stack(rows, axis=0)
     assert np.abs(A).max() > 0
     np.testing.assert_allclose(AT, A, rtol=0, atol=1e-12 * np.abs(A).max())
+
+
+# ---------------------------------------------------------------- filter windows
+# The windows of genFilter (src/Filters.cpp:151-472) over the nb = N/2 + 1
+# bins, checked against independent library windows (scipy.signal.windows)
+# where the reference's formula is a textbook window on nb (or 2 nb - 1)
+# points, and against closed-form values at w = pi / the centre bin for the
+# frequency-domain ones.
+
+def _win(name, n_det=50, **kw):
+    n = OF.next_pow2(2 * n_det)
+    nb = n // 2 + 1
+    return OF.filter_window(name, nb, n, **kw), nb, n
+
+
+def test_windows_equal_library_windows():
+    from scipy.signal import windows as W
+    w, nb, _ = _win("gaussian")
+    np.testing.assert_allclose(w, W.gaussian(nb, 0.4 * (nb - 1) / 2)[1:], atol=1e-12)
+    np.testing.assert_allclose(_win("blackman")[0], W.blackman(2 * nb - 1)[1:nb], atol=1e-12)
+    np.testing.assert_allclose(_win("blackman-harris")[0], W.blackmanharris(nb)[1:], atol=1e-12)
+    np.testing.assert_allclose(_win("triangular")[0], W.bartlett(nb)[1:], atol=1e-12)
+    np.testing.assert_allclose(_win("barlett-hann")[0], W.barthann(nb)[1:], atol=1e-12)
+    np.testing.assert_allclose(_win("lanczos")[0], W.lanczos(nb)[1:], atol=1e-12)
+    np.testing.assert_allclose(_win("tukey")[0], W.tukey(nb + 1, 0.5)[1:nb], atol=1e-12)
+    # cosine-sum windows with Nuttall's published coefficients (Nuttall 1981)
+    np.testing.assert_allclose(_win("nuttall")[0],
+                               W.general_cosine(nb, [0.355768, 0.487396, 0.144232, 0.012604])[1:],
+                               atol=1e-12)
+    np.testing.assert_allclose(_win("blackman-nuttall")[0],
+                               W.general_cosine(nb, [0.3635819, 0.4891775, 0.1365995, 0.0106411])[1:],
+                               atol=1e-12)
+
+
+def test_frequency_windows_closed_forms():
+    # last bin: w = pi (d = 1)
+    assert abs(_win("hann")[0][-1]) < 1e-15
+    assert abs(_win("hamming")[0][-1] - 0.08) < 1e-15
+    assert abs(_win("cosine")[0][-1]) < 1e-15
+    assert abs(_win("shepp-logan")[0][-1] - 2.0 / math.pi) < 1e-15
+    # d = 0.5 stretches the window: hann reaches 0 at w = pi / 2 (bin N/4)
+    w, nb, n = _win("hann", d=0.5)
+    assert abs(w[n // 4 - 1]) < 1e-15
+    # kaiser (the reference evaluates J0): 1 at the centre bin, 1/J0(3 pi) at the end
+    from scipy.special import j0
+    w, nb, n = _win("kaiser")
+    assert abs(w[(nb - 1) // 2 - 1] - 1.0) < 1e-12
+    assert abs(w[-1] - 1.0 / j0(3 * math.pi)) < 1e-9
+    # parzen: continuous at q = 1/2 (0.25 from both branches), 0 at q = 1
+    w, nb, n = _win("parzen")
+    assert abs(w[(nb - 1) // 2 - 1] - 0.25) < 1e-15 and abs(w[-1]) < 1e-15
+
+
+def test_windowed_filter_cutoff_and_ramlak():
+    H = OF.windowed_filter("ram-lak", 40)
+    np.testing.assert_allclose(H, OF.ramp_filter(40), atol=1e-15)
+    H = OF.windowed_filter("hamming", 40, d=0.5)
+    n = OF.next_pow2(80)
+    w = 2 * math.pi * np.arange(n // 2 + 1) / n
+    assert np.all(H[w > math.pi * 0.5] == 0) and np.all(H[1:][w[1:] <= math.pi * 0.5] != 0)
+
+
+def test_real_space_filter_is_centred_convolution():
+    """A real-space kernel of width W is centred at W/2: filtering equals
+    the linear convolution sum_j row[i - (j - W/2)] kernel[j]."""
+    rng = np.random.default_rng(11)
+    n_det, width = 37, 21
+    row = rng.standard_normal(n_det)
+    kern = rng.standard_normal(width)
+    got = OF.filter_rows(row[None, :], OF.custom_filter(kern[None, :], n_det, True))[0]
+    ref = np.zeros(n_det)
+    for i in range(n_det):
+        for j in range(width):
+            t = i - (j - width // 2)
+            if 0 <= t < n_det:
+                ref[i] += row[t] * kern[j]
+    np.testing.assert_allclose(got, ref, atol=1e-12)
+
+
+def test_per_row_filters_index_rows_modulo():
+    rng = np.random.default_rng(12)
+    rows = rng.standard_normal((2, 3, 30))  # e.g. [v][angle][u]
+    Hs = np.stack([OF.windowed_filter(t, 30) for t in ("hann", "cosine", "parzen")])
+    got = OF.filter_rows(rows, Hs)
+    for v in range(2):
+        for a in range(3):
+            np.testing.assert_allclose(got[v, a], OF.filter_rows(rows[v, a], Hs[a]), atol=1e-12)
