@@ -182,8 +182,7 @@ class FBP_CUDA(_Algo):
 
     def __init__(self, cfg):
         super().__init__(cfg)
-        if _truthy(_opt(cfg, "ShortScan", False)):
-            raise AstraError("FBP_CUDA: ShortScan (Parker weighting) is not part of this build")
+        self.short_scan = 1 if _truthy(_opt(cfg, "ShortScan", False)) else 0
         self.filter = filter_config(cfg)
 
     def run(self, iterations=1):
@@ -197,8 +196,8 @@ class FBP_CUDA(_Algo):
         with torch.cuda.device(dev):
             f = _filter_handle(nd, self.proj.proj_shape[0], self.filter)
             try:
-                _lib.call("astra_amd_fbp2d", h, _core.ptr(s), _core.ptr(v), scale, f,
-                          _lib.MODE_SET, _core.stream_ptr(dev))
+                _lib.call("astra_amd_fbp2d", h, _core.ptr(s), _core.ptr(v), scale, self.short_scan,
+                          f, _lib.MODE_SET, _core.stream_ptr(dev))
                 _core.sync(dev)
             finally:
                 if f is not None:
@@ -214,8 +213,7 @@ class FDK_CUDA(_Algo):
 
     def __init__(self, cfg):
         super().__init__(cfg)
-        if _truthy(_opt(cfg, "ShortScan", False)):
-            raise AstraError("FDK_CUDA: ShortScan (Parker weighting) is not part of this build")
+        self.short_scan = _truthy(_opt(cfg, "ShortScan", False))
         self.filter = filter_config(cfg)
         if self.proj.kind != "cone":
             raise AstraError("FDK_CUDA requires a cone geometry")
@@ -224,22 +222,23 @@ class FDK_CUDA(_Algo):
         dev = self.proj.device()
         p = self.sino.on_device(dev).clone()  # FDK filters in place
         v = self.vol.on_device(dev, init=(mode == _lib.MODE_ADD))
-        fdk(self.proj, p, v, mode, self.filter)
+        fdk(self.proj, p, v, mode, self.filter, self.short_scan)
         self.vol.write_back(v)
         _core.sync(dev)
 
 
-def fdk(proj, p, v, mode, filter=None):
+def fdk(proj, p, v, mode, filter=None, short_scan=False):
     """FDK of device tensor p (overwritten by the filtered data) into v;
-    `filter` is a filter_config tuple (None: Ram-Lak)."""
+    `filter` is a filter_config tuple (None: Ram-Lak); short_scan adds
+    Parker weights."""
     dev = v.device
     h = ctypes.c_void_p(proj.handle(dev))
     nv, na, nu = proj.proj_shape
     with torch.cuda.device(dev):
         f = None if filter is None else _filter_handle(nu, na, filter)
         try:
-            _lib.call("astra_amd_fdk", h, _core.ptr(p), _core.ptr(v), 1.0, 0, f, mode,
-                      _core.stream_ptr(dev))
+            _lib.call("astra_amd_fdk", h, _core.ptr(p), _core.ptr(v), 1.0, 1 if short_scan else 0,
+                      f, mode, _core.stream_ptr(dev))
             _core.sync(dev)
         finally:
             if f is not None:

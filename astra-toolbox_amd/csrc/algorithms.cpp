@@ -335,7 +335,7 @@ static void fdk_prepare(const Geom3D &g) {
 
 // astraCUDA3d::FDK (cuda/3d/fdk.cu:415-478): pre-weight, filter, FDK BP
 void fdk_run(const Geom3D &g, float *proj, float *vol, float out_scale, const Filter *f,
-             int mode, hipStream_t s) {
+             bool short_scan, int mode, hipStream_t s) {
     AMX_REQUIRE(g.kind == ASTRA_AMD_GEOM_CONE, "FDK requires a cone geometry");
     fdk_prepare(g);
     const Filter *flt = f ? f : g.fdk_filter.get();
@@ -343,8 +343,21 @@ void fdk_run(const Geom3D &g, float *proj, float *vol, float out_scale, const Fi
                 flt->n_det, g.nu);
     AMX_REQUIRE(flt->n_spec == 1 || flt->n_spec == g.na,
                 "FDK: a per-angle filter needs %u rows, got %u", g.na, flt->n_spec);
-    // fused cosine pre-weight + ramp filter over all nv * na rows
-    filter_rows_launch(*flt, proj, (size_t)g.nv * g.na, g.fdk_w.as<float>(), g.na, s);
+    const float *parker = nullptr;
+    if (short_scan && g.na > 1) {
+        std::lock_guard<std::mutex> lk(g.fdk_mu);
+        if (!g.fdk_parker.bytes()) {
+            std::vector<float> w;
+            fdk_parker(g, w);
+            DevArray dp(sizeof(float) * w.size());
+            dp.upload(w.data(), sizeof(float) * w.size());
+            g.fdk_parker = std::move(dp);
+        }
+        parker = g.fdk_parker.as<float>();
+    }
+    // fused cosine (and Parker) pre-weights + ramp filter over all nv * na rows
+    filter_rows_launch(*flt, proj, (size_t)g.nv * g.na, g.fdk_w.as<float>(), g.na, s, parker,
+                       g.na);
     bp3d_launch(g, proj, vol, out_scale, ASTRA_AMD_KER3D_FDK_WEIGHTING, mode, nullptr,
                 -INFINITY, INFINITY, s);
 }
@@ -360,7 +373,7 @@ void fdk_preweight_only(const Geom3D &g, float *proj, hipStream_t s) {
 // CCudaFilteredBackProjectionAlgorithm / astraCUDA::FBP
 // (src/CudaFilteredBackProjectionAlgorithm.cpp:174-239, cuda/2d/fbp.cu:77-190)
 void fbp2d_run(const Geom2D &g, float *sino, float *vol, float out_scale, const Filter *f,
-               int mode, hipStream_t s) {
+               bool short_scan, int mode, hipStream_t s) {
     std::unique_ptr<Filter> own;
     if (!f) {
         own.reset(ramlak_filter(g.nd));
@@ -388,7 +401,20 @@ void fbp2d_run(const Geom2D &g, float *sino, float *vol, float out_scale, const 
         }
         DevArray dw(sizeof(float) * g.nd);
         dw.upload(w.data(), sizeof(float) * g.nd);
-        filter_rows_launch(*f, sino, g.na, dw.as<float>(), g.na, s);
+        DevArray dp;
+        if (short_scan && g.na > 1) {
+            // Parker weights with each angle's getFanParameters angle
+            // atan2(uy, ux) and the last angle's distances (cuda/2d/fbp.cu:117-144)
+            std::vector<double> ang(g.na);
+            for (unsigned a = 0; a < g.na; ++a)
+                ang[a] = std::atan2((double)g.vecs[6 * a + 5], (double)g.vecs[6 * a + 4]);
+            std::vector<float> pw;
+            parker_weights(ang, g.nd, sod, odd, du, pw);
+            dp = DevArray(sizeof(float) * pw.size());
+            dp.upload(pw.data(), sizeof(float) * pw.size());
+        }
+        filter_rows_launch(*f, sino, g.na, dw.as<float>(), g.na, s,
+                           dp.bytes() ? dp.as<float>() : nullptr, g.na);
         bp2d_launch(g, sino, vol, out_scale, true, mode, nullptr, -INFINITY, INFINITY, s);
         AMX_HIP(hipStreamSynchronize(s));  // dw and own are released on return
     } else {

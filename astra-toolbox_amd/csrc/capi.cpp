@@ -20,10 +20,10 @@ void algo_run(AlgoBase *a, const float *b, float *x, int iters, hipStream_t s);
 double algo_residual(AlgoBase *a, const float *b, const float *x, hipStream_t s);
 Filter *filter_build(unsigned n_det, const float *half, unsigned n_bins);
 void fdk_run(const Geom3D &g, float *proj, float *vol, float out_scale, const Filter *f,
-             int mode, hipStream_t s);
+             bool short_scan, int mode, hipStream_t s);
 void fdk_preweight_only(const Geom3D &g, float *proj, hipStream_t s);
 void fbp2d_run(const Geom2D &g, float *sino, float *vol, float out_scale, const Filter *f,
-               int mode, hipStream_t s);
+               bool short_scan, int mode, hipStream_t s);
 struct Batch2D;
 Batch2D *batch2d_build(unsigned n, const int *kinds, const unsigned *ws, const unsigned *hs,
                        const unsigned *nas, const unsigned *nds, const float *vecs,
@@ -325,21 +325,19 @@ int astra_amd_fdk(astra_amd_geom3d g, float *proj, float *vol, float output_scal
         AMX_REQUIRE(g && g->g, "geometry handle is NULL");
         AMX_REQUIRE(proj && vol, "fdk: NULL data pointer");
         check_device(g->g->device);
-        if (short_scan)
-            amx::fail(ASTRA_AMD_ERR_UNSUPPORTED, "fdk: short-scan (Parker) weighting is not built");
-        amx::fdk_run(*g->g, proj, vol, output_scale, f ? f->f : nullptr, out_mode(mode),
-                     as_stream(stream));
+        amx::fdk_run(*g->g, proj, vol, output_scale, f ? f->f : nullptr, short_scan != 0,
+                     out_mode(mode), as_stream(stream));
     });
 }
 
 int astra_amd_fbp2d(astra_amd_geom2d g, float *sino, float *vol, float output_scale,
-                    astra_amd_filter f, int mode, void *stream) {
+                    int short_scan, astra_amd_filter f, int mode, void *stream) {
     return guarded([&] {
         AMX_REQUIRE(g && g->g, "geometry handle is NULL");
         AMX_REQUIRE(sino && vol, "fbp2d: NULL data pointer");
         check_device(g->g->device);
-        amx::fbp2d_run(*g->g, sino, vol, output_scale, f ? f->f : nullptr, out_mode(mode),
-                       as_stream(stream));
+        amx::fbp2d_run(*g->g, sino, vol, output_scale, f ? f->f : nullptr, short_scan != 0,
+                       out_mode(mode), as_stream(stream));
     });
 }
 

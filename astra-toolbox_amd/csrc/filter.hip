@@ -30,6 +30,8 @@ struct FilterArgs {
     const float2 *tw;     // [n_fft/2] exp(-2 pi i k / n_fft)
     const float *weights; // optional [n_rows / rows_per_weight][n_det]
     size_t rows_per_weight;
+    const float *weights2; // optional second factor [w2_rows][n_det]: row r uses r % w2_rows
+    unsigned w2_rows;
 };
 
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
@@ -56,6 +58,8 @@ __global__ __launch_bounds__(256) void filter_rows_kernel(FilterArgs A) {
     float *row1 = row0 + A.n_det;
     const float *w0 = A.weights ? A.weights + (r0 / A.rows_per_weight) * A.n_det : nullptr;
     const float *w1 = (A.weights && has1) ? A.weights + ((r0 + 1) / A.rows_per_weight) * A.n_det : nullptr;
+    const float *v0 = A.weights2 ? A.weights2 + (r0 % A.w2_rows) * A.n_det : nullptr;
+    const float *v1 = (A.weights2 && has1) ? A.weights2 + ((r0 + 1) % A.w2_rows) * A.n_det : nullptr;
     const unsigned shift = 32 - A.log2n;
 
     // load (bit-reversed positions), zero padding beyond n_det
@@ -64,9 +68,11 @@ __global__ __launch_bounds__(256) void filter_rows_kernel(FilterArgs A) {
         if (j < A.n_det) {
             a = row0[j];
             if (w0) a *= w0[j];
+            if (v0) a *= v0[j];
             if (has1) {
                 b = row1[j];
                 if (w1) b *= w1[j];
+                if (v1) b *= v1[j];
             }
         }
         X[__brev(j) >> shift] = make_float2(a, b);
@@ -136,11 +142,35 @@ __global__ __launch_bounds__(256) void filter_rows_kernel(FilterArgs A) {
     }
 }
 
+// rows *= w[r / div] (optional) * w2[r % mod] (optional): the weights of an
+// identity ('none') filter
+__global__ void k_row_weights(float *rows, unsigned len, const float *w, size_t div,
+                              const float *w2, unsigned mod) {
+    const size_t r = blockIdx.x;
+    float *row = rows + r * len;
+    const float *a = w ? w + (r / div) * len : nullptr;
+    const float *b = w2 ? w2 + (r % mod) * len : nullptr;
+    for (unsigned i = threadIdx.x; i < len; i += blockDim.x) {
+        float v = row[i];
+        if (a) v *= a[i];
+        if (b) v *= b[i];
+        row[i] = v;
+    }
+}
+
 void filter_rows_launch(const Filter &f, float *rows, size_t n_rows, const float *weights,
-                        size_t rows_per_weight, hipStream_t s) {
+                        size_t rows_per_weight, hipStream_t s, const float *weights2,
+                        unsigned w2_rows) {
     if (n_rows == 0) return;
-    if (f.identity) {  // no filtering; the pre-weights still apply
-        if (weights) rowscale_launch(rows, n_rows, f.n_det, weights, rows_per_weight, s);
+    if (!rows_per_weight) rows_per_weight = 1;
+    if (!w2_rows) w2_rows = 1;
+    if (f.identity) {  // no filtering; the weights still apply
+        if (weights || weights2) {
+            AMX_REQUIRE(n_rows < (1ull << 31), "filter: too many rows");
+            k_row_weights<<<(unsigned)n_rows, 256, 0, s>>>(rows, f.n_det, weights, rows_per_weight,
+                                                           weights2, w2_rows);
+            AMX_LAUNCHED();
+        }
         return;
     }
     FilterArgs A{};
@@ -156,7 +186,9 @@ void filter_rows_launch(const Filter &f, float *rows, size_t n_rows, const float
     A.n_spec = f.n_spec;
     A.tw = f.twiddle.as<float2>();
     A.weights = weights;
-    A.rows_per_weight = rows_per_weight ? rows_per_weight : 1;
+    A.rows_per_weight = rows_per_weight;
+    A.weights2 = weights2;
+    A.w2_rows = w2_rows;
     const size_t lds = (size_t)f.n_fft * sizeof(float2);
     AMX_REQUIRE(lds <= 65536, "filter: n_fft %u too large for one workgroup", f.n_fft);
     const size_t groups = (n_rows + 1) / 2;
@@ -168,6 +200,20 @@ void filter_rows_launch(const Filter &f, float *rows, size_t n_rows, const float
         filter_rows_kernel<false><<<(unsigned)groups, 256, lds, s>>>(A);
     AMX_LAUNCHED();
     prof_end(pt, s);
+}
+
+void fdk_parker(const Geom3D &g, std::vector<float> &w) {
+    // angles of FDK() (cuda/3d/fdk.cu:447-450): -atan2(srcX, srcY) + pi, and
+    // the scalar geometry of angle 0 (cuda/3d/fdk.cu:427-445)
+    const double *p = g.vecs.data();
+    std::vector<double> ang(g.na);
+    for (unsigned a = 0; a < g.na; ++a)
+        ang[a] = -std::atan2(p[12 * a + 0], p[12 * a + 1]) + M_PI;
+    const double dcx = p[3] + 0.5 * g.nu * p[6], dcy = p[4] + 0.5 * g.nu * p[7];
+    const double sod = std::sqrt(p[0] * p[0] + p[1] * p[1]);
+    const double odd = std::sqrt(dcx * dcx + dcy * dcy);
+    const double du = std::sqrt(p[6] * p[6] + p[7] * p[7]);
+    parker_weights(ang, g.nu, sod, odd, du, w);
 }
 
 void fdk_weights(const Geom3D &g, std::vector<float> &w) {

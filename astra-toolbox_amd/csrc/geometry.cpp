@@ -501,6 +501,53 @@ void custom_filter_spectra(unsigned n_det, unsigned n_rows, int real_space, cons
     }
 }
 
+void parker_weights(const std::vector<double> &angles, unsigned nu, double sod, double odd,
+                    double du, std::vector<float> &out) {
+    // FDK_PreWeight short-scan branch + devFDK_ParkerWeight
+    // (cuda/3d/fdk.cu:115-163, 201-251): angles relative to the start of
+    // the scanned range, wrapped into [0, 2 pi); weight of ray (beta, gamma)
+    // from the central fan's Radon-space position, times range / pi.
+    const size_t na = angles.size();
+    out.assign(na * nu, 1.f);
+    if (na < 2) return;
+    double da = angles[1] - angles[0];
+    while (da < -M_PI) da += 2 * M_PI;
+    while (da >= M_PI) da -= 2 * M_PI;
+    const double base = da >= 0 ? angles[0] : angles[na - 1];
+    std::vector<double> rel(na);
+    for (size_t i = 0; i < na; ++i) {
+        double f = angles[i] - base;
+        while (f >= 2 * M_PI) f -= 2 * M_PI;
+        while (f < 0) f += 2 * M_PI;
+        rel[i] = f;
+    }
+    const double range = std::fabs(rel[na - 1] - rel[0]) / (double)(na - 1) * (double)na;
+    const double scale = range / M_PI;
+    const double D = sod + odd;
+    const double fan = std::fabs(std::atan(du * (nu * 0.5) / D));
+    for (size_t i = 0; i < na; ++i) {
+        const double beta = rel[i];
+        for (unsigned u = 0; u < nu; ++u) {
+            const double gamma = std::atan((u - 0.5 * nu + 0.5) * du / D);
+            double w;
+            if (beta <= 0.0) {
+                w = 0.0;
+            } else if (beta <= 2.0 * (fan + gamma)) {
+                w = std::sin((M_PI / 4.0) * beta / (fan + gamma));
+                w *= w;
+            } else if (beta <= M_PI + 2 * gamma) {
+                w = 1.0;
+            } else if (beta <= M_PI + 2 * fan) {
+                w = std::sin((M_PI / 4.0) * (M_PI + 2.0 * fan - beta) / (fan - gamma));
+                w *= w;
+            } else {
+                w = 0.0;
+            }
+            out[i * nu + u] = (float)(w * scale);
+        }
+    }
+}
+
 void ramlak_half_spectrum(unsigned n_det, double d, std::vector<double> &out) {
     // genFilter(FILTER_RAMLAK) (src/Filters.cpp:133-160, 473-485) with the
     // spatial kernel of generateRampFilter (src/Filters.cpp:41-68):

@@ -150,6 +150,7 @@ struct Geom3D {
     // FDK pre-weights [nv][nu] and default Ram-Lak filter, built on first use
     mutable std::mutex fdk_mu;
     mutable DevArray fdk_w;
+    mutable DevArray fdk_parker;  // [na][nu] short-scan weights, built on first use
     mutable std::unique_ptr<Filter> fdk_filter;
 };
 
@@ -203,8 +204,10 @@ void bp2d_matched_launch(const Geom2D &g, const float *sino, float *vol, float o
                          int mode, hipStream_t s);
 // filtering
 void filter_rows_launch(const Filter &f, float *rows, size_t n_rows, const float *weights,
-                        size_t rows_per_weight, hipStream_t s);
+                        size_t rows_per_weight, hipStream_t s, const float *weights2 = nullptr,
+                        unsigned w2_rows = 1);
 void fdk_weights(const Geom3D &g, std::vector<float> &w);  // [nv][nu] host
+void fdk_parker(const Geom3D &g, std::vector<float> &w);  // [na][nu] host
 // elementwise
 void fill_launch(float *p, size_t n, float v, hipStream_t s);
 void invert_launch(float *p, size_t n, float scale, hipStream_t s);
@@ -232,6 +235,9 @@ void angle2d_params(int kind, const float *p, unsigned a, FP2DAngle &f, BP2DAngl
 void ramlak_half_spectrum(unsigned n_det, double d, std::vector<double> &out);
 void filter_half_spectrum(int type, unsigned n_det, double d, double param,
                           std::vector<double> &out);
+// short-scan Parker weights [angle][u]
+void parker_weights(const std::vector<double> &angles, unsigned nu, double sod, double odd,
+                    double du, std::vector<float> &out);
 // complex spectra float2[n_rows][n_fft] (interleaved) of user filters
 void custom_filter_spectra(unsigned n_det, unsigned n_rows, int real_space, const float *data,
                            unsigned width, std::vector<float> &out);

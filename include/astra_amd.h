@@ -200,17 +200,20 @@ int astra_amd_fdk_preweight(astra_amd_geom3d g, float *proj, void *stream);
 /* Full FDK: pre-weight + Ram-Lak filter (filter handle, NULL = Ram-Lak
  * built internally) + FDK-weighted cone BP, astraCUDA3d::FDK
  * (cuda/3d/fdk.cu:415-478).  `proj` is used as scratch and overwritten
- * with the filtered projections.  Short scan is UNSUPPORTED. */
+ * with the filtered projections.  short_scan != 0 adds the Parker
+ * weights of FDK_PreWeight (cuda/3d/fdk.cu:115-163, 201-251). */
 int astra_amd_fdk(astra_amd_geom3d g, float *proj, float *vol, float output_scale,
                   int short_scan, astra_amd_filter f, int mode, void *stream);
 /* 2D FBP: astraCUDA::FBP (cuda/2d/fbp.cu:77-190).  FAN: one-row cone
- * pre-weighting (fbp.cu:106-148), filter, FBP-weighted fan BP; PAR: filter,
- * BP with output_scale * (pi/2)/n_angles.  The caller folds the
- * 1/pixel-area factor of CCudaFilteredBackProjectionAlgorithm::run
+ * pre-weighting (fbp.cu:106-148, with Parker weights when short_scan != 0),
+ * filter, FBP-weighted fan BP; PAR: filter, BP with output_scale *
+ * (pi/2)/n_angles (short_scan has no effect, as in the reference).  The
+ * caller folds the 1/pixel-area factor of
+ * CCudaFilteredBackProjectionAlgorithm::run
  * (src/CudaFilteredBackProjectionAlgorithm.cpp:210-212) into output_scale.
  * `sino` is overwritten with the filtered sinogram.  f = NULL: Ram-Lak. */
 int astra_amd_fbp2d(astra_amd_geom2d g, float *sino, float *vol, float output_scale,
-                    astra_amd_filter f, int mode, void *stream);
+                    int short_scan, astra_amd_filter f, int mode, void *stream);
 
 /* ------------------------------------------------- batched 2D SIRT --- */
 /* Many independent 2D slices (parallel or fan, each with its own volume

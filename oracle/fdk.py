@@ -158,6 +158,42 @@ def custom_filter(data, n_det, real_space):
     return np.fft.rfft(out, axis=-1)
 
 
+def parker_weight(beta, gamma, fan):
+    """Parker weight of ray (beta, gamma) for half fan angle `fan`
+    (devFDK_ParkerWeight, cuda/3d/fdk.cu:136-156), vectorised."""
+    beta, gamma = np.broadcast_arrays(np.asarray(beta, np.float64), np.asarray(gamma, np.float64))
+    w = np.zeros(beta.shape)
+    r1 = (beta > 0) & (beta <= 2.0 * (fan + gamma))
+    r2 = ~r1 & (beta > 0) & (beta <= math.pi + 2 * gamma)
+    r3 = ~r1 & ~r2 & (beta > 0) & (beta <= math.pi + 2 * fan)
+    w[r1] = np.sin((math.pi / 4.0) * beta[r1] / (fan + gamma[r1])) ** 2
+    w[r2] = 1.0
+    w[r3] = np.sin((math.pi / 4.0) * (math.pi + 2.0 * fan - beta[r3]) / (fan - gamma[r3])) ** 2
+    return w
+
+
+def parker_weights(angles, n_u, sod, odd, du):
+    """[angle][u] short-scan weights of FDK_PreWeight (cuda/3d/fdk.cu:201-251):
+    angles made relative to the start of the scanned range (wrapped into
+    [0, 2 pi)), scaled by range / pi."""
+    angles = np.asarray(angles, dtype=np.float64)
+    na = len(angles)
+    if na < 2:
+        return np.ones((na, n_u))
+    da = angles[1] - angles[0]
+    while da < -math.pi:
+        da += 2 * math.pi
+    while da >= math.pi:
+        da -= 2 * math.pi
+    base = angles[0] if da >= 0 else angles[-1]
+    rel = np.mod(angles - base, 2 * math.pi)
+    rng = abs(rel[-1] - rel[0]) / (na - 1) * na
+    D = sod + odd
+    fan = abs(math.atan(du * (n_u * 0.5) / D))
+    gamma = np.arctan((np.arange(n_u) - 0.5 * n_u + 0.5) * du / D)
+    return parker_weight(rel[:, None], gamma[None, :], fan) * (rng / math.pi)
+
+
 def fdk_geometry(vecs, n_u, n_v):
     """Scalar cone parameters extracted from angles[0] by ``FDK``
     (cuda/3d/fdk.cu:427-445), all in normalised units."""
@@ -188,17 +224,23 @@ def fdk_preweight(proj, vecs):
     return proj * weight[:, None, :]
 
 
-def fdk(proj, vecs, shape, vol_scale=(1.0, 1.0, 1.0), out_scale=1.0, d=1.0, H=None):
-    """Full FDK (cuda/3d/fdk.cu:415-478): pre-weight, row filter (Ram-Lak
-    unless H is given; per-angle spectra index the angle), FDK-weighted
-    cone BP."""
+def fdk(proj, vecs, shape, vol_scale=(1.0, 1.0, 1.0), out_scale=1.0, d=1.0, H=None,
+        short_scan=False):
+    """Full FDK (cuda/3d/fdk.cu:415-478): pre-weight (and Parker weights for
+    a short scan, with the angles -atan2(srcX, srcY) + pi of fdk.cu:447-450),
+    row filter (Ram-Lak unless H is given; per-angle spectra index the
+    angle), FDK-weighted cone BP."""
     n_v, n_a, n_u = proj.shape
     p = fdk_preweight(proj, vecs)
+    if short_scan:
+        sod, odd, du, dv, zshift = fdk_geometry(vecs, n_u, n_v)
+        ang = -np.arctan2(vecs[:, 0], vecs[:, 1]) + math.pi
+        p = p * parker_weights(ang, n_u, sod, odd, du)[None, :, :]
     p = filter_rows(p, ramp_filter(n_u, d) if H is None else H)
     return proj3d.cone_bp(p, vecs, shape, vol_scale, out_scale, kernel="fdk")
 
 
-def fbp_fan(sino, vecs, shape, out_scale, pixel_area, H=None, filt=None):
+def fbp_fan(sino, vecs, shape, out_scale, pixel_area, H=None, filt=None, short_scan=False):
     """Fan-beam FBP (cuda/2d/fbp.cu:106-148, src/CudaFilteredBackProjection
     Algorithm.cpp:174-239): the fan sinogram is pre-weighted as a one-row
     cone, filtered, and back-projected with the FBP (fr^2) weighting;
@@ -210,6 +252,10 @@ def fbp_fan(sino, vecs, shape, out_scale, pixel_area, H=None, filt=None):
     w = D * (D / (du * sod)) * (math.pi / 2.0) / n_a
     U = (np.arange(n_det) - 0.5 * n_det + 0.5) * du
     p = np.asarray(sino, dtype=np.float64) * (w / np.sqrt(D * D + U * U))[None, :]
+    if short_scan:
+        # each angle's getFanParameters angle atan2(uy, ux) (cuda/2d/fbp.cu:117-128)
+        ang = np.arctan2(vecs[:, 5], vecs[:, 4])
+        p = p * parker_weights(ang, n_det, sod, odd, du)
     if H is not None or filt is None:
         p = filter_rows(p, ramp_filter(n_det) if H is None else H)
     return proj2d.fan_bp(p, vecs, shape, out_scale / pixel_area, fbp_weighted=True)

@@ -154,3 +154,51 @@ def test_fdk_filters(kind):
     astra.algorithm.run(astra.algorithm.create(cfg))
     ref = OF.fdk(proj, ovec, (n, n, n), vs, H=H)
     assert rel_l2(astra.data3d.get(rid), ref) <= 1e-5
+
+
+def test_fdk_short_scan():
+    """FDK over pi + 2 x fan angle with Parker weights (ShortScan)."""
+    import math
+    import astra
+    from astra import phantom
+    from oracle import fdk as OF, geometry as OG, proj3d as O3
+    n, nd, sod, odd = 32, 48, 150.0, 75.0
+    fan = math.atan(nd / 2 / (sod + odd))
+    angles = np.linspace(0, math.pi + 2 * fan, 60, endpoint=False)
+    vg = astra.create_vol_geom(n, n, n)
+    pg = astra.create_proj_geom("cone", 1.0, 1.0, nd, nd, angles, sod, odd)
+    ovec, vs = OG.normalise_3d(OG.gen_cone_projections(angles, nd, nd, sod, odd, 1.0, 1.0),
+                               "cone", OG.vol_geom_3d(n, n, n))
+    proj = O3.cone_fp(phantom.ellipsoids_3d(n, seed=36), ovec, nd, nd, vs).astype(np.float32)
+    pid = astra.create_projector("cuda3d", pg, vg)
+    sid = astra.data3d.create("-sino", pg, proj)
+    rid = astra.data3d.create("-vol", vg, 0)
+    cfg = astra.astra_dict("FDK_CUDA")
+    cfg.update(ProjectorId=pid, ProjectionDataId=sid, ReconstructionDataId=rid,
+               option={"ShortScan": True})
+    astra.algorithm.run(astra.algorithm.create(cfg))
+    ref = OF.fdk(proj, ovec, (n, n, n), vs, short_scan=True)
+    assert rel_l2(astra.data3d.get(rid), ref) <= 1e-5
+
+
+def test_fan_fbp_short_scan():
+    import math
+    import astra
+    from astra import phantom
+    from oracle import fdk as OF, geometry as OG, proj2d as O2
+    n, nd, sod, odd = 64, 96, 200.0, 100.0
+    fan = math.atan(nd / 2 / (sod + odd))
+    angles = np.linspace(0, math.pi + 2 * fan, 120, endpoint=False)
+    vg = astra.create_vol_geom(n, n)
+    pg = astra.create_proj_geom("fanflat", 1.0, nd, angles, sod, odd)
+    ovec, osc = OG.normalise_2d(OG.gen_fan_projections(angles, nd, sod, odd, 1.0), "fan", vg)
+    sino = O2.fan_fp(phantom.ellipses_2d(n, seed=37), ovec, nd, osc).astype(np.float32)
+    pid = astra.create_projector("cuda", pg, vg)
+    sid = astra.data2d.create("-sino", pg, sino)
+    rid = astra.data2d.create("-vol", vg, 0)
+    cfg = astra.astra_dict("FBP_CUDA")
+    cfg.update(ProjectorId=pid, ProjectionDataId=sid, ReconstructionDataId=rid,
+               option={"ShortScan": True})
+    astra.algorithm.run(astra.algorithm.create(cfg))
+    ref = OF.fbp_fan(sino, ovec, (n, n), osc, 1.0, short_scan=True)
+    assert rel_l2(astra.data2d.get(rid), ref) <= 1e-5

@@ -469,3 +469,51 @@ def test_per_row_filters_index_rows_modulo():
     for v in range(2):
         for a in range(3):
             np.testing.assert_allclose(got[v, a], OF.filter_rows(rows[v, a], Hs[a]), atol=1e-12)
+
+
+# ---------------------------------------------------------------- Parker weights
+
+def test_parker_conjugate_rays_sum_to_one():
+    """Short-scan weights of conjugate rays add up to one: in the fan
+    geometry ray (a, U) is the line of ray (a + pi - 2 gamma, -U) with
+    gamma = atan(U / (SOD + ODD)) (checked on the oracle's own geometry),
+    and the Parker weights of the two sum to 1 over the whole scan."""
+    sod, odd, nd = 100.0, 50.0, 41
+    D = sod + odd
+    U = 10.0
+    g = math.atan(U / D)
+    a = 0.3
+    vecs = OG.gen_fan_projections(np.array([a, a + math.pi - 2 * g]), nd, sod, odd, 1.0)
+
+    def line(v, u):
+        s = np.array(v[0:2])
+        d = np.array(v[2:4]) + (u + 0.5) * np.array(v[4:6])
+        n = (d - s) / np.linalg.norm(d - s)
+        return n, s[0] * n[1] - s[1] * n[0]
+
+    u = nd / 2.0 - 0.5 + U  # detector coordinate of U (pixel 1)
+    n1, o1 = line(vecs[0], u)
+    n2, o2 = line(vecs[1], nd - 1 - u)
+    assert min(np.abs(n1 + n2).max() + abs(o1 + o2), np.abs(n1 - n2).max() + abs(o1 - o2)) < 1e-4
+    fan = 0.3
+    rng = np.random.default_rng(21)
+    gam = rng.uniform(-0.99 * fan, 0.99 * fan, 4000)
+    beta = rng.uniform(1e-6, math.pi + 2 * gam)  # conjugate stays inside the scan
+    tot = OF.parker_weight(beta, gam, fan) + OF.parker_weight(beta + math.pi - 2 * gam, -gam, fan)
+    np.testing.assert_allclose(tot, 1.0, atol=1e-12)
+
+
+def test_parker_weights_scan_range_scale():
+    """Equally spaced scan over pi + 2 fan: weights in [0, range/pi], zero
+    at the first angle, and the angle-sum of the weights of each detector
+    column equals the count of a half turn (n_a * pi / range) times the
+    scale, up to discretisation."""
+    sod, odd, nd, du = 100.0, 50.0, 64, 1.0
+    fan = math.atan(du * nd / 2 / (sod + odd))
+    na = 400
+    ang = np.linspace(0, math.pi + 2 * fan, na, endpoint=False)
+    w = OF.parker_weights(ang, nd, sod, odd, du)
+    scale = (ang[-1] - ang[0]) / (na - 1) * na / math.pi
+    assert np.all(w >= 0) and np.all(w <= scale + 1e-12) and np.all(w[0] == 0)
+    col = w.sum(axis=0) / scale            # sum over angles of w / scale
+    np.testing.assert_allclose(col, na * math.pi / (ang[1] - ang[0]) / na, rtol=2e-2)
