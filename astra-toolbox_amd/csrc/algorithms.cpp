@@ -27,7 +27,8 @@ struct Op {
     Geom3D *g3 = nullptr;
     float out_scale = 1.f;
     int kernel = ASTRA_AMD_KER3D_DEFAULT;
-    unsigned rays = 1;
+    unsigned rays = 1;  // FP: rays per detector (2D) / per detector dimension (3D)
+    unsigned vss = 1;   // BP: sub-pixels / sub-voxels per dimension
 
     size_t nvol() const {
         return g3 ? (size_t)g3->nx * g3->ny * g3->nz : (size_t)g2->w * g2->h;
@@ -41,7 +42,7 @@ struct Op {
     void fp(const float *x, float *out_padded, float scale, int mode, const float *b,
             const float *w, hipStream_t s) const {
         if (g3)
-            fp3d_launch(*g3, x, out_padded, out_scale * scale, mode, b, w, true, s);
+            fp3d_launch(*g3, x, out_padded, out_scale * scale, mode, b, w, true, s, rays);
         else
             fp2d_launch(*g2, x, out_padded, out_scale * scale, mode, b, w, true, rays, s);
     }
@@ -49,9 +50,9 @@ struct Op {
             float hi, hipStream_t s) const {
         if (g3) {
             int k = (kernel == ASTRA_AMD_KER3D_2D_WEIGHTING) ? kernel : ASTRA_AMD_KER3D_DEFAULT;
-            bp3d_launch_padded(*g3, pp, x, out_scale * scale, k, mode, uw, lo, hi, s);
+            bp3d_launch_padded(*g3, pp, x, out_scale * scale, k, mode, uw, lo, hi, s, vss);
         } else {
-            bp2d_launch_padded(*g2, pp, x, out_scale * scale, false, mode, uw, lo, hi, s);
+            bp2d_launch_padded(*g2, pp, x, out_scale * scale, false, mode, uw, lo, hi, s, vss);
         }
     }
     void to_padded(const float *proj, float *pp, hipStream_t s) const {
@@ -205,7 +206,7 @@ struct Cgls : AlgoBase {
 };
 
 AlgoBase *algo_build(int type, Geom2D *g2, Geom3D *g3, float out_scale, int kernel,
-                     unsigned rays) {
+                     unsigned rays, unsigned vss) {
     AlgoBase *a = nullptr;
     if (type == ASTRA_AMD_ALGO_SIRT)
         a = new Sirt();
@@ -219,6 +220,7 @@ AlgoBase *algo_build(int type, Geom2D *g2, Geom3D *g3, float out_scale, int kern
     a->op.out_scale = out_scale;
     a->op.kernel = kernel;
     a->op.rays = rays;
+    a->op.vss = vss;
     AMX_HIP(hipGetDevice(&a->device));
     return a;
 }

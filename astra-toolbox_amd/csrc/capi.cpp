@@ -12,7 +12,7 @@
 namespace amx {
 struct AlgoBase;
 AlgoBase *algo_build(int type, Geom2D *g2, Geom3D *g3, float out_scale, int kernel,
-                     unsigned rays);
+                     unsigned rays, unsigned vss);
 void algo_destroy(AlgoBase *a);
 void algo_set_option(AlgoBase *a, const char *key, double v);
 void algo_set_mask(AlgoBase *a, const char *key, const float *m);
@@ -149,10 +149,9 @@ int astra_amd_bp2d(astra_amd_geom2d g, const float *sino, float *vol,
     return guarded([&] {
         check2d(g, p);
         AMX_REQUIRE(vol && sino, "bp2d: NULL data pointer");
-        if (p->rays_per_pixel_dim > 1)
-            amx::fail(ASTRA_AMD_ERR_UNSUPPORTED, "bp2d: VoxelSuperSampling > 1 is not built");
         amx::bp2d_launch(*g->g, sino, vol, p->output_scale, false, out_mode(mode), nullptr,
-                         -INFINITY, INFINITY, as_stream(stream));
+                         -INFINITY, INFINITY, as_stream(stream),
+                         p->rays_per_pixel_dim ? p->rays_per_pixel_dim : 1);
     });
 }
 
@@ -173,7 +172,7 @@ int astra_amd_bp2d_matched(astra_amd_geom2d g, const float *sino, float *vol,
         check2d(g, p);
         AMX_REQUIRE(vol && sino, "bp2d_matched: NULL data pointer");
         amx::bp2d_matched_launch(*g->g, sino, vol, p->output_scale, out_mode(mode),
-                                 as_stream(stream));
+                                 as_stream(stream), p->rays_per_det ? p->rays_per_det : 1);
     });
 }
 
@@ -203,9 +202,9 @@ static void check3d(astra_amd_geom3d g, const astra_amd_params3d *p) {
     AMX_REQUIRE(g && g->g, "geometry handle is NULL");
     AMX_REQUIRE(p != nullptr, "params is NULL");
     check_device(g->g->device);
-    if (p->rays_per_det_dim > 1 || p->rays_per_voxel_dim > 1)
-        amx::fail(ASTRA_AMD_ERR_UNSUPPORTED, "3D supersampling > 1 is not built");
 }
+
+static unsigned ss_of(unsigned v) { return v ? v : 1; }
 
 int astra_amd_fp3d(astra_amd_geom3d g, const float *vol, float *proj,
                    const astra_amd_params3d *p, int mode, void *stream) {
@@ -217,7 +216,7 @@ int astra_amd_fp3d(astra_amd_geom3d g, const float *vol, float *proj,
         AMX_REQUIRE(p->kernel >= 0 && p->kernel <= ASTRA_AMD_KER3D_MATCHED_BP,
                     "fp3d: unknown kernel %d", p->kernel);
         amx::fp3d_launch(*g->g, vol, proj, p->output_scale, out_mode(mode), nullptr, nullptr,
-                         false, as_stream(stream));
+                         false, as_stream(stream), ss_of(p->rays_per_det_dim));
     });
 }
 
@@ -231,7 +230,8 @@ int astra_amd_bp3d(astra_amd_geom3d g, const float *proj, float *vol,
             case ASTRA_AMD_KER3D_DEFAULT:
             case ASTRA_AMD_KER3D_2D_WEIGHTING:
                 amx::bp3d_launch(*g->g, proj, vol, p->output_scale, p->kernel, m, nullptr,
-                                 -INFINITY, INFINITY, as_stream(stream));
+                                 -INFINITY, INFINITY, as_stream(stream),
+                                 ss_of(p->rays_per_voxel_dim));
                 break;
             case ASTRA_AMD_KER3D_FDK_WEIGHTING:
                 AMX_REQUIRE(g->g->kind == ASTRA_AMD_GEOM_CONE,
@@ -240,7 +240,8 @@ int astra_amd_bp3d(astra_amd_geom3d g, const float *proj, float *vol,
                                  -INFINITY, INFINITY, as_stream(stream));
                 break;
             case ASTRA_AMD_KER3D_MATCHED_BP:
-                amx::bp3d_matched_launch(*g->g, proj, vol, p->output_scale, m, as_stream(stream));
+                amx::bp3d_matched_launch(*g->g, proj, vol, p->output_scale, m, as_stream(stream),
+                                         ss_of(p->rays_per_det_dim));
                 break;
             case ASTRA_AMD_KER3D_SUM_SQUARE_WEIGHTS:
                 amx::fail(ASTRA_AMD_ERR_UNSUPPORTED, "bp3d: sum_square_weights is FP-only");
@@ -428,10 +429,8 @@ int astra_amd_algo2d_create(int type, astra_amd_geom2d g, const astra_amd_params
         AMX_REQUIRE(out != nullptr, "algo2d_create: out is NULL");
         *out = nullptr;
         check2d(g, p);
-        if (p->rays_per_pixel_dim > 1)
-            amx::fail(ASTRA_AMD_ERR_UNSUPPORTED, "VoxelSuperSampling > 1 is not built");
-        unsigned rays = p->rays_per_det ? p->rays_per_det : 1;
-        amx::AlgoBase *a = amx::algo_build(type, g->g, nullptr, p->output_scale, 0, rays);
+        amx::AlgoBase *a = amx::algo_build(type, g->g, nullptr, p->output_scale, 0,
+                                           ss_of(p->rays_per_det), ss_of(p->rays_per_pixel_dim));
         *out = new astra_amd_algo_s{a, g->g->device};
     });
 }
@@ -444,7 +443,8 @@ int astra_amd_algo3d_create(int type, astra_amd_geom3d g, const astra_amd_params
         check3d(g, p);
         AMX_REQUIRE(p->kernel == ASTRA_AMD_KER3D_DEFAULT || p->kernel == ASTRA_AMD_KER3D_2D_WEIGHTING,
                     "algo3d: projection kernel %d not usable for SIRT/CGLS", p->kernel);
-        amx::AlgoBase *a = amx::algo_build(type, nullptr, g->g, p->output_scale, p->kernel, 1);
+        amx::AlgoBase *a = amx::algo_build(type, nullptr, g->g, p->output_scale, p->kernel,
+                                           ss_of(p->rays_per_det_dim), ss_of(p->rays_per_voxel_dim));
         *out = new astra_amd_algo_s{a, g->g->device};
     });
 }

@@ -178,30 +178,37 @@ bool prof_read(const char *name, double &ms, double &units, unsigned long long &
 // 3D
 // mode: OUT_ADD / OUT_SET / OUT_RESID (out = (b + scale*Ax) * w); padded_out
 // writes the padded angle-major layout [a][v+4][u+4] consumed by the BP.
+// ss: rays per detector pixel and dimension (FP) / sub-voxels per voxel
+// dimension (BP) -- DetectorSuperSampling / VoxelSuperSampling
 void fp3d_launch(const Geom3D &g, const float *vol, float *proj, float out_scale, int mode,
-                 const float *resid_b, const float *resid_w, bool padded_out, hipStream_t s);
+                 const float *resid_b, const float *resid_w, bool padded_out, hipStream_t s,
+                 unsigned ss = 1);
 // BP from user-layout projections (makes the padded copy internally)
 void bp3d_launch(const Geom3D &g, const float *proj, float *vol, float out_scale, int kernel,
-                 int mode, const float *upd_w, float lo, float hi, hipStream_t s);
+                 int mode, const float *upd_w, float lo, float hi, hipStream_t s,
+                 unsigned ss = 1);
 // BP from an existing padded angle-major copy
 void bp3d_launch_padded(const Geom3D &g, const float *pp, float *vol, float out_scale, int kernel,
-                        int mode, const float *upd_w, float lo, float hi, hipStream_t s);
+                        int mode, const float *upd_w, float lo, float hi, hipStream_t s,
+                        unsigned ss = 1);
 void proj_to_padded_am(const Geom3D &g, const float *proj, float *pp, hipStream_t s);
 size_t padded_am_elems(const Geom3D &g);
 void bp3d_matched_launch(const Geom3D &g, const float *proj, float *vol, float out_scale,
-                         int mode, hipStream_t s);
+                         int mode, hipStream_t s, unsigned ss = 1);
 // 2D
 void fp2d_launch(const Geom2D &g, const float *vol, float *sino, float out_scale, int mode,
                  const float *resid_b, const float *resid_w, bool padded_out, unsigned rays,
                  hipStream_t s);
 void bp2d_launch(const Geom2D &g, const float *sino, float *vol, float out_scale, bool fbp,
-                 int mode, const float *upd_w, float lo, float hi, hipStream_t s);
+                 int mode, const float *upd_w, float lo, float hi, hipStream_t s,
+                 unsigned ss = 1);
 void bp2d_launch_padded(const Geom2D &g, const float *sp, float *vol, float out_scale, bool fbp,
-                        int mode, const float *upd_w, float lo, float hi, hipStream_t s);
+                        int mode, const float *upd_w, float lo, float hi, hipStream_t s,
+                        unsigned ss = 1);
 void sino_to_padded(const Geom2D &g, const float *sino, float *sp, hipStream_t s);
 size_t padded_sino_elems(const Geom2D &g);
 void bp2d_matched_launch(const Geom2D &g, const float *sino, float *vol, float out_scale,
-                         int mode, hipStream_t s);
+                         int mode, hipStream_t s, unsigned rays = 1);
 // filtering
 void filter_rows_launch(const Filter &f, float *rows, size_t n_rows, const float *weights,
                         size_t rows_per_weight, hipStream_t s, const float *weights2 = nullptr,

@@ -97,6 +97,41 @@ __global__ __launch_bounds__(256) void bp2d_kernel(BP2DArgs A) {
     }
 }
 
+// Supersampled BP (devBP_SS / devFanBP_SS, cuda/2d/par_bp.cu:90-137,
+// cuda/2d/fan_bp.cu:106-161): R x R sub-pixels per pixel at offsets
+// (k + 0.5) / R - 0.5, averaged.  KIND 0: parallel (angle weight), 1: fan
+// (weight fr).  One thread per pixel; 64 x 4 pixels per group.
+template <int KIND>
+__global__ __launch_bounds__(256) void bp2d_ss_kernel(BP2DArgs A, int R) {
+    const int X = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int Y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (X >= (int)A.w || Y >= (int)A.h) return;
+    const float sub = 1.0f / (float)R;
+    const float fx0 = X - 0.5f * A.w + 0.5f * sub;
+    const float fy0 = -(Y - 0.5f * A.h + 0.5f * sub);  // world y of the first sub-row
+    const unsigned pitch = A.nd + 2 * kPad;
+    const float hi = (float)A.nd + 2.5f;
+    float acc = 0.f;
+    const float *row = A.sp;
+    for (unsigned a = 0; a < A.na; ++a, row += pitch) {
+        const BP2DAngle c = A.ang[a];
+        for (int i = 0; i < R; ++i) {
+            const float fx = fx0 + i * sub;
+            for (int j = 0; j < R; ++j) {
+                const float fy = fy0 - j * sub;
+                float num = fmaf(fy, c.num[2], fmaf(fx, c.num[1], c.num[0]));
+                float fr = 1.f;
+                if (KIND != 0) fr = __builtin_amdgcn_rcpf(fmaf(fy, c.den[2], fmaf(fx, c.den[1], c.den[0])));
+                float t = clamp_pad(num * fr, hi);
+                float f = floorf(t);
+                float2 q = ld2b(row, (unsigned)f << 2);
+                acc = fmaf((KIND == 0) ? c.w : fr, lerp(q.x, q.y, t - f), acc);
+            }
+        }
+    }
+    store_bp(A.vol, (size_t)Y * A.w + X, acc * A.osc / (float)(R * R), A.mode, A.uw, A.lo, A.hi);
+}
+
 // --------------------------------------------------- matched BP ----------
 // Exact transpose of fp2d_kernel: every sample scatters its two linear
 // weights into the padded slice copies (fp32 atomics).
@@ -188,7 +223,9 @@ void fp2d_launch(const Geom2D &g, const float *vol, float *sino, float out_scale
 }
 
 void bp2d_launch_padded(const Geom2D &g, const float *sp, float *vol, float out_scale, bool fbp,
-                        int mode, const float *upd_w, float lo, float hi, hipStream_t s) {
+                        int mode, const float *upd_w, float lo, float hi, hipStream_t s,
+                        unsigned ss) {
+    AMX_REQUIRE(ss >= 1 && ss <= 16, "bp2d: sub-pixels per dimension must be in [1, 16]");
     BP2DArgs A{};
     A.sp = sp;
     A.w = g.w; A.h = g.h; A.na = g.na; A.nd = g.nd;
@@ -201,7 +238,14 @@ void bp2d_launch_padded(const Geom2D &g, const float *sp, float *vol, float out_
     dim3 grid((g.w + 63) / 64, (g.h + 15) / 16);
     ProfToken pt = prof_begin(g.kind == ASTRA_AMD_GEOM_PAR ? "bp2d_par" : (fbp ? "bp2d_fbp" : "bp2d_fan"),
                               (double)g.w * g.h * g.na, s);
-    if (g.kind == ASTRA_AMD_GEOM_PAR) {
+    if (ss > 1 && !fbp) {  // the FBP-weighted BP has no supersampled form
+        A.ang = g.bp.as<BP2DAngle>();
+        dim3 sgrid((g.w + 63) / 64, (g.h + 3) / 4);
+        if (g.kind == ASTRA_AMD_GEOM_PAR)
+            bp2d_ss_kernel<0><<<sgrid, 256, 0, s>>>(A, (int)ss);
+        else
+            bp2d_ss_kernel<1><<<sgrid, 256, 0, s>>>(A, (int)ss);
+    } else if (g.kind == ASTRA_AMD_GEOM_PAR) {
         A.ang = g.bp.as<BP2DAngle>();
         bp2d_kernel<0><<<grid, 256, 0, s>>>(A);
     } else if (!fbp) {
@@ -216,14 +260,15 @@ void bp2d_launch_padded(const Geom2D &g, const float *sp, float *vol, float out_
 }
 
 void bp2d_launch(const Geom2D &g, const float *sino, float *vol, float out_scale, bool fbp,
-                 int mode, const float *upd_w, float lo, float hi, hipStream_t s) {
+                 int mode, const float *upd_w, float lo, float hi, hipStream_t s, unsigned ss) {
     Scratch sp(padded_sino_elems(g) * sizeof(float), s);
     sino_to_padded(g, sino, sp.as<float>(), s);
-    bp2d_launch_padded(g, sp.as<float>(), vol, out_scale, fbp, mode, upd_w, lo, hi, s);
+    bp2d_launch_padded(g, sp.as<float>(), vol, out_scale, fbp, mode, upd_w, lo, hi, s, ss);
 }
 
 void bp2d_matched_launch(const Geom2D &g, const float *sino, float *vol, float out_scale,
-                         int mode, hipStream_t s) {
+                         int mode, hipStream_t s, unsigned rays) {
+    AMX_REQUIRE(rays >= 1 && rays <= 64, "bp2d: rays per detector must be in [1, 64]");
     const size_t npix = (size_t)g.w * g.h;
     if (mode == OUT_SET) fill_launch(vol, npix, 0.f, s);
     Scratch copies((rows_elems(g) + cols_elems(g)) * sizeof(float), s);
@@ -236,7 +281,7 @@ void bp2d_matched_launch(const Geom2D &g, const float *sino, float *vol, float o
     A.w = g.w; A.h = g.h; A.na = g.na; A.nd = g.nd;
     A.ang = g.fp.as<FP2DAngle>();
     A.osc = out_scale;
-    A.rays = 1;
+    A.rays = rays;
     dim3 grid((g.nd + 63) / 64, (g.na + 3) / 4);
     if (g.kind == ASTRA_AMD_GEOM_FAN)
         fp2d_transpose_kernel<true><<<grid, 256, 0, s>>>(A, sino);

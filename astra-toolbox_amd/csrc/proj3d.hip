@@ -113,6 +113,7 @@ struct FPArgs {
     // workgroup map (fp_map): detector tiles, angles per block, and the
     // range [lfirst, lfirst + lcount) of alist covered by this launch
     unsigned tiles, ablk, lfirst, lcount;
+    unsigned ss;  // rays per detector pixel and dimension (DetectorSuperSampling)
 };
 
 // Workgroup -> (detector tile, angle) map.  Workgroups are handed out
@@ -142,12 +143,14 @@ __device__ __forceinline__ bool fp_map(const FPArgs &A, unsigned &tile, int &ang
 // the slice loop is wave-uniform over the union of the lanes' clipped slice
 // ranges, so the slice base address is scalar and every fetch is one
 // dwordx2 load with a 32-bit vector offset.
+// (u, v): detector coordinates of the ray relative to pixel (0, 0)'s centre
+// (fractional for sub-rays).
 template <bool CONE>
-__device__ __forceinline__ void fp3d_ray(const FPArgs &A, const FP3DAngle &G, int u, int v,
+__device__ __forceinline__ void fp3d_ray(const FPArgs &A, const FP3DAngle &G, double u, double v,
                                          float &a1f, float &a2f, float &c1f, float &c2f,
                                          float &scale, int &lo, int &hi) {
     double P[3], d[3];
-    for (int k = 0; k < 3; ++k) P[k] = G.p0[k] + (double)u * G.u[k] + (double)v * G.v[k];
+    for (int k = 0; k < 3; ++k) P[k] = G.p0[k] + u * G.u[k] + v * G.v[k];
     for (int k = 0; k < 3; ++k) d[k] = CONE ? (G.s[k] - P[k]) : G.s[k];
     const double *B = CONE ? G.s : P;
     double a1 = d[1] / d[0], a2 = d[2] / d[0];
@@ -166,25 +169,10 @@ __device__ __forceinline__ void fp3d_ray(const FPArgs &A, const FP3DAngle &G, in
     scale = (float)(sqrt(a1 * a1 * A.fs1 + a2 * a2 * A.fs2 + 1.0) * A.osc);
 }
 
-template <bool CONE>
-__global__ __launch_bounds__(256) void fp3d_kernel(FPArgs A) {
-    unsigned tile;
-    int angle;
-    if (!fp_map(A, tile, angle)) return;  // uniform over the workgroup
-    const unsigned tiles_u = (A.nu + 15) / 16;
-    const unsigned tu = tile % tiles_u, tv = tile / tiles_u;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int u = tu * 16 + (lane & 15);
-    const int v = tv * 16 + wave * 4 + (lane >> 4);
-    const FP3DAngle &G = A.ang[angle];
-    const bool active = u < (int)A.nu && v < (int)A.nv;
-
-    float a1f = 0.f, a2f = 0.f, c1f = 0.5f, c2f = 0.5f, scale = 0.f;
-    int lo = (int)A.n0, hi = -1;
-    if (active) fp3d_ray<CONE>(A, G, u, v, a1f, a2f, c1f, c2f, scale, lo, hi);
-    const int slo = __builtin_amdgcn_readfirstlane(wave_min(lo));
-    const int shi = __builtin_amdgcn_readfirstlane(wave_max(hi));
-
+// Sum of the bilinear samples of slices [slo, shi] (wave-uniform) along the
+// ray (c1, c2) = a s + c through the padded group copy.
+__device__ __forceinline__ float march3d(const FPArgs &A, float a1f, float a2f, float c1f,
+                                         float c2f, int slo, int shi) {
     const unsigned pitch = A.n1 + 2 * kPad;
     const size_t plane = (size_t)(A.n2 + 2 * kPad) * pitch;
     const float pitchf = (float)pitch;
@@ -218,8 +206,50 @@ __global__ __launch_bounds__(256) void fp3d_kernel(FPArgs A) {
         float2 r0 = ld2b(pl, op), r1 = ld2b(pl, op + rowb);
         acc0 += lerp(lerp(r0.x, r0.y, p1 - fp1), lerp(r1.x, r1.y, p1 - fp1), p2 - fp2);
     }
+    return acc0 + acc1;
+}
+
+// SS: A.ss x A.ss sub-rays per detector pixel at offsets (k + 0.5) / ss - 0.5
+// in u and v, each with its own distance weight, averaged
+// (cone_FP_SS_t / par3D_FP_SS_t, cuda/3d/cone_fp.cu:233-323,
+// cuda/3d/par3d_fp.cu:234-325).
+template <bool CONE, bool SS>
+__global__ __launch_bounds__(256) void fp3d_kernel(FPArgs A) {
+    unsigned tile;
+    int angle;
+    if (!fp_map(A, tile, angle)) return;  // uniform over the workgroup
+    const unsigned tiles_u = (A.nu + 15) / 16;
+    const unsigned tu = tile % tiles_u, tv = tile / tiles_u;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int u = tu * 16 + (lane & 15);
+    const int v = tv * 16 + wave * 4 + (lane >> 4);
+    const FP3DAngle &G = A.ang[angle];
+    const bool active = u < (int)A.nu && v < (int)A.nv;
+
+    float val = 0.f;
+    if (!SS) {
+        float a1f = 0.f, a2f = 0.f, c1f = 0.5f, c2f = 0.5f, scale = 0.f;
+        int lo = (int)A.n0, hi = -1;
+        if (active) fp3d_ray<CONE>(A, G, u, v, a1f, a2f, c1f, c2f, scale, lo, hi);
+        const int slo = __builtin_amdgcn_readfirstlane(wave_min(lo));
+        const int shi = __builtin_amdgcn_readfirstlane(wave_max(hi));
+        val = march3d(A, a1f, a2f, c1f, c2f, slo, shi) * scale;
+    } else {
+        const int R = (int)A.ss;
+        for (int k = 0; k < R; ++k)
+            for (int l = 0; l < R; ++l) {
+                float a1f = 0.f, a2f = 0.f, c1f = 0.5f, c2f = 0.5f, scale = 0.f;
+                int lo = (int)A.n0, hi = -1;
+                if (active)
+                    fp3d_ray<CONE>(A, G, u - 0.5 + (k + 0.5) / R, v - 0.5 + (l + 0.5) / R, a1f,
+                                   a2f, c1f, c2f, scale, lo, hi);
+                const int slo = __builtin_amdgcn_readfirstlane(wave_min(lo));
+                const int shi = __builtin_amdgcn_readfirstlane(wave_max(hi));
+                val += march3d(A, a1f, a2f, c1f, c2f, slo, shi) * scale;
+            }
+        val /= (float)(R * R);
+    }
     if (!active) return;
-    const float val = (acc0 + acc1) * scale;
     size_t idx = ((size_t)v * A.na + angle) * A.nu + u;
     size_t oidx = A.padded_out
                       ? ((size_t)angle * (A.nv + 2 * kPad) + (v + kPad)) * (A.nu + 2 * kPad) + u + kPad
@@ -341,6 +371,66 @@ __global__ __launch_bounds__(256) void bp3d_kernel(BPArgs A) {
     }
 }
 
+// Supersampled BP (dev_cone_BP_SS / dev_par3D_BP_SS,
+// cuda/3d/cone_bp.cu:138-217, cuda/3d/par3d_bp.cu:123-198): R^3 sub-voxels
+// per voxel at offsets (k + 0.5) / R - 0.5, averaged.  KIND 0: cone, weight
+// fr^2; KIND 2: par3d, the angle's weight.  One thread per voxel.
+template <int KIND>
+__global__ __launch_bounds__(256) void bp3d_ss_kernel(BPArgs A, int R) {
+    const unsigned tiles_x = (A.nx + 15) / 16;
+    const int X = (blockIdx.x % tiles_x) * 16 + (threadIdx.x & 15);
+    const int Y = (blockIdx.x / tiles_x) * 16 + (threadIdx.x >> 4);
+    const int Z = blockIdx.y;
+    if (X >= (int)A.nx || Y >= (int)A.ny) return;
+    const float sub = 1.0f / (float)R;
+    const float fx0 = X - 0.5f * A.nx + 0.5f * sub;
+    const float fy0 = Y - 0.5f * A.ny + 0.5f * sub;
+    const float fz0 = Z - 0.5f * A.nz + 0.5f * sub;
+    const unsigned pitch = A.nu + 2 * kPad;
+    const size_t plane = (size_t)(A.nv + 2 * kPad) * pitch;
+    const float pitchf = (float)pitch;
+    const unsigned rowb = pitch * 4u;
+    const float hiu = (float)A.nu + 2.5f, hiv = (float)A.nv + 2.5f;
+    float acc = 0.f;
+    const float *pl = A.pp;
+    for (unsigned a = 0; a < A.na; ++a, pl += plane) {
+        const BP3DAngle c = A.ang[a];
+        const float wa = (KIND == 2) ? A.aw[a] : 1.f;
+        for (int i = 0; i < R; ++i) {
+            const float fx = fx0 + i * sub;
+            for (int j = 0; j < R; ++j) {
+                const float fy = fy0 + j * sub;
+                for (int k = 0; k < R; ++k) {
+                    const float fz = fz0 + k * sub;
+                    float un = fmaf(fz, c.nu[2], fmaf(fy, c.nu[1], fmaf(fx, c.nu[0], c.nu[3])));
+                    float vn = fmaf(fz, c.nv[2], fmaf(fy, c.nv[1], fmaf(fx, c.nv[0], c.nv[3])));
+                    float fr = 1.f;
+                    if (KIND != 2) {
+                        float dn = fmaf(fz, c.dn[2], fmaf(fy, c.dn[1], fmaf(fx, c.dn[0], c.dn[3])));
+                        fr = __builtin_amdgcn_rcpf(dn);
+                    }
+                    float pu = clamp_pad(un * fr, hiu), pv = clamp_pad(vn * fr, hiv);
+                    float fu = floorf(pu), fv = floorf(pv);
+                    unsigned o = texel_off(fu, fv, pitchf);
+                    float2 r0 = ld2b(pl, o), r1 = ld2b(pl, o + rowb);
+                    float val = lerp(lerp(r0.x, r0.y, pu - fu), lerp(r1.x, r1.y, pu - fu), pv - fv);
+                    acc = fmaf((KIND == 2) ? wa : fr * fr, val, acc);
+                }
+            }
+        }
+    }
+    const size_t idx = ((size_t)Z * A.ny + Y) * A.nx + X;
+    const float val = acc * A.osc / (float)(R * R * R);
+    if (A.mode == OUT_ADD) {
+        A.vol[idx] += val;
+    } else if (A.mode == OUT_SET) {
+        A.vol[idx] = val;
+    } else {
+        float w = A.uw ? A.uw[idx] : 1.f;
+        A.vol[idx] = fminf(fmaxf(fmaf(w, val, A.vol[idx]), A.lo), A.hi);
+    }
+}
+
 // --------------------------------------------------- matched BP ----------
 // Exact transpose of fp3d_kernel: every (ray, slice) sample scatters its
 // four bilinear weights into the padded slice-major accumulation copy of
@@ -357,28 +447,37 @@ __global__ __launch_bounds__(256) void fp3d_transpose_kernel(FPArgs A, const flo
     const int v = tv * 16 + wave * 4 + (lane >> 4);
     const FP3DAngle &G = A.ang[angle];
     if (u >= (int)A.nu || v >= (int)A.nv) return;
-    float a1f, a2f, c1f, c2f, scale;
-    int lo, hi;
-    fp3d_ray<CONE>(A, G, u, v, a1f, a2f, c1f, c2f, scale, lo, hi);
-    const float y = proj[((size_t)v * A.na + angle) * A.nu + u] * scale;
-    if (y == 0.f) return;
+    const float y0 = proj[((size_t)v * A.na + angle) * A.nu + u];
+    if (y0 == 0.f) return;
     const unsigned pitch = A.n1 + 2 * kPad;
     const size_t plane = (size_t)(A.n2 + 2 * kPad) * pitch;
     const float pitchf = (float)pitch;
     const float hi1 = (float)A.n1 + 2.5f, hi2 = (float)A.n2 + 2.5f;
     float *acc = const_cast<float *>(A.volp);
-    for (int s = lo; s <= hi; ++s) {
-        const float sf = (float)s;
-        float p1 = clamp_pad(fmaf(a1f, sf, c1f), hi1);
-        float p2 = clamp_pad(fmaf(a2f, sf, c2f), hi2);
-        float f1 = floorf(p1), f2 = floorf(p2);
-        float w1 = p1 - f1, w2 = p2 - f2;
-        float *q = acc + (size_t)s * plane + (texel_off(f1, f2, pitchf) >> 2);
-        atomicAdd(q, y * (1.f - w1) * (1.f - w2));
-        atomicAdd(q + 1, y * w1 * (1.f - w2));
-        atomicAdd(q + pitch, y * (1.f - w1) * w2);
-        atomicAdd(q + pitch + 1, y * w1 * w2);
-    }
+    const int R = (int)A.ss;
+    for (int k = 0; k < R; ++k)
+        for (int l = 0; l < R; ++l) {
+            float a1f, a2f, c1f, c2f, scale;
+            int lo, hi;
+            if (R == 1)
+                fp3d_ray<CONE>(A, G, u, v, a1f, a2f, c1f, c2f, scale, lo, hi);
+            else
+                fp3d_ray<CONE>(A, G, u - 0.5 + (k + 0.5) / R, v - 0.5 + (l + 0.5) / R, a1f, a2f,
+                               c1f, c2f, scale, lo, hi);
+            const float y = y0 * scale / (float)(R * R);
+            for (int s = lo; s <= hi; ++s) {
+                const float sf = (float)s;
+                float p1 = clamp_pad(fmaf(a1f, sf, c1f), hi1);
+                float p2 = clamp_pad(fmaf(a2f, sf, c2f), hi2);
+                float f1 = floorf(p1), f2 = floorf(p2);
+                float w1 = p1 - f1, w2 = p2 - f2;
+                float *q = acc + (size_t)s * plane + (texel_off(f1, f2, pitchf) >> 2);
+                atomicAdd(q, y * (1.f - w1) * (1.f - w2));
+                atomicAdd(q + 1, y * w1 * (1.f - w2));
+                atomicAdd(q + pitch, y * (1.f - w1) * w2);
+                atomicAdd(q + pitch + 1, y * w1 * w2);
+            }
+        }
 }
 
 // vol[z][y][x] (+)= interior of a padded slice-major group copy
@@ -508,7 +607,9 @@ FPArgs fp_args(const Geom3D &g, int grp, float out_scale) {
 }  // namespace
 
 void fp3d_launch(const Geom3D &g, const float *vol, float *proj, float out_scale, int mode,
-                 const float *resid_b, const float *resid_w, bool padded_out, hipStream_t s) {
+                 const float *resid_b, const float *resid_w, bool padded_out, hipStream_t s,
+                 unsigned ss) {
+    AMX_REQUIRE(ss >= 1 && ss <= 16, "fp3d: rays per detector dimension must be in [1, 16]");
     size_t need = 0;
     int groups[3], ng = 0;
     for (int k = 0; k < 3; ++k)
@@ -530,13 +631,21 @@ void fp3d_launch(const Geom3D &g, const float *vol, float *proj, float out_scale
         // is written by exactly one group, which applies the full mode.
         A.mode = mode;
         ProfToken pt = prof_begin(g.kind == ASTRA_AMD_GEOM_CONE ? "fp3d_cone" : "fp3d_par3d",
-                                  (double)g.dir_count[k] * g.nu * g.nv * A.n0, s);
+                                  (double)g.dir_count[k] * g.nu * g.nv * A.n0 * ss * ss, s);
         const bool cone = g.kind == ASTRA_AMD_GEOM_CONE;
+        A.ss = ss;
         fp_group_launches(A, g.dir_count[k], [&](const FPArgs &a, unsigned groups) {
-            if (cone)
-                fp3d_kernel<true><<<groups, 256, 0, s>>>(a);
-            else
-                fp3d_kernel<false><<<groups, 256, 0, s>>>(a);
+            if (ss > 1) {
+                if (cone)
+                    fp3d_kernel<true, true><<<groups, 256, 0, s>>>(a);
+                else
+                    fp3d_kernel<false, true><<<groups, 256, 0, s>>>(a);
+            } else {
+                if (cone)
+                    fp3d_kernel<true, false><<<groups, 256, 0, s>>>(a);
+                else
+                    fp3d_kernel<false, false><<<groups, 256, 0, s>>>(a);
+            }
         }, s);
         prof_end(pt, s);
     }
@@ -553,7 +662,9 @@ size_t padded_am_elems(const Geom3D &g) {
 }
 
 void bp3d_launch_padded(const Geom3D &g, const float *pp, float *vol, float out_scale, int kernel,
-                        int mode, const float *upd_w, float lo, float hi, hipStream_t s) {
+                        int mode, const float *upd_w, float lo, float hi, hipStream_t s,
+                        unsigned ss) {
+    AMX_REQUIRE(ss >= 1 && ss <= 16, "bp3d: rays per voxel dimension must be in [1, 16]");
     BPArgs A{};
     A.pp = pp;
     A.nu = g.nu; A.nv = g.nv; A.na = g.na;
@@ -604,7 +715,17 @@ void bp3d_launch_padded(const Geom3D &g, const float *pp, float *vol, float out_
         kind = 2;
     }
     if (zc_disabled()) zc = false;
-    if (kind == 0)
+    // supersampling applies to the default cone kernel and to par3d, as in
+    // ConeBP_Array / Par3DBP_Array (cuda/3d/cone_bp.cu:351-370,
+    // cuda/3d/par3d_bp.cu:285-293)
+    const bool use_ss = ss > 1 && (g.kind != ASTRA_AMD_GEOM_CONE || kernel == ASTRA_AMD_KER3D_DEFAULT);
+    if (use_ss) {
+        dim3 sgrid(((g.nx + 15) / 16) * ((g.ny + 15) / 16), g.nz);
+        if (kind == 0)
+            bp3d_ss_kernel<0><<<sgrid, 256, 0, s>>>(A, (int)ss);
+        else
+            bp3d_ss_kernel<2><<<sgrid, 256, 0, s>>>(A, (int)ss);
+    } else if (kind == 0)
         zc ? bp3d_kernel<0, ZS, true><<<grid, 256, 0, s>>>(A) : bp3d_kernel<0, ZS, false><<<grid, 256, 0, s>>>(A);
     else if (kind == 1)
         zc ? bp3d_kernel<1, ZS, true><<<grid, 256, 0, s>>>(A) : bp3d_kernel<1, ZS, false><<<grid, 256, 0, s>>>(A);
@@ -615,14 +736,15 @@ void bp3d_launch_padded(const Geom3D &g, const float *pp, float *vol, float out_
 }
 
 void bp3d_launch(const Geom3D &g, const float *proj, float *vol, float out_scale, int kernel,
-                 int mode, const float *upd_w, float lo, float hi, hipStream_t s) {
+                 int mode, const float *upd_w, float lo, float hi, hipStream_t s, unsigned ss) {
     Scratch pp(padded_am_elems(g) * sizeof(float), s);
     proj_to_padded_am(g, proj, pp.as<float>(), s);
-    bp3d_launch_padded(g, pp.as<float>(), vol, out_scale, kernel, mode, upd_w, lo, hi, s);
+    bp3d_launch_padded(g, pp.as<float>(), vol, out_scale, kernel, mode, upd_w, lo, hi, s, ss);
 }
 
 void bp3d_matched_launch(const Geom3D &g, const float *proj, float *vol, float out_scale,
-                         int mode, hipStream_t s) {
+                         int mode, hipStream_t s, unsigned ss) {
+    AMX_REQUIRE(ss >= 1 && ss <= 16, "bp3d: rays per detector dimension must be in [1, 16]");
     const size_t nvox = (size_t)g.nx * g.ny * g.nz;
     if (mode == OUT_SET) fill_launch(vol, nvox, 0.f, s);
     size_t need = 0;
@@ -634,6 +756,7 @@ void bp3d_matched_launch(const Geom3D &g, const float *proj, float *vol, float o
         fill_launch(acc.as<float>(), group_elems(g, k), 0.f, s);
         FPArgs A = fp_args(g, k, out_scale);
         A.volp = acc.as<float>();
+        A.ss = ss;
         const bool cone = g.kind == ASTRA_AMD_GEOM_CONE;
         fp_group_launches(A, g.dir_count[k], [&](const FPArgs &a, unsigned groups) {
             if (cone)

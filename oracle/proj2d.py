@@ -110,14 +110,18 @@ def par_parameters(vec, n_det):
     return angle, det_size, offset
 
 
-def par_fp(vol, vecs, n_det, out_scale=1.0):
+def par_fp(vol, vecs, n_det, out_scale=1.0, rays=1):
     """2D parallel Joseph FP.
 
     ``FP_simple_internal`` + ``FPhorizontal_simple`` / ``FPvertical_simple``
     (cuda/2d/par_fp.cu:53-125, 130-200, 233-301).  Orientation per angle:
     vertical iff |rayX| <= |rayY| (par_fp.cu:272).  The oracle uses exact
     cos/sin of the recovered angle (the reference uses fast intrinsics).
+    rays > 1: sub-rays at detector offsets (k + 0.5) / rays - 0.5, averaged
+    (par_fp.cu:94-111).
     """
+    if rays > 1:
+        return sum(par_fp(vol, _shift_det(vecs, o), n_det, out_scale) for o in _sub_det(rays)) / rays
     vol = np.asarray(vol, dtype=np.float64)
     H, W = vol.shape
     nA = vecs.shape[0]
@@ -148,6 +152,19 @@ def par_fp(vol, vecs, n_det, out_scale=1.0):
     return sino
 
 
+def _sub_det(rays):
+    """Sub-ray detector offsets relative to the pixel centre."""
+    return (np.arange(rays) + 0.5) / rays - 0.5
+
+
+def _shift_det(vecs, o):
+    """Vectors with the detector moved by o pixels along U (the parallel
+    orientation test uses only the ray direction, so it is unchanged)."""
+    v = np.array(vecs, dtype=np.float64)
+    v[:, 2:4] += o * v[:, 4:6]
+    return v
+
+
 def par_fp_T(sino, vecs, shape, out_scale=1.0):
     """Exact transpose of ``par_fp`` (the matched adjoint: the same samples
     and weights, scattered)."""
@@ -173,28 +190,31 @@ def par_fp_T(sino, vecs, shape, out_scale=1.0):
     return vol
 
 
-def par_bp(sino, vecs, shape, out_scale=1.0):
+def par_bp(sino, vecs, shape, out_scale=1.0, ss=1):
     """2D parallel voxel-driven BP.
 
     ``devBP`` + ``transferConstants`` (cuda/2d/par_bp.cu:52-87, 163-191).
-    T = x c - y_row s + off, weight |r|/|d| per angle.
+    T = x c - y_row s + off, weight |r|/|d| per angle.  ss > 1: ss x ss
+    sub-pixels at offsets (k + 0.5) / ss, averaged (``devBP_SS``, :90-137).
     """
     sino = np.asarray(sino, dtype=np.float64)
     H, W = shape
-    X = np.arange(W) - 0.5 * W + 0.5
-    Y = np.arange(H) - 0.5 * H + 0.5
-    fx, fy = np.meshgrid(X, Y)
     vol = np.zeros((H, W))
-    for a in range(vecs.shape[0]):
-        rx, ry, dsx, dsy, ux, uy = (float(x) for x in vecs[a])
-        d = ux * ry - uy * rx
-        cs = ry / d
-        sn = -rx / d
-        off = (dsy * rx - dsx * ry) / d
-        scale = math.sqrt(rx * rx + ry * ry) / abs(d)
-        T = fx * cs - fy * sn + off
-        vol += tex1d(sino[a], T) * scale
-    return vol * out_scale
+    for ox in (np.arange(ss) + 0.5) / ss:
+        for oy in (np.arange(ss) + 0.5) / ss:
+            X = np.arange(W) - 0.5 * W + ox
+            Y = np.arange(H) - 0.5 * H + oy
+            fx, fy = np.meshgrid(X, Y)
+            for a in range(vecs.shape[0]):
+                rx, ry, dsx, dsy, ux, uy = (float(x) for x in vecs[a])
+                d = ux * ry - uy * rx
+                cs = ry / d
+                sn = -rx / d
+                off = (dsy * rx - dsx * ry) / d
+                scale = math.sqrt(rx * rx + ry * ry) / abs(d)
+                T = fx * cs - fy * sn + off
+                vol += tex1d(sino[a], T) * scale
+    return vol * out_scale / (ss * ss)
 
 
 def _fma32(a, b, c):
@@ -216,13 +236,21 @@ def fan_ray_is_horizontal(vec, det):
     return not (fdy > fdx)
 
 
-def fan_fp(vol, vecs, n_det, out_scale=1.0):
+def fan_fp(vol, vecs, n_det, out_scale=1.0, rays=1):
     """2D fan-beam Joseph FP.
 
-    ``FanFPhorizontal`` / ``FanFPvertical`` (cuda/2d/fan_fp.cu:55-193) with
-    one ray per detector.  Horizontal rays sample columns (y = alpha x + beta),
-    vertical rays sample rows (x = alpha y + beta).
+    ``FanFPhorizontal`` / ``FanFPvertical`` (cuda/2d/fan_fp.cu:55-193).
+    Horizontal rays sample columns (y = alpha x + beta), vertical rays sample
+    rows (x = alpha y + beta).  rays > 1: sub-rays through det + (k + 0.5) /
+    rays, averaged; the orientation is decided once per detector
+    (fan_fp.cu:79-96).
     """
+    if rays > 1:
+        return sum(_fan_fp_one(vol, vecs, n_det, out_scale, o) for o in (np.arange(rays) + 0.5) / rays) / rays
+    return _fan_fp_one(vol, vecs, n_det, out_scale, 0.5)
+
+
+def _fan_fp_one(vol, vecs, n_det, out_scale, off):
     vol = np.asarray(vol, dtype=np.float64)
     H, W = vol.shape
     nA = vecs.shape[0]
@@ -233,7 +261,7 @@ def fan_fp(vol, vecs, n_det, out_scale=1.0):
     for a in range(nA):
         sx, sy, dsx, dsy, ux, uy = (float(x) for x in vecs[a])
         hor = np.array([fan_ray_is_horizontal(vecs[a], t) for t in tdet])
-        fdet = tdet + 0.5
+        fdet = tdet + off
         dx = dsx + fdet * ux
         dy = dsy + fdet * uy
         if hor.any():
@@ -310,22 +338,27 @@ def fan_bp_params(vec, fbp=False):
     return num_c, num_x, num_y, den_c, den_x, den_y
 
 
-def fan_bp(sino, vecs, shape, out_scale=1.0, fbp_weighted=False):
+def fan_bp(sino, vecs, shape, out_scale=1.0, fbp_weighted=False, ss=1):
     """2D fan-beam voxel-driven BP: ``devFanBP<FBPWEIGHT>``
     (cuda/2d/fan_bp.cu:57-103).  Weight fr (BP) or fr^2 with DenC = 1 (FBP).
+    ss > 1 (BP only): ss x ss sub-pixels, averaged (``devFanBP_SS``, :106-161).
     """
     sino = np.asarray(sino, dtype=np.float64)
     H, W = shape
-    X = np.arange(W) - 0.5 * W + 0.5
-    Y = -(np.arange(H) - 0.5 * H + 0.5)
-    fx, fy = np.meshgrid(X, Y)
+    if fbp_weighted:
+        ss = 1
     vol = np.zeros((H, W))
-    for a in range(vecs.shape[0]):
-        nc, nx, ny, dc, dx, dy = fan_bp_params(vecs[a], fbp_weighted)
-        num = nc + nx * fx + ny * fy
-        den = (1.0 if fbp_weighted else dc) + dx * fx + dy * fy
-        fr = 1.0 / den
-        T = num * fr
-        w = fr * fr if fbp_weighted else fr
-        vol += tex1d(sino[a], T) * w
-    return vol * out_scale
+    for ox in (np.arange(ss) + 0.5) / ss:
+        for oy in (np.arange(ss) + 0.5) / ss:
+            X = np.arange(W) - 0.5 * W + ox
+            Y = -(np.arange(H) - 0.5 * H + oy)
+            fx, fy = np.meshgrid(X, Y)
+            for a in range(vecs.shape[0]):
+                nc, nx, ny, dc, dx, dy = fan_bp_params(vecs[a], fbp_weighted)
+                num = nc + nx * fx + ny * fy
+                den = (1.0 if fbp_weighted else dc) + dx * fx + dy * fy
+                fr = 1.0 / den
+                T = num * fr
+                w = fr * fr if fbp_weighted else fr
+                vol += tex1d(sino[a], T) * w
+    return vol * out_scale / (ss * ss)

@@ -91,53 +91,67 @@ def _ray_sum(vol, d, a1, a2, b1, b2):
     return acc
 
 
-def cone_fp(vol, vecs, n_u, n_v, vol_scale=(1.0, 1.0, 1.0), out_scale=1.0):
+def _sub_offsets(ss):
+    """Sub-ray / sub-voxel offsets (k + 0.5) / ss within a unit cell."""
+    return (np.arange(ss) + 0.5) / ss
+
+
+def cone_fp(vol, vecs, n_u, n_v, vol_scale=(1.0, 1.0, 1.0), out_scale=1.0, ss=1):
     """Cone-beam ray-driven FP: ``cone_FP_t`` (cuda/3d/cone_fp.cu:154-231).
 
     Ray to the centre of detector pixel (u, v) (DetS + (u+.5)U + (v+.5)V),
     written as (c1, c2) = a * c0 + b along the principal axis, sampled at
     every slice centre; the sum is scaled by sqrt(a1^2 s1 + a2^2 s2 + 1).
+    ss > 1: ss x ss sub-rays through DetS + (u + (k+.5)/ss) U + (v + (l+.5)/ss) V,
+    averaged (``cone_FP_SS_t``, cuda/3d/cone_fp.cu:233-323).
     """
     vol = np.asarray(vol, dtype=np.float64)
     nA = vecs.shape[0]
     proj = np.zeros((n_v, nA, n_u))
-    uu, vv = np.meshgrid(np.arange(n_u) + 0.5, np.arange(n_v) + 0.5)
     for a in range(nA):
         vec = vecs[a]
         d = cone_direction(vec, n_u, n_v)
         c0, c1, c2 = _DIRS[d]
         src = vec[0:3]
-        det = [vec[3 + k] + uu * vec[6 + k] + vv * vec[9 + k] for k in range(3)]
-        den = src[c0] - det[c0]
-        a1 = (src[c1] - det[c1]) / den
-        a2 = (src[c2] - det[c2]) / den
-        b1 = src[c1] - a1 * src[c0]
-        b2 = src[c2] - a2 * src[c0]
         s1, s2, osc = _fp_scale(d, vol_scale, out_scale)
-        proj[:, a, :] = _ray_sum(vol, d, a1, a2, b1, b2) * np.sqrt(a1 * a1 * s1 + a2 * a2 * s2 + 1.0) * osc
+        acc = np.zeros((n_v, n_u))
+        for ou in _sub_offsets(ss):
+            for ov in _sub_offsets(ss):
+                uu, vv = np.meshgrid(np.arange(n_u) + ou, np.arange(n_v) + ov)
+                det = [vec[3 + k] + uu * vec[6 + k] + vv * vec[9 + k] for k in range(3)]
+                den = src[c0] - det[c0]
+                a1 = (src[c1] - det[c1]) / den
+                a2 = (src[c2] - det[c2]) / den
+                b1 = src[c1] - a1 * src[c0]
+                b2 = src[c2] - a2 * src[c0]
+                acc += _ray_sum(vol, d, a1, a2, b1, b2) * np.sqrt(a1 * a1 * s1 + a2 * a2 * s2 + 1.0)
+        proj[:, a, :] = acc * osc / (ss * ss)
     return proj
 
 
-def par3d_fp(vol, vecs, n_u, n_v, vol_scale=(1.0, 1.0, 1.0), out_scale=1.0):
-    """Parallel-beam 3D FP: ``par3D_FP_t`` (cuda/3d/par3d_fp.cu:152-230)."""
+def par3d_fp(vol, vecs, n_u, n_v, vol_scale=(1.0, 1.0, 1.0), out_scale=1.0, ss=1):
+    """Parallel-beam 3D FP: ``par3D_FP_t`` (cuda/3d/par3d_fp.cu:152-230);
+    ss > 1 as in ``par3D_FP_SS_t`` (:234-325)."""
     vol = np.asarray(vol, dtype=np.float64)
     nA = vecs.shape[0]
     proj = np.zeros((n_v, nA, n_u))
-    uu, vv = np.meshgrid(np.arange(n_u) + 0.5, np.arange(n_v) + 0.5)
     for a in range(nA):
         vec = vecs[a]
         d = par3d_direction(vec)
         c0, c1, c2 = _DIRS[d]
         ray = vec[0:3]
-        det = [vec[3 + k] + uu * vec[6 + k] + vv * vec[9 + k] for k in range(3)]
         a1 = ray[c1] / ray[c0]
         a2 = ray[c2] / ray[c0]
-        b1 = det[c1] - a1 * det[c0]
-        b2 = det[c2] - a2 * det[c0]
         s1, s2, osc = _fp_scale(d, vol_scale, out_scale, par3d=True)
-        a1f = np.full_like(b1, a1)
-        a2f = np.full_like(b2, a2)
-        proj[:, a, :] = _ray_sum(vol, d, a1f, a2f, b1, b2) * math.sqrt(a1 * a1 * s1 + a2 * a2 * s2 + 1.0) * osc
+        acc = np.zeros((n_v, n_u))
+        for ou in _sub_offsets(ss):
+            for ov in _sub_offsets(ss):
+                uu, vv = np.meshgrid(np.arange(n_u) + ou, np.arange(n_v) + ov)
+                det = [vec[3 + k] + uu * vec[6 + k] + vv * vec[9 + k] for k in range(3)]
+                b1 = det[c1] - a1 * det[c0]
+                b2 = det[c2] - a2 * det[c0]
+                acc += _ray_sum(vol, d, np.full_like(b1, a1), np.full_like(b2, a2), b1, b2)
+        proj[:, a, :] = acc * math.sqrt(a1 * a1 * s1 + a2 * a2 * s2 + 1.0) * osc / (ss * ss)
     return proj
 
 
@@ -247,30 +261,41 @@ def cone_bp_params(vec, kernel, vol_scale):
     return num_u, num_v, den
 
 
-def _voxel_grid(shape):
+def _voxel_grid(shape, off=(0.5, 0.5, 0.5)):
+    """Voxel (sub-)positions: X - nx/2 + off_x etc. (off = 0.5: centres)."""
     nz, ny, nx = shape
-    X = np.arange(nx) - 0.5 * nx + 0.5
-    Y = np.arange(ny) - 0.5 * ny + 0.5
-    Z = np.arange(nz) - 0.5 * nz + 0.5
+    X = np.arange(nx) - 0.5 * nx + off[0]
+    Y = np.arange(ny) - 0.5 * ny + off[1]
+    Z = np.arange(nz) - 0.5 * nz + off[2]
     return np.meshgrid(Z, Y, X, indexing="ij")
 
 
-def cone_bp(proj, vecs, shape, vol_scale=(1.0, 1.0, 1.0), out_scale=1.0, kernel="default"):
+def _sub_voxels(ss):
+    o = _sub_offsets(ss)
+    return [(a, b, c) for a in o for b in o for c in o]
+
+
+def cone_bp(proj, vecs, shape, vol_scale=(1.0, 1.0, 1.0), out_scale=1.0, kernel="default", ss=1):
     """Cone-beam voxel-driven BP: ``dev_cone_BP<KERNEL>``
     (cuda/3d/cone_bp.cu:55-133) and its output scale (``ConeBP_Array``,
     :322-328).  kernel: 'default' (weight fr^2), 'fdk' (Den.w = 1, fr^2,
-    output / voxel x), '2d' (weight fr)."""
+    output / voxel x), '2d' (weight fr).  ss > 1 (default kernel only, as
+    ``ConeBP_Array`` dispatches, :351-370): ss^3 sub-voxels at offsets
+    (k + 0.5) / ss, averaged (``dev_cone_BP_SS``, :138-217)."""
     proj = np.asarray(proj, dtype=np.float64)
-    fz, fy, fx = _voxel_grid(shape)
+    if kernel != "default":
+        ss = 1
     vol = np.zeros(shape)
     for a in range(vecs.shape[0]):
         nu, nv, dn = cone_bp_params(vecs[a], kernel, vol_scale)
-        U = nu[3] + fx * nu[0] + fy * nu[1] + fz * nu[2]
-        V = nv[3] + fx * nv[0] + fy * nv[1] + fz * nv[2]
-        D = (1.0 if kernel == "fdk" else dn[3]) + fx * dn[0] + fy * dn[1] + fz * dn[2]
-        fr = 1.0 / D
-        val = tex2d(proj[:, a, :], U * fr, V * fr)
-        vol += (fr if kernel == "2d" else fr * fr) * val
+        for off in _sub_voxels(ss):
+            fz, fy, fx = _voxel_grid(shape, off)
+            U = nu[3] + fx * nu[0] + fy * nu[1] + fz * nu[2]
+            V = nv[3] + fx * nv[0] + fy * nv[1] + fz * nv[2]
+            D = (1.0 if kernel == "fdk" else dn[3]) + fx * dn[0] + fy * dn[1] + fz * dn[2]
+            fr = 1.0 / D
+            val = tex2d(proj[:, a, :], U * fr, V * fr)
+            vol += (fr if kernel == "2d" else fr * fr) * val / ss ** 3
     vx, vy, vz = (f32(x) for x in vol_scale)
     if kernel == "fdk":
         osc = float(f32(out_scale) / vx)
@@ -295,16 +320,18 @@ def par3d_bp_params(vec, kernel, vol_scale):
     return num_u, num_v, s
 
 
-def par3d_bp(proj, vecs, shape, vol_scale=(1.0, 1.0, 1.0), out_scale=1.0, kernel="default"):
+def par3d_bp(proj, vecs, shape, vol_scale=(1.0, 1.0, 1.0), out_scale=1.0, kernel="default", ss=1):
     """Parallel-beam 3D BP: ``dev_par3D_BP`` (cuda/3d/par3d_bp.cu:55-120),
-    output scale volScale x*y*z (:263)."""
+    output scale volScale x*y*z (:263); ss > 1 as ``dev_par3D_BP_SS``
+    (:123-198)."""
     proj = np.asarray(proj, dtype=np.float64)
-    fz, fy, fx = _voxel_grid(shape)
     vol = np.zeros(shape)
     for a in range(vecs.shape[0]):
         nu, nv, s = par3d_bp_params(vecs[a], kernel, vol_scale)
-        U = nu[3] + fx * nu[0] + fy * nu[1] + fz * nu[2]
-        V = nv[3] + fx * nv[0] + fy * nv[1] + fz * nv[2]
-        vol += tex2d(proj[:, a, :], U, V) * s
+        for off in _sub_voxels(ss):
+            fz, fy, fx = _voxel_grid(shape, off)
+            U = nu[3] + fx * nu[0] + fy * nu[1] + fz * nu[2]
+            V = nv[3] + fx * nv[0] + fy * nv[1] + fz * nv[2]
+            vol += tex2d(proj[:, a, :], U, V) * s / ss ** 3
     vx, vy, vz = (f32(x) for x in vol_scale)
     return vol * float(f32(out_scale) * (vx * vy * vz))

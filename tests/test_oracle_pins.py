@@ -517,3 +517,82 @@ def test_parker_weights_scan_range_scale():
     assert np.all(w >= 0) and np.all(w <= scale + 1e-12) and np.all(w[0] == 0)
     col = w.sum(axis=0) / scale            # sum over angles of w / scale
     np.testing.assert_allclose(col, na * math.pi / (ang[1] - ang[0]) / na, rtol=2e-2)
+
+
+# ---------------------------------------------------------------- supersampling
+# Sub-ray / sub-voxel sampling written out explicitly in the oracle is
+# checked against the average of the plain projector over shifted detector
+# or volume geometries (an independent construction of the same samples).
+
+def _shift_window_3d(n, d):
+    vg = OG.vol_geom_3d(n, n, n)
+    vg["option"] = {"WindowMinX": -n / 2 + d[0], "WindowMaxX": n / 2 + d[0],
+                    "WindowMinY": -n / 2 + d[1], "WindowMaxY": n / 2 + d[1],
+                    "WindowMinZ": -n / 2 + d[2], "WindowMaxZ": n / 2 + d[2]}
+    return vg
+
+
+def test_cone_fp_supersampling_equals_shifted_detectors():
+    n, nu, nv, ss = 12, 10, 9, 3
+    ang = np.array([0.2, 1.3, 2.0])  # away from the 45-degree direction switch
+    v, vs = OG.normalise_3d(OG.gen_cone_projections(ang, nu, nv, 40.0, 20.0, 1.5, 1.2), "cone",
+                            OG.vol_geom_3d(n, n, n))
+    vol = np.random.default_rng(41).random((n, n, n))
+    ref = np.zeros((nv, len(ang), nu))
+    for k in range(ss):
+        for l in range(ss):
+            w = v.copy()
+            w[:, 3:6] += ((k + 0.5) / ss - 0.5) * v[:, 6:9] + ((l + 0.5) / ss - 0.5) * v[:, 9:12]
+            ref += O3.cone_fp(vol, w, nu, nv, vs)
+    np.testing.assert_allclose(O3.cone_fp(vol, v, nu, nv, vs, ss=ss), ref / ss ** 2, rtol=1e-12)
+
+
+def test_bp_supersampling_equals_shifted_volumes():
+    n, na, nu, ss = 8, 5, 12, 2
+    ang = np.linspace(0, 2 * np.pi, na, endpoint=False)
+    proj = np.random.default_rng(42).random((nu, na, nu))
+    vecs = OG.gen_cone_projections(ang, nu, nu, 30.0, 15.0, 1.0, 1.0)
+    v0, vs = OG.normalise_3d(vecs, "cone", OG.vol_geom_3d(n, n, n))
+    pv0, _ = OG.normalise_3d(OG.gen_par3d_projections(ang, nu, nu, 1.0, 1.0), "parallel3d",
+                             OG.vol_geom_3d(n, n, n))
+    refc = np.zeros((n, n, n))
+    refp = np.zeros((n, n, n))
+    offs = [(a, b, c) for a in range(ss) for b in range(ss) for c in range(ss)]
+    for a, b, c in offs:
+        d = tuple((q + 0.5) / ss - 0.5 for q in (a, b, c))
+        vc, _ = OG.normalise_3d(vecs, "cone", _shift_window_3d(n, d))
+        refc += O3.cone_bp(proj, vc, (n, n, n), vs)
+        vp, _ = OG.normalise_3d(OG.gen_par3d_projections(ang, nu, nu, 1.0, 1.0), "parallel3d",
+                                _shift_window_3d(n, d))
+        refp += O3.par3d_bp(proj, vp, (n, n, n), vs)
+    np.testing.assert_allclose(O3.cone_bp(proj, v0, (n, n, n), vs, ss=ss), refc / len(offs), rtol=1e-12)
+    np.testing.assert_allclose(O3.par3d_bp(proj, pv0, (n, n, n), vs, ss=ss), refp / len(offs),
+                               rtol=1e-12, atol=1e-12)
+
+
+def test_2d_supersampling_equals_shifted_geometries():
+    n, na, nd, ss = 16, 7, 24, 3
+    ang = np.linspace(0.1, 0.5, na)  # near-vertical rays: no orientation change
+    vol = np.random.default_rng(43).random((n, n))
+    vg = OG.vol_geom_2d(n, n)
+    for kind, vecs in (("parallel", OG.gen_par_projections(ang, nd, 1.2)),
+                       ("fan", OG.gen_fan_projections(ang, nd, 400.0, 200.0, 1.0))):
+        v, osc = OG.normalise_2d(vecs, kind, vg)
+        fp = O2.par_fp if kind == "parallel" else O2.fan_fp
+        ref = sum(fp(vol, O2._shift_det(v, (k + 0.5) / ss - 0.5), nd, osc) for k in range(ss)) / ss
+        np.testing.assert_allclose(fp(vol, v, nd, osc, rays=ss), ref, rtol=1e-12, atol=1e-12)
+        y = np.random.default_rng(44).random((na, nd))
+        bp = O2.par_bp if kind == "parallel" else O2.fan_bp
+        refb = 0
+        for a in range(ss):
+            for b in range(ss):
+                # sub-pixel (dx, dy) in (column, row) units sits at world
+                # (dx, -dy) from the pixel centre: move the geometry by the
+                # opposite amount
+                dx, dy = (a + 0.5) / ss - 0.5, (b + 0.5) / ss - 0.5
+                vv = np.array(v, dtype=np.float64)
+                vv[:, 2:4] -= (dx, -dy)
+                if kind == "fan":
+                    vv[:, 0:2] -= (dx, -dy)
+                refb = refb + bp(y, vv, (n, n), osc)
+        np.testing.assert_allclose(bp(y, v, (n, n), osc, ss=ss), refb / ss ** 2, rtol=1e-10)
