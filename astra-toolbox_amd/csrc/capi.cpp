@@ -211,10 +211,15 @@ int astra_amd_fp3d(astra_amd_geom3d g, const float *vol, float *proj,
     return guarded([&] {
         check3d(g, p);
         AMX_REQUIRE(vol && proj, "fp3d: NULL data pointer");
-        if (p->kernel == ASTRA_AMD_KER3D_SUM_SQUARE_WEIGHTS)
-            amx::fail(ASTRA_AMD_ERR_UNSUPPORTED, "fp3d: sum_square_weights kernel is not built");
         AMX_REQUIRE(p->kernel >= 0 && p->kernel <= ASTRA_AMD_KER3D_MATCHED_BP,
                     "fp3d: unknown kernel %d", p->kernel);
+        if (p->kernel == ASTRA_AMD_KER3D_SUM_SQUARE_WEIGHTS) {
+            if (ss_of(p->rays_per_det_dim) > 1)
+                amx::fail(ASTRA_AMD_ERR_UNSUPPORTED,
+                          "fp3d: sum_square_weights with DetectorSuperSampling > 1 is not built");
+            amx::fp3d_sumsq_launch(*g->g, proj, p->output_scale, out_mode(mode), as_stream(stream));
+            return;
+        }
         amx::fp3d_launch(*g->g, vol, proj, p->output_scale, out_mode(mode), nullptr, nullptr,
                          false, as_stream(stream), ss_of(p->rays_per_det_dim));
     });

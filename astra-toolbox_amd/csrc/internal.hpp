@@ -192,6 +192,8 @@ void bp3d_launch_padded(const Geom3D &g, const float *pp, float *vol, float out_
                         int mode, const float *upd_w, float lo, float hi, hipStream_t s,
                         unsigned ss = 1);
 void proj_to_padded_am(const Geom3D &g, const float *proj, float *pp, hipStream_t s);
+// par3d FP with the sum_square_weights kernel (squared weights, no volume)
+void fp3d_sumsq_launch(const Geom3D &g, float *proj, float out_scale, int mode, hipStream_t s);
 size_t padded_am_elems(const Geom3D &g);
 void bp3d_matched_launch(const Geom3D &g, const float *proj, float *vol, float out_scale,
                          int mode, hipStream_t s, unsigned ss = 1);

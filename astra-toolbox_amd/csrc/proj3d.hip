@@ -269,6 +269,50 @@ __global__ __launch_bounds__(256) void fp3d_kernel(FPArgs A) {
     }
 }
 
+// Squared weights of the par3d FP: per ray, the sum over slices of the
+// squared bilinear weights of its samples (separable: per axis
+// sum w^2 = t^2 + (1-t)^2, or the one tap inside near the border), times
+// the squared distance weight (par3D_FP_SumSqW_t and dirWeights,
+// cuda/3d/par3d_fp.cu:329-419).  No volume data is read.
+__device__ __forceinline__ float dir_weight_sq(float f, float n) {
+    // f: texture coordinate (texel i covers [i, i + 1))
+    if (f <= -0.5f) return 0.f;
+    if (f <= 0.5f) return (f + 0.5f) * (f + 0.5f);
+    if (f <= n - 0.5f) {
+        float t = f + 0.5f - floorf(f + 0.5f);
+        return t * t + (1.f - t) * (1.f - t);
+    }
+    if (f <= n + 0.5f) return (n + 0.5f - f) * (n + 0.5f - f);
+    return 0.f;
+}
+
+__global__ __launch_bounds__(256) void fp3d_sumsq_kernel(FPArgs A) {
+    unsigned tile;
+    int angle;
+    if (!fp_map(A, tile, angle)) return;
+    const unsigned tiles_u = (A.nu + 15) / 16;
+    const unsigned tu = tile % tiles_u, tv = tile / tiles_u;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int u = tu * 16 + (lane & 15);
+    const int v = tv * 16 + wave * 4 + (lane >> 4);
+    if (u >= (int)A.nu || v >= (int)A.nv) return;
+    float a1f, a2f, c1f, c2f, scale;
+    int lo, hi;
+    fp3d_ray<false>(A, A.ang[angle], u, v, a1f, a2f, c1f, c2f, scale, lo, hi);
+    // padded texel coordinate p = f - 0.5 + kPad
+    const float o1 = c1f - kPad + 0.5f, o2 = c2f - kPad + 0.5f;
+    const float n1 = (float)A.n1, n2 = (float)A.n2;
+    float acc = 0.f;
+    for (int s = lo; s <= hi; ++s)
+        acc += dir_weight_sq(fmaf(a1f, (float)s, o1), n1) * dir_weight_sq(fmaf(a2f, (float)s, o2), n2);
+    const float val = acc * scale * scale;
+    const size_t idx = ((size_t)v * A.na + angle) * A.nu + u;
+    if (A.mode == OUT_ADD)
+        A.out[idx] += val;
+    else
+        A.out[idx] = val;
+}
+
 // ------------------------------------------------------------- BP --------
 struct BPArgs {
     const float *pp;  // padded angle-major projections
@@ -648,6 +692,22 @@ void fp3d_launch(const Geom3D &g, const float *vol, float *proj, float out_scale
             }
         }, s);
         prof_end(pt, s);
+    }
+}
+
+void fp3d_sumsq_launch(const Geom3D &g, float *proj, float out_scale, int mode, hipStream_t s) {
+    AMX_REQUIRE(g.kind == ASTRA_AMD_GEOM_PAR3D,
+                "fp3d: sum_square_weights is defined for parallel3d geometries only");
+    AMX_REQUIRE(mode == OUT_SET || mode == OUT_ADD, "fp3d: sum_square_weights mode");
+    for (int k = 0; k < 3; ++k) {
+        if (!g.dir_count[k]) continue;
+        FPArgs A = fp_args(g, k, out_scale);
+        A.out = proj;
+        A.mode = mode;
+        A.ss = 1;
+        fp_group_launches(A, g.dir_count[k], [&](const FPArgs &a, unsigned groups) {
+            fp3d_sumsq_kernel<<<groups, 256, 0, s>>>(a);
+        }, s);
     }
 }
 

@@ -52,7 +52,7 @@ extern "C" {
 /* 3D projection kernels: Cuda3DProjectionKernel
  * (include/astra/cuda/3d/dims3d.h:45-51). */
 #define ASTRA_AMD_KER3D_DEFAULT 0
-#define ASTRA_AMD_KER3D_SUM_SQUARE_WEIGHTS 1 /* not built: UNSUPPORTED     */
+#define ASTRA_AMD_KER3D_SUM_SQUARE_WEIGHTS 1 /* par3d FP: squared weights  */
 #define ASTRA_AMD_KER3D_FDK_WEIGHTING 2
 #define ASTRA_AMD_KER3D_2D_WEIGHTING 3
 #define ASTRA_AMD_KER3D_MATCHED_BP 4 /* BP = exact transpose of the FP     */

@@ -215,6 +215,44 @@ def par3d_fp_T(proj, vecs, shape, vol_scale=(1.0, 1.0, 1.0), out_scale=1.0):
     return vol
 
 
+def _dir_weight_sq(f, n):
+    """``dirWeights`` (cuda/3d/par3d_fp.cu:329-341): sum of the squared
+    linear weights of texture coordinate f on an axis of n texels."""
+    f = np.asarray(f, dtype=np.float64)
+    t = f + 0.5 - np.floor(f + 0.5)
+    return np.select([f <= -0.5, f <= 0.5, f <= n - 0.5, f <= n + 0.5],
+                     [0.0, (f + 0.5) ** 2, t * t + (1 - t) ** 2, (n + 0.5 - f) ** 2], 0.0)
+
+
+def par3d_fp_sumsq(vecs, n_u, n_v, shape, vol_scale=(1.0, 1.0, 1.0), out_scale=1.0):
+    """par3d FP with the sum_square_weights kernel (``par3D_FP_SumSqW_t``,
+    cuda/3d/par3d_fp.cu:344-419): per ray, the sum over slices of the squared
+    bilinear weights, times the squared distance weight."""
+    nz, ny, nx = shape
+    n = (nx, ny, nz)
+    nA = vecs.shape[0]
+    proj = np.zeros((n_v, nA, n_u))
+    uu, vv = np.meshgrid(np.arange(n_u) + 0.5, np.arange(n_v) + 0.5)
+    for a in range(nA):
+        vec = vecs[a]
+        d = par3d_direction(vec)
+        c0, c1, c2 = _DIRS[d]
+        n0, n1, n2 = n[c0], n[c1], n[c2]
+        ray = vec[0:3]
+        det = [vec[3 + k] + uu * vec[6 + k] + vv * vec[9 + k] for k in range(3)]
+        a1 = ray[c1] / ray[c0]
+        a2 = ray[c2] / ray[c0]
+        b1 = det[c1] - a1 * det[c0]
+        b2 = det[c2] - a2 * det[c0]
+        s1, s2, osc = _fp_scale(d, vol_scale, out_scale, par3d=True)
+        acc = np.zeros_like(b1)
+        for sl in range(n0):
+            p = sl - 0.5 * n0 + 0.5
+            acc += _dir_weight_sq(a1 * p + b1 + 0.5 * n1, n1) * _dir_weight_sq(a2 * p + b2 + 0.5 * n2, n2)
+        proj[:, a, :] = acc * (math.sqrt(a1 * a1 * s1 + a2 * a2 * s2 + 1.0) * osc) ** 2
+    return proj
+
+
 # ---------------------------------------------------------------- BP ----
 
 def _det3x(b, c):

@@ -112,3 +112,20 @@ def test_matched_adjoint_of_supersampled_fp():
     lhs = float((AG.forward_project(x2, pid2).double() * y2.double()).sum())
     rhs = float((x2.double() * AG.back_project(y2, pid2).double()).sum())
     assert abs(lhs - rhs) / abs(lhs) <= 1e-5
+
+
+def test_par3d_sum_square_weights():
+    import torch
+    import astra
+    from astra import experimental
+    from oracle import geometry as OG, proj3d as O3
+    n, na, nd = 20, 18, 26
+    ang = np.linspace(0, np.pi, na, endpoint=False)
+    vg = astra.create_vol_geom(n, n, n)
+    pg = astra.create_proj_geom("parallel3d", 1.0, 1.0, nd, nd, ang)
+    v, vs = OG.normalise_3d(OG.gen_par3d_projections(ang, nd, nd, 1.0, 1.0), "parallel3d",
+                            OG.vol_geom_3d(n, n, n))
+    pid = astra.create_projector("cuda3d", pg, vg, {"ProjectionKernel": "sum_square_weights"})
+    proj = torch.empty((nd, na, nd), dtype=torch.float32, device="cuda")
+    experimental.direct_FP3D(pid, torch.zeros((n, n, n), device="cuda"), proj)
+    assert rel_l2(proj.cpu().numpy(), O3.par3d_fp_sumsq(v, nd, nd, (n, n, n), vs)) <= 1e-5

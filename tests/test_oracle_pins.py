@@ -596,3 +596,21 @@ def test_2d_supersampling_equals_shifted_geometries():
                     vv[:, 0:2] -= (dx, -dy)
                 refb = refb + bp(y, vv, (n, n), osc)
         np.testing.assert_allclose(bp(y, v, (n, n), osc, ss=ss), refb / ss ** 2, rtol=1e-10)
+
+
+def test_sum_square_weights_equals_squared_matrix_rows():
+    """sum_square_weights = row sums of the element-wise squared system
+    matrix of the par3d FP (built column by column from unit volumes)."""
+    n, na, nu, nv = 4, 3, 6, 5
+    ang = np.array([0.3, 1.2, 2.5])
+    v, vs = OG.normalise_3d(OG.gen_par3d_projections(ang, nu, nv, 0.9, 1.1), "parallel3d",
+                            OG.vol_geom_3d(n, n, n))
+    cols = []
+    for i in range(n ** 3):
+        e = np.zeros(n ** 3)
+        e[i] = 1.0
+        cols.append(O3.par3d_fp(e.reshape(n, n, n), v, nu, nv, vs, out_scale=1.3).ravel())
+    A = np.stack(cols, axis=1)
+    ref = (A * A).sum(axis=1).reshape(nv, na, nu)
+    got = O3.par3d_fp_sumsq(v, nu, nv, (n, n, n), vs, out_scale=1.3)
+    np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-14)
