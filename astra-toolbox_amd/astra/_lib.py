@@ -19,7 +19,7 @@ ERR_ARG, ERR_HIP, ERR_UNSUPPORTED, ERR_ALLOC, ERR_INTERNAL = 1, 2, 3, 4, 5
 MODE_ADD, MODE_SET = 0, 1
 GEOM_PAR, GEOM_FAN, GEOM_PAR3D, GEOM_CONE = 0, 1, 2, 3
 KER3D_DEFAULT, KER3D_SUM_SQUARE_WEIGHTS, KER3D_FDK_WEIGHTING, KER3D_2D_WEIGHTING, KER3D_MATCHED_BP = range(5)
-ALGO_SIRT, ALGO_CGLS = 0, 1
+ALGO_SIRT, ALGO_CGLS, ALGO_EM, ALGO_SART = 0, 1, 2, 3
 
 
 class Params2D(ctypes.Structure):
@@ -67,6 +67,7 @@ SIGNATURES = {
     "astra_amd_algo3d_create": [_i, _vp, ctypes.POINTER(Params3D), ctypes.POINTER(_vp)],
     "astra_amd_algo_set_option": [_vp, ctypes.c_char_p, _d],
     "astra_amd_algo_set_mask": [_vp, ctypes.c_char_p, _vp],
+    "astra_amd_algo_set_order": [_vp, _vp, _u],
     "astra_amd_algo_run": [_vp, _vp, _vp, _i, _vp],
     "astra_amd_algo_residual_norm": [_vp, _vp, _vp, ctypes.POINTER(_d), _vp],
     "astra_amd_algo_destroy": [_vp],

@@ -353,9 +353,57 @@ class CGLS3D_CUDA(_Iterative):
     is3d = True
 
 
-TYPES = {c.__name__: c for c in (FP_CUDA, BP_CUDA, FBP_CUDA, SIRT_CUDA, CGLS_CUDA, FP3D_CUDA,
-                                  BP3D_CUDA, FDK_CUDA, SIRT3D_CUDA, CGLS3D_CUDA)}
-NOT_BUILT = ("FP", "BP", "FBP", "SIRT", "SART", "ART", "CGLS", "SART_CUDA", "EM_CUDA",
+class EM_CUDA(_Iterative):
+    """CCudaEMAlgorithm (src/CudaEMAlgorithm.cpp:63-253): multiplicative
+    updates; no mask options."""
+    algo_type = _lib.ALGO_EM
+
+    def __init__(self, cfg):
+        super().__init__(cfg)
+        if self.masks:
+            raise AstraError("EM_CUDA: mask options are not supported")
+
+
+class SART_CUDA(_Iterative):
+    """CCudaSartAlgorithm (src/CudaSartAlgorithm.cpp:63-316): one angle per
+    iteration.  ProjectionOrder 'sequential', 'custom' (ProjectionOrderList)
+    or 'random' (default): a permutation of the angles, drawn here from a
+    fixed-seed generator (the reference shuffles with the C library's
+    rand()), so 'random' runs are reproducible but not the reference's
+    sequence."""
+    algo_type = _lib.ALGO_SART
+
+    def __init__(self, cfg):
+        super().__init__(cfg)
+        if "SinogramMask" in self.masks:
+            raise AstraError("SART_CUDA: Sinogram mask option is not supported.")
+        self.options["Relaxation"] = float(_opt(cfg, "Relaxation", 1.0))
+        na = self.proj.proj_shape[0]
+        order = str(_opt(cfg, "ProjectionOrder", "random")).lower()
+        if order == "sequential":
+            self.order = list(range(na))
+        elif order == "random":
+            self.order = [int(i) for i in np.random.default_rng(0).permutation(na)]
+        elif order == "custom":
+            lst = _opt(cfg, "ProjectionOrderList")
+            if lst is None:
+                raise AstraError("SART_CUDA: ProjectionOrder 'custom' needs ProjectionOrderList")
+            self.order = [int(i) for i in np.asarray(lst).ravel()]
+        else:
+            raise AstraError("Unknown ProjectionOrder")
+
+    def _handle(self, dev):
+        new = dev.index not in self._handles
+        h = super()._handle(dev)
+        if new:
+            arr = np.ascontiguousarray(self.order, dtype=np.int32)
+            _lib.call("astra_amd_algo_set_order", h, arr.ctypes.data_as(ctypes.c_void_p), len(arr))
+        return h
+
+
+TYPES = {c.__name__: c for c in (FP_CUDA, BP_CUDA, FBP_CUDA, SIRT_CUDA, CGLS_CUDA, EM_CUDA, SART_CUDA,
+                                  FP3D_CUDA, BP3D_CUDA, FDK_CUDA, SIRT3D_CUDA, CGLS3D_CUDA)}
+NOT_BUILT = ("FP", "BP", "FBP", "SIRT", "SART", "ART", "CGLS",
              "DART_MASK_CUDA", "DART_SMOOTHING_CUDA", "DART_MASK_CUDA3D",
              "DART_SMOOTHING_CUDA3D", "ROI_SELECT_CUDA", "DATA_OPERATION_CUDA")
 

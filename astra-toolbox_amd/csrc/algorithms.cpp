@@ -104,7 +104,7 @@ struct AlgoBase {
         // padded buffers must have zero borders: they are only ever written
         // in their interior
         AMX_HIP(hipMemset(r.as<float>(), 0, np));
-        if (type == ASTRA_AMD_ALGO_SIRT) {
+        if (type != ASTRA_AMD_ALGO_CGLS) {
             line_w = DevArray(np);
             pixel_w = DevArray(nv);
             AMX_HIP(hipMemset(line_w.as<float>(), 0, np));
@@ -205,6 +205,90 @@ struct Cgls : AlgoBase {
     }
 };
 
+// EM: CCudaEMAlgorithm::precomputeWeights / run (src/CudaEMAlgorithm.cpp:
+// 149-253): pixelW = inv(A^T 1); x *= pixelW A^T (b / A x), the ratio set
+// to 0 where A x <= 1e-6.  Both steps fuse into the projector epilogues.
+struct Em : AlgoBase {
+    void run(const float *b, float *x, int iters, hipStream_t s) override {
+        AMX_REQUIRE(!recon_mask && !sino_mask && !min_mask && !max_mask,
+                    "EM: mask options are not supported");
+        ensure_buffers();
+        if (!weights_ready) {
+            const size_t np = op.nproj();
+            fill_launch(t.as<float>(), np, 1.f, s);
+            op.to_padded(t.as<float>(), r.as<float>(), s);
+            op.bp(r.as<float>(), pixel_w.as<float>(), 1.f, OUT_SET, nullptr, lo, hi, s);
+            invert_launch(pixel_w.as<float>(), op.nvol(), 1.f, s);
+            weights_ready = true;
+        }
+        for (int it = 0; it < iters; ++it) {
+            op.fp(x, r.as<float>(), 1.f, OUT_RATIO, b, nullptr, s);
+            op.bp(r.as<float>(), x, 1.f, OUT_EMUPD, pixel_w.as<float>(), lo, hi, s);
+        }
+    }
+};
+
+// SART: CCudaSartAlgorithm (src/CudaSartAlgorithm.cpp:196-316), 2D only.
+// One angle per iteration, in the given projection order (or 0, 1, 2, ...),
+// continuing across run() calls: r_a = (b_a - A_a x) lineW_a;
+// x = clamp(x + lambda * mask * A_a^T r_a) with SART's unweighted BP.
+struct Sart : AlgoBase {
+    std::vector<int> order;
+    long long iteration = 0;
+
+    void precompute(hipStream_t s) {
+        const size_t nv = op.nvol();
+        float *tv = t.as<float>();
+        if (recon_mask)
+            copy_launch(tv, recon_mask, nv, s);
+        else
+            fill_launch(tv, nv, 1.f, s);
+        op.fp(tv, line_w.as<float>(), 1.f, OUT_SET, nullptr, nullptr, s);
+        invert_launch(line_w.as<float>(), op.npad(), 1.f, s);
+    }
+
+    void run(const float *b, float *x, int iters, hipStream_t s) override {
+        AMX_REQUIRE(op.g2 != nullptr, "SART: 2D geometries only");
+        AMX_REQUIRE(!sino_mask, "SART: the sinogram mask option is not supported");
+        ensure_buffers();
+        if (!weights_ready || recon_mask) {
+            precompute(s);
+            weights_ready = true;
+        }
+        const Geom2D &g = *op.g2;
+        const size_t nv = op.nvol();
+        const size_t pitch = g.nd + 2 * 2;
+        for (int it = 0; it < iters; ++it, ++iteration) {
+            const int a = order.empty() ? (int)(iteration % g.na)
+                                        : order[(size_t)(iteration % (long long)order.size())];
+            const float *xin = x;
+            if (recon_mask) {
+                copy_launch(t.as<float>(), x, nv, s);
+                mul_launch(t.as<float>(), recon_mask, nv, s);
+                xin = t.as<float>();
+            }
+            fp2d_launch_range(g, xin, r.as<float>(), -op.out_scale, OUT_RESID, b,
+                              line_w.as<float>(), true, op.rays, (unsigned)a, 1, s);
+            bp2d_sart_launch(g, r.as<float>() + (size_t)a * pitch, x, (unsigned)a,
+                             op.out_scale * (float)relaxation, recon_mask, lo, hi, s);
+            if (min_mask || max_mask)
+                clamp_launch(x, nv, -INFINITY, INFINITY, min_mask, max_mask, s);
+        }
+    }
+};
+
+void algo_set_order(AlgoBase *a, const int *order, unsigned n) {
+    std::lock_guard<std::mutex> lk(a->mu);
+    auto *sart = dynamic_cast<Sart *>(a);
+    AMX_REQUIRE(sart != nullptr, "ProjectionOrder applies to SART only");
+    AMX_REQUIRE(order != nullptr || n == 0, "ProjectionOrder: NULL list");
+    const unsigned na = a->op.g2->na;
+    for (unsigned i = 0; i < n; ++i)
+        AMX_REQUIRE(order[i] >= 0 && (unsigned)order[i] < na,
+                    "ProjectionOrderList entry %d out of range [0, %u)", order[i], na);
+    sart->order.assign(order, order + n);
+}
+
 AlgoBase *algo_build(int type, Geom2D *g2, Geom3D *g3, float out_scale, int kernel,
                      unsigned rays, unsigned vss) {
     AlgoBase *a = nullptr;
@@ -212,7 +296,13 @@ AlgoBase *algo_build(int type, Geom2D *g2, Geom3D *g3, float out_scale, int kern
         a = new Sirt();
     else if (type == ASTRA_AMD_ALGO_CGLS)
         a = new Cgls();
-    else
+    else if (type == ASTRA_AMD_ALGO_EM)
+        a = new Em();
+    else if (type == ASTRA_AMD_ALGO_SART) {
+        AMX_REQUIRE(g2 != nullptr, "SART: 2D geometries only");
+        AMX_REQUIRE(vss == 1, "SART: VoxelSuperSampling is not supported");
+        a = new Sart();
+    } else
         fail(ASTRA_AMD_ERR_ARG, "algorithm: unknown type %d", type);
     a->type = type;
     a->op.g2 = g2;

@@ -16,6 +16,7 @@ AlgoBase *algo_build(int type, Geom2D *g2, Geom3D *g3, float out_scale, int kern
 void algo_destroy(AlgoBase *a);
 void algo_set_option(AlgoBase *a, const char *key, double v);
 void algo_set_mask(AlgoBase *a, const char *key, const float *m);
+void algo_set_order(AlgoBase *a, const int *order, unsigned n);
 void algo_run(AlgoBase *a, const float *b, float *x, int iters, hipStream_t s);
 double algo_residual(AlgoBase *a, const float *b, const float *x, hipStream_t s);
 Filter *filter_build(unsigned n_det, const float *half, unsigned n_bins);
@@ -458,6 +459,13 @@ int astra_amd_algo_set_option(astra_amd_algo a, const char *key, double value) {
     return guarded([&] {
         AMX_REQUIRE(a && a->a, "algorithm handle is NULL");
         amx::algo_set_option(a->a, key, value);
+    });
+}
+
+int astra_amd_algo_set_order(astra_amd_algo a, const int *order, unsigned int n) {
+    return guarded([&] {
+        AMX_REQUIRE(a && a->a, "algorithm handle is NULL");
+        amx::algo_set_order(a->a, order, n);
     });
 }
 

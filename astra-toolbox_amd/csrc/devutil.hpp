@@ -31,16 +31,6 @@ __device__ __forceinline__ unsigned texel_off(float fl1, float fl2, float pitch)
     return (unsigned)fmaf(fl2, pitch, fl1) << 2;
 }
 
-// Pair of adjacent floats at a 4-byte aligned address (one dwordx2 load).
-typedef float f2a4 __attribute__((ext_vector_type(2), aligned(4)));
-
-__device__ __forceinline__ float clampc(float t, float hi) {
-    // fmaxf first: a NaN coordinate becomes -1.5 (stays in bounds)
-    return fminf(fmaxf(t, -1.5f), hi);
-}
-
-__device__ __forceinline__ f2a4 ld2(const float *p) { return *reinterpret_cast<const f2a4 *>(p); }
-
 __device__ __forceinline__ float lerp(float a, float b, float w) { return fmaf(w, b - a, a); }
 
 __device__ __forceinline__ int wave_min(int v) {
@@ -73,6 +63,12 @@ __device__ __forceinline__ void clip_range(double a, double c, double n, int n0,
 // OUT_RESID  out  = (b + v) * w            (w optional)
 // OUT_UPDATE out  = clamp(out + w * v)     (w optional)
 // OUT_ADDMUL out  = (out + v) * w          (w optional)
-enum { OUT_ADD = 0, OUT_SET = 1, OUT_RESID = 2, OUT_UPDATE = 3, OUT_ADDMUL = 4 };
+// OUT_RATIO  out  = v > 1e-6 ? b / v : 0   (FP; EM's sinogram ratio)
+// OUT_EMUPD  out  = out * (v * w)          (BP; EM's multiplicative update)
+enum { OUT_ADD = 0, OUT_SET = 1, OUT_RESID = 2, OUT_UPDATE = 3, OUT_ADDMUL = 4, OUT_RATIO = 5,
+       OUT_EMUPD = 6 };
+
+// EM's sinogram ratio (opDividedBy, cuda/2d/arith.cu:91-98)
+__device__ __forceinline__ float em_ratio(float b, float v) { return v > 0.000001f ? b / v : 0.f; }
 
 }  // namespace amx

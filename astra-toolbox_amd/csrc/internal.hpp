@@ -201,6 +201,13 @@ void bp3d_matched_launch(const Geom3D &g, const float *proj, float *vol, float o
 void fp2d_launch(const Geom2D &g, const float *vol, float *sino, float out_scale, int mode,
                  const float *resid_b, const float *resid_w, bool padded_out, unsigned rays,
                  hipStream_t s);
+// FP of angles [a0, a0 + na_sub) only (rows a0.. of the output / residual)
+void fp2d_launch_range(const Geom2D &g, const float *vol, float *sino, float out_scale, int mode,
+                       const float *resid_b, const float *resid_w, bool padded_out, unsigned rays,
+                       unsigned a0, unsigned na_sub, hipStream_t s);
+// SART's unweighted BP of one padded sinogram row: vol = clamp(vol + w * BP)
+void bp2d_sart_launch(const Geom2D &g, const float *sp_row, float *vol, unsigned angle,
+                      float out_scale, const float *upd_w, float lo, float hi, hipStream_t s);
 void bp2d_launch(const Geom2D &g, const float *sino, float *vol, float out_scale, bool fbp,
                  int mode, const float *upd_w, float lo, float hi, hipStream_t s,
                  unsigned ss = 1);

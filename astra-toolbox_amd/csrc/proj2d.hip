@@ -192,7 +192,15 @@ void sino_to_padded(const Geom2D &g, const float *sino, float *sp, hipStream_t s
 void fp2d_launch(const Geom2D &g, const float *vol, float *sino, float out_scale, int mode,
                  const float *resid_b, const float *resid_w, bool padded_out, unsigned rays,
                  hipStream_t s) {
+    fp2d_launch_range(g, vol, sino, out_scale, mode, resid_b, resid_w, padded_out, rays, 0, g.na,
+                      s);
+}
+
+void fp2d_launch_range(const Geom2D &g, const float *vol, float *sino, float out_scale, int mode,
+                       const float *resid_b, const float *resid_w, bool padded_out, unsigned rays,
+                       unsigned a0, unsigned na_sub, hipStream_t s) {
     AMX_REQUIRE(rays >= 1 && rays <= 64, "fp2d: rays per detector must be in [1, 64]");
+    AMX_REQUIRE(a0 + na_sub <= g.na && na_sub > 0, "fp2d: angle range out of bounds");
     Scratch copies((rows_elems(g) + cols_elems(g)) * sizeof(float), s);
     float *rows = copies.as<float>();
     float *cols = rows + rows_elems(g);
@@ -203,17 +211,20 @@ void fp2d_launch(const Geom2D &g, const float *vol, float *sino, float out_scale
     A.rows = rows;
     A.cols = cols;
     A.w = g.w; A.h = g.h; A.na = g.na; A.nd = g.nd;
-    A.ang = g.fp.as<FP2DAngle>();
+    // angles [a0, a0 + na_sub): offset the angle table and the row pointers
+    const size_t out_pitch = padded_out ? g.nd + 2 * kPad : g.nd;
+    A.ang = g.fp.as<FP2DAngle>() + a0;
+    A.na = na_sub;
     A.osc = out_scale;
     A.rays = rays;
-    A.out = sino;
+    A.out = sino + (size_t)a0 * out_pitch;
     A.mode = mode;
-    A.rb = resid_b;
-    A.rw = resid_w;
+    A.rb = resid_b ? resid_b + (size_t)a0 * g.nd : nullptr;
+    A.rw = resid_w ? resid_w + (size_t)a0 * out_pitch : nullptr;
     A.padded_out = padded_out ? 1 : 0;
-    dim3 grid((g.nd + 63) / 64, (g.na + 3) / 4);
+    dim3 grid((g.nd + 63) / 64, (na_sub + 3) / 4);
     ProfToken pt = prof_begin(g.kind == ASTRA_AMD_GEOM_FAN ? "fp2d_fan" : "fp2d_par",
-                              (double)g.na * g.nd * rays * std::max(g.w, g.h), s);
+                              (double)na_sub * g.nd * rays * std::max(g.w, g.h), s);
     if (g.kind == ASTRA_AMD_GEOM_FAN)
         fp2d_kernel<true><<<grid, 256, 0, s>>>(A);
     else
@@ -257,6 +268,27 @@ void bp2d_launch_padded(const Geom2D &g, const float *sp, float *vol, float out_
     }
     AMX_LAUNCHED();
     prof_end(pt, s);
+}
+
+void bp2d_sart_launch(const Geom2D &g, const float *sp_row, float *vol, unsigned angle,
+                      float out_scale, const float *upd_w, float lo, float hi, hipStream_t s) {
+    AMX_REQUIRE(angle < g.na, "bp2d_sart: angle out of range");
+    BP2DArgs A{};
+    A.sp = sp_row;
+    A.w = g.w; A.h = g.h; A.na = 1; A.nd = g.nd;
+    A.ang = g.bp.as<BP2DAngle>() + angle;
+    A.osc = out_scale;
+    A.vol = vol;
+    A.mode = OUT_UPDATE;
+    A.uw = upd_w;
+    A.lo = lo;
+    A.hi = hi;
+    dim3 grid((g.w + 63) / 64, (g.h + 15) / 16);
+    if (g.kind == ASTRA_AMD_GEOM_PAR)
+        bp2d_kernel<3><<<grid, 256, 0, s>>>(A);
+    else
+        bp2d_kernel<4><<<grid, 256, 0, s>>>(A);
+    AMX_LAUNCHED();
 }
 
 void bp2d_launch(const Geom2D &g, const float *sino, float *vol, float out_scale, bool fbp,

@@ -126,6 +126,8 @@ __device__ __forceinline__ void store_fp(float *out, size_t oidx, float val, int
         float r = rb[idx] + val;
         if (rw) r *= rw[oidx];
         out[oidx] = r;
+    } else if (mode == OUT_RATIO) {
+        out[oidx] = em_ratio(rb[idx], val);
     } else {
         float r = out[oidx] + val;
         if (rw) r *= rw[oidx];
@@ -135,7 +137,10 @@ __device__ __forceinline__ void store_fp(float *out, size_t oidx, float val, int
 
 // Back projection of PY consecutive rows (Y0 .. Y0+PY-1) of column X over
 // all angles from a padded sinogram [na][nd+4].
-// KIND: 0 parallel (per-angle weight), 1 fan (fr), 2 fan FBP (fr^2).
+// KIND: 0 parallel (per-angle weight), 1 fan (fr), 2 fan FBP (fr^2),
+// 3 parallel unweighted, 4 fan unweighted (SART's BP, whose per-angle and
+// fr weights cancel against its line weights: cuda/2d/par_bp.cu:139-159,
+// cuda/2d/fan_bp.cu:167-199).
 template <int KIND, int PY>
 __device__ __forceinline__ void bp2d_column(int X, int Y0, unsigned w, unsigned h, unsigned na,
                                             unsigned nd, const float *__restrict__ sp,
@@ -153,12 +158,12 @@ __device__ __forceinline__ void bp2d_column(int X, int Y0, unsigned w, unsigned 
         float den = fmaf(fy0, c.den[2], fmaf(fx, c.den[1], c.den[0]));
 #pragma unroll
         for (int i = 0; i < PY; ++i) {
-            float fr = (KIND == 0) ? 1.f : __builtin_amdgcn_rcpf(den);
+            float fr = (KIND == 0 || KIND == 3) ? 1.f : __builtin_amdgcn_rcpf(den);
             float t = clamp_pad(num * fr, hi);
             float f = floorf(t);
             float2 q = ld2b(row, (unsigned)f << 2);
             float val = lerp(q.x, q.y, t - f);
-            float wgt = (KIND == 0) ? c.w : (KIND == 1 ? fr : fr * fr);
+            float wgt = (KIND == 0) ? c.w : (KIND == 1 ? fr : (KIND == 2 ? fr * fr : 1.f));
             acc[i] = fmaf(wgt, val, acc[i]);
             num -= c.num[2];  // next row: world y decreases by 1
             den -= c.den[2];
@@ -172,6 +177,8 @@ __device__ __forceinline__ void store_bp(float *vol, size_t idx, float val, int 
         vol[idx] += val;
     } else if (mode == OUT_SET) {
         vol[idx] = val;
+    } else if (mode == OUT_EMUPD) {
+        vol[idx] *= val * uw[idx];
     } else {  // OUT_UPDATE: clamp(x + w * val)
         float wv = uw ? uw[idx] : 1.f;
         vol[idx] = fminf(fmaxf(fmaf(wv, val, vol[idx]), lo), hi);

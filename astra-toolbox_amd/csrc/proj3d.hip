@@ -262,6 +262,8 @@ __global__ __launch_bounds__(256) void fp3d_kernel(FPArgs A) {
         float r = A.rb[idx] + val;
         if (A.rw) r *= A.rw[oidx];
         A.out[oidx] = r;
+    } else if (A.mode == OUT_RATIO) {
+        A.out[oidx] = em_ratio(A.rb[idx], val);
     } else {  // OUT_ADDMUL: (out + val) * w
         float r = A.out[oidx] + val;
         if (A.rw) r *= A.rw[oidx];
@@ -408,6 +410,8 @@ __global__ __launch_bounds__(256) void bp3d_kernel(BPArgs A) {
             A.vol[idx] += val;
         } else if (A.mode == OUT_SET) {
             A.vol[idx] = val;
+        } else if (A.mode == OUT_EMUPD) {
+            A.vol[idx] *= val * A.uw[idx];
         } else {  // OUT_UPDATE: clamp(x + w * val)
             float w = A.uw ? A.uw[idx] : 1.f;
             A.vol[idx] = fminf(fmaxf(fmaf(w, val, A.vol[idx]), A.lo), A.hi);
@@ -469,6 +473,8 @@ __global__ __launch_bounds__(256) void bp3d_ss_kernel(BPArgs A, int R) {
         A.vol[idx] += val;
     } else if (A.mode == OUT_SET) {
         A.vol[idx] = val;
+    } else if (A.mode == OUT_EMUPD) {
+        A.vol[idx] *= val * A.uw[idx];
     } else {
         float w = A.uw ? A.uw[idx] : 1.f;
         A.vol[idx] = fminf(fmaxf(fmaf(w, val, A.vol[idx]), A.lo), A.hi);

@@ -271,6 +271,8 @@ int astra_amd_dot(const float *a, const float *b, size_t n, double *out, void *s
  * the algorithm. */
 #define ASTRA_AMD_ALGO_SIRT 0
 #define ASTRA_AMD_ALGO_CGLS 1
+#define ASTRA_AMD_ALGO_EM 2   /* CCudaEMAlgorithm (src/CudaEMAlgorithm.cpp)   */
+#define ASTRA_AMD_ALGO_SART 3 /* CCudaSartAlgorithm (src/CudaSartAlgorithm.cpp), 2D */
 int astra_amd_algo2d_create(int type, astra_amd_geom2d g, const astra_amd_params2d *p,
                             astra_amd_algo *out);
 int astra_amd_algo3d_create(int type, astra_amd_geom3d g, const astra_amd_params3d *p,
@@ -281,6 +283,11 @@ int astra_amd_algo_set_option(astra_amd_algo a, const char *key, double value);
 /* Optional masks (device pointers, lifetime >= algorithm): "ReconstructionMask"
  * (volume shape), "SinogramMask" (projection shape), "MinMask", "MaxMask". */
 int astra_amd_algo_set_mask(astra_amd_algo a, const char *key, const float *mask);
+/* SART projection order (ProjectionOrder 'sequential' / 'random' /
+ * 'custom' with ProjectionOrderList, src/CudaSartAlgorithm.cpp:79-107):
+ * the angle of iteration k is order[k % n]; n = 0 restores 0, 1, 2, ...
+ * Host array, copied. */
+int astra_amd_algo_set_order(astra_amd_algo a, const int *order, unsigned int n);
 /* run(iterations): vol is read as the initial iterate and overwritten. */
 int astra_amd_algo_run(astra_amd_algo a, const float *sino, float *vol, int iterations,
                        void *stream);

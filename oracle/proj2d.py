@@ -362,3 +362,25 @@ def fan_bp(sino, vecs, shape, out_scale=1.0, fbp_weighted=False, ss=1):
                 w = fr * fr if fbp_weighted else fr
                 vol += tex1d(sino[a], T) * w
     return vol * out_scale / (ss * ss)
+
+
+def sart_bp(row, vec, shape, kind, out_scale=1.0):
+    """SART's BP of one sinogram row (``devBP_SART`` / ``devFanBP_SART``,
+    cuda/2d/par_bp.cu:139-159, cuda/2d/fan_bp.cu:167-199): the detector
+    position of every pixel as in the full BP, without the per-angle or fr
+    weight."""
+    row = np.asarray(row, dtype=np.float64)
+    H, W = shape
+    X = np.arange(W) - 0.5 * W + 0.5
+    if kind == "parallel":
+        Y = np.arange(H) - 0.5 * H + 0.5
+        fx, fy = np.meshgrid(X, Y)
+        rx, ry, dsx, dsy, ux, uy = (float(x) for x in vec)
+        d = ux * ry - uy * rx
+        T = fx * (ry / d) - fy * (-rx / d) + (dsy * rx - dsx * ry) / d
+    else:
+        Y = -(np.arange(H) - 0.5 * H + 0.5)
+        fx, fy = np.meshgrid(X, Y)
+        nc, nx, ny, dc, dx, dy = fan_bp_params(vec, False)
+        T = (nc + nx * fx + ny * fy) / (dc + dx * fx + dy * fy)
+    return tex1d(row, T) * out_scale

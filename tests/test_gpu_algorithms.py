@@ -169,3 +169,70 @@ def test_fdk(cone3d):
     assert e <= 1e-5
     # the input projections are not modified
     assert np.array_equal(astra.data3d.get(sid), c["proj"])
+
+
+@pytest.mark.parametrize("kind", ["parallel", "fanflat"])
+def test_em(kind):
+    """EM_CUDA against the oracle's EM over the oracle projectors."""
+    import astra
+    from astra import phantom
+    from oracle import algorithms as OA, geometry as OG, proj2d as O2
+    n, na, nd = 48, 60, 72
+    vg = astra.create_vol_geom(n, n)
+    if kind == "parallel":
+        ang = np.linspace(0, np.pi, na, endpoint=False)
+        pg = astra.create_proj_geom("parallel", 1.0, nd, ang)
+        v, osc = OG.normalise_2d(OG.gen_par_projections(ang, nd, 1.0), "parallel", vg)
+        fp, bp = (lambda x: O2.par_fp(x, v, nd, osc)), (lambda y: O2.par_bp(y, v, (n, n), osc))
+    else:
+        ang = np.linspace(0, 2 * np.pi, na, endpoint=False)
+        pg = astra.create_proj_geom("fanflat", 1.0, nd, ang, 150.0, 75.0)
+        v, osc = OG.normalise_2d(OG.gen_fan_projections(ang, nd, 150.0, 75.0, 1.0), "fan", vg)
+        fp, bp = (lambda x: O2.fan_fp(x, v, nd, osc)), (lambda y: O2.fan_bp(y, v, (n, n), osc))
+    sino = fp(phantom.ellipses_2d(n, seed=63)).astype(np.float32)
+    pid = astra.create_projector("cuda", pg, vg)
+    sid = astra.data2d.create("-sino", pg, sino)
+    rid = astra.data2d.create("-vol", vg, 1.0)
+    aid = _run_algo(astra, "EM_CUDA", pid, sid, rid, 10)
+    ref = OA.em(fp, bp, sino, np.ones((n, n)), 10)
+    assert rel_l2(astra.data2d.get(rid), ref) <= 1e-4
+    rn = astra.algorithm.get_res_norm(aid)
+    assert abs(rn - OA.residual_norm(fp, sino, astra.data2d.get(rid))) <= 1e-4 * np.linalg.norm(sino)
+
+
+@pytest.mark.parametrize("kind", ["parallel", "fanflat"])
+def test_sart(kind):
+    """SART_CUDA (custom order, constraint, two runs continuing the order)
+    against the oracle's SART with SART's unweighted BP."""
+    import astra
+    from astra import phantom
+    from oracle import algorithms as OA, geometry as OG, proj2d as O2
+    n, na, nd = 40, 30, 60
+    vg = astra.create_vol_geom(n, n)
+    if kind == "parallel":
+        ang = np.linspace(0, np.pi, na, endpoint=False)
+        pg = astra.create_proj_geom("parallel", 1.0, nd, ang)
+        v, osc = OG.normalise_2d(OG.gen_par_projections(ang, nd, 1.0), "parallel", vg)
+        fp1 = lambda x, a: O2.par_fp(x, v[a:a + 1], nd, osc)[0]
+        full = lambda x: O2.par_fp(x, v, nd, osc)
+        okind = "parallel"
+    else:
+        ang = np.linspace(0, 2 * np.pi, na, endpoint=False)
+        pg = astra.create_proj_geom("fanflat", 1.0, nd, ang, 120.0, 60.0)
+        v, osc = OG.normalise_2d(OG.gen_fan_projections(ang, nd, 120.0, 60.0, 1.0), "fan", vg)
+        fp1 = lambda x, a: O2.fan_fp(x, v[a:a + 1], nd, osc)[0]
+        full = lambda x: O2.fan_fp(x, v, nd, osc)
+        okind = "fan"
+    bp1 = lambda r, a: O2.sart_bp(r, v[a], (n, n), okind, osc)
+    sino = full(phantom.ellipses_2d(n, seed=64)).astype(np.float32)
+    order = [int(i) for i in np.random.default_rng(65).permutation(na)]
+    pid = astra.create_projector("cuda", pg, vg)
+    sid = astra.data2d.create("-sino", pg, sino)
+    rid = astra.data2d.create("-vol", vg, 0)
+    aid = _run_algo(astra, "SART_CUDA", pid, sid, rid, 25, ProjectionOrder="custom",
+                    ProjectionOrderList=order, Relaxation=0.8, MinConstraint=0.0)
+    astra.algorithm.run(aid, 20)  # continues the order at iteration 25
+    seq = [order[k % na] for k in range(45)]
+    ref = OA.sart(fp1, bp1, sino, np.zeros((n, n)), seq, relaxation=0.8, min_c=0.0,
+                  line_w=OA._inv(full(np.ones((n, n)))))
+    assert rel_l2(astra.data2d.get(rid), ref) <= 1e-4

@@ -143,7 +143,18 @@ def test_algorithm_config_errors():
     with pytest.raises(astra.AstraError):
         astra.algorithm.create({"type": "NOT_AN_ALGORITHM"})
     with pytest.raises(astra.AstraError):
-        astra.algorithm.create({"type": "SART_CUDA", "ProjectionDataId": s, "ReconstructionDataId": r})
+        astra.algorithm.create({"type": "ART", "ProjectionDataId": s, "ReconstructionDataId": r})
+    with pytest.raises(astra.AstraError):
+        astra.algorithm.create({"type": "SART_CUDA", "ProjectionDataId": s, "ReconstructionDataId": r,
+                                "option": {"ProjectionOrder": "backwards"}})
+    m = astra.data2d.create("-vol", vg, 1.0)
+    with pytest.raises(astra.AstraError):
+        astra.algorithm.create({"type": "EM_CUDA", "ProjectionDataId": s, "ReconstructionDataId": r,
+                                "option": {"ReconstructionMaskId": m}})
+    sart = astra.algorithm.create({"type": "SART_CUDA", "ProjectionDataId": s, "ReconstructionDataId": r,
+                                   "option": {"ProjectionOrder": "custom", "ProjectionOrderList": [2, 0, 1]}})
+    assert astra.algorithm._reg.get(sart).order == [2, 0, 1]
+    astra.algorithm.delete(sart)
     with pytest.raises(astra.AstraError):
         astra.create_projector("linear", pg, vg)
     cfg = astra.astra_dict("SIRT_CUDA")

@@ -614,3 +614,54 @@ def test_sum_square_weights_equals_squared_matrix_rows():
     ref = (A * A).sum(axis=1).reshape(nv, na, nu)
     got = O3.par3d_fp_sumsq(v, nu, nv, (n, n, n), vs, out_scale=1.3)
     np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-14)
+
+
+# ---------------------------------------------------------------- EM / SART
+
+def test_em_fixed_point_and_sart_reduces_residual():
+    """With the exact transpose, EM leaves the solution of a consistent
+    positive system in place (every ray through a positive pixel has ratio
+    1) and decreases the Kullback-Leibler divergence from any start; SART
+    sweeps reduce the residual."""
+    n, na, nd = 16, 20, 24
+    ang = np.linspace(0, np.pi, na, endpoint=False)
+    v, osc = OG.normalise_2d(OG.gen_par_projections(ang, nd, 1.0), "parallel", OG.vol_geom_2d(n, n))
+    truth = 0.5 + np.random.default_rng(61).random((n, n))
+    fp = lambda x: O2.par_fp(x, v, nd, osc)
+    bp = lambda y: O2.par_bp(y, v, (n, n), osc)
+    bpT = lambda y: O2.par_fp_T(y, v, (n, n), osc)
+    b = fp(truth)
+    np.testing.assert_allclose(OA.em(fp, bpT, b, truth, 3), truth, rtol=1e-10)
+
+    def kl(x):
+        t = fp(x)
+        m = b > 0
+        return float(np.sum(b[m] * np.log(b[m] / t[m]) - b[m] + t[m]) + np.sum(t[~m]))
+    x = np.ones((n, n))
+    prev = kl(x)
+    for _ in range(5):
+        x = OA.em(fp, bpT, b, x, 1)
+        cur = kl(x)
+        assert cur < prev
+        prev = cur
+    line_w = OA._inv(fp(np.ones((n, n))))
+    fpa = lambda x, a: O2.par_fp(x, v[a:a + 1], nd, osc)[0]
+    bpa = lambda r, a: O2.sart_bp(r, v[a], (n, n), "parallel", osc)
+    r0 = np.linalg.norm(b)
+    x1 = OA.sart(fpa, bpa, b, np.zeros((n, n)), list(range(na)), line_w=line_w)
+    x2 = OA.sart(fpa, bpa, b, x1, list(range(na)), line_w=line_w)
+    r1, r2 = np.linalg.norm(b - fp(x1)), np.linalg.norm(b - fp(x2))
+    assert r2 < r1 < 0.5 * r0
+
+
+def test_sart_parallel_bp_is_the_unweighted_bp():
+    """For parallel beams SART's BP of one row is the full BP of that row
+    divided by the angle's weight |r| / |d|."""
+    n, nd = 15, 21
+    ang = np.array([0.7])
+    v, osc = OG.normalise_2d(OG.gen_par_projections(ang, nd, 1.3), "parallel", OG.vol_geom_2d(n, n))
+    row = np.random.default_rng(62).random(nd)
+    rx, ry, dsx, dsy, ux, uy = v[0]
+    w = math.hypot(rx, ry) / abs(ux * ry - uy * rx)
+    np.testing.assert_allclose(O2.sart_bp(row, v[0], (n, n), "parallel"),
+                               O2.par_bp(row[None, :], v, (n, n)) / w, rtol=1e-12)
