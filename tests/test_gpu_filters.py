@@ -21,7 +21,11 @@ def par():
     vg = astra.create_vol_geom(N, N)
     pg = astra.create_proj_geom("parallel", 1.0, ND, angles)
     ovec, osc = OG.normalise_2d(OG.gen_par_projections(angles, ND, 1.0), "par", vg)
-    sino = np.random.default_rng(31).random((NA, ND), dtype=np.float32)
+    # sinogram data (projections of a phantom): windows that keep only high
+    # frequencies turn white noise into a test of fp32 coordinate rounding
+    from astra import phantom
+    from oracle import proj2d as O2
+    sino = O2.par_fp(phantom.ellipses_2d(N, seed=31), ovec, ND, osc).astype(np.float32)
     return dict(astra=astra, vg=vg, pg=pg, ovec=ovec, osc=osc, sino=sino)
 
 
