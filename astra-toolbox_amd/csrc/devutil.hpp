@@ -26,12 +26,31 @@ __device__ __forceinline__ float2 ld2b(const float *base, unsigned byte_off) {
     return make_float2(v.x, v.y);
 }
 
+// Four floats at base + byte offset (one global_load_dwordx4; 8-byte
+// aligned): the bilinear taps of a row-pair interleaved plane.
+__device__ __forceinline__ float4 ld4b(const float *base, unsigned byte_off) {
+    typedef float f4u __attribute__((ext_vector_type(4), aligned(8)));
+    f4u v = *reinterpret_cast<const f4u *>(reinterpret_cast<const char *>(base) + byte_off);
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+
+// byte offset of texel pair (fl1, fl2) in a row-pair plane (8 bytes per pair)
+__device__ __forceinline__ unsigned pair_off(float fl1, float fl2, float pitch) {
+    return (unsigned)fmaf(fl2, pitch, fl1) << 3;
+}
+
 // byte offset of padded texel (fl1, fl2) in a plane of row pitch `pitch`
 __device__ __forceinline__ unsigned texel_off(float fl1, float fl2, float pitch) {
     return (unsigned)fmaf(fl2, pitch, fl1) << 2;
 }
 
 __device__ __forceinline__ float lerp(float a, float b, float w) { return fmaf(w, b - a, a); }
+
+// bilinear value from the four taps of a row-pair load: q = (v[r][c],
+// v[r+1][c], v[r][c+1], v[r+1][c+1]); w1 along c, w2 along r
+__device__ __forceinline__ float bilerp_pairs(float4 q, float w1, float w2) {
+    return lerp(lerp(q.x, q.z, w1), lerp(q.y, q.w, w1), w2);
+}
 
 __device__ __forceinline__ int wave_min(int v) {
     for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));

@@ -13,6 +13,11 @@
 //                     rays of every group and one slice is one contiguous
 //                     plane (32-bit offsets inside a plane).
 //   BP input copy     angle-major and zero-padded: [angle][v+4][u+4]
+//   Both copies are stored row-pair interleaved: element [r][c] is the
+//   float2 (p[r][c], p[r+1][c]), so the four taps of a bilinear sample at
+//   (r, c) are one 16-byte load of [r][c] and [r][c+1].  The texture
+//   address unit handles one instruction per sample instead of two (it is
+//   the unit that limits both kernels); the copies take twice the memory.
 //
 // Sampling semantics follow cuda/3d/cone_fp.cu:154-231,
 // cuda/3d/par3d_fp.cu:152-230, cuda/3d/cone_bp.cu:55-133 and
@@ -30,57 +35,66 @@
 namespace amx {
 
 // ----------------------------------------------------------- prep --------
-// dst[c0][c2 + 2][c1 + 2] = vol[z][y][x] for the z- and y-groups (the fast
-// axis stays x).  One workgroup per destination row; borders written 0.
+// dst[c0][c2 + 2][c1 + 2] = (vol at c2, vol at c2 + 1) for the z- and
+// y-groups (the fast axis stays x; row pairs as described above).  One
+// workgroup per destination row; borders written 0.
 __global__ __launch_bounds__(256) void pad_copy_xfast(const float *__restrict__ vol,
-                                                      float *__restrict__ dst, unsigned nx,
+                                                      float2 *__restrict__ dst, unsigned nx,
                                                       unsigned ny, unsigned nz, int group) {
     const unsigned px = nx + 2 * kPad;
     const unsigned row = blockIdx.x;  // c0 * n2p + padded c2
     const unsigned n2p = (group == 2) ? ny + 2 * kPad : nz + 2 * kPad;
-    const unsigned n2 = (group == 2) ? ny : nz;
+    const int n2 = (group == 2) ? (int)ny : (int)nz;
     const unsigned c0 = row / n2p;
     const int c2 = (int)(row % n2p) - kPad;
-    float *out = dst + (size_t)row * px;
-    const bool inside = c2 >= 0 && c2 < (int)n2;
-    const float *src = (group == 2) ? vol + ((size_t)c0 * ny + (inside ? c2 : 0)) * nx   // [z=c0][y=c2]
-                                    : vol + ((size_t)(inside ? c2 : 0) * ny + c0) * nx;  // [z=c2][y=c0]
+    float2 *out = dst + (size_t)row * px;
+    const bool in0 = c2 >= 0 && c2 < n2, in1 = c2 + 1 >= 0 && c2 + 1 < n2;
+    auto src_row = [&](int r) {
+        return (group == 2) ? vol + ((size_t)c0 * ny + r) * nx     // [z=c0][y=r]
+                            : vol + ((size_t)r * ny + c0) * nx;    // [z=r][y=c0]
+    };
+    const float *s0 = src_row(in0 ? c2 : 0), *s1 = src_row(in1 ? c2 + 1 : 0);
     for (unsigned i = threadIdx.x; i < px; i += blockDim.x) {
-        int x = (int)i - kPad;
-        out[i] = (inside && x >= 0 && x < (int)nx) ? src[x] : 0.f;
+        const int x = (int)i - kPad;
+        const bool xin = x >= 0 && x < (int)nx;
+        out[i] = make_float2((in0 && xin) ? s0[x] : 0.f, (in1 && xin) ? s1[x] : 0.f);
     }
 }
 
-// x-group: dst[x][z + 2][y + 2] = vol[z][y][x]; per z-plane an (x, y)
-// transpose through a 64 x 64 LDS tile.  grid = (y tiles, x tiles, nz + 4).
+// x-group: dst[x][z + 2][y + 2] = (vol[z][y][x], vol[z+1][y][x]); per
+// padded z row an (x, y) transpose of planes z and z + 1 through two
+// 64 x 64 LDS tiles.  grid = (y tiles, x tiles, nz + 4).
 __global__ __launch_bounds__(256) void pad_transpose_x(const float *__restrict__ vol,
-                                                       float *__restrict__ dst, unsigned nx,
+                                                       float2 *__restrict__ dst, unsigned nx,
                                                        unsigned ny, unsigned nz) {
-    __shared__ float tile[64][65];
+    __shared__ float tile[2][64][65];
     const unsigned pyn = ny + 2 * kPad, pzn = nz + 2 * kPad;
     const int zr = blockIdx.z;  // padded z row
-    const int z = zr - kPad;
     const int x0 = blockIdx.y * 64;
     const int yp0 = blockIdx.x * 64;  // padded y tile origin
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-    for (int r = ty; r < 64; r += 4) {
-        int y = yp0 + r - kPad;
-        int x = x0 + tx;
-        float v = 0.f;
-        if (z >= 0 && z < (int)nz && y >= 0 && y < (int)ny && x < (int)nx)
-            v = vol[((size_t)z * ny + y) * nx + x];
-        tile[r][tx] = v;
+    for (int k = 0; k < 2; ++k) {
+        const int z = zr - kPad + k;
+        for (int r = ty; r < 64; r += 4) {
+            int y = yp0 + r - kPad;
+            int x = x0 + tx;
+            float v = 0.f;
+            if (z >= 0 && z < (int)nz && y >= 0 && y < (int)ny && x < (int)nx)
+                v = vol[((size_t)z * ny + y) * nx + x];
+            tile[k][r][tx] = v;
+        }
     }
     __syncthreads();
     for (int r = ty; r < 64; r += 4) {
         int x = x0 + r;
         int yp = yp0 + tx;
-        if (x < (int)nx && yp < (int)pyn) dst[((size_t)x * pzn + zr) * pyn + yp] = tile[tx][r];
+        if (x < (int)nx && yp < (int)pyn)
+            dst[((size_t)x * pzn + zr) * pyn + yp] = make_float2(tile[0][tx][r], tile[1][tx][r]);
     }
 }
 
-// proj [v][a][u] -> padded angle-major [a][v + 2][u + 2] (borders 0).
-// One workgroup per destination row.
+// proj [v][a][u] -> padded angle-major [a][v + 2][u + 2] (borders 0), the
+// single (not paired) layout of the algorithms' projection buffers.
 __global__ __launch_bounds__(256) void proj_to_am(const float *__restrict__ proj,
                                                   float *__restrict__ pp, unsigned nu,
                                                   unsigned na, unsigned nv) {
@@ -95,6 +109,39 @@ __global__ __launch_bounds__(256) void proj_to_am(const float *__restrict__ proj
         int u = (int)i - kPad;
         out[i] = (inside && u >= 0 && u < (int)nu) ? src[u] : 0.f;
     }
+}
+
+// proj [v][a][u] -> padded angle-major row pairs [a][v + 2][u + 2] =
+// (p[v][u], p[v+1][u]) (borders 0).  One workgroup per destination row.
+__global__ __launch_bounds__(256) void proj_to_am_pairs(const float *__restrict__ proj,
+                                                        float2 *__restrict__ pp, unsigned nu,
+                                                        unsigned na, unsigned nv) {
+    const unsigned pu = nu + 2 * kPad, pv = nv + 2 * kPad;
+    const unsigned row = blockIdx.x;  // a * pv + padded v
+    const unsigned a = row / pv;
+    const int v = (int)(row % pv) - kPad;
+    float2 *out = pp + (size_t)row * pu;
+    const bool in0 = v >= 0 && v < (int)nv, in1 = v + 1 >= 0 && v + 1 < (int)nv;
+    const float *s0 = proj + ((size_t)(in0 ? v : 0) * na + a) * nu;
+    const float *s1 = proj + ((size_t)(in1 ? v + 1 : 0) * na + a) * nu;
+    for (unsigned i = threadIdx.x; i < pu; i += blockDim.x) {
+        const int u = (int)i - kPad;
+        const bool uin = u >= 0 && u < (int)nu;
+        out[i] = make_float2((in0 && uin) ? s0[u] : 0.f, (in1 && uin) ? s1[u] : 0.f);
+    }
+}
+
+// padded single layout [a][v + 4][u + 4] -> row pairs (same padding).
+__global__ __launch_bounds__(256) void am_to_pairs(const float *__restrict__ src,
+                                                   float2 *__restrict__ pp, unsigned nu,
+                                                   unsigned nv) {
+    const unsigned pu = nu + 2 * kPad, pv = nv + 2 * kPad;
+    const unsigned row = blockIdx.x;  // a * pv + padded v
+    const bool last = (row % pv) == pv - 1;
+    const float *s0 = src + (size_t)row * pu;
+    float2 *out = pp + (size_t)row * pu;
+    for (unsigned i = threadIdx.x; i < pu; i += blockDim.x)
+        out[i] = make_float2(s0[i], last ? 0.f : s0[pu + i]);
 }
 
 // ------------------------------------------------------------- FP --------
@@ -173,10 +220,9 @@ __device__ __forceinline__ void fp3d_ray(const FPArgs &A, const FP3DAngle &G, do
 // ray (c1, c2) = a s + c through the padded group copy.
 __device__ __forceinline__ float march3d(const FPArgs &A, float a1f, float a2f, float c1f,
                                          float c2f, int slo, int shi) {
-    const unsigned pitch = A.n1 + 2 * kPad;
-    const size_t plane = (size_t)(A.n2 + 2 * kPad) * pitch;
+    const unsigned pitch = A.n1 + 2 * kPad;                        // pairs per row
+    const size_t plane = (size_t)(A.n2 + 2 * kPad) * pitch * 2;     // floats per slice
     const float pitchf = (float)pitch;
-    const unsigned rowb = pitch * 4u;
     const float hi1 = (float)A.n1 + 2.5f, hi2 = (float)A.n2 + 2.5f;
     float acc0 = 0.f, acc1 = 0.f;
     int s = slo;
@@ -188,13 +234,10 @@ __device__ __forceinline__ float march3d(const FPArgs &A, float a1f, float a2f, 
         float q1 = clamp_pad(fmaf(a1f, sf + 1.f, c1f), hi1);
         float q2 = clamp_pad(fmaf(a2f, sf + 1.f, c2f), hi2);
         float fp1 = floorf(p1), fp2 = floorf(p2), fq1 = floorf(q1), fq2 = floorf(q2);
-        unsigned op = texel_off(fp1, fp2, pitchf), oq = texel_off(fq1, fq2, pitchf);
-        const float *pl2 = pl + plane;
-        float2 r0 = ld2b(pl, op), r1 = ld2b(pl, op + rowb);
-        float2 t0 = ld2b(pl2, oq), t1 = ld2b(pl2, oq + rowb);
-        float w1 = p1 - fp1, w2 = p2 - fp2, x1 = q1 - fq1, x2 = q2 - fq2;
-        acc0 += lerp(lerp(r0.x, r0.y, w1), lerp(r1.x, r1.y, w1), w2);
-        acc1 += lerp(lerp(t0.x, t0.y, x1), lerp(t1.x, t1.y, x1), x2);
+        float4 r = ld4b(pl, pair_off(fp1, fp2, pitchf));
+        float4 t = ld4b(pl + plane, pair_off(fq1, fq2, pitchf));
+        acc0 += bilerp_pairs(r, p1 - fp1, p2 - fp2);
+        acc1 += bilerp_pairs(t, q1 - fq1, q2 - fq2);
         pl += 2 * plane;
     }
     if (s <= shi) {
@@ -202,9 +245,7 @@ __device__ __forceinline__ float march3d(const FPArgs &A, float a1f, float a2f, 
         float p1 = clamp_pad(fmaf(a1f, sf, c1f), hi1);
         float p2 = clamp_pad(fmaf(a2f, sf, c2f), hi2);
         float fp1 = floorf(p1), fp2 = floorf(p2);
-        unsigned op = texel_off(fp1, fp2, pitchf);
-        float2 r0 = ld2b(pl, op), r1 = ld2b(pl, op + rowb);
-        acc0 += lerp(lerp(r0.x, r0.y, p1 - fp1), lerp(r1.x, r1.y, p1 - fp1), p2 - fp2);
+        acc0 += bilerp_pairs(ld4b(pl, pair_off(fp1, fp2, pitchf)), p1 - fp1, p2 - fp2);
     }
     return acc0 + acc1;
 }
@@ -348,10 +389,10 @@ __global__ __launch_bounds__(256) void bp3d_kernel(BPArgs A) {
     const float fx = X - 0.5f * A.nx + 0.5f;
     const float fy = Y - 0.5f * A.ny + 0.5f;
     const float fz = Z0 - 0.5f * A.nz + 0.5f;
-    const unsigned pitch = A.nu + 2 * kPad;
-    const size_t plane = (size_t)(A.nv + 2 * kPad) * pitch;
+    const unsigned pitch = A.nu + 2 * kPad;                    // pairs per row
+    const size_t plane = (size_t)(A.nv + 2 * kPad) * pitch * 2; // floats per angle
     const float pitchf = (float)pitch;
-    const unsigned rowb = pitch * 4u;
+    const unsigned rowb = pitch * 8u;                           // bytes per row
     const float hiu = (float)A.nu + 2.5f, hiv = (float)A.nv + 2.5f;
     float acc[ZS];
 #pragma unroll
@@ -369,7 +410,7 @@ __global__ __launch_bounds__(256) void bp3d_kernel(BPArgs A) {
             const float pu = clamp_pad(un * fr, hiu);
             const float fu = floorf(pu);
             const float wu = pu - fu;
-            const unsigned ou = (unsigned)fu << 2;
+            const unsigned ou = (unsigned)fu << 3;
             const float wgt = (KIND == 0) ? fr * fr : (KIND == 1 ? fr : wa);
 #pragma unroll
             for (int i = 0; i < ZS; ++i) {
@@ -377,9 +418,7 @@ __global__ __launch_bounds__(256) void bp3d_kernel(BPArgs A) {
                 // pv in [0.5, nv + 2.5]: floor is exact and fract = pv - floor(pv)
                 unsigned iv = (unsigned)floorf(pv);
                 float wv = __builtin_amdgcn_fractf(pv);
-                unsigned o = __umul24(iv, rowb) + ou;
-                float2 r0 = ld2b(pl, o), r1 = ld2b(pl, o + rowb);
-                float val = lerp(lerp(r0.x, r0.y, wu), lerp(r1.x, r1.y, wu), wv);
+                float val = bilerp_pairs(ld4b(pl, __umul24(iv, rowb) + ou), wu, wv);
                 acc[i] = fmaf(wgt, val, acc[i]);
                 vn += c.nv[2];
             }
@@ -390,9 +429,7 @@ __global__ __launch_bounds__(256) void bp3d_kernel(BPArgs A) {
                 float pu = clamp_pad(un * fr, hiu);
                 float pv = clamp_pad(vn * fr, hiv);
                 float fu = floorf(pu), fv = floorf(pv);
-                unsigned o = texel_off(fu, fv, pitchf);
-                float2 r0 = ld2b(pl, o), r1 = ld2b(pl, o + rowb);
-                float val = lerp(lerp(r0.x, r0.y, pu - fu), lerp(r1.x, r1.y, pu - fu), pv - fv);
+                float val = bilerp_pairs(ld4b(pl, pair_off(fu, fv, pitchf)), pu - fu, pv - fv);
                 float wgt = (KIND == 0) ? fr * fr : (KIND == 1 ? fr : wa);
                 acc[i] = fmaf(wgt, val, acc[i]);
                 un += c.nu[2];
@@ -435,9 +472,8 @@ __global__ __launch_bounds__(256) void bp3d_ss_kernel(BPArgs A, int R) {
     const float fy0 = Y - 0.5f * A.ny + 0.5f * sub;
     const float fz0 = Z - 0.5f * A.nz + 0.5f * sub;
     const unsigned pitch = A.nu + 2 * kPad;
-    const size_t plane = (size_t)(A.nv + 2 * kPad) * pitch;
+    const size_t plane = (size_t)(A.nv + 2 * kPad) * pitch * 2;
     const float pitchf = (float)pitch;
-    const unsigned rowb = pitch * 4u;
     const float hiu = (float)A.nu + 2.5f, hiv = (float)A.nv + 2.5f;
     float acc = 0.f;
     const float *pl = A.pp;
@@ -459,9 +495,7 @@ __global__ __launch_bounds__(256) void bp3d_ss_kernel(BPArgs A, int R) {
                     }
                     float pu = clamp_pad(un * fr, hiu), pv = clamp_pad(vn * fr, hiv);
                     float fu = floorf(pu), fv = floorf(pv);
-                    unsigned o = texel_off(fu, fv, pitchf);
-                    float2 r0 = ld2b(pl, o), r1 = ld2b(pl, o + rowb);
-                    float val = lerp(lerp(r0.x, r0.y, pu - fu), lerp(r1.x, r1.y, pu - fu), pv - fv);
+                    float val = bilerp_pairs(ld4b(pl, pair_off(fu, fv, pitchf)), pu - fu, pv - fv);
                     acc = fmaf((KIND == 2) ? wa : fr * fr, val, acc);
                 }
             }
@@ -567,7 +601,7 @@ size_t group_elems(const Geom3D &g, int grp) {
     return (size_t)n0 * (n1 + 2 * kPad) * (n2 + 2 * kPad);
 }
 
-void make_group_copy(const Geom3D &g, int grp, const float *vol, float *dst, hipStream_t s) {
+void make_group_copy(const Geom3D &g, int grp, const float *vol, float2 *dst, hipStream_t s) {
     if (grp == 0) {
         dim3 grid((g.ny + 2 * kPad + 63) / 64, (g.nx + 63) / 64, g.nz + 2 * kPad);
         pad_transpose_x<<<grid, 256, 0, s>>>(vol, dst, g.nx, g.ny, g.nz);
@@ -667,10 +701,10 @@ void fp3d_launch(const Geom3D &g, const float *vol, float *proj, float out_scale
             need = std::max(need, group_elems(g, k));
             groups[ng++] = k;
         }
-    Scratch copy(need * sizeof(float), s);
+    Scratch copy(need * sizeof(float2), s);  // row-pair copy
     for (int i = 0; i < ng; ++i) {
         const int k = groups[i];
-        make_group_copy(g, k, vol, copy.as<float>(), s);
+        make_group_copy(g, k, vol, copy.as<float2>(), s);
         FPArgs A = fp_args(g, k, out_scale);
         A.volp = copy.as<float>();
         A.out = proj;
@@ -727,12 +761,13 @@ size_t padded_am_elems(const Geom3D &g) {
     return (size_t)g.na * (g.nv + 2 * kPad) * (g.nu + 2 * kPad);
 }
 
-void bp3d_launch_padded(const Geom3D &g, const float *pp, float *vol, float out_scale, int kernel,
-                        int mode, const float *upd_w, float lo, float hi, hipStream_t s,
-                        unsigned ss) {
+namespace {
+// BP from a row-pair angle-major copy
+void bp3d_run_pairs(const Geom3D &g, const float2 *pairs, float *vol, float out_scale, int kernel,
+                    int mode, const float *upd_w, float lo, float hi, hipStream_t s, unsigned ss) {
     AMX_REQUIRE(ss >= 1 && ss <= 16, "bp3d: rays per voxel dimension must be in [1, 16]");
     BPArgs A{};
-    A.pp = pp;
+    A.pp = reinterpret_cast<const float *>(pairs);
     A.nu = g.nu; A.nv = g.nv; A.na = g.na;
     A.nx = g.nx; A.ny = g.ny; A.nz = g.nz;
     A.vol = vol;
@@ -801,11 +836,24 @@ void bp3d_launch_padded(const Geom3D &g, const float *pp, float *vol, float out_
     prof_end(pt, s);
 }
 
+}  // namespace
+
+void bp3d_launch_padded(const Geom3D &g, const float *pp, float *vol, float out_scale, int kernel,
+                        int mode, const float *upd_w, float lo, float hi, hipStream_t s,
+                        unsigned ss) {
+    Scratch pairs(padded_am_elems(g) * sizeof(float2), s);
+    am_to_pairs<<<g.na * (g.nv + 2 * kPad), 256, 0, s>>>(pp, pairs.as<float2>(), g.nu, g.nv);
+    AMX_LAUNCHED();
+    bp3d_run_pairs(g, pairs.as<float2>(), vol, out_scale, kernel, mode, upd_w, lo, hi, s, ss);
+}
+
 void bp3d_launch(const Geom3D &g, const float *proj, float *vol, float out_scale, int kernel,
                  int mode, const float *upd_w, float lo, float hi, hipStream_t s, unsigned ss) {
-    Scratch pp(padded_am_elems(g) * sizeof(float), s);
-    proj_to_padded_am(g, proj, pp.as<float>(), s);
-    bp3d_launch_padded(g, pp.as<float>(), vol, out_scale, kernel, mode, upd_w, lo, hi, s, ss);
+    Scratch pairs(padded_am_elems(g) * sizeof(float2), s);
+    proj_to_am_pairs<<<g.na * (g.nv + 2 * kPad), 256, 0, s>>>(proj, pairs.as<float2>(), g.nu, g.na,
+                                                               g.nv);
+    AMX_LAUNCHED();
+    bp3d_run_pairs(g, pairs.as<float2>(), vol, out_scale, kernel, mode, upd_w, lo, hi, s, ss);
 }
 
 void bp3d_matched_launch(const Geom3D &g, const float *proj, float *vol, float out_scale,
