@@ -34,6 +34,15 @@ __device__ __forceinline__ float4 ld4b(const float *base, unsigned byte_off) {
     return make_float4(v.x, v.y, v.z, v.w);
 }
 
+// 16-byte aligned float4 / one float at base + byte offset (scalar base,
+// 32-bit vector offset)
+__device__ __forceinline__ float4 ld4a(const float4 *base, unsigned byte_off) {
+    return *reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(base) + byte_off);
+}
+__device__ __forceinline__ float ld1b(const float *base, unsigned byte_off) {
+    return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(base) + byte_off);
+}
+
 // byte offset of texel pair (fl1, fl2) in a row-pair plane (8 bytes per pair)
 __device__ __forceinline__ unsigned pair_off(float fl1, float fl2, float pitch) {
     return (unsigned)fmaf(fl2, pitch, fl1) << 3;

@@ -516,73 +516,199 @@ __global__ __launch_bounds__(256) void bp3d_ss_kernel(BPArgs A, int R) {
 }
 
 // --------------------------------------------------- matched BP ----------
-// Exact transpose of fp3d_kernel: every (ray, slice) sample scatters its
-// four bilinear weights into the padded slice-major accumulation copy of
-// the angle's direction group (fp32 atomics).
+// Exact transpose of fp3d_kernel, voxel-driven (a gather, no atomics).  The
+// forward projector takes, per (sub-)ray r and slice s, the bilinear
+// sample at padded in-slice coordinates p(r, s) = (a1 s + c1, a2 s + c2)
+// (fp32).  Its transpose at the voxel with padded in-slice indices
+// (J1, J2) of slice s is
+//     sum_r y_r * scale_r / R^2 * (1 - |p1 - J1|)+ * (1 - |p2 - J2|)+ ,
+// over the rays of the voxel's direction group whose sample lies within
+// one texel of the voxel along both axes.  Interior voxels only receive
+// samples with unclamped coordinates inside the ray's clipped slice range,
+// so clamping and clipping need no counterpart here.
+//
+// matched_rays_kernel stores, per (sub-)ray of a chunk of angles, the fp32
+// coefficients (a1, c1, a2, c2) exactly as fp3d_ray computes them for the
+// forward projector, and the weighted value y * scale / R^2.  Sub-rays
+// form a lattice (U, V) = (u R + k, v R + l).
 template <bool CONE>
-__global__ __launch_bounds__(256) void fp3d_transpose_kernel(FPArgs A, const float *__restrict__ proj) {
-    unsigned tile;
-    int angle;
-    if (!fp_map(A, tile, angle)) return;
-    const unsigned tiles_u = (A.nu + 15) / 16;
-    const unsigned tu = tile % tiles_u, tv = tile / tiles_u;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int u = tu * 16 + (lane & 15);
-    const int v = tv * 16 + wave * 4 + (lane >> 4);
-    const FP3DAngle &G = A.ang[angle];
-    if (u >= (int)A.nu || v >= (int)A.nv) return;
-    const float y0 = proj[((size_t)v * A.na + angle) * A.nu + u];
-    if (y0 == 0.f) return;
-    const unsigned pitch = A.n1 + 2 * kPad;
-    const size_t plane = (size_t)(A.n2 + 2 * kPad) * pitch;
-    const float pitchf = (float)pitch;
-    const float hi1 = (float)A.n1 + 2.5f, hi2 = (float)A.n2 + 2.5f;
-    float *acc = const_cast<float *>(A.volp);
-    const int R = (int)A.ss;
-    for (int k = 0; k < R; ++k)
-        for (int l = 0; l < R; ++l) {
-            float a1f, a2f, c1f, c2f, scale;
-            int lo, hi;
-            if (R == 1)
-                fp3d_ray<CONE>(A, G, u, v, a1f, a2f, c1f, c2f, scale, lo, hi);
-            else
-                fp3d_ray<CONE>(A, G, u - 0.5 + (k + 0.5) / R, v - 0.5 + (l + 0.5) / R, a1f, a2f,
-                               c1f, c2f, scale, lo, hi);
-            const float y = y0 * scale / (float)(R * R);
-            for (int s = lo; s <= hi; ++s) {
-                const float sf = (float)s;
-                float p1 = clamp_pad(fmaf(a1f, sf, c1f), hi1);
-                float p2 = clamp_pad(fmaf(a2f, sf, c2f), hi2);
-                float f1 = floorf(p1), f2 = floorf(p2);
-                float w1 = p1 - f1, w2 = p2 - f2;
-                float *q = acc + (size_t)s * plane + (texel_off(f1, f2, pitchf) >> 2);
-                atomicAdd(q, y * (1.f - w1) * (1.f - w2));
-                atomicAdd(q + 1, y * w1 * (1.f - w2));
-                atomicAdd(q + pitch, y * (1.f - w1) * w2);
-                atomicAdd(q + pitch + 1, y * w1 * w2);
-            }
-        }
-}
-
-// vol[z][y][x] (+)= interior of a padded slice-major group copy
-__global__ __launch_bounds__(256) void unpad_group_add(const float *__restrict__ src,
-                                                       float *__restrict__ vol, unsigned nx,
-                                                       unsigned ny, unsigned nz, int group) {
-    size_t n = (size_t)nx * ny * nz;
+__global__ __launch_bounds__(256) void matched_rays_kernel(FPArgs A, const float *__restrict__ proj,
+                                                           float4 *__restrict__ rays,
+                                                           float *__restrict__ yw) {
+    const unsigned R = A.ss, NU = A.nu * R, NV = A.nv * R;
+    const size_t per = (size_t)NU * NV;
+    const size_t n = per * A.lcount;
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
          i += (size_t)gridDim.x * blockDim.x) {
-        unsigned x = i % nx;
-        unsigned y = (i / nx) % ny;
-        unsigned z = i / ((size_t)nx * ny);
-        size_t s;
-        if (group == 0)
-            s = ((size_t)x * (nz + 2 * kPad) + z + kPad) * (ny + 2 * kPad) + y + kPad;
-        else if (group == 1)
-            s = ((size_t)y * (nz + 2 * kPad) + z + kPad) * (nx + 2 * kPad) + x + kPad;
+        const unsigned li = (unsigned)(i / per);
+        const unsigned rem = (unsigned)(i - (size_t)li * per);
+        const unsigned V = rem / NU, U = rem - V * NU;
+        const int angle = A.alist[A.lfirst + li];
+        const int u = (int)(U / R), k = (int)(U % R), v = (int)(V / R), l = (int)(V % R);
+        const FP3DAngle &G = A.ang[angle];
+        float a1f, a2f, c1f, c2f, scale;
+        int lo, hi;
+        if (R == 1)
+            fp3d_ray<CONE>(A, G, u, v, a1f, a2f, c1f, c2f, scale, lo, hi);
         else
-            s = ((size_t)z * (ny + 2 * kPad) + y + kPad) * (nx + 2 * kPad) + x + kPad;
-        vol[i] += src[s];
+            fp3d_ray<CONE>(A, G, u - 0.5 + (k + 0.5) / (int)R, v - 0.5 + (l + 0.5) / (int)R, a1f,
+                           a2f, c1f, c2f, scale, lo, hi);
+        rays[i] = make_float4(a1f, c1f, a2f, c2f);
+        const float y = proj[((size_t)v * A.na + angle) * A.nu + u];
+        yw[i] = y * scale / (float)(R * R);
     }
+}
+
+__device__ __forceinline__ void cross3(const double *a, const double *b, double *c) {
+    c[0] = a[1] * b[2] - a[2] * b[1];
+    c[1] = a[2] * b[0] - a[0] * b[2];
+    c[2] = a[0] * b[1] - a[1] * b[0];
+}
+__device__ __forceinline__ double dot3(const double *a, const double *b) {
+    return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+
+// Per angle of the chunk, the projective map from a point W of the group
+// frame (c0, c1, c2; voxel units, volume centre at 0) to continuous
+// sub-ray lattice coordinates: U = nU.(W, 1) / den.(W, 1), likewise V;
+// three float4 (nU, nV, den), scaled so that den is 1 at the centre.
+// Cone: W = S + t (P - S) with P = p0 + u U + v V on the detector gives,
+// with n = U x V and k = (p0 - S).n,
+//     u = (W - S).Au / (|n|^2 (W - S).n),  Au = ((S - p0).(V x n)) n + k V x n,
+// and v likewise with n x U.  Parallel: W - p0 = u U + v V + t d gives
+//     u = (W - p0).(V x d) / U.(V x d),  v = (W - p0).(d x U) / V.(d x U).
+template <bool CONE>
+__global__ void matched_geo_kernel(FPArgs A, float4 *__restrict__ geo) {
+    const unsigned li = blockIdx.x * blockDim.x + threadIdx.x;
+    if (li >= A.lcount) return;
+    const FP3DAngle &G = A.ang[A.alist[A.lfirst + li]];
+    double nu4[4], nv4[4], dn4[4];
+    if (CONE) {
+        double n[3], mu[3], mv[3], sp[3], ps[3], Au[3], Av[3];
+        cross3(G.u, G.v, n);
+        cross3(G.v, n, mu);
+        cross3(n, G.u, mv);
+        for (int i = 0; i < 3; ++i) {
+            sp[i] = G.s[i] - G.p0[i];
+            ps[i] = -sp[i];
+        }
+        const double k = dot3(ps, n), nn = dot3(n, n);
+        const double su = dot3(sp, mu), sv = dot3(sp, mv);
+        for (int i = 0; i < 3; ++i) {
+            Au[i] = su * n[i] + k * mu[i];
+            Av[i] = sv * n[i] + k * mv[i];
+        }
+        for (int i = 0; i < 3; ++i) {
+            nu4[i] = Au[i];
+            nv4[i] = Av[i];
+            dn4[i] = nn * n[i];
+        }
+        nu4[3] = -dot3(G.s, Au);
+        nv4[3] = -dot3(G.s, Av);
+        dn4[3] = -nn * dot3(G.s, n);
+    } else {
+        double bu[3], bv[3];
+        cross3(G.v, G.s, bu);
+        cross3(G.s, G.u, bv);
+        const double iu = 1.0 / dot3(G.u, bu), iv = 1.0 / dot3(G.v, bv);
+        for (int i = 0; i < 3; ++i) {
+            nu4[i] = bu[i] * iu;
+            nv4[i] = bv[i] * iv;
+            dn4[i] = 0.0;
+        }
+        nu4[3] = -dot3(G.p0, bu) * iu;
+        nv4[3] = -dot3(G.p0, bv) * iv;
+        dn4[3] = 1.0;
+    }
+    // detector coordinate u -> lattice coordinate R u + (R - 1) / 2
+    const double R = (double)A.ss, off = 0.5 * (R - 1.0);
+    const double inv = 1.0 / dn4[3];
+    float4 *o = geo + 3 * (size_t)li;
+    double r[3][4];
+    for (int i = 0; i < 4; ++i) {
+        r[0][i] = (R * nu4[i] + off * dn4[i]) * inv;
+        r[1][i] = (R * nv4[i] + off * dn4[i]) * inv;
+        r[2][i] = dn4[i] * inv;
+    }
+    for (int j = 0; j < 3; ++j)
+        o[j] = make_float4((float)r[j][0], (float)r[j][1], (float)r[j][2], (float)r[j][3]);
+}
+
+struct MatchedArgs {
+    const float4 *rays;  // [angle of chunk][V][U] (a1, c1, a2, c2)
+    const float *yw;     // [angle of chunk][V][U] y * scale / R^2
+    const float4 *geo;   // [angle of chunk][3] lattice map
+    unsigned count;      // angles in the chunk
+    unsigned NU, NV;     // sub-ray lattice size
+    unsigned n0, n1, n2, tiles1;
+    unsigned nx, ny;
+    int group;
+    float *vol;  // [z][y][x], += the chunk's contribution
+};
+
+// One thread per voxel: a workgroup is a 16 (c1) x 16 (c2) tile of one
+// slice; consecutive workgroups take consecutive slices of the same tile
+// (their candidate rays overlap).  Per angle the candidate rays are the
+// lattice points inside the image of the voxel's open neighbourhood square
+// (c1 +- 1, c2 +- 1), bounded by its four projected corners (widened by
+// 0.05 lattice units for fp32 rounding of the map; extra candidates get
+// weight 0).
+__global__ __launch_bounds__(256) void bp3d_matched_gather(MatchedArgs M) {
+    const unsigned s = blockIdx.x % M.n0, tile = blockIdx.x / M.n0;
+    const int i1 = (int)((tile % M.tiles1) * 16 + (threadIdx.x & 15));
+    const int i2 = (int)((tile / M.tiles1) * 16 + (threadIdx.x >> 4));
+    if (i1 >= (int)M.n1 || i2 >= (int)M.n2) return;
+    const float w0 = (float)s - 0.5f * M.n0 + 0.5f;
+    const float w1 = (float)i1 - 0.5f * M.n1 + 0.5f;
+    const float w2 = (float)i2 - 0.5f * M.n2 + 0.5f;
+    const float J1 = (float)(i1 + kPad), J2 = (float)(i2 + kPad), sf = (float)s;
+    const float NUm = (float)M.NU - 1.f, NVm = (float)M.NV - 1.f;
+    constexpr float kEps = 0.05f;
+    float acc = 0.f;
+    for (unsigned a = 0; a < M.count; ++a) {
+        const float4 gu = M.geo[3 * a], gv = M.geo[3 * a + 1], gd = M.geo[3 * a + 2];
+        const float cu = fmaf(gu.x, w0, fmaf(gu.y, w1, fmaf(gu.z, w2, gu.w)));
+        const float cv = fmaf(gv.x, w0, fmaf(gv.y, w1, fmaf(gv.z, w2, gv.w)));
+        const float cd = fmaf(gd.x, w0, fmaf(gd.y, w1, fmaf(gd.z, w2, gd.w)));
+        float umin = INFINITY, umax = -INFINITY, vmin = INFINITY, vmax = -INFINITY;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const float d1 = (c & 1) ? 1.f : -1.f, d2 = (c & 2) ? 1.f : -1.f;
+            const float r = __builtin_amdgcn_rcpf(fmaf(d2, gd.z, fmaf(d1, gd.y, cd)));
+            const float U = fmaf(d2, gu.z, fmaf(d1, gu.y, cu)) * r;
+            const float V = fmaf(d2, gv.z, fmaf(d1, gv.y, cv)) * r;
+            umin = fminf(umin, U);
+            umax = fmaxf(umax, U);
+            vmin = fminf(vmin, V);
+            vmax = fmaxf(vmax, V);
+        }
+        // clamped in float before the conversion: a NaN bound becomes a full
+        // (in-range) scan, an infinite one an empty scan
+        const int u0 = (int)fminf(fmaxf(ceilf(umin - kEps), 0.f), NUm + 1.f);
+        const int u1 = (int)fmaxf(fminf(floorf(umax + kEps), NUm), -1.f);
+        const int v0 = (int)fminf(fmaxf(ceilf(vmin - kEps), 0.f), NVm + 1.f);
+        const int v1 = (int)fmaxf(fminf(floorf(vmax + kEps), NVm), -1.f);
+        // per-angle tables at a scalar base; 32-bit byte offsets inside
+        const size_t abase = (size_t)a * M.NV * M.NU;
+        const float4 *ra = M.rays + abase;
+        const float *ya = M.yw + abase;
+        for (int V = v0; V <= v1; ++V) {
+            const unsigned rb = (unsigned)V * M.NU;
+            for (int U = u0; U <= u1; ++U) {
+                const unsigned o = rb + (unsigned)U;
+                const float4 r = ld4a(ra, o << 4);
+                const float y = ld1b(ya, o << 2);
+                const float e1 = fmaxf(1.f - fabsf(fmaf(r.x, sf, r.y) - J1), 0.f);
+                const float e2 = fmaxf(1.f - fabsf(fmaf(r.z, sf, r.w) - J2), 0.f);
+                acc = fmaf(y * e1, e2, acc);
+            }
+        }
+    }
+    unsigned x, y, z;
+    if (M.group == 0) { x = s; y = i1; z = i2; }
+    else if (M.group == 1) { y = s; x = i1; z = i2; }
+    else { z = s; x = i1; y = i2; }
+    M.vol[((size_t)z * M.ny + y) * M.nx + x] += acc;
 }
 
 // ------------------------------------------------------------ launch -----
@@ -861,26 +987,52 @@ void bp3d_matched_launch(const Geom3D &g, const float *proj, float *vol, float o
     AMX_REQUIRE(ss >= 1 && ss <= 16, "bp3d: rays per detector dimension must be in [1, 16]");
     const size_t nvox = (size_t)g.nx * g.ny * g.nz;
     if (mode == OUT_SET) fill_launch(vol, nvox, 0.f, s);
-    size_t need = 0;
-    for (int k = 0; k < 3; ++k)
-        if (g.dir_count[k]) need = std::max(need, group_elems(g, k));
-    Scratch acc(need * sizeof(float), s);
+    const size_t NU = (size_t)g.nu * ss, NV = (size_t)g.nv * ss;
+    // 32-bit byte offsets into one angle's ray table
+    AMX_REQUIRE(NU * NV * sizeof(float4) < (1ull << 32),
+                "bp3d: detector too large for the matched adjoint");
+    // ray tables are built per chunk of angles, at most ~1 GiB at a time
+    const size_t per_angle = NU * NV * (sizeof(float4) + sizeof(float));
+    const unsigned chunk = (unsigned)std::max<size_t>(
+        1, std::min<size_t>(g.na, (size_t(1) << 30) / per_angle));
+    const bool cone = g.kind == ASTRA_AMD_GEOM_CONE;
+    Scratch rays(per_angle * chunk, s), geo((size_t)chunk * 3 * sizeof(float4), s);
+    float4 *rp = rays.as<float4>();
+    float *yp = reinterpret_cast<float *>(rp + NU * NV * chunk);
     for (int k = 0; k < 3; ++k) {
         if (!g.dir_count[k]) continue;
-        fill_launch(acc.as<float>(), group_elems(g, k), 0.f, s);
         FPArgs A = fp_args(g, k, out_scale);
-        A.volp = acc.as<float>();
         A.ss = ss;
-        const bool cone = g.kind == ASTRA_AMD_GEOM_CONE;
-        fp_group_launches(A, g.dir_count[k], [&](const FPArgs &a, unsigned groups) {
-            if (cone)
-                fp3d_transpose_kernel<true><<<groups, 256, 0, s>>>(a, proj);
-            else
-                fp3d_transpose_kernel<false><<<groups, 256, 0, s>>>(a, proj);
-        }, s);
-        unsigned blocks = (unsigned)std::min<size_t>((nvox + 255) / 256, 1u << 20);
-        unpad_group_add<<<blocks, 256, 0, s>>>(acc.as<float>(), vol, g.nx, g.ny, g.nz, k);
-        AMX_LAUNCHED();
+        MatchedArgs M{};
+        M.rays = rp;
+        M.yw = yp;
+        M.geo = geo.as<float4>();
+        M.NU = (unsigned)NU;
+        M.NV = (unsigned)NV;
+        M.n0 = A.n0; M.n1 = A.n1; M.n2 = A.n2;
+        M.tiles1 = (A.n1 + 15) / 16;
+        M.nx = g.nx; M.ny = g.ny;
+        M.group = k;
+        M.vol = vol;
+        const size_t gblocks = (size_t)A.n0 * M.tiles1 * ((A.n2 + 15) / 16);
+        AMX_REQUIRE(gblocks < (1ull << 31), "bp3d: volume too large for one matched launch");
+        for (unsigned first = 0; first < (unsigned)g.dir_count[k]; first += chunk) {
+            A.lfirst = first;
+            A.lcount = std::min((unsigned)g.dir_count[k] - first, chunk);
+            const size_t nrays = NU * NV * A.lcount;
+            const unsigned rblocks = (unsigned)std::min<size_t>((nrays + 255) / 256, 1u << 20);
+            const unsigned gb = (A.lcount + 63) / 64;
+            if (cone) {
+                matched_rays_kernel<true><<<rblocks, 256, 0, s>>>(A, proj, rp, yp);
+                matched_geo_kernel<true><<<gb, 64, 0, s>>>(A, geo.as<float4>());
+            } else {
+                matched_rays_kernel<false><<<rblocks, 256, 0, s>>>(A, proj, rp, yp);
+                matched_geo_kernel<false><<<gb, 64, 0, s>>>(A, geo.as<float4>());
+            }
+            M.count = A.lcount;
+            bp3d_matched_gather<<<(unsigned)gblocks, 256, 0, s>>>(M);
+            AMX_LAUNCHED();
+        }
     }
 }
 

@@ -169,49 +169,54 @@ def _ray_scatter(vol, d, a1, a2, b1, b2, vals):
         plane += tex2d_T(plane.shape, f1, f2, vals)
 
 
-def cone_fp_T(proj, vecs, shape, vol_scale=(1.0, 1.0, 1.0), out_scale=1.0):
+def cone_fp_T(proj, vecs, shape, vol_scale=(1.0, 1.0, 1.0), out_scale=1.0, ss=1):
     """Exact transpose of ``cone_fp`` (the matched adjoint: the same rays,
-    samples and weights as the forward projection, scattered)."""
+    samples and weights as the forward projection, scattered); ss x ss
+    sub-rays per pixel as in ``cone_fp``."""
     proj = np.asarray(proj, dtype=np.float64)
     n_v, nA, n_u = proj.shape
     vol = np.zeros(shape)
-    uu, vv = np.meshgrid(np.arange(n_u) + 0.5, np.arange(n_v) + 0.5)
     for a in range(nA):
         vec = vecs[a]
         d = cone_direction(vec, n_u, n_v)
         c0, c1, c2 = _DIRS[d]
         src = vec[0:3]
-        det = [vec[3 + k] + uu * vec[6 + k] + vv * vec[9 + k] for k in range(3)]
-        den = src[c0] - det[c0]
-        a1 = (src[c1] - det[c1]) / den
-        a2 = (src[c2] - det[c2]) / den
-        b1 = src[c1] - a1 * src[c0]
-        b2 = src[c2] - a2 * src[c0]
         s1, s2, osc = _fp_scale(d, vol_scale, out_scale)
-        vals = proj[:, a, :] * np.sqrt(a1 * a1 * s1 + a2 * a2 * s2 + 1.0) * osc
-        _ray_scatter(vol, d, a1, a2, b1, b2, vals)
+        for ou in _sub_offsets(ss):
+            for ov in _sub_offsets(ss):
+                uu, vv = np.meshgrid(np.arange(n_u) + ou, np.arange(n_v) + ov)
+                det = [vec[3 + k] + uu * vec[6 + k] + vv * vec[9 + k] for k in range(3)]
+                den = src[c0] - det[c0]
+                a1 = (src[c1] - det[c1]) / den
+                a2 = (src[c2] - det[c2]) / den
+                b1 = src[c1] - a1 * src[c0]
+                b2 = src[c2] - a2 * src[c0]
+                vals = proj[:, a, :] * np.sqrt(a1 * a1 * s1 + a2 * a2 * s2 + 1.0) * osc / (ss * ss)
+                _ray_scatter(vol, d, a1, a2, b1, b2, vals)
     return vol
 
 
-def par3d_fp_T(proj, vecs, shape, vol_scale=(1.0, 1.0, 1.0), out_scale=1.0):
-    """Exact transpose of ``par3d_fp``."""
+def par3d_fp_T(proj, vecs, shape, vol_scale=(1.0, 1.0, 1.0), out_scale=1.0, ss=1):
+    """Exact transpose of ``par3d_fp`` (ss x ss sub-rays per pixel)."""
     proj = np.asarray(proj, dtype=np.float64)
     n_v, nA, n_u = proj.shape
     vol = np.zeros(shape)
-    uu, vv = np.meshgrid(np.arange(n_u) + 0.5, np.arange(n_v) + 0.5)
     for a in range(nA):
         vec = vecs[a]
         d = par3d_direction(vec)
         c0, c1, c2 = _DIRS[d]
         ray = vec[0:3]
-        det = [vec[3 + k] + uu * vec[6 + k] + vv * vec[9 + k] for k in range(3)]
         a1 = ray[c1] / ray[c0]
         a2 = ray[c2] / ray[c0]
-        b1 = det[c1] - a1 * det[c0]
-        b2 = det[c2] - a2 * det[c0]
         s1, s2, osc = _fp_scale(d, vol_scale, out_scale, par3d=True)
-        vals = proj[:, a, :] * math.sqrt(a1 * a1 * s1 + a2 * a2 * s2 + 1.0) * osc
-        _ray_scatter(vol, d, np.full_like(b1, a1), np.full_like(b2, a2), b1, b2, vals)
+        vals = proj[:, a, :] * math.sqrt(a1 * a1 * s1 + a2 * a2 * s2 + 1.0) * osc / (ss * ss)
+        for ou in _sub_offsets(ss):
+            for ov in _sub_offsets(ss):
+                uu, vv = np.meshgrid(np.arange(n_u) + ou, np.arange(n_v) + ov)
+                det = [vec[3 + k] + uu * vec[6 + k] + vv * vec[9 + k] for k in range(3)]
+                b1 = det[c1] - a1 * det[c0]
+                b2 = det[c2] - a2 * det[c0]
+                _ray_scatter(vol, d, np.full_like(b1, a1), np.full_like(b2, a2), b1, b2, vals)
     return vol
 
 

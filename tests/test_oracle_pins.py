@@ -346,6 +346,21 @@ def test_par3d_transpose_dot_product_fp64():
     assert rel < 1e-12
 
 
+def test_supersampled_3d_transposes_dot_product_fp64():
+    n, na, nu, nv = 9, 5, 10, 11
+    ang = np.linspace(0, 2 * np.pi, na, False)
+    v, vs = OG.normalise_3d(OG.gen_cone_projections(ang, nu, nv, 30.0, 20.0, 1.5, 1.2),
+                            "cone", OG.vol_geom_3d(n, n, n))
+    rel = _dot_rel(lambda x: O3.cone_fp(x, v, nu, nv, vs, ss=2),
+                   lambda y: O3.cone_fp_T(y, v, (n, n, n), vs, ss=2), (n, n, n), (nv, na, nu), 4)
+    assert rel < 1e-12
+    v3, vs3 = OG.normalise_3d(OG.gen_par3d_projections(ang, nu, nv, 1.1, 0.9), "parallel3d",
+                              OG.vol_geom_3d(n, n, n))
+    rel = _dot_rel(lambda x: O3.par3d_fp(x, v3, nu, nv, vs3, ss=3),
+                   lambda y: O3.par3d_fp_T(y, v3, (n, n, n), vs3, ss=3), (n, n, n), (nv, na, nu), 5)
+    assert rel < 1e-12
+
+
 def test_2d_transposes_dot_product_fp64():
     n, na, nd = 17, 11, 25
     ang = np.linspace(0, np.pi, na, False)
