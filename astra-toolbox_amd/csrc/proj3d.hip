@@ -646,25 +646,33 @@ struct MatchedArgs {
     float *vol;  // [z][y][x], += the chunk's contribution
 };
 
-// One thread per voxel: a workgroup is a 16 (c1) x 16 (c2) tile of one
-// slice; consecutive workgroups take consecutive slices of the same tile
-// (their candidate rays overlap).  Per angle the candidate rays are the
-// lattice points inside the image of the voxel's open neighbourhood square
-// (c1 +- 1, c2 +- 1), bounded by its four projected corners (widened by
+// One thread per S consecutive voxels along the slice axis c0 (S slices
+// share one pass over their candidate rays: every candidate is loaded once
+// and weighted for each slice).  A workgroup is a 16 (c1) x 16 (c2) tile
+// of S slices; consecutive workgroups take consecutive slice groups of the
+// same tile (their candidate rays overlap).  Per angle the candidate rays
+// are the lattice points inside the image of the voxels' open
+// neighbourhood squares (c1 +- 1, c2 +- 1) over the S slices, bounded by
+// the projected corners of the first and last slice's squares (a central
+// or parallel projection maps each corner's path to a segment), widened by
 // 0.05 lattice units for fp32 rounding of the map; extra candidates get
-// weight 0).
+// weight 0.
+template <int S>
 __global__ __launch_bounds__(256) void bp3d_matched_gather(MatchedArgs M) {
-    const unsigned s = blockIdx.x % M.n0, tile = blockIdx.x / M.n0;
+    const unsigned sgroups = (M.n0 + S - 1) / S;
+    const unsigned s0 = (blockIdx.x % sgroups) * S, tile = blockIdx.x / sgroups;
     const int i1 = (int)((tile % M.tiles1) * 16 + (threadIdx.x & 15));
     const int i2 = (int)((tile / M.tiles1) * 16 + (threadIdx.x >> 4));
     if (i1 >= (int)M.n1 || i2 >= (int)M.n2) return;
-    const float w0 = (float)s - 0.5f * M.n0 + 0.5f;
+    const float w0 = (float)s0 - 0.5f * M.n0 + 0.5f;
     const float w1 = (float)i1 - 0.5f * M.n1 + 0.5f;
     const float w2 = (float)i2 - 0.5f * M.n2 + 0.5f;
-    const float J1 = (float)(i1 + kPad), J2 = (float)(i2 + kPad), sf = (float)s;
+    const float J1 = (float)(i1 + kPad), J2 = (float)(i2 + kPad), sf = (float)s0;
     const float NUm = (float)M.NU - 1.f, NVm = (float)M.NV - 1.f;
     constexpr float kEps = 0.05f;
-    float acc = 0.f;
+    float acc[S];
+#pragma unroll
+    for (int k = 0; k < S; ++k) acc[k] = 0.f;
     for (unsigned a = 0; a < M.count; ++a) {
         const float4 gu = M.geo[3 * a], gv = M.geo[3 * a + 1], gd = M.geo[3 * a + 2];
         const float cu = fmaf(gu.x, w0, fmaf(gu.y, w1, fmaf(gu.z, w2, gu.w)));
@@ -672,11 +680,13 @@ __global__ __launch_bounds__(256) void bp3d_matched_gather(MatchedArgs M) {
         const float cd = fmaf(gd.x, w0, fmaf(gd.y, w1, fmaf(gd.z, w2, gd.w)));
         float umin = INFINITY, umax = -INFINITY, vmin = INFINITY, vmax = -INFINITY;
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
+        for (int c = 0; c < (S > 1 ? 8 : 4); ++c) {
+            const float d0 = (c & 4) ? (float)(S - 1) : 0.f;
             const float d1 = (c & 1) ? 1.f : -1.f, d2 = (c & 2) ? 1.f : -1.f;
-            const float r = __builtin_amdgcn_rcpf(fmaf(d2, gd.z, fmaf(d1, gd.y, cd)));
-            const float U = fmaf(d2, gu.z, fmaf(d1, gu.y, cu)) * r;
-            const float V = fmaf(d2, gv.z, fmaf(d1, gv.y, cv)) * r;
+            const float r = __builtin_amdgcn_rcpf(
+                fmaf(d2, gd.z, fmaf(d1, gd.y, fmaf(d0, gd.x, cd))));
+            const float U = fmaf(d2, gu.z, fmaf(d1, gu.y, fmaf(d0, gu.x, cu))) * r;
+            const float V = fmaf(d2, gv.z, fmaf(d1, gv.y, fmaf(d0, gv.x, cv))) * r;
             umin = fminf(umin, U);
             umax = fmaxf(umax, U);
             vmin = fminf(vmin, V);
@@ -698,17 +708,26 @@ __global__ __launch_bounds__(256) void bp3d_matched_gather(MatchedArgs M) {
                 const unsigned o = rb + (unsigned)U;
                 const float4 r = ld4a(ra, o << 4);
                 const float y = ld1b(ya, o << 2);
-                const float e1 = fmaxf(1.f - fabsf(fmaf(r.x, sf, r.y) - J1), 0.f);
-                const float e2 = fmaxf(1.f - fabsf(fmaf(r.z, sf, r.w) - J2), 0.f);
-                acc = fmaf(y * e1, e2, acc);
+#pragma unroll
+                for (int k = 0; k < S; ++k) {
+                    const float sk = sf + (float)k;
+                    const float e1 = fmaxf(1.f - fabsf(fmaf(r.x, sk, r.y) - J1), 0.f);
+                    const float e2 = fmaxf(1.f - fabsf(fmaf(r.z, sk, r.w) - J2), 0.f);
+                    acc[k] = fmaf(y * e1, e2, acc[k]);
+                }
             }
         }
     }
-    unsigned x, y, z;
-    if (M.group == 0) { x = s; y = i1; z = i2; }
-    else if (M.group == 1) { y = s; x = i1; z = i2; }
-    else { z = s; x = i1; y = i2; }
-    M.vol[((size_t)z * M.ny + y) * M.nx + x] += acc;
+#pragma unroll
+    for (int k = 0; k < S; ++k) {
+        const unsigned s = s0 + k;
+        if (s >= M.n0) break;
+        unsigned x, y, z;
+        if (M.group == 0) { x = s; y = i1; z = i2; }
+        else if (M.group == 1) { y = s; x = i1; z = i2; }
+        else { z = s; x = i1; y = i2; }
+        M.vol[((size_t)z * M.ny + y) * M.nx + x] += acc[k];
+    }
 }
 
 // ------------------------------------------------------------ launch -----
@@ -982,6 +1001,17 @@ void bp3d_launch(const Geom3D &g, const float *proj, float *vol, float out_scale
     bp3d_run_pairs(g, pairs.as<float2>(), vol, out_scale, kernel, mode, upd_w, lo, hi, s, ss);
 }
 
+// slices per thread of the matched gather (ASTRA_AMD_MATCHED_SLICES:
+// 1, 2, 4, 8 or 16; default 8)
+static int matched_slices() {
+    static const int v = [] {
+        const char *e = std::getenv("ASTRA_AMD_MATCHED_SLICES");
+        const int x = e ? std::atoi(e) : 8;
+        return (x == 1 || x == 2 || x == 4 || x == 16) ? x : 8;
+    }();
+    return v;
+}
+
 void bp3d_matched_launch(const Geom3D &g, const float *proj, float *vol, float out_scale,
                          int mode, hipStream_t s, unsigned ss) {
     AMX_REQUIRE(ss >= 1 && ss <= 16, "bp3d: rays per detector dimension must be in [1, 16]");
@@ -1014,7 +1044,8 @@ void bp3d_matched_launch(const Geom3D &g, const float *proj, float *vol, float o
         M.nx = g.nx; M.ny = g.ny;
         M.group = k;
         M.vol = vol;
-        const size_t gblocks = (size_t)A.n0 * M.tiles1 * ((A.n2 + 15) / 16);
+        const int S = matched_slices();
+        const size_t gblocks = (size_t)((A.n0 + S - 1) / S) * M.tiles1 * ((A.n2 + 15) / 16);
         AMX_REQUIRE(gblocks < (1ull << 31), "bp3d: volume too large for one matched launch");
         for (unsigned first = 0; first < (unsigned)g.dir_count[k]; first += chunk) {
             A.lfirst = first;
@@ -1030,7 +1061,16 @@ void bp3d_matched_launch(const Geom3D &g, const float *proj, float *vol, float o
                 matched_geo_kernel<false><<<gb, 64, 0, s>>>(A, geo.as<float4>());
             }
             M.count = A.lcount;
-            bp3d_matched_gather<<<(unsigned)gblocks, 256, 0, s>>>(M);
+            if (S == 1)
+                bp3d_matched_gather<1><<<(unsigned)gblocks, 256, 0, s>>>(M);
+            else if (S == 2)
+                bp3d_matched_gather<2><<<(unsigned)gblocks, 256, 0, s>>>(M);
+            else if (S == 4)
+                bp3d_matched_gather<4><<<(unsigned)gblocks, 256, 0, s>>>(M);
+            else if (S == 8)
+                bp3d_matched_gather<8><<<(unsigned)gblocks, 256, 0, s>>>(M);
+            else
+                bp3d_matched_gather<16><<<(unsigned)gblocks, 256, 0, s>>>(M);
             AMX_LAUNCHED();
         }
     }

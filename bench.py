@@ -200,6 +200,28 @@ def differentiable_bench(n=256, na=360, nd=384, iters=20):
             "unrolled_sirt": res}
 
 
+TRAFFIC_FILE = "profiles/cone512_fpbp_counters.md"
+
+
+def measured_traffic(kernel, n, na, nd):
+    """HBM bytes per launch of `kernel` at the 512^3 / 720 x 768^2 workload,
+    from the calibrated FETCH_SIZE + WRITE_SIZE profiler passes recorded in
+    TRAFFIC_FILE (profiler counters cannot be read from inside this run);
+    None for any other workload."""
+    if (n, na, nd) != (512, 720, 768):
+        return None
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), TRAFFIC_FILE)
+    try:
+        with open(path) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
+                    return float(parts[2])
+    except OSError:
+        return None
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -335,7 +357,9 @@ def main():
                    "volume": [n, n, n], "angles": na, "detector": [nd, nd],
                    "source_origin": sod, "origin_detector": odd},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(dom, n, na, nd),
+                     "traffic_source": TRAFFIC_FILE if measured_traffic(dom, n, na, nd) else None,
                      "bytes_per_launch": bytes_per_launch,
                      "avg_launch_ms": kd["avg_launch_ms"]},
         "fdk_seconds": fdk_s,
