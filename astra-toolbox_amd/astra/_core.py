@@ -114,7 +114,8 @@ class Projector:
                     nv, na, nu = self.proj_shape
                     v = np.ascontiguousarray(self.vecs, dtype=np.float64)
                     vs = (ctypes.c_float * 3)(*self.vol_scale)
-                    kind = _lib.GEOM_CONE if self.kind == "cone" else _lib.GEOM_PAR3D
+                    kind = {"cone": _lib.GEOM_CONE, "par3d": _lib.GEOM_PAR3D,
+                            "cylcone": _lib.GEOM_CYLCONE}[self.kind]
                     _lib.call("astra_amd_geom3d_create", kind, nx, ny, nz, na, nu, nv,
                               v.ctypes.data_as(ctypes.c_void_p), ctypes.cast(vs, ctypes.c_void_p),
                               ctypes.byref(out))

@@ -17,7 +17,7 @@ LIB_PATH = os.environ.get(
 OK = 0
 ERR_ARG, ERR_HIP, ERR_UNSUPPORTED, ERR_ALLOC, ERR_INTERNAL = 1, 2, 3, 4, 5
 MODE_ADD, MODE_SET = 0, 1
-GEOM_PAR, GEOM_FAN, GEOM_PAR3D, GEOM_CONE = 0, 1, 2, 3
+GEOM_PAR, GEOM_FAN, GEOM_PAR3D, GEOM_CONE, GEOM_CYLCONE = 0, 1, 2, 3, 4
 KER3D_DEFAULT, KER3D_SUM_SQUARE_WEIGHTS, KER3D_FDK_WEIGHTING, KER3D_2D_WEIGHTING, KER3D_MATCHED_BP = range(5)
 ALGO_SIRT, ALGO_CGLS, ALGO_EM, ALGO_SART = 0, 1, 2, 3
 

@@ -167,19 +167,32 @@ def vectors_3d(pg):
         v[:, 3:6] = v[:, 3:6] - 0.5 * nv * v[:, 9:12] - 0.5 * nu * v[:, 6:9]
         return ("cone" if t == "cone_vec" else "par3d"), v
     if t == "cyl_cone_vec":
-        raise AstraError("cyl_cone_vec geometries are not supported by this build")
+        # 13-tuples (src, detector centre, u, v, radius), kept as given
+        # (src/CylConeVecProjectionGeometry3D.cpp:80-112)
+        v = np.array(pg["Vectors"], dtype=np.float64)
+        if v.ndim != 2 or v.shape[1] != 13:
+            raise AstraError("Vectors should be an Nx13 matrix")
+        if not (v[:, 12] > 0).all():
+            raise AstraError("cyl_cone_vec: the detector radius must be positive")
+        return "cylcone", v
     raise AstraError("projection geometry '%s' is not a 3D parallel/cone geometry" % t)
 
 
 def normalise_3d(kind, vecs, vg):
     """Translate by -window centre, scale per axis by 1/voxel size; return
-    (vectors, vol_scale) (src/GeometryUtil3D.cpp:569-593)."""
+    (vectors, vol_scale) (src/GeometryUtil3D.cpp:569-593).  The cylindrical
+    detector is translated only: its kernels work in world units with the
+    voxel sizes (convertAstraGeometry_unscaled_internal, :595-616)."""
     d, fac = [], []
     for ax, n in (("X", vg["GridColCount"]), ("Y", vg["GridRowCount"]), ("Z", vg["GridSliceCount"])):
         lo, hi = _window(vg, ax, n)
         d.append(float(-(lo + hi) / f32(2)))
         fac.append(float(f32(1.0) / f32((hi - lo) / f32(n))))
     out = np.array(vecs, dtype=np.float64)
+    if kind == "cylcone":
+        out[:, 0:3] += d
+        out[:, 3:6] += d
+        return out, tuple(float(p) for p in pixel_lengths(vg))
     if kind == "cone":
         out[:, 0:3] += d
     out[:, 3:6] += d

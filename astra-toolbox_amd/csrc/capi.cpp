@@ -214,6 +214,11 @@ int astra_amd_fp3d(astra_amd_geom3d g, const float *vol, float *proj,
         AMX_REQUIRE(vol && proj, "fp3d: NULL data pointer");
         AMX_REQUIRE(p->kernel >= 0 && p->kernel <= ASTRA_AMD_KER3D_MATCHED_BP,
                     "fp3d: unknown kernel %d", p->kernel);
+        // cuda/3d/mem3d.cu:385-393: default and matched_bp only
+        if (g->g->kind == ASTRA_AMD_GEOM_CYLCONE && p->kernel != ASTRA_AMD_KER3D_DEFAULT &&
+            p->kernel != ASTRA_AMD_KER3D_MATCHED_BP)
+            amx::fail(ASTRA_AMD_ERR_UNSUPPORTED, "fp3d: the cylindrical-detector geometry supports "
+                      "the default and matched_bp kernels only");
         if (p->kernel == ASTRA_AMD_KER3D_SUM_SQUARE_WEIGHTS) {
             if (ss_of(p->rays_per_det_dim) > 1)
                 amx::fail(ASTRA_AMD_ERR_UNSUPPORTED,
@@ -449,6 +454,13 @@ int astra_amd_algo3d_create(int type, astra_amd_geom3d g, const astra_amd_params
         check3d(g, p);
         AMX_REQUIRE(p->kernel == ASTRA_AMD_KER3D_DEFAULT || p->kernel == ASTRA_AMD_KER3D_2D_WEIGHTING,
                     "algo3d: projection kernel %d not usable for SIRT/CGLS", p->kernel);
+        if (g->g->kind == ASTRA_AMD_GEOM_CYLCONE) {
+            // src/CudaSirtAlgorithm3D.cpp:155-159, src/CudaCglsAlgorithm3D.cpp:156-160
+            AMX_REQUIRE(ss_of(p->rays_per_det_dim) == 1 && ss_of(p->rays_per_voxel_dim) == 1,
+                        "Detector/voxel supersampling is not supported for cyl_cone_vec geometry.");
+            AMX_REQUIRE(p->kernel == ASTRA_AMD_KER3D_DEFAULT,
+                        "algo3d: the cylindrical-detector geometry supports the default kernel only");
+        }
         amx::AlgoBase *a = amx::algo_build(type, nullptr, g->g, p->output_scale, p->kernel,
                                            ss_of(p->rays_per_det_dim), ss_of(p->rays_per_voxel_dim));
         *out = new astra_amd_algo_s{a, g->g->device};

@@ -166,12 +166,63 @@ void par3d_bp_coeffs(const double *p, int kernel, const float *vs, BP3DAngle &o,
         w = (float)(1.0 / scaled_cross_norm(u, v, vs));
 }
 
+inline double dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline double norm(V3 a) { return std::sqrt(dot(a, a)); }
+inline V3 scale(V3 a, double f) { return {a.x * f, a.y * f, a.z * f}; }
+inline V3 cross(V3 a, V3 b) { return {det3x(a, b), det3y(a, b), det3z(a, b)}; }
+
+// Cylinder axes of one cyl_cone_vec row [src, detC, u, v, R]
+// (getCylConeAxes, src/GeometryUtil3D.cpp:542-561): radial a from the axis
+// to the detector centre (away from the source), tangential b, centre c.
+void cyl_axes(const double *p, V3 &a, V3 &b, V3 &c) {
+    V3 s{p[0], p[1], p[2]}, d{p[3], p[4], p[5]}, u{p[6], p[7], p[8]}, v{p[9], p[10], p[11]};
+    const double R = p[12];
+    a = scale(cross(u, v), -R / (norm(u) * norm(v)));
+    V3 dm{d.x - a.x - s.x, d.y - a.y - s.y, d.z - a.z - s.z};
+    V3 dp{d.x + a.x - s.x, d.y + a.y - s.y, d.z + a.z - s.z};
+    if (norm(dm) > norm(dp)) a = scale(a, -1.0);
+    c = sub(d, a);
+    b = scale(u, R / norm(u));
+}
+
+// DevCylConeParamsBP (cuda/3d/cone_cyl_bp.cu:165-230)
+void cyl_bp_coeffs(const double *p, CylBPAngle &o) {
+    V3 s{p[0], p[1], p[2]}, u{p[6], p[7], p[8]}, v{p[9], p[10], p[11]};
+    const double R = p[12];
+    V3 a, b, c;
+    cyl_axes(p, a, b, c);
+    const V3 e[3] = {scale(a, 1.0 / norm(a)), scale(b, 1.0 / norm(b)), scale(v, 1.0 / norm(v))};
+    // ray density 1 / t^2 after scaling the ray components by sqrt(|u x v|)
+    const double f = std::sqrt(norm(cross(u, v)));
+    float *rows[3] = {o.ca, o.cb, o.cc};
+    const V3 smc = sub(s, c);
+    double sm[3];
+    for (int k = 0; k < 3; ++k) {
+        rows[k][0] = (float)(e[k].x * f);
+        rows[k][1] = (float)(e[k].y * f);
+        rows[k][2] = (float)(e[k].z * f);
+        rows[k][3] = (float)(-dot(s, e[k]) * f);
+        sm[k] = dot(smc, e[k]);
+    }
+    o.sa = (float)sm[0];
+    o.sb = (float)sm[1];
+    o.sc = (float)sm[2];
+    o.c = (float)(sm[0] * sm[0] + sm[1] * sm[1] - R * R);
+    o.inv_dth = (float)(R / norm(u));
+    o.inv_nv = (float)(1.0 / norm(v));
+    o.inv_r = (float)(1.0 / R);
+    o.pad_ = 0.f;
+}
+
 }  // namespace
 
 Geom3D *geom3d_build(int kind, unsigned nx, unsigned ny, unsigned nz, unsigned na, unsigned nu,
                      unsigned nv, const double *vecs, const float *vol_scale) {
-    AMX_REQUIRE(kind == ASTRA_AMD_GEOM_CONE || kind == ASTRA_AMD_GEOM_PAR3D,
-                "geom3d: kind must be CONE or PAR3D (got %d)", kind);
+    AMX_REQUIRE(kind == ASTRA_AMD_GEOM_CONE || kind == ASTRA_AMD_GEOM_PAR3D ||
+                    kind == ASTRA_AMD_GEOM_CYLCONE,
+                "geom3d: kind must be CONE, PAR3D or CYLCONE (got %d)", kind);
+    const bool cyl = kind == ASTRA_AMD_GEOM_CYLCONE;
+    const size_t per = cyl ? 13 : 12;
     AMX_REQUIRE(nx > 0 && ny > 0 && nz > 0 && na > 0 && nu > 0 && nv > 0,
                 "geom3d: all dimensions must be positive");
     AMX_REQUIRE(vecs != nullptr, "geom3d: vecs is NULL");
@@ -181,23 +232,59 @@ Geom3D *geom3d_build(int kind, unsigned nx, unsigned ny, unsigned nz, unsigned n
     const float one[3] = {1.f, 1.f, 1.f};
     const float *vs = vol_scale ? vol_scale : one;
     AMX_REQUIRE(vs[0] > 0 && vs[1] > 0 && vs[2] > 0, "geom3d: voxel sizes must be positive");
-    for (size_t k = 0; k < 12 * (size_t)na; ++k)
-        AMX_REQUIRE(std::isfinite(vecs[k]), "geom3d: non-finite vector component (angle %zu)", k / 12);
+    for (size_t k = 0; k < per * (size_t)na; ++k)
+        AMX_REQUIRE(std::isfinite(vecs[k]), "geom3d: non-finite vector component (angle %zu)",
+                    k / per);
+    if (cyl)
+        for (unsigned a = 0; a < na; ++a)
+            AMX_REQUIRE(vecs[13 * (size_t)a + 12] > 0.0, "geom3d: cylinder radius must be positive "
+                        "(angle %u)", a);
 
     auto *g = new Geom3D();
     g->kind = kind;
     g->nx = nx; g->ny = ny; g->nz = nz; g->na = na; g->nu = nu; g->nv = nv;
     for (int k = 0; k < 3; ++k) g->vol_scale[k] = vs[k];
     AMX_HIP(hipGetDevice(&g->device));
-    g->vecs.assign(vecs, vecs + 12 * (size_t)na);
+    g->vecs.assign(vecs, vecs + per * (size_t)na);
     g->fp_host.resize(na);
     g->dir_count.assign(3, 0);
 
     std::vector<BP3DAngle> bd(na), bf(na), b2(na);
+    std::vector<CylBPAngle> bc(cyl ? na : 0);
     std::vector<float> wd(na, 0.f), w2(na, 0.f);
     for (unsigned a = 0; a < na; ++a) {
-        const double *p = vecs + 12 * (size_t)a;
+        const double *p = vecs + per * (size_t)a;
         int dir;
+        if (cyl) {
+            // cuda/3d/cone_cyl.cu:486-500: |src - detector centre| per axis
+            // in world units, compared as float
+            dir = principal_axis(std::fabs((float)(p[0] - p[3])), std::fabs((float)(p[1] - p[4])),
+                                 std::fabs((float)(p[2] - p[5])));
+            FP3DAngle &f = g->fp_host[a];
+            f = FP3DAngle{};
+            f.dir = dir;
+            g->dir_count[dir]++;
+            // rays in voxel units: every point scales by 1 / voxel size per
+            // axis (the world-unit ray setup of cylcone_FP_t,
+            // cuda/3d/cone_cyl.cu:260-284, is the same line)
+            V3 ax, bx, cx;
+            cyl_axes(p, ax, bx, cx);
+            const double A3[3] = {ax.x, ax.y, ax.z}, B3[3] = {bx.x, bx.y, bx.z},
+                         C3[3] = {cx.x, cx.y, cx.z};
+            const int *pm = kPerm[dir];
+            for (int k = 0; k < 3; ++k) {
+                const int c = pm[k];
+                const double iv = 1.0 / vs[c];
+                f.s[k] = p[c] * iv;
+                f.p0[k] = C3[c] * iv;
+                f.u[k] = A3[c] * iv;
+                f.b[k] = B3[c] * iv;
+                f.v[k] = p[9 + c] * iv;
+            }
+            f.dth = std::sqrt(p[6] * p[6] + p[7] * p[7] + p[8] * p[8]) / p[12];
+            cyl_bp_coeffs(p, bc[a]);
+            continue;
+        }
         if (kind == ASTRA_AMD_GEOM_CONE) {
             // cuda/3d/cone_fp.cu:378-387: |src - detector centre| per axis,
             // evaluated in double and compared as float
@@ -212,8 +299,8 @@ Geom3D *geom3d_build(int kind, unsigned nx, unsigned ny, unsigned nz, unsigned n
                                  std::fabs((float)p[2]));
         }
         FP3DAngle &f = g->fp_host[a];
+        f = FP3DAngle{};
         f.dir = dir;
-        f.pad_ = 0;
         g->dir_count[dir]++;
         const int *pm = kPerm[dir];
         for (int k = 0; k < 3; ++k) {
@@ -238,9 +325,9 @@ Geom3D *geom3d_build(int kind, unsigned nx, unsigned ny, unsigned nz, unsigned n
             if (b[a].nu[2] != 0.f || b[a].dn[2] != 0.f) return false;
         return true;
     };
-    g->zc_default = zconst(bd);
-    g->zc_fdk = zconst(bf);
-    g->zc_2d = zconst(b2);
+    g->zc_default = !cyl && zconst(bd);
+    g->zc_fdk = !cyl && zconst(bf);
+    g->zc_2d = !cyl && zconst(b2);
     try {
         g->fp = DevArray(sizeof(FP3DAngle) * na);
         g->fp.upload(g->fp_host.data(), sizeof(FP3DAngle) * na);
@@ -254,6 +341,10 @@ Geom3D *geom3d_build(int kind, unsigned nx, unsigned ny, unsigned nz, unsigned n
         g->bp_w_default.upload(wd.data(), sizeof(float) * na);
         g->bp_w_2d = DevArray(sizeof(float) * na);
         g->bp_w_2d.upload(w2.data(), sizeof(float) * na);
+        if (cyl) {
+            g->bp_cyl = DevArray(sizeof(CylBPAngle) * na);
+            g->bp_cyl.upload(bc.data(), sizeof(CylBPAngle) * na);
+        }
         for (int k = 0; k < 3; ++k) {
             std::vector<int> idx;
             for (unsigned a = 0; a < na; ++a)

@@ -83,13 +83,31 @@ class DevArray {
 // Per-angle forward-projection ray data, FP64, already permuted into the
 // (c0, c1, c2) = (principal axis, in-slice fast axis, in-slice slow axis)
 // order of the angle's direction group.
+// Cylindrical detector (cyl_cone_vec): s = source, p0 = cylinder centre,
+// u = radial vector (axis -> detector centre, length R), b = tangential
+// vector (length R), v = one pixel step along the axis, dth = angular step
+// per pixel; pixel (iu, iv) sits at p0 + u cos(th) + b sin(th) +
+// v (iv - nv/2 + 0.5) with th = (iu - nu/2 + 0.5) dth.
 struct FP3DAngle {
-    double s[3];   // source (cone) or ray direction (par3d)
-    double p0[3];  // centre of detector pixel (0, 0)
-    double u[3];   // one detector pixel step along u
+    double s[3];   // source (cone, cylinder) or ray direction (par3d)
+    double p0[3];  // centre of detector pixel (0, 0) (cylinder: axis centre)
+    double u[3];   // one detector pixel step along u (cylinder: radial)
     double v[3];   // one detector pixel step along v
+    double b[3];   // cylinder: tangential
+    double dth;    // cylinder: angular step per pixel
     int dir;       // 0 = x, 1 = y, 2 = z principal axis
     int pad_;
+};
+
+// Per-angle cylindrical-detector BP coefficients (float32, world units of
+// the translated frame), DevCylConeParamsBP of cuda/3d/cone_cyl_bp.cu:
+// 165-230: rows (x, y, z, w) of the source-relative ray in the (radial,
+// tangential, axial) frame scaled by sqrt(|u x v|); the source in that
+// frame; c = |s - centre|^2 in the radial-tangential plane - R^2.
+struct CylBPAngle {
+    float ca[4], cb[4], cc[4];
+    float sa, sb, sc, c;
+    float inv_dth, inv_nv, inv_r, pad_;
 };
 
 // Per-angle voxel-driven back-projection coefficients (float32), in padded
@@ -132,7 +150,7 @@ struct Filter {
 };
 
 struct Geom3D {
-    int kind;  // ASTRA_AMD_GEOM_PAR3D / _CONE
+    int kind;  // ASTRA_AMD_GEOM_PAR3D / _CONE / _CYLCONE
     unsigned nx, ny, nz, na, nu, nv;
     float vol_scale[3];
     int device;
@@ -142,6 +160,7 @@ struct Geom3D {
     DevArray fp;                 // FP3DAngle[na]
     DevArray bp_default, bp_fdk, bp_2d;  // BP3DAngle[na]
     DevArray bp_w_default, bp_w_2d;      // float[na] par3d weights
+    DevArray bp_cyl;                     // CylBPAngle[na] (cylindrical detector)
     // per BP coefficient set: U and the denominator do not depend on z for
     // any angle (nu[2] == dn[2] == 0, e.g. circular cone with a vertical
     // detector), so the BP hoists them out of its z loop

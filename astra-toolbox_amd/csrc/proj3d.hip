@@ -192,12 +192,24 @@ __device__ __forceinline__ bool fp_map(const FPArgs &A, unsigned &tile, int &ang
 // dwordx2 load with a 32-bit vector offset.
 // (u, v): detector coordinates of the ray relative to pixel (0, 0)'s centre
 // (fractional for sub-rays).
-template <bool CONE>
+// GK: GEO_PAR3D, GEO_CONE (flat detector) or GEO_CYL (cylindrical detector:
+// pixel centre on the cylinder, cuda/3d/cone_cyl.cu:243-258).
+enum { GEO_PAR3D = 0, GEO_CONE = 1, GEO_CYL = 2 };
+
+template <int GK>
 __device__ __forceinline__ void fp3d_ray(const FPArgs &A, const FP3DAngle &G, double u, double v,
                                          float &a1f, float &a2f, float &c1f, float &c2f,
                                          float &scale, int &lo, int &hi) {
+    constexpr bool CONE = GK != GEO_PAR3D;
     double P[3], d[3];
-    for (int k = 0; k < 3; ++k) P[k] = G.p0[k] + u * G.u[k] + v * G.v[k];
+    if (GK == GEO_CYL) {
+        const double th = (u - 0.5 * A.nu + 0.5) * G.dth;
+        const double fv = v - 0.5 * A.nv + 0.5;
+        const double ct = cos(th), st = sin(th);
+        for (int k = 0; k < 3; ++k) P[k] = G.p0[k] + ct * G.u[k] + st * G.b[k] + fv * G.v[k];
+    } else {
+        for (int k = 0; k < 3; ++k) P[k] = G.p0[k] + u * G.u[k] + v * G.v[k];
+    }
     for (int k = 0; k < 3; ++k) d[k] = CONE ? (G.s[k] - P[k]) : G.s[k];
     const double *B = CONE ? G.s : P;
     double a1 = d[1] / d[0], a2 = d[2] / d[0];
@@ -254,7 +266,7 @@ __device__ __forceinline__ float march3d(const FPArgs &A, float a1f, float a2f, 
 // in u and v, each with its own distance weight, averaged
 // (cone_FP_SS_t / par3D_FP_SS_t, cuda/3d/cone_fp.cu:233-323,
 // cuda/3d/par3d_fp.cu:234-325).
-template <bool CONE, bool SS>
+template <int GK, bool SS>
 __global__ __launch_bounds__(256) void fp3d_kernel(FPArgs A) {
     unsigned tile;
     int angle;
@@ -271,7 +283,7 @@ __global__ __launch_bounds__(256) void fp3d_kernel(FPArgs A) {
     if (!SS) {
         float a1f = 0.f, a2f = 0.f, c1f = 0.5f, c2f = 0.5f, scale = 0.f;
         int lo = (int)A.n0, hi = -1;
-        if (active) fp3d_ray<CONE>(A, G, u, v, a1f, a2f, c1f, c2f, scale, lo, hi);
+        if (active) fp3d_ray<GK>(A, G, u, v, a1f, a2f, c1f, c2f, scale, lo, hi);
         const int slo = __builtin_amdgcn_readfirstlane(wave_min(lo));
         const int shi = __builtin_amdgcn_readfirstlane(wave_max(hi));
         val = march3d(A, a1f, a2f, c1f, c2f, slo, shi) * scale;
@@ -282,7 +294,7 @@ __global__ __launch_bounds__(256) void fp3d_kernel(FPArgs A) {
                 float a1f = 0.f, a2f = 0.f, c1f = 0.5f, c2f = 0.5f, scale = 0.f;
                 int lo = (int)A.n0, hi = -1;
                 if (active)
-                    fp3d_ray<CONE>(A, G, u - 0.5 + (k + 0.5) / R, v - 0.5 + (l + 0.5) / R, a1f,
+                    fp3d_ray<GK>(A, G, u - 0.5 + (k + 0.5) / R, v - 0.5 + (l + 0.5) / R, a1f,
                                    a2f, c1f, c2f, scale, lo, hi);
                 const int slo = __builtin_amdgcn_readfirstlane(wave_min(lo));
                 const int shi = __builtin_amdgcn_readfirstlane(wave_max(hi));
@@ -341,7 +353,7 @@ __global__ __launch_bounds__(256) void fp3d_sumsq_kernel(FPArgs A) {
     if (u >= (int)A.nu || v >= (int)A.nv) return;
     float a1f, a2f, c1f, c2f, scale;
     int lo, hi;
-    fp3d_ray<false>(A, A.ang[angle], u, v, a1f, a2f, c1f, c2f, scale, lo, hi);
+    fp3d_ray<GEO_PAR3D>(A, A.ang[angle], u, v, a1f, a2f, c1f, c2f, scale, lo, hi);
     // padded texel coordinate p = f - 0.5 + kPad
     const float o1 = c1f - kPad + 0.5f, o2 = c2f - kPad + 0.5f;
     const float n1 = (float)A.n1, n2 = (float)A.n2;
@@ -515,6 +527,80 @@ __global__ __launch_bounds__(256) void bp3d_ss_kernel(BPArgs A, int R) {
     }
 }
 
+// ------------------------------------------------ cylindrical BP --------
+// Voxel-driven BP from a cylindrical detector (dev_cylcone_BP,
+// cuda/3d/cone_cyl_bp.cu:58-158), in world units of the translated frame:
+// the ray r from the source through the voxel, in the (radial, tangential,
+// axial) frame, meets the cylinder at the larger root t of
+// A t^2 + 2 B t + C = 0 (A = ra^2 + rb^2, B = ra sa + rb sb); the detector
+// coordinates there are u = asin((t rb + sb) / R) / angular step + nu / 2
+// and v = (t rc + sc) / |v| + nv / 2 (texture coordinates); weight t^2.
+// A voxel whose ray misses the cylinder (B^2 < A C) gets nothing from that
+// angle (its coordinates are replaced by an in-bounds dummy).  One thread
+// per (x, y) column of ZS voxels, 16 x 16 threads per group, as bp3d_kernel.
+template <int ZS>
+__global__ __launch_bounds__(256) void bp3d_cyl_kernel(BPArgs A, const CylBPAngle *__restrict__ ang,
+                                                       float vx, float vy, float vz) {
+    const unsigned tiles_x = (A.nx + 15) / 16;
+    const int X = (blockIdx.x % tiles_x) * 16 + (threadIdx.x & 15);
+    const int Y = (blockIdx.x / tiles_x) * 16 + (threadIdx.x >> 4);
+    const int Z0 = blockIdx.y * ZS;
+    if (X >= (int)A.nx || Y >= (int)A.ny) return;
+    const float fx = vx * (X - 0.5f * A.nx + 0.5f);
+    const float fy = vy * (Y - 0.5f * A.ny + 0.5f);
+    const float fz = vz * (Z0 - 0.5f * A.nz + 0.5f);
+    const unsigned pitch = A.nu + 2 * kPad;
+    const size_t plane = (size_t)(A.nv + 2 * kPad) * pitch * 2;
+    const float pitchf = (float)pitch;
+    const float hiu = (float)A.nu + 2.5f, hiv = (float)A.nv + 2.5f;
+    // texture coordinate -> padded texel coordinate: - 0.5 + 2
+    const float ou = 0.5f * A.nu + 1.5f, ov = 0.5f * A.nv + 1.5f;
+    float acc[ZS];
+#pragma unroll
+    for (int i = 0; i < ZS; ++i) acc[i] = 0.f;
+    const float *pl = A.pp;
+    for (unsigned a = 0; a < A.na; ++a, pl += plane) {
+        const CylBPAngle c = ang[a];
+        float ra = fmaf(fz, c.ca[2], fmaf(fy, c.ca[1], fmaf(fx, c.ca[0], c.ca[3])));
+        float rb = fmaf(fz, c.cb[2], fmaf(fy, c.cb[1], fmaf(fx, c.cb[0], c.cb[3])));
+        float rc = fmaf(fz, c.cc[2], fmaf(fy, c.cc[1], fmaf(fx, c.cc[0], c.cc[3])));
+        const float dra = vz * c.ca[2], drb = vz * c.cb[2], drc = vz * c.cc[2];
+#pragma unroll
+        for (int i = 0; i < ZS; ++i) {
+            const float qa = fmaf(ra, ra, rb * rb);
+            const float qb = fmaf(ra, c.sa, rb * c.sb);
+            const float disc = fmaf(qb, qb, -qa * c.c);
+            const bool hit = disc >= 0.f;
+            const float t = hit ? (sqrtf(disc) - qb) / qa : 0.f;
+            const float sn = fminf(fmaxf(fmaf(t, rb, c.sb) * c.inv_r, -1.f), 1.f);
+            const float pu = hit ? clamp_pad(fmaf(asinf(sn), c.inv_dth, ou), hiu) : 0.5f;
+            const float pv = hit ? clamp_pad(fmaf(fmaf(t, rc, c.sc), c.inv_nv, ov), hiv) : 0.5f;
+            const float fu = floorf(pu), fv = floorf(pv);
+            const float val = bilerp_pairs(ld4b(pl, pair_off(fu, fv, pitchf)), pu - fu, pv - fv);
+            acc[i] = fmaf(t * t, val, acc[i]);
+            ra += dra;
+            rb += drb;
+            rc += drc;
+        }
+    }
+    const int nzr = min(ZS, (int)A.nz - Z0);
+    for (int i = 0; i < ZS; ++i) {
+        if (i >= nzr) break;
+        size_t idx = ((size_t)(Z0 + i) * A.ny + Y) * A.nx + X;
+        float val = acc[i] * A.osc;
+        if (A.mode == OUT_ADD) {
+            A.vol[idx] += val;
+        } else if (A.mode == OUT_SET) {
+            A.vol[idx] = val;
+        } else if (A.mode == OUT_EMUPD) {
+            A.vol[idx] *= val * A.uw[idx];
+        } else {  // OUT_UPDATE: clamp(x + w * val)
+            float w = A.uw ? A.uw[idx] : 1.f;
+            A.vol[idx] = fminf(fmaxf(fmaf(w, val, A.vol[idx]), A.lo), A.hi);
+        }
+    }
+}
+
 // --------------------------------------------------- matched BP ----------
 // Exact transpose of fp3d_kernel, voxel-driven (a gather, no atomics).  The
 // forward projector takes, per (sub-)ray r and slice s, the bilinear
@@ -531,7 +617,7 @@ __global__ __launch_bounds__(256) void bp3d_ss_kernel(BPArgs A, int R) {
 // coefficients (a1, c1, a2, c2) exactly as fp3d_ray computes them for the
 // forward projector, and the weighted value y * scale / R^2.  Sub-rays
 // form a lattice (U, V) = (u R + k, v R + l).
-template <bool CONE>
+template <int GK>
 __global__ __launch_bounds__(256) void matched_rays_kernel(FPArgs A, const float *__restrict__ proj,
                                                            float4 *__restrict__ rays,
                                                            float *__restrict__ yw) {
@@ -549,9 +635,9 @@ __global__ __launch_bounds__(256) void matched_rays_kernel(FPArgs A, const float
         float a1f, a2f, c1f, c2f, scale;
         int lo, hi;
         if (R == 1)
-            fp3d_ray<CONE>(A, G, u, v, a1f, a2f, c1f, c2f, scale, lo, hi);
+            fp3d_ray<GK>(A, G, u, v, a1f, a2f, c1f, c2f, scale, lo, hi);
         else
-            fp3d_ray<CONE>(A, G, u - 0.5 + (k + 0.5) / (int)R, v - 0.5 + (l + 0.5) / (int)R, a1f,
+            fp3d_ray<GK>(A, G, u - 0.5 + (k + 0.5) / (int)R, v - 0.5 + (l + 0.5) / (int)R, a1f,
                            a2f, c1f, c2f, scale, lo, hi);
         rays[i] = make_float4(a1f, c1f, a2f, c2f);
         const float y = proj[((size_t)v * A.na + angle) * A.nu + u];
@@ -637,7 +723,10 @@ __global__ void matched_geo_kernel(FPArgs A, float4 *__restrict__ geo) {
 struct MatchedArgs {
     const float4 *rays;  // [angle of chunk][V][U] (a1, c1, a2, c2)
     const float *yw;     // [angle of chunk][V][U] y * scale / R^2
-    const float4 *geo;   // [angle of chunk][3] lattice map
+    const float4 *geo;   // [angle of chunk][3] lattice map (flat detector)
+    const CylBPAngle *cyl;  // cylindrical detector: per-angle map (world units)
+    const int *alist;       // cylindrical detector: angle indices of the chunk
+    float vs[3];            // cylindrical detector: voxel sizes (x, y, z)
     unsigned count;      // angles in the chunk
     unsigned NU, NV;     // sub-ray lattice size
     unsigned n0, n1, n2, tiles1;
@@ -657,7 +746,11 @@ struct MatchedArgs {
 // or parallel projection maps each corner's path to a segment), widened by
 // 0.05 lattice units for fp32 rounding of the map; extra candidates get
 // weight 0.
-template <int S>
+// CYL: cylindrical detector; the corners map through the cylinder hit of
+// bp3d_cyl_kernel (world units), which bends the square's edges slightly,
+// so the bound is widened by 0.25 lattice units, and a corner whose ray
+// misses the cylinder makes the scan cover the whole detector.
+template <int S, bool CYL>
 __global__ __launch_bounds__(256) void bp3d_matched_gather(MatchedArgs M) {
     const unsigned sgroups = (M.n0 + S - 1) / S;
     const unsigned s0 = (blockIdx.x % sgroups) * S, tile = blockIdx.x / sgroups;
@@ -669,28 +762,66 @@ __global__ __launch_bounds__(256) void bp3d_matched_gather(MatchedArgs M) {
     const float w2 = (float)i2 - 0.5f * M.n2 + 0.5f;
     const float J1 = (float)(i1 + kPad), J2 = (float)(i2 + kPad), sf = (float)s0;
     const float NUm = (float)M.NU - 1.f, NVm = (float)M.NV - 1.f;
-    constexpr float kEps = 0.05f;
+    constexpr float kEps = CYL ? 0.25f : 0.05f;
     float acc[S];
 #pragma unroll
     for (int k = 0; k < S; ++k) acc[k] = 0.f;
     for (unsigned a = 0; a < M.count; ++a) {
-        const float4 gu = M.geo[3 * a], gv = M.geo[3 * a + 1], gd = M.geo[3 * a + 2];
-        const float cu = fmaf(gu.x, w0, fmaf(gu.y, w1, fmaf(gu.z, w2, gu.w)));
-        const float cv = fmaf(gv.x, w0, fmaf(gv.y, w1, fmaf(gv.z, w2, gv.w)));
-        const float cd = fmaf(gd.x, w0, fmaf(gd.y, w1, fmaf(gd.z, w2, gd.w)));
         float umin = INFINITY, umax = -INFINITY, vmin = INFINITY, vmax = -INFINITY;
+        if (CYL) {
+            const CylBPAngle c = M.cyl[M.alist[a]];
+            bool miss = false;
 #pragma unroll
-        for (int c = 0; c < (S > 1 ? 8 : 4); ++c) {
-            const float d0 = (c & 4) ? (float)(S - 1) : 0.f;
-            const float d1 = (c & 1) ? 1.f : -1.f, d2 = (c & 2) ? 1.f : -1.f;
-            const float r = __builtin_amdgcn_rcpf(
-                fmaf(d2, gd.z, fmaf(d1, gd.y, fmaf(d0, gd.x, cd))));
-            const float U = fmaf(d2, gu.z, fmaf(d1, gu.y, fmaf(d0, gu.x, cu))) * r;
-            const float V = fmaf(d2, gv.z, fmaf(d1, gv.y, fmaf(d0, gv.x, cv))) * r;
-            umin = fminf(umin, U);
-            umax = fmaxf(umax, U);
-            vmin = fminf(vmin, V);
-            vmax = fmaxf(vmax, V);
+            for (int q = 0; q < (S > 1 ? 8 : 4); ++q) {
+                const float g0 = w0 + ((q & 4) ? (float)(S - 1) : 0.f);
+                const float g1 = w1 + ((q & 1) ? 1.f : -1.f), g2 = w2 + ((q & 2) ? 1.f : -1.f);
+                float x, y, z;
+                if (M.group == 0) { x = g0; y = g1; z = g2; }
+                else if (M.group == 1) { y = g0; x = g1; z = g2; }
+                else { z = g0; x = g1; y = g2; }
+                x *= M.vs[0];
+                y *= M.vs[1];
+                z *= M.vs[2];
+                const float ra = fmaf(z, c.ca[2], fmaf(y, c.ca[1], fmaf(x, c.ca[0], c.ca[3])));
+                const float rb = fmaf(z, c.cb[2], fmaf(y, c.cb[1], fmaf(x, c.cb[0], c.cb[3])));
+                const float rc = fmaf(z, c.cc[2], fmaf(y, c.cc[1], fmaf(x, c.cc[0], c.cc[3])));
+                const float qa = fmaf(ra, ra, rb * rb);
+                const float qb = fmaf(ra, c.sa, rb * c.sb);
+                const float disc = fmaf(qb, qb, -qa * c.c);
+                miss |= !(disc >= 0.f);
+                const float t = (sqrtf(fmaxf(disc, 0.f)) - qb) / qa;
+                const float sn = fminf(fmaxf(fmaf(t, rb, c.sb) * c.inv_r, -1.f), 1.f);
+                // lattice coordinate = texture coordinate - 0.5
+                const float U = fmaf(asinf(sn), c.inv_dth, 0.5f * (M.NU - 1.f));
+                const float V = fmaf(fmaf(t, rc, c.sc), c.inv_nv, 0.5f * (M.NV - 1.f));
+                umin = fminf(umin, U);
+                umax = fmaxf(umax, U);
+                vmin = fminf(vmin, V);
+                vmax = fmaxf(vmax, V);
+            }
+            if (miss) {
+                umin = vmin = 0.f;
+                umax = NUm;
+                vmax = NVm;
+            }
+        } else {
+            const float4 gu = M.geo[3 * a], gv = M.geo[3 * a + 1], gd = M.geo[3 * a + 2];
+            const float cu = fmaf(gu.x, w0, fmaf(gu.y, w1, fmaf(gu.z, w2, gu.w)));
+            const float cv = fmaf(gv.x, w0, fmaf(gv.y, w1, fmaf(gv.z, w2, gv.w)));
+            const float cd = fmaf(gd.x, w0, fmaf(gd.y, w1, fmaf(gd.z, w2, gd.w)));
+#pragma unroll
+            for (int c = 0; c < (S > 1 ? 8 : 4); ++c) {
+                const float d0 = (c & 4) ? (float)(S - 1) : 0.f;
+                const float d1 = (c & 1) ? 1.f : -1.f, d2 = (c & 2) ? 1.f : -1.f;
+                const float r = __builtin_amdgcn_rcpf(
+                    fmaf(d2, gd.z, fmaf(d1, gd.y, fmaf(d0, gd.x, cd))));
+                const float U = fmaf(d2, gu.z, fmaf(d1, gu.y, fmaf(d0, gu.x, cu))) * r;
+                const float V = fmaf(d2, gv.z, fmaf(d1, gv.y, fmaf(d0, gv.x, cv))) * r;
+                umin = fminf(umin, U);
+                umax = fmaxf(umax, U);
+                vmin = fminf(vmin, V);
+                vmax = fmaxf(vmax, V);
+            }
         }
         // clamped in float before the conversion: a NaN bound becomes a full
         // (in-range) scan, an infinite one an empty scan
@@ -839,6 +970,9 @@ void fp3d_launch(const Geom3D &g, const float *vol, float *proj, float out_scale
                  const float *resid_b, const float *resid_w, bool padded_out, hipStream_t s,
                  unsigned ss) {
     AMX_REQUIRE(ss >= 1 && ss <= 16, "fp3d: rays per detector dimension must be in [1, 16]");
+    // the cylindrical-detector FP has no sub-rays (ConeCylFP ignores
+    // iRaysPerDetDim, cuda/3d/cone_cyl.cu:613-666)
+    if (g.kind == ASTRA_AMD_GEOM_CYLCONE) ss = 1;
     size_t need = 0;
     int groups[3], ng = 0;
     for (int k = 0; k < 3; ++k)
@@ -859,21 +993,24 @@ void fp3d_launch(const Geom3D &g, const float *vol, float *proj, float out_scale
         // The direction groups partition the angles: every output element
         // is written by exactly one group, which applies the full mode.
         A.mode = mode;
-        ProfToken pt = prof_begin(g.kind == ASTRA_AMD_GEOM_CONE ? "fp3d_cone" : "fp3d_par3d",
+        ProfToken pt = prof_begin(g.kind == ASTRA_AMD_GEOM_CONE     ? "fp3d_cone"
+                                  : g.kind == ASTRA_AMD_GEOM_CYLCONE ? "fp3d_cyl"
+                                                                     : "fp3d_par3d",
                                   (double)g.dir_count[k] * g.nu * g.nv * A.n0 * ss * ss, s);
-        const bool cone = g.kind == ASTRA_AMD_GEOM_CONE;
         A.ss = ss;
         fp_group_launches(A, g.dir_count[k], [&](const FPArgs &a, unsigned groups) {
-            if (ss > 1) {
-                if (cone)
-                    fp3d_kernel<true, true><<<groups, 256, 0, s>>>(a);
+            if (g.kind == ASTRA_AMD_GEOM_CYLCONE)
+                fp3d_kernel<GEO_CYL, false><<<groups, 256, 0, s>>>(a);
+            else if (ss > 1) {
+                if (g.kind == ASTRA_AMD_GEOM_CONE)
+                    fp3d_kernel<GEO_CONE, true><<<groups, 256, 0, s>>>(a);
                 else
-                    fp3d_kernel<false, true><<<groups, 256, 0, s>>>(a);
+                    fp3d_kernel<GEO_PAR3D, true><<<groups, 256, 0, s>>>(a);
             } else {
-                if (cone)
-                    fp3d_kernel<true, false><<<groups, 256, 0, s>>>(a);
+                if (g.kind == ASTRA_AMD_GEOM_CONE)
+                    fp3d_kernel<GEO_CONE, false><<<groups, 256, 0, s>>>(a);
                 else
-                    fp3d_kernel<false, false><<<groups, 256, 0, s>>>(a);
+                    fp3d_kernel<GEO_PAR3D, false><<<groups, 256, 0, s>>>(a);
             }
         }, s);
         prof_end(pt, s);
@@ -929,6 +1066,19 @@ void bp3d_run_pairs(const Geom3D &g, const float2 *pairs, float *vol, float out_
         A.osc = out_scale * (vx * vy * vz);
     constexpr int ZS = 8;
     dim3 grid(((g.nx + 15) / 16) * ((g.ny + 15) / 16), (g.nz + ZS - 1) / ZS);
+    if (g.kind == ASTRA_AMD_GEOM_CYLCONE) {
+        // default kernel only, no voxel supersampling (ConeCylBP_Array,
+        // cuda/3d/cone_cyl_bp.cu:234-296; cuda/3d/mem3d.cu:463-479)
+        if (kernel != ASTRA_AMD_KER3D_DEFAULT)
+            fail(ASTRA_AMD_ERR_UNSUPPORTED,
+                 "bp3d: the cylindrical-detector geometry supports the default and matched_bp "
+                 "kernels only");
+        ProfToken pt = prof_begin("bp3d_cyl", (double)g.nx * g.ny * g.nz * g.na, s);
+        bp3d_cyl_kernel<ZS><<<grid, 256, 0, s>>>(A, g.bp_cyl.as<CylBPAngle>(), vx, vy, vz);
+        AMX_LAUNCHED();
+        prof_end(pt, s);
+        return;
+    }
     const char *pname = g.kind != ASTRA_AMD_GEOM_CONE ? "bp3d_par3d"
                         : (kernel == ASTRA_AMD_KER3D_FDK_WEIGHTING ? "bp3d_fdk" : "bp3d_cone");
     ProfToken pt = prof_begin(pname, (double)g.nx * g.ny * g.nz * g.na, s);
@@ -1015,6 +1165,8 @@ static int matched_slices() {
 void bp3d_matched_launch(const Geom3D &g, const float *proj, float *vol, float out_scale,
                          int mode, hipStream_t s, unsigned ss) {
     AMX_REQUIRE(ss >= 1 && ss <= 16, "bp3d: rays per detector dimension must be in [1, 16]");
+    // the transpose of the FP actually taken (no sub-rays on a cylinder)
+    if (g.kind == ASTRA_AMD_GEOM_CYLCONE) ss = 1;
     const size_t nvox = (size_t)g.nx * g.ny * g.nz;
     if (mode == OUT_SET) fill_launch(vol, nvox, 0.f, s);
     const size_t NU = (size_t)g.nu * ss, NV = (size_t)g.nv * ss;
@@ -1025,7 +1177,7 @@ void bp3d_matched_launch(const Geom3D &g, const float *proj, float *vol, float o
     const size_t per_angle = NU * NV * (sizeof(float4) + sizeof(float));
     const unsigned chunk = (unsigned)std::max<size_t>(
         1, std::min<size_t>(g.na, (size_t(1) << 30) / per_angle));
-    const bool cone = g.kind == ASTRA_AMD_GEOM_CONE;
+    const bool cyl = g.kind == ASTRA_AMD_GEOM_CYLCONE;
     Scratch rays(per_angle * chunk, s), geo((size_t)chunk * 3 * sizeof(float4), s);
     float4 *rp = rays.as<float4>();
     float *yp = reinterpret_cast<float *>(rp + NU * NV * chunk);
@@ -1044,7 +1196,9 @@ void bp3d_matched_launch(const Geom3D &g, const float *proj, float *vol, float o
         M.nx = g.nx; M.ny = g.ny;
         M.group = k;
         M.vol = vol;
-        const int S = matched_slices();
+        M.cyl = g.bp_cyl.as<CylBPAngle>();
+        for (int q = 0; q < 3; ++q) M.vs[q] = g.vol_scale[q];
+        const int S = cyl ? 8 : matched_slices();
         const size_t gblocks = (size_t)((A.n0 + S - 1) / S) * M.tiles1 * ((A.n2 + 15) / 16);
         AMX_REQUIRE(gblocks < (1ull << 31), "bp3d: volume too large for one matched launch");
         for (unsigned first = 0; first < (unsigned)g.dir_count[k]; first += chunk) {
@@ -1053,24 +1207,30 @@ void bp3d_matched_launch(const Geom3D &g, const float *proj, float *vol, float o
             const size_t nrays = NU * NV * A.lcount;
             const unsigned rblocks = (unsigned)std::min<size_t>((nrays + 255) / 256, 1u << 20);
             const unsigned gb = (A.lcount + 63) / 64;
-            if (cone) {
-                matched_rays_kernel<true><<<rblocks, 256, 0, s>>>(A, proj, rp, yp);
+            if (cyl) {
+                matched_rays_kernel<GEO_CYL><<<rblocks, 256, 0, s>>>(A, proj, rp, yp);
+            } else if (g.kind == ASTRA_AMD_GEOM_CONE) {
+                matched_rays_kernel<GEO_CONE><<<rblocks, 256, 0, s>>>(A, proj, rp, yp);
                 matched_geo_kernel<true><<<gb, 64, 0, s>>>(A, geo.as<float4>());
             } else {
-                matched_rays_kernel<false><<<rblocks, 256, 0, s>>>(A, proj, rp, yp);
+                matched_rays_kernel<GEO_PAR3D><<<rblocks, 256, 0, s>>>(A, proj, rp, yp);
                 matched_geo_kernel<false><<<gb, 64, 0, s>>>(A, geo.as<float4>());
             }
             M.count = A.lcount;
-            if (S == 1)
-                bp3d_matched_gather<1><<<(unsigned)gblocks, 256, 0, s>>>(M);
+            M.alist = A.alist + first;
+            const unsigned gbl = (unsigned)gblocks;
+            if (cyl)
+                bp3d_matched_gather<8, true><<<gbl, 256, 0, s>>>(M);
+            else if (S == 1)
+                bp3d_matched_gather<1, false><<<gbl, 256, 0, s>>>(M);
             else if (S == 2)
-                bp3d_matched_gather<2><<<(unsigned)gblocks, 256, 0, s>>>(M);
+                bp3d_matched_gather<2, false><<<gbl, 256, 0, s>>>(M);
             else if (S == 4)
-                bp3d_matched_gather<4><<<(unsigned)gblocks, 256, 0, s>>>(M);
+                bp3d_matched_gather<4, false><<<gbl, 256, 0, s>>>(M);
             else if (S == 8)
-                bp3d_matched_gather<8><<<(unsigned)gblocks, 256, 0, s>>>(M);
+                bp3d_matched_gather<8, false><<<gbl, 256, 0, s>>>(M);
             else
-                bp3d_matched_gather<16><<<(unsigned)gblocks, 256, 0, s>>>(M);
+                bp3d_matched_gather<16, false><<<gbl, 256, 0, s>>>(M);
             AMX_LAUNCHED();
         }
     }

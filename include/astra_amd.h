@@ -48,6 +48,9 @@ extern "C" {
 #define ASTRA_AMD_GEOM_FAN 1   /* 2D SFanProjection                       */
 #define ASTRA_AMD_GEOM_PAR3D 2 /* 3D SPar3DProjection                     */
 #define ASTRA_AMD_GEOM_CONE 3  /* 3D SConeProjection                      */
+#define ASTRA_AMD_GEOM_CYLCONE 4 /* 3D SCylConeProjection (cylindrical
+                                    detector, include/astra/GeometryUtil3D.h:
+                                    125-150)                                */
 
 /* 3D projection kernels: Cuda3DProjectionKernel
  * (include/astra/cuda/3d/dims3d.h:45-51). */
@@ -125,8 +128,12 @@ int astra_amd_geom2d_destroy(astra_amd_geom2d g);
 /* ----------------------------------------------------- 3D geometry ---- */
 /* Build a device-resident 3D geometry from NORMALISED vectors: 12 doubles
  * per angle [s, dS, u, v] (s = source for CONE, ray for PAR3D).
+ * CYLCONE takes 13 doubles per angle [src, detector centre, u, v, radius],
+ * TRANSLATED by the volume window centre but not scaled
+ * (convertAstraGeometry_unscaled_internal, src/GeometryUtil3D.cpp:595-616).
  * Replaces Geometry3DParameters (include/astra/GeometryUtil3D.h:160-224)
- * and the transferConstants of cuda/3d/{cone,par3d}_{fp,bp}.cu.
+ * and the transferConstants of cuda/3d/{cone,par3d}_{fp,bp}.cu,
+ * cuda/3d/cone_cyl.cu:75-123 and cuda/3d/cone_cyl_bp.cu:162-232.
  * vol_scale = SVolScale3D (voxel size x, y, z).  Synchronous. */
 int astra_amd_geom3d_create(int kind, unsigned int vol_x, unsigned int vol_y,
                             unsigned int vol_z, unsigned int n_angles, unsigned int det_u,

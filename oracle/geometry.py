@@ -183,6 +183,21 @@ def normalise_3d(vecs, kind, vol_geom):
     return out, tuple(float(p) for p in pix)
 
 
+def translate_cylcone(vecs, vol_geom):
+    """``convertAstraGeometry_unscaled_internal`` (src/GeometryUtil3D.cpp:
+    595-616) for ``cyl_cone_vec``: translate source and detector centre by
+    the float32 window centre; no scaling (the kernels take voxel sizes).
+    Rows are 13-tuples (src, detector centre, u, v, radius).  Returns
+    ``(vecs, vol_scale)``."""
+    win = _vol3d_window(vol_geom)
+    d = [float(f32(-(f32(lo) + f32(hi)) / f32(2))) for lo, hi, _ in win]
+    pix = [f32((f32(hi) - f32(lo)) / f32(n)) for lo, hi, n in win]
+    out = np.array(vecs, dtype=np.float64)
+    out[:, 0:3] += d
+    out[:, 3:6] += d
+    return out, tuple(float(p) for p in pix)
+
+
 def _vol3d_window(vg):
     nx = vg["GridColCount"]
     ny = vg["GridRowCount"]

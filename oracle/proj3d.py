@@ -378,3 +378,128 @@ def par3d_bp(proj, vecs, shape, vol_scale=(1.0, 1.0, 1.0), out_scale=1.0, kernel
             vol += tex2d(proj[:, a, :], U, V) * s / ss ** 3
     vx, vy, vz = (f32(x) for x in vol_scale)
     return vol * float(f32(out_scale) * (vx * vy * vz))
+
+
+# ------------------------------------------------ cylindrical detector ------
+def cyl_axes(vec):
+    """``getCylConeAxes`` (src/GeometryUtil3D.cpp:542-561): radial a (from
+    the cylinder axis to the detector centre, pointing away from the
+    source), tangential b = u R / |u|, cylinder centre c = d - a."""
+    s = np.asarray(vec[0:3], dtype=np.float64)
+    d = np.asarray(vec[3:6], dtype=np.float64)
+    u = np.asarray(vec[6:9], dtype=np.float64)
+    v = np.asarray(vec[9:12], dtype=np.float64)
+    R = float(vec[12])
+    a = -np.cross(u, v) * (R / (np.linalg.norm(u) * np.linalg.norm(v)))
+    if np.linalg.norm(d - a - s) > np.linalg.norm(d + a - s):
+        a = -a
+    return a, u * (R / np.linalg.norm(u)), d - a
+
+
+def cylcone_direction(vec):
+    """Principal axis of ``ConeCylFP_Array_internal`` (cuda/3d/cone_cyl.cu:
+    486-500): largest |src - detector centre| component (world units,
+    compared as float32, ties to the lower axis)."""
+    dX, dY, dZ = (abs(f32(vec[k] - vec[3 + k])) for k in range(3))
+    if dX >= dY and dX >= dZ:
+        return 0
+    if dY >= dX and dY >= dZ:
+        return 1
+    return 2
+
+
+def cylcone_pixels(vec, n_u, n_v):
+    """World positions [3][n_v][n_u] of the detector pixel centres:
+    c + a cos(theta) + b sin(theta) + v (row - n_v/2 + 0.5), theta =
+    (col - n_u/2 + 0.5) |u| / R (cuda/3d/cone_cyl.cu:243-258)."""
+    a, b, c = cyl_axes(vec)
+    v = np.asarray(vec[9:12], dtype=np.float64)
+    dth = np.linalg.norm(vec[6:9]) / float(vec[12])
+    uu, vv = np.meshgrid(np.arange(n_u), np.arange(n_v))
+    th = (uu - 0.5 * n_u + 0.5) * dth
+    fv = vv - 0.5 * n_v + 0.5
+    return [c[k] + a[k] * np.cos(th) + b[k] * np.sin(th) + v[k] * fv for k in range(3)]
+
+
+def cylcone_fp(vol, vecs, n_u, n_v, vol_scale=(1.0, 1.0, 1.0), out_scale=1.0):
+    """Cone-beam FP onto a cylindrical detector: ``cylcone_FP_t``
+    (cuda/3d/cone_cyl.cu:197-298), with ideal bilinear sampling.  Vectors are
+    the translated (unscaled) 13-tuples of ``translate_cylcone``; the ray
+    from the source to each pixel centre is written per slice in texel
+    units of the principal-axis group and weighted by its world length per
+    slice, sqrt(a1^2 + a2^2 + 1) x voxel size along the axis."""
+    vol = np.asarray(vol, dtype=np.float64)
+    vs = [float(f32(x)) for x in vol_scale]
+    nA = vecs.shape[0]
+    proj = np.zeros((n_v, nA, n_u))
+    for a in range(nA):
+        vec = vecs[a]
+        d = cylcone_direction(vec)
+        c0, c1, c2 = _DIRS[d]
+        src = vec[0:3]
+        det = cylcone_pixels(vec, n_u, n_v)
+        A1 = (src[c1] - det[c1]) / (src[c0] - det[c0])
+        A2 = (src[c2] - det[c2]) / (src[c0] - det[c0])
+        dist = np.sqrt(A1 * A1 + A2 * A2 + 1.0) * vs[c0]
+        a1 = A1 / vs[c1]
+        a2 = A2 / vs[c2]
+        b1 = src[c1] / vs[c1] - a1 * src[c0]
+        b2 = src[c2] / vs[c2] - a2 * src[c0]
+        proj[:, a, :] = _ray_sum(vol, d, a1 * vs[c0], a2 * vs[c0], b1, b2) * dist * out_scale
+    return proj
+
+
+def cylcone_bp_params(vec):
+    """``DevCylConeParamsBP`` (cuda/3d/cone_cyl_bp.cu:165-230), float64:
+    rows (x, y, z, w) of the source-relative ray in the cylinder frame
+    (radial, tangential, axial), scaled by sqrt(|u x v|) so that the ray
+    density is 1 / t^2; the source in that frame; 1 / angular step; 1/|v|."""
+    s = np.asarray(vec[0:3], dtype=np.float64)
+    u = np.asarray(vec[6:9], dtype=np.float64)
+    v = np.asarray(vec[9:12], dtype=np.float64)
+    R = float(vec[12])
+    a, b, c = cyl_axes(vec)
+    an, bn, vn = a / np.linalg.norm(a), b / np.linalg.norm(b), v / np.linalg.norm(v)
+    sc = math.sqrt(np.linalg.norm(np.cross(u, v)))
+    rows = [np.r_[e * sc, -np.dot(s, e) * sc] for e in (an, bn, vn)]
+    sm = [float(np.dot(s - c, e)) for e in (an, bn, vn)]
+    return rows, sm, np.linalg.norm(u) / R, 1.0 / np.linalg.norm(v), R
+
+
+def cylcone_detector_coords(vec, n_u, n_v, X, Y, Z):
+    """Texture coordinates (fU, fV) on the cylindrical detector of the ray
+    from the source through world points (X, Y, Z) (translated frame), the
+    ray parameter t of the hit and the hit mask (cuda/3d/cone_cyl_bp.cu:
+    104-134)."""
+    (ca, cb, cc), (sa, sb, sc), dth, inv_nv, R = cylcone_bp_params(vec)
+    ra = ca[3] + X * ca[0] + Y * ca[1] + Z * ca[2]
+    rb = cb[3] + X * cb[0] + Y * cb[1] + Z * cb[2]
+    rc = cc[3] + X * cc[0] + Y * cc[1] + Z * cc[2]
+    A = ra * ra + rb * rb
+    B = ra * sa + rb * sb
+    C = sa * sa + sb * sb - R * R
+    D = B * B - A * C
+    ok = D >= 0
+    t = (-B + np.sqrt(np.where(ok, D, 0.0))) / A
+    th = np.arcsin(np.clip((t * rb + sb) / R, -1.0, 1.0)) / dth
+    return th + 0.5 * n_u, (t * rc + sc) * inv_nv + 0.5 * n_v, t, ok
+
+
+def cylcone_bp(proj, vecs, shape, vol_scale=(1.0, 1.0, 1.0), out_scale=1.0):
+    """Voxel-driven BP from a cylindrical detector: ``dev_cylcone_BP``
+    (cuda/3d/cone_cyl_bp.cu:58-158) and its output scale (:246).  The ray
+    from the source through the voxel meets the cylinder at parameter t
+    (larger root of A t^2 + 2 B t + C = 0); the detector coordinates there
+    are (asin(b / R) / angular step + n_u / 2, axial / |v| + n_v / 2)
+    (texture coordinates); weight t^2.  A voxel whose ray misses the
+    cylinder contributes nothing."""
+    proj = np.asarray(proj, dtype=np.float64)
+    n_v, nA, n_u = proj.shape
+    vx, vy, vz = (f32(x) for x in vol_scale)
+    fz, fy, fx = _voxel_grid(shape)
+    fx, fy, fz = fx * float(vx), fy * float(vy), fz * float(vz)
+    vol = np.zeros(shape)
+    for a in range(nA):
+        fU, fV, t, ok = cylcone_detector_coords(vecs[a], n_u, n_v, fx, fy, fz)
+        vol += np.where(ok, tex2d(proj[:, a, :], fU, fV) * t * t, 0.0)
+    return vol * float(f32(out_scale) * (vx * vy * vz))

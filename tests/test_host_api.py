@@ -173,3 +173,19 @@ def test_add_noise_to_sino_statistics():
     # b = -max log(Poisson(I0 e^{-1}) / I0): mean ~2, sd ~ 2 / sqrt(I0 / e)
     assert abs(noisy.mean() - 2.0) < 1e-3
     assert 0.5 < noisy.std() / (2.0 / np.sqrt(1e5 / np.e)) < 1.5
+
+
+def test_cylcone_geometry_matches_oracle_translation():
+    rng = np.random.default_rng(5)
+    vecs = np.c_[rng.normal(size=(7, 12)) * 20, np.full(7, 80.0)]
+    vg = astra.create_vol_geom(10, 12, 14, -6, 6, -4, 6, -7, 7)
+    pg = astra.create_proj_geom("cyl_cone_vec", 9, 8, vecs)
+    k, v = G.vectors_3d(pg)
+    assert k == "cylcone"
+    v, vs = G.normalise_3d(k, v, vg)
+    ov, ovs = OG.translate_cylcone(vecs, vg)
+    assert np.array_equal(v, ov) and vs == ovs
+    with pytest.raises(astra.AstraError):
+        G.vectors_3d(astra.create_proj_geom("cyl_cone_vec", 9, 8, np.c_[vecs[:, :12], -np.ones(7)]))
+    with pytest.raises(astra.AstraError):
+        astra.create_proj_geom("cyl_cone_vec", 9, 8, vecs[:, :12])

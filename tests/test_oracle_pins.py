@@ -680,3 +680,74 @@ def test_sart_parallel_bp_is_the_unweighted_bp():
     w = math.hypot(rx, ry) / abs(ux * ry - uy * rx)
     np.testing.assert_allclose(O2.sart_bp(row, v[0], (n, n), "parallel"),
                                O2.par_bp(row[None, :], v, (n, n)) / w, rtol=1e-12)
+
+
+# ---------------------------------------------------------------- cyl_cone_vec
+# The cylindrical-detector cone projector (cylcone_fp / cylcone_bp) is pinned
+# by (1) its flat-detector limit: as the radius grows the cylinder section
+# becomes the flat cone_vec detector with the same centre and pixel vectors,
+# and the FP and BP reduce to cone_fp / cone_bp (independent formulas: ray
+# setup in world units vs normalised vectors, quadratic cylinder hit + asin
+# vs the projective map), with anisotropic voxels and a shifted window;
+# (2) the BP's detector mapping inverting the FP's pixel rays exactly at a
+# strongly curved detector; (3) the reference's adjoint criterion.
+
+def _cyl_vecs(na, sod, odd, R, du=1.1, dv=0.9, tilt=0.0):
+    out = []
+    for t in np.linspace(0, 2 * np.pi, na, endpoint=False):
+        c, s = np.cos(t), np.sin(t)
+        src = np.array([s * sod, -c * sod, tilt * sod])
+        det = np.array([-s * odd, c * odd, -tilt * odd])
+        out.append(np.r_[src, det, np.array([c, s, 0.0]) * du, np.array([0.0, 0.0, dv]), R])
+    return np.array(out)
+
+
+def test_cylcone_flat_detector_limit():
+    n, na, nu, nv = 14, 5, 18, 16
+    vg = OG.vol_geom_3d(n, n, n)
+    vg["option"] = {"WindowMinX": -8.0, "WindowMaxX": 9.5, "WindowMinY": -7.0, "WindowMaxY": 7.0,
+                    "WindowMinZ": -5.0, "WindowMaxZ": 6.2}
+    vec13 = _cyl_vecs(na, 60.0, 30.0, 1e6, tilt=0.05)
+    tv, vs = OG.translate_cylcone(vec13, vg)
+    cv, cvs = OG.normalise_3d(OG.conevec_from_astra(vec13[:, :12], nu, nv), "cone", vg)
+    assert vs == cvs
+    rng = np.random.default_rng(81)
+    vol = rng.random((n, n, n))
+    a = O3.cylcone_fp(vol, tv, nu, nv, vs)
+    b = O3.cone_fp(vol, cv, nu, nv, cvs)
+    assert np.linalg.norm(a - b) / np.linalg.norm(b) < 1e-5
+    y = rng.random((nv, na, nu))
+    a = O3.cylcone_bp(y, tv, (n, n, n), vs)
+    b = O3.cone_bp(y, cv, (n, n, n), cvs)
+    assert np.linalg.norm(a - b) / np.linalg.norm(b) < 1e-5
+
+
+def test_cylcone_bp_mapping_inverts_fp_rays():
+    nu, nv = 40, 24
+    vec13 = _cyl_vecs(7, 50.0, 40.0, 90.0, du=2.0, dv=1.3, tilt=0.1)   # 50 deg fan
+    rng = np.random.default_rng(82)
+    for vec in vec13:
+        src = vec[0:3]
+        pix = O3.cylcone_pixels(vec, nu, nv)
+        iu = rng.integers(0, nu, 30)
+        iv = rng.integers(0, nv, 30)
+        tau = rng.uniform(0.2, 0.9, 30)
+        P = [pix[k][iv, iu] for k in range(3)]
+        W = [src[k] + tau * (P[k] - src[k]) for k in range(3)]
+        fU, fV, t, ok = O3.cylcone_detector_coords(vec, nu, nv, *W)
+        assert ok.all()
+        assert np.abs(fU - (iu + 0.5)).max() < 1e-9 and np.abs(fV - (iv + 0.5)).max() < 1e-9
+
+
+def test_cylcone_bp_adjoint_criterion():
+    # the reference's BP-vs-FP adjoint criterion (<= 1e-1,
+    # tests/python/functional/test_line2d.py:591-640)
+    n, na, nu, nv = 12, 8, 20, 16
+    vg = OG.vol_geom_3d(n, n, n)
+    tv, vs = OG.translate_cylcone(_cyl_vecs(na, 40.0, 30.0, 70.0), vg)
+    rng = np.random.default_rng(83)
+    x = rng.random((n, n, n))
+    y = rng.random((nv, na, nu))
+    lhs = float(np.sum(O3.cylcone_fp(x, tv, nu, nv, vs) * y))
+    rhs = float(np.sum(x * O3.cylcone_bp(y, tv, (n, n, n), vs)))
+    assert abs(lhs - rhs) / abs(lhs) < 1e-1
