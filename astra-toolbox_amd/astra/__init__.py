@@ -8,14 +8,15 @@ runs in libastra_amd (hand-written HIP kernels for gfx950).
 """
 import os
 
-from . import (algorithm, autograd, batched, data2d, data3d, experimental, functions, log, projector,
-               projector3d)
+from . import (algorithm, autograd, batched, composite, data2d, data3d, experimental, functions, log,
+               projector, projector3d, pythonutils)
 from ._core import set_gpu_index
 from .creators import (astra_dict, create_backprojection, create_backprojection3d_gpu,
                        create_proj_geom, create_projector, create_reconstruction, create_sino,
                        create_sino3d_gpu, create_vol_geom)
 from .functions import add_noise_to_sino, clear, data_op, geom_2vec, geom_size, move_vol_geom
 from .log import AstraError
+from .pythonutils import GPULink
 
 __version__ = "2.5.99+mi355x"
 

@@ -27,13 +27,25 @@ _default_gpu = [0]
 
 def set_gpu_index(idx, memory=0):
     """astra.set_gpu_index (python/astra/astra_c.pyx:84-118).  A list picks its
-    first entry: one device per process here.  `memory` (the CGM split limit)
-    has no effect: jobs run unchunked."""
+    first entry: one device per process here.  `memory` (bytes, 0 = the
+    device's free memory) bounds the GPU memory of a 3D FP / BP / FDK job on
+    host data; larger jobs are split (astra.composite)."""
+    if memory != 0 and memory < 1024 * 1024:
+        raise AstraError("Setting GPU memory lower than 1MB is not supported")
     if isinstance(idx, (list, tuple)):
         if not idx:
             raise AstraError("set_gpu_index: empty GPU list")
         idx = idx[0]
     _default_gpu[0] = int(idx)
+    _memory_limit[0] = int(memory)
+
+
+_memory_limit = [0]
+
+
+def gpu_memory_limit():
+    """The job memory limit set by set_gpu_index (0 = not set)."""
+    return _memory_limit[0]
 
 
 def device(index=None):
@@ -262,9 +274,14 @@ class Data:
     # -- device access for one operation
     def on_device(self, dev, init=True):
         """Tensor on `dev` holding this object's data (the linked tensor
-        itself when it already lives there)."""
+        itself when it already lives there and is contiguous; a pitched
+        GPULink view goes through a contiguous copy and write_back)."""
         if self.is_gpu() and self.linked.device == dev:
-            return self.linked
+            if self.linked.is_contiguous():
+                return self.linked
+            if init:
+                return self.linked.contiguous()
+            return torch.empty(self.shape, dtype=torch.float32, device=dev)
         src = self.linked if self.linked is not None else torch.from_numpy(self.host)
         if init:
             return src.to(dev, non_blocking=False).contiguous()
@@ -288,8 +305,76 @@ def new_data(kind, geom, shape, data=None):
     return d
 
 
+# ---------------------------------------------------- raw device pointers -
+class _DLDevice(ctypes.Structure):
+    _fields_ = [("device_type", ctypes.c_int32), ("device_id", ctypes.c_int32)]
+
+
+class _DLDataType(ctypes.Structure):
+    _fields_ = [("code", ctypes.c_uint8), ("bits", ctypes.c_uint8), ("lanes", ctypes.c_uint16)]
+
+
+class _DLTensor(ctypes.Structure):
+    _fields_ = [("data", ctypes.c_void_p), ("device", _DLDevice), ("ndim", ctypes.c_int32),
+                ("dtype", _DLDataType), ("shape", ctypes.POINTER(ctypes.c_int64)),
+                ("strides", ctypes.POINTER(ctypes.c_int64)), ("byte_offset", ctypes.c_uint64)]
+
+
+class _DLManagedTensor(ctypes.Structure):
+    _fields_ = [("dl_tensor", _DLTensor), ("manager_ctx", ctypes.c_void_p),
+                ("deleter", ctypes.c_void_p)]
+
+
+_KDL_ROCM = 10
+_dl_keepalive = []  # the descriptors torch reads at free time (no deleter)
+
+
+def _tensor_from_pointer(ptr, shape, row_pitch_elems, dev_index):
+    """A torch view of device memory at `ptr` (float32, [z][y][x] with rows
+    `row_pitch_elems` apart), through a DLPack capsule without a deleter:
+    the caller owns the memory."""
+    z, y, x = shape
+    shp = (ctypes.c_int64 * 3)(z, y, x)
+    strides = (ctypes.c_int64 * 3)(y * row_pitch_elems, row_pitch_elems, 1)
+    mt = _DLManagedTensor()
+    mt.dl_tensor.data = ctypes.c_void_p(int(ptr))
+    mt.dl_tensor.device = _DLDevice(_KDL_ROCM, int(dev_index))
+    mt.dl_tensor.ndim = 3
+    mt.dl_tensor.dtype = _DLDataType(2, 32, 1)  # kDLFloat, 32 bits
+    mt.dl_tensor.shape = shp
+    mt.dl_tensor.strides = strides
+    mt.dl_tensor.byte_offset = 0
+    mt.manager_ctx = None
+    mt.deleter = None
+    _dl_keepalive.append((mt, shp, strides))
+    new_capsule = ctypes.pythonapi.PyCapsule_New
+    new_capsule.restype = ctypes.py_object
+    new_capsule.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_void_p]
+    cap = new_capsule(ctypes.addressof(mt), b"dltensor", None)
+    return torch.from_dlpack(cap)
+
+
+def _link_gpulink(kind, geom, shape, link):
+    """utils.pyx:305-317 / 335-347: shape check, then wrap the pointer."""
+    data_shape = (int(link.z), int(link.y), int(link.x))
+    if tuple(shape) != data_shape:
+        raise ValueError("The dimensions of the data {} do not match those specified in the "
+                         "geometry {}".format(data_shape, tuple(shape)))
+    if int(link.pitch) % 4 or int(link.pitch) < 4 * int(link.x):
+        raise ValueError("GPULink pitch must be a multiple of 4 bytes and at least 4 * x")
+    if torch is None or not torch.cuda.is_available():
+        raise AstraError("no GPU available: the ASTRA GPU path needs a ROCm device")
+    dev = device()
+    t = _tensor_from_pointer(link.ptr, data_shape, int(link.pitch) // 4, dev.index)
+    return Data(kind, geom, shape, linked=t)
+
+
 def link_data(kind, geom, shape, obj):
-    """Wrap a DLPack producer without copying (python/astra/utils.pyx:237-349)."""
+    """Wrap a DLPack producer without copying (python/astra/utils.pyx:237-349);
+    3D data also takes a GPULink (a raw, possibly pitched, device pointer)."""
+    from .pythonutils import GPULink
+    if isinstance(obj, GPULink) and len(shape) == 3:
+        return _link_gpulink(kind, geom, shape, obj)
     if not hasattr(obj, "__dlpack__"):
         raise TypeError("Object of type %s does not support the DLPack protocol" % type(obj).__name__)
     if isinstance(obj, np.ndarray) and not obj.flags.writeable:

@@ -61,6 +61,7 @@ SIGNATURES = {
     "astra_amd_filter_rows": [_vp, _vp, _sz, _vp],
     "astra_amd_fdk_preweight": [_vp, _vp, _vp],
     "astra_amd_fdk": [_vp, _vp, _vp, _f, _i, _vp, _i, _vp],
+    "astra_amd_fdk_filter": [_vp, _vp, _i, _vp, _vp],
     "astra_amd_fbp2d": [_vp, _vp, _vp, _f, _i, _vp, _i, _vp],
     "astra_amd_dot": [_vp, _vp, _sz, ctypes.POINTER(_d), _vp],
     "astra_amd_algo2d_create": [_i, _vp, ctypes.POINTER(Params2D), ctypes.POINTER(_vp)],

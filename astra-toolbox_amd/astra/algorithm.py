@@ -10,7 +10,7 @@ import math
 
 import numpy as np
 
-from . import _core, _lib
+from . import _core, _lib, composite
 from . import geometry as G
 from ._registry import algorithms as _reg
 from ._registry import data2d as _d2, data3d as _d3
@@ -80,6 +80,9 @@ class _Algo:
 # ---------------------------------------------------------- FP / BP -------
 class _FP(_Algo):
     def run(self, iterations=1):
+        if self.is3d:  # CCompositeGeometryManager job: split when it does not fit
+            composite.fp(self.proj, self.vol, self.sino, _lib.MODE_SET)
+            return
         dev = self.proj.device()
         v = self.vol.on_device(dev)
         s = self.sino.on_device(dev, init=False)
@@ -90,6 +93,9 @@ class _FP(_Algo):
 
 class _BP(_Algo):
     def run(self, iterations=1):
+        if self.is3d:
+            composite.bp(self.proj, self.sino, self.vol, _lib.MODE_SET)
+            return
         dev = self.proj.device()
         s = self.sino.on_device(dev)
         v = self.vol.on_device(dev, init=False)
@@ -207,8 +213,8 @@ class FBP_CUDA(_Algo):
 
 
 class FDK_CUDA(_Algo):
-    """CCudaFDKAlgorithm3D (src/CudaFDKAlgorithm3D.cpp:183-198) via a
-    single-GPU, unchunked FDK job."""
+    """CCudaFDKAlgorithm3D (src/CudaFDKAlgorithm3D.cpp:183-198) via one
+    FDK job (split over volume slabs when it does not fit)."""
     is3d = True
 
     def __init__(self, cfg):
@@ -219,12 +225,10 @@ class FDK_CUDA(_Algo):
             raise AstraError("FDK_CUDA requires a cone geometry")
 
     def run(self, iterations=1, mode=_lib.MODE_SET):
-        dev = self.proj.device()
-        p = self.sino.on_device(dev).clone()  # FDK filters in place
-        v = self.vol.on_device(dev, init=(mode == _lib.MODE_ADD))
-        fdk(self.proj, p, v, mode, self.filter, self.short_scan)
-        self.vol.write_back(v)
-        _core.sync(dev)
+        nv, na, nu = self.proj.proj_shape
+        flt = self.filter
+        composite.fdk(self.proj, self.sino, self.vol, mode,
+                      lambda: None if flt is None else _filter_handle(nu, na, flt), self.short_scan)
 
 
 def fdk(proj, p, v, mode, filter=None, short_scan=False):

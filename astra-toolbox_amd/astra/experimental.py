@@ -6,7 +6,7 @@ producer; GPU tensors are used in place (zero copy), host arrays are
 staged through the projector's GPU.  Calls are synchronous, like the
 reference's CompositeGeometryManager jobs.
 """
-from . import _core, _lib
+from . import _core, _lib, composite
 from ._registry import data3d as _d3
 from ._registry import projectors2d as _p2, projectors3d as _p3
 from .log import AstraError
@@ -16,6 +16,14 @@ MODE_SET = _lib.MODE_SET
 
 
 def _run(p, dvol, dproj, mode, t):
+    if p.is3d:  # CCompositeGeometryManager jobs: split when they do not fit
+        if t == "FP":
+            composite.fp(p, dvol, dproj, mode)
+        elif t == "BP":
+            composite.bp(p, dproj, dvol, mode)
+        else:
+            raise AstraError("Internal error: wrong op type")
+        return
     dev = p.device()
     if t == "FP":
         v = dvol.on_device(dev)
@@ -70,7 +78,7 @@ def do_composite(projector_id, vol_ids, proj_ids, mode, t):
     """FP / BP between lists of 3D data objects
     (CCompositeGeometryManager::doFP/doBP): every projection receives the
     contribution of every volume (FP), every volume that of every projection
-    (BP).  One GPU, no chunking."""
+    (BP).  One GPU; each job is split when it does not fit."""
     if mode not in (MODE_ADD, MODE_SET):
         raise AstraError("Internal error: wrong composite mode")
     base = _p3.get(projector_id) if projector_id is not None and projector_id in _p3.objects else None
@@ -99,12 +107,6 @@ def do_composite_BP(projector_id, vol_ids, proj_ids):
 def accumulate_FDK(projector_id, vol_id, proj_id):
     """vol += FDK(proj) with a Ram-Lak filter (CCompositeGeometryManager::doFDK,
     MODE_ADD); the projection data object is left unchanged."""
-    from .algorithm import fdk
     p = _p3.get(projector_id)
     dv, dp = _d3.get(vol_id), _d3.get(proj_id)
-    dev = p.device()
-    pt = dp.on_device(dev).clone()
-    vt = dv.on_device(dev)
-    fdk(p, pt, vt, MODE_ADD)
-    dv.write_back(vt)
-    _core.sync(dev)
+    composite.fdk(p, dp, dv, MODE_ADD, lambda: None, False)

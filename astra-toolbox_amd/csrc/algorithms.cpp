@@ -426,8 +426,8 @@ static void fdk_prepare(const Geom3D &g) {
 }
 
 // astraCUDA3d::FDK (cuda/3d/fdk.cu:415-478): pre-weight, filter, FDK BP
-void fdk_run(const Geom3D &g, float *proj, float *vol, float out_scale, const Filter *f,
-             bool short_scan, int mode, hipStream_t s) {
+void fdk_filter_run(const Geom3D &g, float *proj, const Filter *f, bool short_scan,
+                    hipStream_t s) {
     AMX_REQUIRE(g.kind == ASTRA_AMD_GEOM_CONE, "FDK requires a cone geometry");
     fdk_prepare(g);
     const Filter *flt = f ? f : g.fdk_filter.get();
@@ -450,6 +450,11 @@ void fdk_run(const Geom3D &g, float *proj, float *vol, float out_scale, const Fi
     // fused cosine (and Parker) pre-weights + ramp filter over all nv * na rows
     filter_rows_launch(*flt, proj, (size_t)g.nv * g.na, g.fdk_w.as<float>(), g.na, s, parker,
                        g.na);
+}
+
+void fdk_run(const Geom3D &g, float *proj, float *vol, float out_scale, const Filter *f,
+             bool short_scan, int mode, hipStream_t s) {
+    fdk_filter_run(g, proj, f, short_scan, s);
     bp3d_launch(g, proj, vol, out_scale, ASTRA_AMD_KER3D_FDK_WEIGHTING, mode, nullptr,
                 -INFINITY, INFINITY, s);
 }

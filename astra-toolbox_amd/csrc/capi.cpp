@@ -23,6 +23,8 @@ Filter *filter_build(unsigned n_det, const float *half, unsigned n_bins);
 void fdk_run(const Geom3D &g, float *proj, float *vol, float out_scale, const Filter *f,
              bool short_scan, int mode, hipStream_t s);
 void fdk_preweight_only(const Geom3D &g, float *proj, hipStream_t s);
+void fdk_filter_run(const Geom3D &g, float *proj, const Filter *f, bool short_scan,
+                    hipStream_t s);
 void fbp2d_run(const Geom2D &g, float *sino, float *vol, float out_scale, const Filter *f,
                bool short_scan, int mode, hipStream_t s);
 struct Batch2D;
@@ -339,6 +341,16 @@ int astra_amd_fdk(astra_amd_geom3d g, float *proj, float *vol, float output_scal
         check_device(g->g->device);
         amx::fdk_run(*g->g, proj, vol, output_scale, f ? f->f : nullptr, short_scan != 0,
                      out_mode(mode), as_stream(stream));
+    });
+}
+
+int astra_amd_fdk_filter(astra_amd_geom3d g, float *proj, int short_scan, astra_amd_filter f,
+                         void *stream) {
+    return guarded([&] {
+        AMX_REQUIRE(g && g->g, "geometry handle is NULL");
+        AMX_REQUIRE(proj, "fdk_filter: NULL data pointer");
+        check_device(g->g->device);
+        amx::fdk_filter_run(*g->g, proj, f ? f->f : nullptr, short_scan != 0, as_stream(stream));
     });
 }
 

@@ -211,6 +211,14 @@ int astra_amd_fdk_preweight(astra_amd_geom3d g, float *proj, void *stream);
  * weights of FDK_PreWeight (cuda/3d/fdk.cu:115-163, 201-251). */
 int astra_amd_fdk(astra_amd_geom3d g, float *proj, float *vol, float output_scale,
                   int short_scan, astra_amd_filter f, int mode, void *stream);
+/* The filtering half of astra_amd_fdk, in place: pre-weights (+ Parker
+ * weights when short_scan != 0) and the row filter (cuda/3d/fdk.cu:
+ * 415-465).  An FDK over a volume split into slabs filters once and then
+ * back projects into each slab with astra_amd_bp3d (kernel FDK_WEIGHTING)
+ * on the slab's geometry (CCompositeGeometryManager::doFDK splits the
+ * volume the same way, src/CompositeGeometryManager.cpp:1283-1304). */
+int astra_amd_fdk_filter(astra_amd_geom3d g, float *proj, int short_scan, astra_amd_filter f,
+                         void *stream);
 /* 2D FBP: astraCUDA::FBP (cuda/2d/fbp.cu:77-190).  FAN: one-row cone
  * pre-weighting (fbp.cu:106-148, with Parker weights when short_scan != 0),
  * filter, FBP-weighted fan BP; PAR: filter, BP with output_scale *

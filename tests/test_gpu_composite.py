@@ -1,0 +1,128 @@
+"""Memory-bounded 3D jobs on host data (astra.composite: the reference's
+CompositeGeometryManager splitting) and GPULink raw-pointer linking, on the
+GPU: a split job equals the unsplit one, for FP, BP (default and matched
+kernels), FDK, cone / par3d / cylindrical detectors."""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture
+def small_memory():
+    import astra
+    from astra import composite as C
+    calls = []
+    orig = C._sub_projector
+
+    def spy(p, pg, vg):
+        calls.append((pg["Vectors"].shape[0] if "Vectors" in pg else None, vg["GridSliceCount"]))
+        return orig(p, pg, vg)
+
+    C._sub_projector = spy
+    try:
+        yield astra, calls
+    finally:
+        C._sub_projector = orig
+        astra.set_gpu_index(0, memory=0)
+
+
+def _geoms(kind, n=40, na=48, nd=56):
+    import astra
+    vg = astra.create_vol_geom(n, n + 4, n - 6, -22, 22, -20, 20, -18, 16)
+    ang = np.linspace(0, 2 * np.pi, na, endpoint=False)
+    if kind == "cone":
+        pg = astra.create_proj_geom("cone", 1.0, 1.1, nd, nd + 2, ang, 150.0, 80.0)
+    elif kind == "parallel3d":
+        pg = astra.create_proj_geom("parallel3d", 1.2, 1.0, nd, nd + 2, ang[: na // 2])
+    else:
+        v = []
+        for t in ang:
+            c, s = np.cos(t), np.sin(t)
+            v.append([s * 150, -c * 150, 0, -s * 80, c * 80, 0, c * 1.1, s * 1.1, 0, 0, 0, 1.0, 230.0])
+        pg = astra.create_proj_geom("cyl_cone_vec", nd, nd + 2, np.array(v))
+    return vg, pg
+
+
+@pytest.mark.parametrize("kind", ["cone", "parallel3d", "cylcone"])
+def test_split_fp_bp_equal_unsplit(small_memory, kind):
+    astra, calls = small_memory
+    from astra import phantom
+    vg, pg = _geoms(kind)
+    vol = np.zeros((34, 40, 44), dtype=np.float32)   # [z][y][x] of the 44 x 40 x 34 grid
+    vol[:, :, 2:42] = phantom.ellipsoids_3d(40, seed=101)[:34]
+    _, ref = astra.create_sino3d_gpu(vol, pg, vg)
+    _, refb = astra.create_backprojection3d_gpu(ref, pg, vg)
+    astra.set_gpu_index(0, memory=1024 * 1024)
+    _, got = astra.create_sino3d_gpu(vol, pg, vg)
+    n_fp = len(calls)
+    _, gotb = astra.create_backprojection3d_gpu(ref, pg, vg)
+    assert n_fp >= 2 and len(calls) >= n_fp + 2  # both jobs were split
+    assert rel_l2(got, ref) <= 1e-6
+    assert rel_l2(gotb, refb) <= 1e-6
+
+
+def test_split_matched_bp_and_mode_add(small_memory):
+    astra, calls = small_memory
+    from astra import experimental
+    vg, pg = _geoms("cone")
+    rng = np.random.default_rng(102)
+    proj = rng.random((56, 48, 58), dtype=np.float32)
+    pid = astra.create_projector("cuda3d", pg, vg, {"ProjectionKernel": "matched_bp"})
+    base = rng.random((34, 40, 44), dtype=np.float32)
+    ref = base.copy()
+    experimental.direct_FPBP3D(pid, ref, proj.copy(), experimental.MODE_ADD, "BP")
+    astra.set_gpu_index(0, memory=1024 * 1024)
+    got = base.copy()
+    experimental.direct_FPBP3D(pid, got, proj.copy(), experimental.MODE_ADD, "BP")
+    assert len(calls) > 2
+    assert rel_l2(got, ref) <= 1e-6
+
+
+def test_split_fdk(small_memory):
+    astra, calls = small_memory
+    from astra import phantom
+    from oracle import fdk as OF, geometry as OG, proj3d as O3
+    n, na, nd = 32, 40, 48
+    vg = astra.create_vol_geom(n, n, n)
+    ang = np.linspace(0, 2 * np.pi, na, endpoint=False)
+    pg = astra.create_proj_geom("cone", 1.0, 1.0, nd, nd, ang, 150.0, 75.0)
+    ov, vs = OG.normalise_3d(OG.gen_cone_projections(ang, nd, nd, 150.0, 75.0, 1.0, 1.0), "cone",
+                             OG.vol_geom_3d(n, n, n))
+    proj = O3.cone_fp(phantom.ellipsoids_3d(n, seed=103), ov, nd, nd, vs).astype(np.float32)
+    fixed = proj.nbytes + na * (nd + 4) * (nd + 4) * 8
+    astra.set_gpu_index(0, memory=int((fixed + n * n * 4 * 10) / 0.9))
+    sid = astra.data3d.create("-sino", pg, proj)
+    rid = astra.data3d.create("-vol", vg, 0)
+    cfg = astra.astra_dict("FDK_CUDA")
+    cfg.update(ProjectionDataId=sid, ReconstructionDataId=rid)
+    astra.algorithm.run(astra.algorithm.create(cfg))
+    assert len(calls) >= 3
+    assert rel_l2(astra.data3d.get(rid), OF.fdk(proj, ov, (n, n, n), vs)) <= 1e-5
+    assert np.array_equal(astra.data3d.get(sid), proj)  # input untouched
+
+
+def test_gpulink_pitched_volume_and_projections():
+    import torch
+    import astra
+    from astra import phantom
+    vg, pg = _geoms("cone")
+    vol = np.ascontiguousarray(phantom.ellipsoids_3d(44, seed=104)[:34, :40, :44])
+    _, ref = astra.create_sino3d_gpu(vol, pg, vg)
+    vbuf = torch.zeros((34, 40, 48), device="cuda")       # rows padded to 48 floats
+    vbuf[:, :, :44] = torch.from_numpy(vol).cuda()
+    pbuf = torch.full((56, 48, 64), 7.0, device="cuda")   # rows padded to 64 floats
+    vid = astra.data3d.link("-vol", vg, astra.GPULink(vbuf.data_ptr(), 44, 40, 34, 48 * 4))
+    sid = astra.data3d.link("-sino", pg, astra.GPULink(pbuf.data_ptr(), 58, 48, 56, 64 * 4))
+    cfg = astra.astra_dict("FP3D_CUDA")
+    cfg.update(ProjectionDataId=sid, VolumeDataId=vid)
+    astra.algorithm.run(astra.algorithm.create(cfg))
+    torch.cuda.synchronize()
+    assert rel_l2(pbuf[:, :, :58].cpu().numpy(), ref) <= 1e-6
+    assert float(pbuf[:, :, 58:].min()) == 7.0 and float(pbuf[:, :, 58:].max()) == 7.0
+    assert rel_l2(astra.data3d.get(sid), ref) <= 1e-6
+    with pytest.raises(ValueError):
+        astra.data3d.link("-vol", vg, astra.GPULink(vbuf.data_ptr(), 44, 40, 33, 48 * 4))
+    astra.data3d.delete([vid, sid])
