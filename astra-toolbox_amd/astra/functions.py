@@ -14,19 +14,30 @@ def clear():
 
 def add_noise_to_sino(sinogram_in, I0, seed=None):
     """Poisson noise for I0 incident photons (functions.py:83-121):
-    b = -max * log(Poisson(I0 exp(-p / max)) / I0)."""
-    if isinstance(sinogram_in, np.ndarray):
-        sino = np.array(sinogram_in, dtype=np.float64)
-    else:
-        from . import data2d
-        sino = np.array(data2d.get(sinogram_in), dtype=np.float64)
-    mx = sino.max()
-    if mx <= 0:
-        return sino.astype(np.float32)
-    rng = np.random.default_rng(seed)
-    counts = rng.poisson(I0 * np.exp(-sino / mx))
-    counts = np.maximum(counts, 1)
-    return (-mx * np.log(counts / I0)).astype(np.float32)
+    b = -max * log(Poisson(I0 exp(-p / max)) / I0), drawn from numpy's
+    global legacy generator (seeded and restored when `seed` is given), one
+    draw per element in C order, so a seed reproduces the reference's
+    output.  A data id is updated in place."""
+    if seed is not None:
+        state = np.random.get_state()
+        np.random.seed(seed)
+    try:
+        if isinstance(sinogram_in, np.ndarray):
+            raw = sinogram_in
+        else:
+            from . import data2d
+            raw = data2d.get(sinogram_in)
+        mx = raw.max()
+        ct = I0 * np.exp(-(raw / mx))
+        counts = np.random.poisson(ct).astype(ct.dtype)
+        with np.errstate(divide="ignore"):
+            out = -mx * np.log(counts / I0)
+        if not isinstance(sinogram_in, np.ndarray):
+            data2d.store(sinogram_in, out)
+        return out
+    finally:
+        if seed is not None:
+            np.random.set_state(state)
 
 
 def geom_2vec(proj_geom):

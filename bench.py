@@ -200,6 +200,99 @@ def differentiable_bench(n=256, na=360, nd=384, iters=20):
             "unrolled_sirt": res}
 
 
+def fan2d_bench(sirt_iters=200):
+    """2D fan beam at 1024^2: seeded 10-ellipse phantom, 720 angles over
+    [0, 2pi), 1536 detectors of width 1, SOD 1000, ODD 500, Poisson noise
+    for 1e5 incident photons; FBP (Ram-Lak) and SIRT_CUDA (200 iterations),
+    data linked on the GPU (DLPack), seconds per job after one warm-up."""
+    import torch
+
+    import astra
+    from astra import phantom
+
+    n, na, nd = 1024, 720, 1536
+    ang = np.linspace(0, 2 * np.pi, na, endpoint=False)
+    vg = astra.create_vol_geom(n, n)
+    pg = astra.create_proj_geom("fanflat", 1.0, nd, ang, 1000.0, 500.0)
+    pid = astra.create_projector("cuda", pg, vg)
+    _, clean = astra.create_sino(phantom.ellipses_2d(n, seed=0), pid)
+    noisy = astra.add_noise_to_sino(clean, 1e5, seed=0).astype(np.float32)
+    sino_t = torch.from_numpy(noisy).cuda()
+    rec_t = torch.zeros((n, n), dtype=torch.float32, device="cuda")
+    sid = astra.data2d.link("-sino", pg, sino_t)
+    rid = astra.data2d.link("-vol", vg, rec_t)
+    res = {"workload": "fan 1024^2, 720 angles over [0,2pi), 1536 dets, SOD 1000, ODD 500, "
+                       "Poisson 1e5 photons"}
+    for name, iters in (("FBP_CUDA", 1), ("SIRT_CUDA", sirt_iters)):
+        cfg = astra.astra_dict(name)
+        cfg.update(ProjectorId=pid, ProjectionDataId=sid, ReconstructionDataId=rid)
+        aid = astra.algorithm.create(cfg)
+        astra.algorithm.run(aid, 1 if iters == 1 else 2)  # warm-up
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        astra.algorithm.run(aid, iters)
+        torch.cuda.synchronize()
+        res[name.split("_")[0].lower() + "_seconds"] = time.perf_counter() - t
+        astra.algorithm.delete(aid)
+    fp_samples = float(na) * nd * n
+    bp_updates = float(n) * n * na
+    res["sirt_iterations"] = sirt_iters
+    res["sirt_Gupdates_per_s"] = sirt_iters * (fp_samples + bp_updates) / res["sirt_seconds"] / 1e9
+    log("bench: fan 1024^2: FBP %.4f s; SIRT %d its %.3f s (%.0f Gupdates/s)"
+        % (res["fbp_seconds"], sirt_iters, res["sirt_seconds"], res["sirt_Gupdates_per_s"]))
+    astra.data2d.delete([sid, rid])
+    astra.projector.delete(pid)
+    return res
+
+
+def large_cone_bench(n=1024, na=1440, nd=1536, steps=2):
+    """Cone FP+BP and FDK at 1024^3 / 1440 x 1536^2, unchunked on one GPU
+    (volume, projections and the internal copies resident in HBM)."""
+    import torch
+
+    import astra
+    from astra import _core, _lib, phantom
+    from astra.algorithm import fdk
+
+    dev = torch.device("cuda", 0)
+    angles = np.linspace(0, 2 * np.pi, na, endpoint=False)
+    vg = astra.create_vol_geom(n, n, n)
+    pg = astra.create_proj_geom("cone", 1.0, 1.0, nd, nd, angles, 1000.0, 500.0)
+    pid = astra.create_projector("cuda3d", pg, vg)
+    P = astra.projector3d.get(pid)
+    vol = phantom.ellipsoids_3d_device(n, count=20, seed=0, device=dev)
+    proj = torch.empty((nd, na, nd), dtype=torch.float32, device=dev)
+    bpv = torch.empty_like(vol)
+
+    def step():
+        _core.fp(P, vol, proj, _lib.MODE_SET)
+        _core.bp(P, proj, bpv, _lib.MODE_SET)
+
+    step()
+    torch.cuda.synchronize(dev)
+    t = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize(dev)
+    dt = (time.perf_counter() - t) / steps
+    pcopy = proj.clone()
+    torch.cuda.synchronize(dev)
+    t = time.perf_counter()
+    fdk(P, pcopy, bpv, _lib.MODE_SET)
+    torch.cuda.synchronize(dev)
+    fdk_s = time.perf_counter() - t
+    peak = torch.cuda.max_memory_allocated(dev) / 2 ** 30
+    ups = (float(na) * nd * nd * n + float(n) ** 3 * na) / dt / 1e9
+    log("bench: cone 1024^3 / 1440 x 1536^2: FP+BP %.3f s/step (%.0f Gupdates/s), FDK %.3f s"
+        % (dt, ups, fdk_s))
+    del vol, proj, bpv, pcopy
+    astra.projector3d.delete(pid)
+    torch.cuda.empty_cache()
+    return {"workload": "cone %d^3, %d angles, %dx%d detector, SOD 1000, ODD 500" % (n, na, nd, nd),
+            "fpbp_seconds_per_step": dt, "Gupdates_per_s": ups, "fdk_seconds": fdk_s,
+            "torch_peak_GiB": peak}
+
+
 TRAFFIC_FILE = "profiles/cone512_fpbp_counters.md"
 
 
@@ -238,6 +331,8 @@ def main():
     ap.add_argument("--batched-iters", type=int, default=50)
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-diff", action="store_true")
+    ap.add_argument("--no-fan2d", action="store_true")
+    ap.add_argument("--no-large", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 1 GiB device copy first (profiler counter scaling)")
     args = ap.parse_args()
@@ -333,6 +428,8 @@ def main():
     diff_res = None
     if not args.no_diff:
         diff_res = differentiable_bench()
+    fan_res = None if args.no_fan2d else fan2d_bench()
+    large_res = None if args.no_large else large_cone_bench()
 
     cpu = None
     if not args.no_cpu:
@@ -365,6 +462,8 @@ def main():
         "fdk_seconds": fdk_s,
         "batched_sirt": batched_res,
         "differentiable": diff_res,
+        "fan2d_1024": fan_res,
+        "cone_1024": large_res,
         "kernels": kernels,
         "cpu_baseline": cpu,
     }

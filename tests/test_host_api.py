@@ -189,3 +189,24 @@ def test_cylcone_geometry_matches_oracle_translation():
         G.vectors_3d(astra.create_proj_geom("cyl_cone_vec", 9, 8, np.c_[vecs[:, :12], -np.ones(7)]))
     with pytest.raises(astra.AstraError):
         astra.create_proj_geom("cyl_cone_vec", 9, 8, vecs[:, :12])
+
+
+def test_add_noise_to_sino_matches_reference_draw_order():
+    # the reference draws one np.random.poisson per element in nested loops
+    # from the seeded global legacy generator and restores its state
+    # (python/astra/functions.py:94-119)
+    rng = np.random.default_rng(9)
+    sino = (rng.random((13, 17)) * 3).astype(np.float32)
+    np.random.seed(1234)
+    before = np.random.random()
+    np.random.seed(1234)
+    got = astra.add_noise_to_sino(sino, 1e4, seed=77)
+    assert np.random.random() == before  # global state restored
+    np.random.seed(77)
+    ct = 1e4 * np.exp(-(sino / sino.max()))
+    c = np.zeros_like(ct)
+    for i in range(c.shape[0]):
+        for j in range(c.shape[1]):
+            c[i, j] = np.random.poisson(ct[i, j])
+    ref = -sino.max() * np.log(c / 1e4)
+    assert got.dtype == ref.dtype and np.array_equal(got, ref)
