@@ -751,3 +751,20 @@ def test_cylcone_bp_adjoint_criterion():
     lhs = float(np.sum(O3.cylcone_fp(x, tv, nu, nv, vs) * y))
     rhs = float(np.sum(x * O3.cylcone_bp(y, tv, (n, n, n), vs)))
     assert abs(lhs - rhs) / abs(lhs) < 1e-1
+
+
+def test_vector_forms_reproduce_angle_geometries():
+    # *_vec rows hold the detector centre; the conversions to corner form
+    # (DetS = centre - n/2 u [- n_v/2 v]) must reproduce the angle-based
+    # generators' corner vectors exactly (a dropped half or sign fails)
+    ang = np.linspace(0.1, 5.9, 11)
+    c3 = OG.gen_cone_projections(ang, 13, 9, 70.0, 30.0, 1.3, 0.7)
+    centre = c3.copy()
+    centre[:, 3:6] += 0.5 * 13 * c3[:, 6:9] + 0.5 * 9 * c3[:, 9:12]
+    assert np.allclose(OG.conevec_from_astra(centre, 13, 9), c3, rtol=0, atol=1e-12)
+    for gen, conv in ((lambda: OG.gen_fan_projections(ang, 17, 60.0, 25.0, 1.1), OG.fanvec_from_astra),
+                      (lambda: OG.gen_par_projections(ang, 17, 1.1), OG.parvec_from_astra)):
+        v = gen()
+        centre = v.copy()
+        centre[:, 2:4] += 0.5 * 17 * v[:, 4:6]
+        assert np.allclose(conv(centre, 17), v, rtol=0, atol=1e-5)
