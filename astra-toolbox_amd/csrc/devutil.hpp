@@ -43,10 +43,16 @@ __device__ __forceinline__ float ld1b(const float *base, unsigned byte_off) {
     return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(base) + byte_off);
 }
 
-// byte offset of texel pair (fl1, fl2) in a row-pair plane (8 bytes per pair)
-__device__ __forceinline__ unsigned pair_off(float fl1, float fl2, float pitch) {
-    return (unsigned)fmaf(fl2, pitch, fl1) << 3;
+// Byte offset of the tap pair of padded coordinates (p1, p2) in a
+// row-pair plane of `pitch` pairs per row (8 bytes per pair).  p >= 0.5
+// after clamp_pad, so truncation (v_cvt_u32_f32) is the floor; NaN
+// converts to 0, which is in bounds.
+__device__ __forceinline__ unsigned pair_off(float p1, float p2, unsigned pitch) {
+    return (__umul24((unsigned)p2, pitch) + (unsigned)p1) << 3;
 }
+
+// fractional part of a clamped (>= 0.5) padded coordinate: p - floor(p)
+__device__ __forceinline__ float frac(float p) { return __builtin_amdgcn_fractf(p); }
 
 // byte offset of padded texel (fl1, fl2) in a plane of row pitch `pitch`
 __device__ __forceinline__ unsigned texel_off(float fl1, float fl2, float pitch) {
@@ -55,10 +61,13 @@ __device__ __forceinline__ unsigned texel_off(float fl1, float fl2, float pitch)
 
 __device__ __forceinline__ float lerp(float a, float b, float w) { return fmaf(w, b - a, a); }
 
-// bilinear value from the four taps of a row-pair load: q = (v[r][c],
-// v[r+1][c], v[r][c+1], v[r+1][c+1]); w1 along c, w2 along r
+// Bilinear value from the load of two difference pairs: q = (v[r][c],
+// v[r+1][c] - v[r][c], v[r][c+1], v[r+1][c+1] - v[r][c+1]); w1 along c, w2
+// along r.  The stored differences are the ones lerp() would compute, so
+// each row-direction lerp is one fma.
 __device__ __forceinline__ float bilerp_pairs(float4 q, float w1, float w2) {
-    return lerp(lerp(q.x, q.z, w1), lerp(q.y, q.w, w1), w2);
+    const float l0 = fmaf(w2, q.y, q.x), l1 = fmaf(w2, q.w, q.z);
+    return fmaf(w1, l1 - l0, l0);
 }
 
 __device__ __forceinline__ int wave_min(int v) {

@@ -13,11 +13,12 @@
 //                     rays of every group and one slice is one contiguous
 //                     plane (32-bit offsets inside a plane).
 //   BP input copy     angle-major and zero-padded: [angle][v+4][u+4]
-//   Both copies are stored row-pair interleaved: element [r][c] is the
-//   float2 (p[r][c], p[r+1][c]), so the four taps of a bilinear sample at
-//   (r, c) are one 16-byte load of [r][c] and [r][c+1].  The texture
-//   address unit handles one instruction per sample instead of two (it is
-//   the unit that limits both kernels); the copies take twice the memory.
+//   Both copies are stored as row difference pairs: element [r][c] is the
+//   float2 (p[r][c], p[r+1][c] - p[r][c]), so the four taps of a bilinear
+//   sample at (r, c) are one 16-byte load of [r][c] and [r][c+1] (the
+//   texture-address unit handles one instruction per sample instead of
+//   two), and each row-direction lerp is one fma; the copies take twice
+//   the memory.
 //
 // Sampling semantics follow cuda/3d/cone_fp.cu:154-231,
 // cuda/3d/par3d_fp.cu:152-230, cuda/3d/cone_bp.cu:55-133 and
@@ -35,9 +36,9 @@
 namespace amx {
 
 // ----------------------------------------------------------- prep --------
-// dst[c0][c2 + 2][c1 + 2] = (vol at c2, vol at c2 + 1) for the z- and
-// y-groups (the fast axis stays x; row pairs as described above).  One
-// workgroup per destination row; borders written 0.
+// dst[c0][c2 + 2][c1 + 2] = (vol at c2, vol at c2 + 1 - vol at c2) for the
+// z- and y-groups (the fast axis stays x; difference pairs as described
+// above).  One workgroup per destination row; borders read 0.
 __global__ __launch_bounds__(256) void pad_copy_xfast(const float *__restrict__ vol,
                                                       float2 *__restrict__ dst, unsigned nx,
                                                       unsigned ny, unsigned nz, int group) {
@@ -57,11 +58,12 @@ __global__ __launch_bounds__(256) void pad_copy_xfast(const float *__restrict__ 
     for (unsigned i = threadIdx.x; i < px; i += blockDim.x) {
         const int x = (int)i - kPad;
         const bool xin = x >= 0 && x < (int)nx;
-        out[i] = make_float2((in0 && xin) ? s0[x] : 0.f, (in1 && xin) ? s1[x] : 0.f);
+        const float a = (in0 && xin) ? s0[x] : 0.f, b = (in1 && xin) ? s1[x] : 0.f;
+        out[i] = make_float2(a, b - a);
     }
 }
 
-// x-group: dst[x][z + 2][y + 2] = (vol[z][y][x], vol[z+1][y][x]); per
+// x-group: dst[x][z + 2][y + 2] = (vol[z][y][x], vol[z+1][y][x] - vol[z][y][x]); per
 // padded z row an (x, y) transpose of planes z and z + 1 through two
 // 64 x 64 LDS tiles.  grid = (y tiles, x tiles, nz + 4).
 __global__ __launch_bounds__(256) void pad_transpose_x(const float *__restrict__ vol,
@@ -89,7 +91,8 @@ __global__ __launch_bounds__(256) void pad_transpose_x(const float *__restrict__
         int x = x0 + r;
         int yp = yp0 + tx;
         if (x < (int)nx && yp < (int)pyn)
-            dst[((size_t)x * pzn + zr) * pyn + yp] = make_float2(tile[0][tx][r], tile[1][tx][r]);
+            dst[((size_t)x * pzn + zr) * pyn + yp] =
+                make_float2(tile[0][tx][r], tile[1][tx][r] - tile[0][tx][r]);
     }
 }
 
@@ -111,8 +114,9 @@ __global__ __launch_bounds__(256) void proj_to_am(const float *__restrict__ proj
     }
 }
 
-// proj [v][a][u] -> padded angle-major row pairs [a][v + 2][u + 2] =
-// (p[v][u], p[v+1][u]) (borders 0).  One workgroup per destination row.
+// proj [v][a][u] -> padded angle-major difference pairs [a][v + 2][u + 2] =
+// (p[v][u], p[v+1][u] - p[v][u]) (borders read 0).  One workgroup per
+// destination row.
 __global__ __launch_bounds__(256) void proj_to_am_pairs(const float *__restrict__ proj,
                                                         float2 *__restrict__ pp, unsigned nu,
                                                         unsigned na, unsigned nv) {
@@ -127,11 +131,12 @@ __global__ __launch_bounds__(256) void proj_to_am_pairs(const float *__restrict_
     for (unsigned i = threadIdx.x; i < pu; i += blockDim.x) {
         const int u = (int)i - kPad;
         const bool uin = u >= 0 && u < (int)nu;
-        out[i] = make_float2((in0 && uin) ? s0[u] : 0.f, (in1 && uin) ? s1[u] : 0.f);
+        const float a = (in0 && uin) ? s0[u] : 0.f, b = (in1 && uin) ? s1[u] : 0.f;
+        out[i] = make_float2(a, b - a);
     }
 }
 
-// padded single layout [a][v + 4][u + 4] -> row pairs (same padding).
+// padded single layout [a][v + 4][u + 4] -> difference pairs (same padding).
 __global__ __launch_bounds__(256) void am_to_pairs(const float *__restrict__ src,
                                                    float2 *__restrict__ pp, unsigned nu,
                                                    unsigned nv) {
@@ -140,8 +145,10 @@ __global__ __launch_bounds__(256) void am_to_pairs(const float *__restrict__ src
     const bool last = (row % pv) == pv - 1;
     const float *s0 = src + (size_t)row * pu;
     float2 *out = pp + (size_t)row * pu;
-    for (unsigned i = threadIdx.x; i < pu; i += blockDim.x)
-        out[i] = make_float2(s0[i], last ? 0.f : s0[pu + i]);
+    for (unsigned i = threadIdx.x; i < pu; i += blockDim.x) {
+        const float a = s0[i], b = last ? 0.f : s0[pu + i];
+        out[i] = make_float2(a, b - a);
+    }
 }
 
 // ------------------------------------------------------------- FP --------
@@ -234,7 +241,6 @@ __device__ __forceinline__ float march3d(const FPArgs &A, float a1f, float a2f, 
                                          float c2f, int slo, int shi) {
     const unsigned pitch = A.n1 + 2 * kPad;                        // pairs per row
     const size_t plane = (size_t)(A.n2 + 2 * kPad) * pitch * 2;     // floats per slice
-    const float pitchf = (float)pitch;
     const float hi1 = (float)A.n1 + 2.5f, hi2 = (float)A.n2 + 2.5f;
     float acc0 = 0.f, acc1 = 0.f;
     int s = slo;
@@ -245,19 +251,17 @@ __device__ __forceinline__ float march3d(const FPArgs &A, float a1f, float a2f, 
         float p2 = clamp_pad(fmaf(a2f, sf, c2f), hi2);
         float q1 = clamp_pad(fmaf(a1f, sf + 1.f, c1f), hi1);
         float q2 = clamp_pad(fmaf(a2f, sf + 1.f, c2f), hi2);
-        float fp1 = floorf(p1), fp2 = floorf(p2), fq1 = floorf(q1), fq2 = floorf(q2);
-        float4 r = ld4b(pl, pair_off(fp1, fp2, pitchf));
-        float4 t = ld4b(pl + plane, pair_off(fq1, fq2, pitchf));
-        acc0 += bilerp_pairs(r, p1 - fp1, p2 - fp2);
-        acc1 += bilerp_pairs(t, q1 - fq1, q2 - fq2);
+        float4 r = ld4b(pl, pair_off(p1, p2, pitch));
+        float4 t = ld4b(pl + plane, pair_off(q1, q2, pitch));
+        acc0 += bilerp_pairs(r, frac(p1), frac(p2));
+        acc1 += bilerp_pairs(t, frac(q1), frac(q2));
         pl += 2 * plane;
     }
     if (s <= shi) {
         const float sf = (float)s;
         float p1 = clamp_pad(fmaf(a1f, sf, c1f), hi1);
         float p2 = clamp_pad(fmaf(a2f, sf, c2f), hi2);
-        float fp1 = floorf(p1), fp2 = floorf(p2);
-        acc0 += bilerp_pairs(ld4b(pl, pair_off(fp1, fp2, pitchf)), p1 - fp1, p2 - fp2);
+        acc0 += bilerp_pairs(ld4b(pl, pair_off(p1, p2, pitch)), frac(p1), frac(p2));
     }
     return acc0 + acc1;
 }
@@ -403,7 +407,6 @@ __global__ __launch_bounds__(256) void bp3d_kernel(BPArgs A) {
     const float fz = Z0 - 0.5f * A.nz + 0.5f;
     const unsigned pitch = A.nu + 2 * kPad;                    // pairs per row
     const size_t plane = (size_t)(A.nv + 2 * kPad) * pitch * 2; // floats per angle
-    const float pitchf = (float)pitch;
     const unsigned rowb = pitch * 8u;                           // bytes per row
     const float hiu = (float)A.nu + 2.5f, hiv = (float)A.nv + 2.5f;
     float acc[ZS];
@@ -420,17 +423,14 @@ __global__ __launch_bounds__(256) void bp3d_kernel(BPArgs A) {
         if (ZC) {
             const float fr = (KIND == 2) ? 1.f : __builtin_amdgcn_rcpf(dn);
             const float pu = clamp_pad(un * fr, hiu);
-            const float fu = floorf(pu);
-            const float wu = pu - fu;
-            const unsigned ou = (unsigned)fu << 3;
+            const float wu = frac(pu);
+            const unsigned ou = (unsigned)pu << 3;
             const float wgt = (KIND == 0) ? fr * fr : (KIND == 1 ? fr : wa);
 #pragma unroll
             for (int i = 0; i < ZS; ++i) {
+                // pv in [0.5, nv + 2.5]: truncation is the floor
                 float pv = clamp_pad(vn * fr, hiv);
-                // pv in [0.5, nv + 2.5]: floor is exact and fract = pv - floor(pv)
-                unsigned iv = (unsigned)floorf(pv);
-                float wv = __builtin_amdgcn_fractf(pv);
-                float val = bilerp_pairs(ld4b(pl, __umul24(iv, rowb) + ou), wu, wv);
+                float val = bilerp_pairs(ld4b(pl, __umul24((unsigned)pv, rowb) + ou), wu, frac(pv));
                 acc[i] = fmaf(wgt, val, acc[i]);
                 vn += c.nv[2];
             }
@@ -440,8 +440,7 @@ __global__ __launch_bounds__(256) void bp3d_kernel(BPArgs A) {
                 float fr = (KIND == 2) ? 1.f : __builtin_amdgcn_rcpf(dn);
                 float pu = clamp_pad(un * fr, hiu);
                 float pv = clamp_pad(vn * fr, hiv);
-                float fu = floorf(pu), fv = floorf(pv);
-                float val = bilerp_pairs(ld4b(pl, pair_off(fu, fv, pitchf)), pu - fu, pv - fv);
+                float val = bilerp_pairs(ld4b(pl, pair_off(pu, pv, pitch)), frac(pu), frac(pv));
                 float wgt = (KIND == 0) ? fr * fr : (KIND == 1 ? fr : wa);
                 acc[i] = fmaf(wgt, val, acc[i]);
                 un += c.nu[2];
@@ -485,7 +484,6 @@ __global__ __launch_bounds__(256) void bp3d_ss_kernel(BPArgs A, int R) {
     const float fz0 = Z - 0.5f * A.nz + 0.5f * sub;
     const unsigned pitch = A.nu + 2 * kPad;
     const size_t plane = (size_t)(A.nv + 2 * kPad) * pitch * 2;
-    const float pitchf = (float)pitch;
     const float hiu = (float)A.nu + 2.5f, hiv = (float)A.nv + 2.5f;
     float acc = 0.f;
     const float *pl = A.pp;
@@ -506,8 +504,7 @@ __global__ __launch_bounds__(256) void bp3d_ss_kernel(BPArgs A, int R) {
                         fr = __builtin_amdgcn_rcpf(dn);
                     }
                     float pu = clamp_pad(un * fr, hiu), pv = clamp_pad(vn * fr, hiv);
-                    float fu = floorf(pu), fv = floorf(pv);
-                    float val = bilerp_pairs(ld4b(pl, pair_off(fu, fv, pitchf)), pu - fu, pv - fv);
+                    float val = bilerp_pairs(ld4b(pl, pair_off(pu, pv, pitch)), frac(pu), frac(pv));
                     acc = fmaf((KIND == 2) ? wa : fr * fr, val, acc);
                 }
             }
@@ -551,7 +548,6 @@ __global__ __launch_bounds__(256) void bp3d_cyl_kernel(BPArgs A, const CylBPAngl
     const float fz = vz * (Z0 - 0.5f * A.nz + 0.5f);
     const unsigned pitch = A.nu + 2 * kPad;
     const size_t plane = (size_t)(A.nv + 2 * kPad) * pitch * 2;
-    const float pitchf = (float)pitch;
     const float hiu = (float)A.nu + 2.5f, hiv = (float)A.nv + 2.5f;
     // texture coordinate -> padded texel coordinate: - 0.5 + 2
     const float ou = 0.5f * A.nu + 1.5f, ov = 0.5f * A.nv + 1.5f;
@@ -575,8 +571,7 @@ __global__ __launch_bounds__(256) void bp3d_cyl_kernel(BPArgs A, const CylBPAngl
             const float sn = fminf(fmaxf(fmaf(t, rb, c.sb) * c.inv_r, -1.f), 1.f);
             const float pu = hit ? clamp_pad(fmaf(asinf(sn), c.inv_dth, ou), hiu) : 0.5f;
             const float pv = hit ? clamp_pad(fmaf(fmaf(t, rc, c.sc), c.inv_nv, ov), hiv) : 0.5f;
-            const float fu = floorf(pu), fv = floorf(pv);
-            const float val = bilerp_pairs(ld4b(pl, pair_off(fu, fv, pitchf)), pu - fu, pv - fv);
+            const float val = bilerp_pairs(ld4b(pl, pair_off(pu, pv, pitch)), frac(pu), frac(pv));
             acc[i] = fmaf(t * t, val, acc[i]);
             ra += dra;
             rb += drb;
