@@ -53,6 +53,10 @@ SIGNATURES = {
     "astra_amd_bp2d_matched": [_vp, _vp, _vp, ctypes.POINTER(Params2D), _i, _vp],
     "astra_amd_fp3d": [_vp, _vp, _vp, ctypes.POINTER(Params3D), _i, _vp],
     "astra_amd_bp3d": [_vp, _vp, _vp, ctypes.POINTER(Params3D), _i, _vp],
+    "astra_amd_fp2d_pitched": [_vp, _vp, _sz, _vp, _sz, ctypes.POINTER(Params2D), _i, _vp],
+    "astra_amd_bp2d_pitched": [_vp, _vp, _sz, _vp, _sz, ctypes.POINTER(Params2D), _i, _vp],
+    "astra_amd_fp3d_pitched": [_vp, _vp, _sz, _vp, _sz, ctypes.POINTER(Params3D), _i, _vp],
+    "astra_amd_bp3d_pitched": [_vp, _vp, _sz, _vp, _sz, ctypes.POINTER(Params3D), _i, _vp],
     "astra_amd_ramlak_spectrum": [_u, _f, _vp, _u],
     "astra_amd_filter_spectrum": [_i, _u, _f, _f, _vp, _u],
     "astra_amd_filter_create": [_u, _vp, _u, ctypes.POINTER(_vp)],

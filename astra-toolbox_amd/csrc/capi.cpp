@@ -4,6 +4,7 @@
 // reference (include/astra/Logging.h:169).
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <string>
 
 #include "devutil.hpp"
@@ -93,6 +94,40 @@ int out_mode(int mode) {
 }
 
 hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+// A pitched device array (`rows` rows of `width` floats, rows `pitch` floats
+// apart) seen as a contiguous one: the array itself when pitch == width,
+// else a stream-ordered scratch copy (filled from the array when `load`),
+// written back by store().
+class Contig {
+  public:
+    Contig(const float *p, size_t width, size_t rows, size_t pitch, bool load, hipStream_t s)
+        : src_(const_cast<float *>(p)), w_(width), rows_(rows), pitch_(pitch), s_(s) {
+        AMX_REQUIRE(pitch >= width, "pitch (%zu floats) is smaller than the row width (%zu)",
+                    pitch, width);
+        if (pitch == width) {
+            ptr_ = src_;
+            return;
+        }
+        scr_.reset(new amx::Scratch(width * rows * sizeof(float), s));
+        ptr_ = scr_->as<float>();
+        if (load)
+            AMX_HIP(hipMemcpy2DAsync(ptr_, w_ * sizeof(float), src_, pitch_ * sizeof(float),
+                                     w_ * sizeof(float), rows_, hipMemcpyDeviceToDevice, s_));
+    }
+    float *get() const { return ptr_; }
+    void store() {
+        if (ptr_ != src_)
+            AMX_HIP(hipMemcpy2DAsync(src_, pitch_ * sizeof(float), ptr_, w_ * sizeof(float),
+                                     w_ * sizeof(float), rows_, hipMemcpyDeviceToDevice, s_));
+    }
+
+  private:
+    float *src_, *ptr_ = nullptr;
+    size_t w_, rows_, pitch_;
+    hipStream_t s_;
+    std::unique_ptr<amx::Scratch> scr_;
+};
 
 }  // namespace
 
@@ -526,6 +561,71 @@ int astra_amd_algo_destroy(astra_amd_algo a) {
         if (!a) return;
         amx::algo_destroy(a->a);
         delete a;
+    });
+}
+
+// ------------------------------------------------- pitched buffers -----
+int astra_amd_fp2d_pitched(astra_amd_geom2d g, const float *vol, size_t vol_pitch, float *sino,
+                           size_t sino_pitch, const astra_amd_params2d *p, int mode,
+                           void *stream) {
+    return guarded([&] {
+        check2d(g, p);
+        AMX_REQUIRE(vol && sino, "fp2d: NULL data pointer");
+        const amx::Geom2D &G = *g->g;
+        hipStream_t st = as_stream(stream);
+        Contig v(vol, G.w, G.h, vol_pitch, true, st);
+        Contig o(sino, G.nd, G.na, sino_pitch, mode == ASTRA_AMD_MODE_ADD, st);
+        int rc = astra_amd_fp2d(g, v.get(), o.get(), p, mode, stream);
+        if (rc) amx::fail(rc, "%s", t_err.c_str());
+        o.store();
+    });
+}
+
+int astra_amd_bp2d_pitched(astra_amd_geom2d g, const float *sino, size_t sino_pitch, float *vol,
+                           size_t vol_pitch, const astra_amd_params2d *p, int mode,
+                           void *stream) {
+    return guarded([&] {
+        check2d(g, p);
+        AMX_REQUIRE(vol && sino, "bp2d: NULL data pointer");
+        const amx::Geom2D &G = *g->g;
+        hipStream_t st = as_stream(stream);
+        Contig i(sino, G.nd, G.na, sino_pitch, true, st);
+        Contig o(vol, G.w, G.h, vol_pitch, mode == ASTRA_AMD_MODE_ADD, st);
+        int rc = astra_amd_bp2d(g, i.get(), o.get(), p, mode, stream);
+        if (rc) amx::fail(rc, "%s", t_err.c_str());
+        o.store();
+    });
+}
+
+int astra_amd_fp3d_pitched(astra_amd_geom3d g, const float *vol, size_t vol_pitch, float *proj,
+                           size_t proj_pitch, const astra_amd_params3d *p, int mode,
+                           void *stream) {
+    return guarded([&] {
+        check3d(g, p);
+        AMX_REQUIRE(vol && proj, "fp3d: NULL data pointer");
+        const amx::Geom3D &G = *g->g;
+        hipStream_t st = as_stream(stream);
+        Contig v(vol, G.nx, (size_t)G.ny * G.nz, vol_pitch, true, st);
+        Contig o(proj, G.nu, (size_t)G.na * G.nv, proj_pitch, mode == ASTRA_AMD_MODE_ADD, st);
+        int rc = astra_amd_fp3d(g, v.get(), o.get(), p, mode, stream);
+        if (rc) amx::fail(rc, "%s", t_err.c_str());
+        o.store();
+    });
+}
+
+int astra_amd_bp3d_pitched(astra_amd_geom3d g, const float *proj, size_t proj_pitch, float *vol,
+                           size_t vol_pitch, const astra_amd_params3d *p, int mode,
+                           void *stream) {
+    return guarded([&] {
+        check3d(g, p);
+        AMX_REQUIRE(vol && proj, "bp3d: NULL data pointer");
+        const amx::Geom3D &G = *g->g;
+        hipStream_t st = as_stream(stream);
+        Contig i(proj, G.nu, (size_t)G.na * G.nv, proj_pitch, true, st);
+        Contig o(vol, G.nx, (size_t)G.ny * G.nz, vol_pitch, mode == ASTRA_AMD_MODE_ADD, st);
+        int rc = astra_amd_bp3d(g, i.get(), o.get(), p, mode, stream);
+        if (rc) amx::fail(rc, "%s", t_err.c_str());
+        o.store();
     });
 }
 

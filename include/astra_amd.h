@@ -154,6 +154,24 @@ int astra_amd_bp2d(astra_amd_geom2d g, const float *sino, float *vol,
  * (cuda/2d/fan_bp.cu:311-336). FAN geometries only. */
 int astra_amd_bp2d_fbp_weighted(astra_amd_geom2d g, const float *sino, float *vol,
                                 const astra_amd_params2d *p, int mode, void *stream);
+/* Pitched variants of fp2d / bp2d / fp3d / bp3d: the device arrays have
+ * rows `*_pitch` floats apart (>= the row width: nx / nu / nd / w), as the
+ * reference's pitched GPU data objects (cudaMalloc3D / cudaMallocPitch,
+ * cuda/3d/mem3d.cu:76-118, cuda/2d/mem2d.cu:54-92).  A
+ * pitch equal to the width runs in place; otherwise the array goes through
+ * a stream-ordered contiguous copy (padding columns are left untouched). */
+int astra_amd_fp2d_pitched(astra_amd_geom2d g, const float *vol, size_t vol_pitch, float *sino,
+                           size_t sino_pitch, const astra_amd_params2d *p, int mode,
+                           void *stream);
+int astra_amd_bp2d_pitched(astra_amd_geom2d g, const float *sino, size_t sino_pitch, float *vol,
+                           size_t vol_pitch, const astra_amd_params2d *p, int mode,
+                           void *stream);
+int astra_amd_fp3d_pitched(astra_amd_geom3d g, const float *vol, size_t vol_pitch, float *proj,
+                           size_t proj_pitch, const astra_amd_params3d *p, int mode,
+                           void *stream);
+int astra_amd_bp3d_pitched(astra_amd_geom3d g, const float *proj, size_t proj_pitch, float *vol,
+                           size_t vol_pitch, const astra_amd_params3d *p, int mode,
+                           void *stream);
 /* 3D forward projection: astraCUDA3d::FP (include/astra/cuda/3d/mem3d.h:107-108,
  * cuda/3d/mem3d.cu:334-398). */
 int astra_amd_fp3d(astra_amd_geom3d g, const float *vol, float *proj,

@@ -126,3 +126,61 @@ def test_gpulink_pitched_volume_and_projections():
     with pytest.raises(ValueError):
         astra.data3d.link("-vol", vg, astra.GPULink(vbuf.data_ptr(), 44, 40, 33, 48 * 4))
     astra.data3d.delete([vid, sid])
+
+
+def test_pitched_entry_points_equal_contiguous():
+    """astra_amd_{fp,bp}{2,3}d_pitched on padded rows equal the contiguous
+    calls, MODE_ADD included, and leave the padding columns untouched."""
+    import ctypes
+    import torch
+    import astra
+    from astra import _core, _lib
+    vg, pg = _geoms("cone")
+    p3 = _core.Projector("cuda3d", pg, vg)
+    dev = torch.device("cuda", 0)
+    h3 = ctypes.c_void_p(p3.handle(dev))
+    s = _core.stream_ptr(dev)
+    rng = np.random.default_rng(105)
+    vol = torch.from_numpy(rng.random((34, 40, 44), dtype=np.float32)).cuda()
+    prm = p3.params3d()
+    ref = torch.empty((56, 48, 58), device="cuda")
+    _lib.call("astra_amd_fp3d", h3, _core.ptr(vol), _core.ptr(ref), ctypes.byref(prm), _lib.MODE_SET, s)
+    vpad = torch.full((34, 40, 50), 3.0, device="cuda")
+    vpad[:, :, :44] = vol
+    ppad = torch.full((56, 48, 61), 5.0, device="cuda")
+    _lib.call("astra_amd_fp3d_pitched", h3, _core.ptr(vpad), 50, _core.ptr(ppad), 61,
+              ctypes.byref(prm), _lib.MODE_SET, s)
+    torch.cuda.synchronize()
+    assert torch.equal(ppad[:, :, :58], ref) and bool((ppad[:, :, 58:] == 5.0).all())
+    refb = torch.ones((34, 40, 44), device="cuda")
+    _lib.call("astra_amd_bp3d", h3, _core.ptr(ref), _core.ptr(refb), ctypes.byref(prm), _lib.MODE_ADD, s)
+    vout = torch.full((34, 40, 50), 1.0, device="cuda")
+    _lib.call("astra_amd_bp3d_pitched", h3, _core.ptr(ppad), 61, _core.ptr(vout), 50,
+              ctypes.byref(prm), _lib.MODE_ADD, s)
+    torch.cuda.synchronize()
+    assert torch.equal(vout[:, :, :44], refb) and bool((vout[:, :, 44:] == 1.0).all())
+    # 2D
+    vg2 = astra.create_vol_geom(30, 26)
+    pg2 = astra.create_proj_geom("fanflat", 1.0, 40, np.linspace(0, 6.2, 33), 80.0, 40.0)
+    p2 = _core.Projector("cuda", pg2, vg2)
+    h2 = ctypes.c_void_p(p2.handle(dev))
+    prm2 = p2.params2d()
+    img = torch.from_numpy(rng.random((30, 26), dtype=np.float32)).cuda()
+    sino = torch.empty((33, 40), device="cuda")
+    _lib.call("astra_amd_fp2d", h2, _core.ptr(img), _core.ptr(sino), ctypes.byref(prm2), _lib.MODE_SET, s)
+    ipad = torch.zeros((30, 32), device="cuda")
+    ipad[:, :26] = img
+    spad = torch.full((33, 45), 2.0, device="cuda")
+    _lib.call("astra_amd_fp2d_pitched", h2, _core.ptr(ipad), 32, _core.ptr(spad), 45,
+              ctypes.byref(prm2), _lib.MODE_SET, s)
+    bref = torch.empty((30, 26), device="cuda")
+    _lib.call("astra_amd_bp2d", h2, _core.ptr(sino), _core.ptr(bref), ctypes.byref(prm2), _lib.MODE_SET, s)
+    bpad = torch.full((30, 32), 4.0, device="cuda")
+    _lib.call("astra_amd_bp2d_pitched", h2, _core.ptr(spad), 45, _core.ptr(bpad), 32,
+              ctypes.byref(prm2), _lib.MODE_SET, s)
+    torch.cuda.synchronize()
+    assert torch.equal(spad[:, :40], sino) and bool((spad[:, 40:] == 2.0).all())
+    assert torch.equal(bpad[:, :26], bref) and bool((bpad[:, 26:] == 4.0).all())
+    with pytest.raises(astra.AstraError):
+        _lib.call("astra_amd_fp2d_pitched", h2, _core.ptr(ipad), 20, _core.ptr(spad), 45,
+                  ctypes.byref(prm2), _lib.MODE_SET, s)
