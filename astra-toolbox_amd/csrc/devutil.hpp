@@ -54,11 +54,6 @@ __device__ __forceinline__ unsigned pair_off(float p1, float p2, unsigned pitch)
 // fractional part of a clamped (>= 0.5) padded coordinate: p - floor(p)
 __device__ __forceinline__ float frac(float p) { return __builtin_amdgcn_fractf(p); }
 
-// byte offset of padded texel (fl1, fl2) in a plane of row pitch `pitch`
-__device__ __forceinline__ unsigned texel_off(float fl1, float fl2, float pitch) {
-    return (unsigned)fmaf(fl2, pitch, fl1) << 2;
-}
-
 __device__ __forceinline__ float lerp(float a, float b, float w) { return fmaf(w, b - a, a); }
 
 // Bilinear value from the load of two difference pairs: q = (v[r][c],

@@ -133,47 +133,102 @@ __global__ __launch_bounds__(256) void bp2d_ss_kernel(BP2DArgs A, int R) {
 }
 
 // --------------------------------------------------- matched BP ----------
-// Exact transpose of fp2d_kernel: every sample scatters its two linear
-// weights into the padded slice copies (fp32 atomics).
+// Exact transpose of fp2d_kernel as a gather (deterministic, no atomics).
+// The FP takes, per (sub-)ray of lattice index L = R det + k and slice s of
+// the ray's orientation class, the linear sample at padded coordinate
+// t = a s + c (fp32) of the rows copy (vertical rays: slice = row, t along
+// the columns) or of the columns copy (horizontal rays: slice = column, t
+// along the rows).  Pixel (X, Y) therefore receives, per angle,
+//     sum over vertical rays   of y' (1 - |a Y + c - (X + 2)|)+
+//   + sum over horizontal rays of y' (1 - |a X + c - (Y + 2)|)+,
+// y' = y * out_scale * distance weight / R.  Candidates are the lattice
+// points between the images of the two neighbouring pixel centres along the
+// class's in-slice axis under the BP's detector map (padded texel
+// coordinate t_pad = num / den; lattice = R (t_pad - 1.5) - 0.5), widened by
+// 0.05 for fp32 rounding; extra candidates get weight 0.
+//
+// matched2d_rays_kernel: per (angle, lattice index) the ray's fp32 (a, c),
+// y' and its class (1 = vertical).
 template <bool FAN>
-__global__ __launch_bounds__(256) void fp2d_transpose_kernel(FP2DArgs A, const float *__restrict__ sino) {
-    const int det = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int angle = blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (det >= (int)A.nd || angle >= (int)A.na) return;
-    const FP2DAngle &G = A.ang[angle];
-    const bool vertical = ray_vertical<FAN>(G, det);
-    const float y0 = sino[(size_t)angle * A.nd + det] * (A.osc / (float)A.rays);
-    if (y0 == 0.f) return;
-    float *rows = const_cast<float *>(A.rows);
-    float *cols = const_cast<float *>(A.cols);
-    for (unsigned k = 0; k < A.rays; ++k) {
+__global__ __launch_bounds__(256) void matched2d_rays_kernel(FP2DArgs A, const float *__restrict__ sino,
+                                                             float4 *__restrict__ rays) {
+    const unsigned NL = A.nd * A.rays;
+    const size_t n = (size_t)NL * A.na;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const unsigned angle = (unsigned)(i / NL), L = (unsigned)(i - (size_t)angle * NL);
+        const int det = (int)(L / A.rays), k = (int)(L % A.rays);
+        const FP2DAngle &G = A.ang[angle];
+        const bool vertical = ray_vertical<FAN>(G, det);
         const double fdet = det + (0.5 + k) / (double)A.rays;
-        Ray2D R = ray2d<FAN>(G, fdet, vertical, A.w, A.h);
-        const float y = y0 * R.scale;
-        float *base = vertical ? rows : cols;
-        const unsigned pitch = vertical ? A.w + 2 * kPad : A.h + 2 * kPad;
-        const float hi = vertical ? (float)A.w + 2.5f : (float)A.h + 2.5f;
-        for (int s = R.lo; s <= R.hi; ++s) {
-            float t = clamp_pad(fmaf(R.a, (float)s, R.c), hi);
-            float f = floorf(t);
-            float wt = t - f;
-            float *p = base + (size_t)s * pitch + (unsigned)f;
-            atomicAdd(p, y * (1.f - wt));
-            atomicAdd(p + 1, y * wt);
-        }
+        const Ray2D R = ray2d<FAN>(G, fdet, vertical, A.w, A.h);
+        const float y0 = sino[(size_t)angle * A.nd + det] * (A.osc / (float)A.rays);
+        rays[i] = make_float4(R.a, R.c, y0 * R.scale, vertical ? 1.f : 0.f);
     }
 }
 
-// vol[r][c] (+)= rows[r][c+2] + cols[c][r+2]
-__global__ __launch_bounds__(256) void unpad2d_add(const float *__restrict__ rows,
-                                                   const float *__restrict__ cols,
-                                                   float *__restrict__ vol, unsigned w,
-                                                   unsigned h) {
-    const unsigned c = blockIdx.x * 64 + (threadIdx.x & 63);
-    const unsigned r = blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (c >= w || r >= h) return;
-    vol[(size_t)r * w + c] += rows[(size_t)r * (w + 2 * kPad) + c + kPad] +
-                              cols[(size_t)c * (h + 2 * kPad) + r + kPad];
+// lattice coordinate of world point (x, y) under the BP's detector map
+template <bool FAN>
+__device__ __forceinline__ float lattice2d(const BP2DAngle &c, float x, float y, float R) {
+    const float num = fmaf(y, c.num[2], fmaf(x, c.num[1], c.num[0]));
+    const float t = FAN ? num / fmaf(y, c.den[2], fmaf(x, c.den[1], c.den[0])) : num;
+    return fmaf(R, t - 1.5f, -0.5f);
+}
+
+// candidates [l0, l1] of one orientation class for slice `sf` and in-slice
+// padded target J, weighted into acc
+__device__ __forceinline__ float gather2d_class(const float4 *__restrict__ ra, float la, float lb,
+                                                float NLm, float want, float sf, float J) {
+    constexpr float kEps = 0.05f;
+    const int l0 = (int)fminf(fmaxf(ceilf(fminf(la, lb) - kEps), 0.f), NLm + 1.f);
+    const int l1 = (int)fmaxf(fminf(floorf(fmaxf(la, lb) + kEps), NLm), -1.f);
+    float acc = 0.f;
+    for (int L = l0; L <= l1; ++L) {
+        const float4 r = ra[L];
+        const float e = fmaxf(1.f - fabsf(fmaf(r.x, sf, r.y) - J), 0.f);
+        acc = fmaf(r.w == want ? r.z : 0.f, e, acc);
+    }
+    return acc;
+}
+
+struct Matched2DArgs {
+    const float4 *rays;      // [angle][nd * R]
+    const BP2DAngle *bpa;    // detector map per angle (num / den)
+    const FP2DAngle *fpa;    // per-angle class (parallel beams)
+    unsigned w, h, na, nd, R;
+    float *vol;
+    int mode;
+};
+
+// One thread per pixel, 64 x 4 pixels per group, all angles.
+template <bool FAN>
+__global__ __launch_bounds__(256) void matched2d_gather(Matched2DArgs M) {
+    const int X = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int Y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (X >= (int)M.w || Y >= (int)M.h) return;
+    const float x = X - 0.5f * M.w + 0.5f, y = -(Y - 0.5f * M.h + 0.5f);
+    const float Rf = (float)M.R;
+    const unsigned NL = M.nd * M.R;
+    const float NLm = (float)NL - 1.f;
+    const float Jc = (float)(X + kPad), Jr = (float)(Y + kPad);
+    float acc = 0.f;
+    for (unsigned a = 0; a < M.na; ++a) {
+        const BP2DAngle c = M.bpa[a];
+        const float4 *ra = M.rays + (size_t)a * NL;
+        // vertical rays: slice Y, neighbours X -/+ 1 along the columns
+        if (FAN || M.fpa[a].vertical)
+            acc += gather2d_class(ra, lattice2d<FAN>(c, x - 1.f, y, Rf),
+                                  lattice2d<FAN>(c, x + 1.f, y, Rf), NLm, 1.f, (float)Y, Jc);
+        // horizontal rays: slice X, neighbours Y -/+ 1 along the rows
+        if (FAN || !M.fpa[a].vertical)
+            acc += gather2d_class(ra, lattice2d<FAN>(c, x, y + 1.f, Rf),
+                                  lattice2d<FAN>(c, x, y - 1.f, Rf), NLm, 0.f, (float)X, Jr);
+    }
+    float *o = M.vol + (size_t)Y * M.w + X;
+    if (M.mode == OUT_SET)
+        *o = acc;
+    else
+        *o += acc;
 }
 
 // ------------------------------------------------------------ launch -----
@@ -301,27 +356,30 @@ void bp2d_launch(const Geom2D &g, const float *sino, float *vol, float out_scale
 void bp2d_matched_launch(const Geom2D &g, const float *sino, float *vol, float out_scale,
                          int mode, hipStream_t s, unsigned rays) {
     AMX_REQUIRE(rays >= 1 && rays <= 64, "bp2d: rays per detector must be in [1, 64]");
-    const size_t npix = (size_t)g.w * g.h;
-    if (mode == OUT_SET) fill_launch(vol, npix, 0.f, s);
-    Scratch copies((rows_elems(g) + cols_elems(g)) * sizeof(float), s);
-    float *rows = copies.as<float>();
-    float *cols = rows + rows_elems(g);
-    fill_launch(rows, rows_elems(g) + cols_elems(g), 0.f, s);
+    AMX_REQUIRE((size_t)g.nd * rays < (1u << 24), "bp2d: too many rays per angle");
     FP2DArgs A{};
-    A.rows = rows;
-    A.cols = cols;
     A.w = g.w; A.h = g.h; A.na = g.na; A.nd = g.nd;
     A.ang = g.fp.as<FP2DAngle>();
     A.osc = out_scale;
     A.rays = rays;
-    dim3 grid((g.nd + 63) / 64, (g.na + 3) / 4);
-    if (g.kind == ASTRA_AMD_GEOM_FAN)
-        fp2d_transpose_kernel<true><<<grid, 256, 0, s>>>(A, sino);
-    else
-        fp2d_transpose_kernel<false><<<grid, 256, 0, s>>>(A, sino);
-    AMX_LAUNCHED();
-    dim3 ug((g.w + 63) / 64, (g.h + 3) / 4);
-    unpad2d_add<<<ug, 256, 0, s>>>(rows, cols, vol, g.w, g.h);
+    const size_t n = (size_t)g.na * g.nd * rays;
+    Scratch table(n * sizeof(float4), s);
+    const unsigned rb = (unsigned)std::min<size_t>((n + 255) / 256, 1u << 20);
+    Matched2DArgs M{};
+    M.rays = table.as<float4>();
+    M.bpa = g.bp.as<BP2DAngle>();
+    M.fpa = g.fp.as<FP2DAngle>();
+    M.w = g.w; M.h = g.h; M.na = g.na; M.nd = g.nd; M.R = rays;
+    M.vol = vol;
+    M.mode = mode;
+    dim3 grid((g.w + 63) / 64, (g.h + 3) / 4);
+    if (g.kind == ASTRA_AMD_GEOM_FAN) {
+        matched2d_rays_kernel<true><<<rb, 256, 0, s>>>(A, sino, table.as<float4>());
+        matched2d_gather<true><<<grid, 256, 0, s>>>(M);
+    } else {
+        matched2d_rays_kernel<false><<<rb, 256, 0, s>>>(A, sino, table.as<float4>());
+        matched2d_gather<false><<<grid, 256, 0, s>>>(M);
+    }
     AMX_LAUNCHED();
 }
 

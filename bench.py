@@ -276,6 +276,8 @@ def large_cone_bench(n=1024, na=1440, nd=1536, steps=2):
     torch.cuda.synchronize(dev)
     dt = (time.perf_counter() - t) / steps
     pcopy = proj.clone()
+    fdk(P, pcopy, bpv, _lib.MODE_SET)  # warm-up (weights, filter, scratch pool)
+    pcopy.copy_(proj)
     torch.cuda.synchronize(dev)
     t = time.perf_counter()
     fdk(P, pcopy, bpv, _lib.MODE_SET)
