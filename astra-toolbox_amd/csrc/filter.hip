@@ -189,8 +189,23 @@ void filter_rows_launch(const Filter &f, float *rows, size_t n_rows, const float
     A.rows_per_weight = rows_per_weight;
     A.weights2 = weights2;
     A.w2_rows = w2_rows;
+    // One row pair's transform lives in LDS: n_fft <= 8192 fits the default
+    // 64 KiB; larger transforms (up to the CU's LDS, 160 KiB on gfx950)
+    // raise the kernel's dynamic LDS limit first.
     const size_t lds = (size_t)f.n_fft * sizeof(float2);
-    AMX_REQUIRE(lds <= 65536, "filter: n_fft %u too large for one workgroup", f.n_fft);
+    int dev = 0, max_lds = 0;
+    AMX_HIP(hipGetDevice(&dev));
+    AMX_HIP(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
+    AMX_REQUIRE(lds <= (size_t)max_lds, "filter: n_fft %u needs %zu bytes of LDS, the device has %d",
+                f.n_fft, lds, max_lds);
+    if (lds > 65536) {
+        if (f.general)
+            AMX_HIP(hipFuncSetAttribute((const void *)filter_rows_kernel<true>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        else
+            AMX_HIP(hipFuncSetAttribute((const void *)filter_rows_kernel<false>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    }
     const size_t groups = (n_rows + 1) / 2;
     AMX_REQUIRE(groups < (1ull << 31), "filter: too many rows");
     ProfToken pt = prof_begin("filter_rows", (double)n_rows, s);

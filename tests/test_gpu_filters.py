@@ -206,3 +206,28 @@ def test_fan_fbp_short_scan():
     astra.algorithm.run(astra.algorithm.create(cfg))
     ref = OF.fbp_fan(sino, ovec, (n, n), osc, 1.0, short_scan=True)
     assert rel_l2(astra.data2d.get(rid), ref) <= 1e-5
+
+
+def test_fbp_wide_detector_large_fft():
+    """5000 detector bins: the row transform has 16384 points (128 KiB of
+    LDS per workgroup, above the default 64 KiB).  Padded detector
+    coordinates reach 5000, where an fp32 step is 4.9e-4 texel, so the BP
+    of the filtered (high-pass) data is held to 5e-5 (the 1e-5 cases use
+    detectors of a few hundred bins)."""
+    import astra
+    from oracle import fdk as OF, geometry as OG, proj2d as O2
+    n, na, nd = 64, 24, 5000
+    angles = np.linspace(0, np.pi, na, endpoint=False)
+    vg = astra.create_vol_geom(n, n)
+    pg = astra.create_proj_geom("parallel", 0.02, nd, angles)
+    ovec, osc = OG.normalise_2d(OG.gen_par_projections(angles, nd, 0.02), "par", vg)
+    from astra import phantom
+    sino = O2.par_fp(phantom.ellipses_2d(n, seed=38), ovec, nd, osc).astype(np.float32)
+    pid = astra.create_projector("cuda", pg, vg)
+    sid = astra.data2d.create("-sino", pg, sino)
+    rid = astra.data2d.create("-vol", vg, 0)
+    cfg = astra.astra_dict("FBP_CUDA")
+    cfg.update(ProjectorId=pid, ProjectionDataId=sid, ReconstructionDataId=rid)
+    astra.algorithm.run(astra.algorithm.create(cfg))
+    ref = OF.fbp_par(sino, ovec, (n, n), osc, 1.0)
+    assert rel_l2(astra.data2d.get(rid), ref) <= 5e-5
