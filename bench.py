@@ -63,6 +63,49 @@ def cpu_baseline(iters):
     }
 
 
+def _cpu_sirt_worker(job):
+    """One independent CPU SIRT run (a process of the all-cores baseline) on
+    the phantom image `img`; returns its seconds.  Imports numpy and the
+    CPU library only."""
+    img, iters = job
+    from oracle import cpu
+
+    lib = cpu.reference() or cpu.port()
+    n, na, nd = 256, 180, 384
+    ang = np.linspace(0, np.pi, na, endpoint=False)
+    sino = lib.fp(img, ang, nd)
+    lib.sirt(sino, ang, (n, n), 1)
+    t0 = time.perf_counter()
+    lib.sirt(sino, ang, (n, n), iters)
+    return time.perf_counter() - t0
+
+
+def cpu_all_cores(iters, workers):
+    """Throughput of `workers` concurrent independent CPU SIRT runs (the
+    reference CPU path is single-threaded, so the host's cores are used as
+    independent jobs).  Runs before the GPU is touched."""
+    import multiprocessing as mp
+
+    from astra import phantom
+
+    img = phantom.ellipses_2d(256, seed=0)
+    ctx = mp.get_context("spawn")
+    t0 = time.perf_counter()
+    with ctx.Pool(workers) as pool:
+        secs = pool.map(_cpu_sirt_worker, [(img, iters)] * workers)
+    wall = time.perf_counter() - t0
+    n, na, nd = 256, 180, 384
+    updates = workers * iters * 2.0 * na * nd * n
+    slowest = max(secs)
+    from oracle import cpu
+
+    kind = "reference" if cpu.reference() is not None else "port"
+    return {"value": updates / slowest / 1e9, "unit": "Gupdates/s", "cores": workers, "kind": kind,
+            "sample": "%d concurrent CPU SIRT runs (256x256, 180 x 384, %d iterations each); "
+                      "slowest run %.2f s, pool wall time incl. start-up %.2f s"
+                      % (workers, iters, slowest, wall)}
+
+
 def copy_calibration(dev, nbytes=1 << 30):
     """One plain device-to-device copy of `nbytes` (a kernel of known HBM
     traffic: nbytes read + nbytes written) used to scale the profiler's
@@ -245,6 +288,52 @@ def fan2d_bench(sirt_iters=200):
     return res
 
 
+def sirt3d_bench(n=512, na=720, nd=768, iters=50):
+    """SIRT3D_CUDA, 50 iterations on the 512^3 / 720 x 768^2 cone workload
+    (projections = FP of the 20-ellipsoid phantom), data linked on the GPU,
+    after a 2-iteration warm-up on the same algorithm object."""
+    import torch
+
+    import astra
+    from astra import _core, _lib, phantom
+
+    dev = torch.device("cuda", 0)
+    angles = np.linspace(0, 2 * np.pi, na, endpoint=False)
+    vg = astra.create_vol_geom(n, n, n)
+    pg = astra.create_proj_geom("cone", 1.0, 1.0, nd, nd, angles, 1000.0, 500.0)
+    pid = astra.create_projector("cuda3d", pg, vg)
+    P = astra.projector3d.get(pid)
+    truth = phantom.ellipsoids_3d_device(n, count=20, seed=0, device=dev)
+    proj = torch.empty((nd, na, nd), dtype=torch.float32, device=dev)
+    _core.fp(P, truth, proj, _lib.MODE_SET)
+    del truth
+    rec = torch.zeros((n, n, n), dtype=torch.float32, device=dev)
+    sid = astra.data3d.link("-sino", pg, proj)
+    rid = astra.data3d.link("-vol", vg, rec)
+    cfg = astra.astra_dict("SIRT3D_CUDA")
+    cfg.update(ProjectorId=pid, ProjectionDataId=sid, ReconstructionDataId=rid)
+    aid = astra.algorithm.create(cfg)
+    astra.algorithm.run(aid, 2)  # warm-up: weights, copies, scratch pool
+    rec.zero_()
+    torch.cuda.synchronize(dev)
+    t = time.perf_counter()
+    astra.algorithm.run(aid, iters)
+    torch.cuda.synchronize(dev)
+    dt = time.perf_counter() - t
+    res_norm = astra.algorithm.get_res_norm(aid)
+    astra.algorithm.delete(aid)
+    astra.data3d.delete([sid, rid])
+    astra.projector3d.delete(pid)
+    del proj, rec
+    torch.cuda.empty_cache()
+    ups = iters * (float(na) * nd * nd * n + float(n) ** 3 * na) / dt / 1e9
+    log("bench: SIRT3D %d^3, %d iterations in %.3f s (%.1f ms/iteration, %.0f Gupdates/s)"
+        % (n, iters, dt, 1e3 * dt / iters, ups))
+    return {"workload": "cone %d^3, %d angles, %dx%d detector, SOD 1000, ODD 500" % (n, na, nd, nd),
+            "iterations": iters, "seconds": dt, "ms_per_iteration": 1e3 * dt / iters,
+            "Gupdates_per_s": ups, "residual_norm": res_norm}
+
+
 def large_cone_bench(n=1024, na=1440, nd=1536, steps=2):
     """Cone FP+BP and FDK at 1024^3 / 1440 x 1536^2, unchunked on one GPU
     (volume, projections and the internal copies resident in HBM)."""
@@ -335,6 +424,10 @@ def main():
     ap.add_argument("--no-diff", action="store_true")
     ap.add_argument("--no-fan2d", action="store_true")
     ap.add_argument("--no-large", action="store_true")
+    ap.add_argument("--no-sirt3d", action="store_true")
+    ap.add_argument("--sirt3d-iters", type=int, default=50)
+    ap.add_argument("--cpu-workers", type=int, default=0,
+                    help="concurrent CPU SIRT runs for the all-cores figure (0: min(16, cores))")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 1 GiB device copy first (profiler counter scaling)")
     args = ap.parse_args()
@@ -342,6 +435,12 @@ def main():
         print("bench.py runs on one GPU only: the projection path is measured on a single "
               "MI355X (multi-GPU sharding is design-only); use --gpus 1", file=sys.stderr)
         sys.exit(2)
+
+    cpu_all = None
+    if not args.no_cpu:
+        workers = args.cpu_workers or min(16, os.cpu_count() or 1)
+        log("bench: CPU baseline, %d concurrent runs" % workers)
+        cpu_all = cpu_all_cores(max(args.cpu_iters // 5, 1), workers)
 
     import torch
 
@@ -431,12 +530,14 @@ def main():
     if not args.no_diff:
         diff_res = differentiable_bench()
     fan_res = None if args.no_fan2d else fan2d_bench()
+    sirt3d_res = None if args.no_sirt3d else sirt3d_bench(n, na, nd, args.sirt3d_iters)
     large_res = None if args.no_large else large_cone_bench()
 
     cpu = None
     if not args.no_cpu:
         log("bench: CPU baseline (%d SIRT iterations)" % args.cpu_iters)
         cpu = cpu_baseline(args.cpu_iters)
+        cpu["all_cores"] = cpu_all
 
     result = {
         "metric": "cone FP+BP Gupdates/s & FDK sec at 512^3; batched 2D SIRT slices/s vs plain",
@@ -465,6 +566,7 @@ def main():
         "batched_sirt": batched_res,
         "differentiable": diff_res,
         "fan2d_1024": fan_res,
+        "sirt3d_512": sirt3d_res,
         "cone_1024": large_res,
         "kernels": kernels,
         "cpu_baseline": cpu,
