@@ -7,6 +7,7 @@ staged through the projector's GPU.  Calls are synchronous, like the
 reference's CompositeGeometryManager jobs.
 """
 from . import _core, _lib, composite
+from . import geometry as G
 from ._registry import data3d as _d3
 from ._registry import projectors2d as _p2, projectors3d as _p3
 from .log import AstraError
@@ -102,6 +103,34 @@ def do_composite_FP(projector_id, vol_ids, proj_ids):
 
 def do_composite_BP(projector_id, vol_ids, proj_ids):
     do_composite(projector_id, vol_ids, proj_ids, MODE_SET, "BP")
+
+
+def accumulate_FP(projector_id, vol_id, proj_id):
+    """proj += FP(vol) between two 3D data objects (MODE_ADD composite job)."""
+    do_composite(projector_id, [vol_id], [proj_id], MODE_ADD, "FP")
+
+
+def accumulate_BP(projector_id, vol_id, proj_id):
+    """vol += BP(proj) between two 3D data objects (MODE_ADD composite job)."""
+    do_composite(projector_id, [vol_id], [proj_id], MODE_ADD, "BP")
+
+
+def getProjectedBBox(geometry, minx, maxx, miny, maxy, minz, maxz):
+    """(vmin, vmax): the detector rows that the box [minx, maxx] x [miny,
+    maxy] x [minz, maxz] projects onto, over all projections of a 3D
+    projection geometry (row index coordinates, 0 = centre of the first
+    row)."""
+    _, _, vmin, vmax = G.projected_bbox_3d(geometry, minx, maxx, miny, maxy, minz, maxz)
+    return (vmin, vmax)
+
+
+def projectPoint(geometry, x, y, z, angle):
+    """(u, v) detector index coordinates of world point (x, y, z) in
+    projection number `angle` of a 3D projection geometry."""
+    n = G.proj_shape(geometry)[1]
+    if not 0 <= int(angle) < n:
+        raise AstraError("projection index out of range")
+    return G.project_point_3d(geometry, x, y, z, int(angle))
 
 
 def accumulate_FDK(projector_id, vol_id, proj_id):

@@ -221,3 +221,133 @@ def geom_2vec(pg):
     if t.endswith("_vec"):
         return dict(pg)
     raise AstraError("geom_2vec: unsupported geometry type '%s'" % t)
+
+
+# ------------------------------------------------ point projection -------
+def _det_index(offset, spacing, n):
+    """detectorOffset{X,Y}To{Col,Row}IndexFloat
+    (include/astra/ProjectionGeometry3D.h:442-458): float32 arithmetic."""
+    return float(f32(f32(offset) / f32(spacing)) + f32((f32(n) - f32(1.0)) / f32(2.0)))
+
+
+def _cyl_frame(vec, flip):
+    """Cylinder centre, radial (a), tangential (b) and axis vectors of one
+    cyl_cone_vec row; `flip` orients a away from the source as
+    getCylConeAxes does (src/GeometryUtil3D.cpp:561-583)."""
+    s, d, u, v, r = vec[0:3], vec[3:6], vec[6:9], vec[9:12], vec[12]
+    a = -np.cross(u, v) * (r / (np.linalg.norm(u) * np.linalg.norm(v)))
+    if flip and np.linalg.norm(d - a - s) > np.linalg.norm(d + a - s):
+        a = -a
+    return d - a, a, u * (r / np.linalg.norm(u)), v
+
+
+def project_point_3d(pg, x, y, z, i):
+    """(u, v) detector index coordinates (0 = centre of the first column /
+    row) of world point (x, y, z) in projection i: projectPoint of the 3D
+    geometry classes (src/ParallelProjectionGeometry3D.cpp:153-167,
+    src/ConeProjectionGeometry3D.cpp:176-200,
+    src/ParallelVecProjectionGeometry3D.cpp:201-215,
+    src/ConeVecProjectionGeometry3D.cpp:199-214,
+    src/CylConeVecProjectionGeometry3D.cpp:343-448).  A cylindrical
+    detector that the ray misses gives u = -inf / +inf (the side it passes)
+    and v = NaN."""
+    t = pg["type"]
+    p = np.array([x, y, z], dtype=np.float64)
+    nu, nv = int(pg["DetectorColCount"]), int(pg["DetectorRowCount"])
+    if t in ("parallel3d", "cone"):
+        a = float(_angles(pg)[i])
+        su, sv = pg["DetectorSpacingX"], pg["DetectorSpacingY"]
+        lateral = np.cos(a) * x + np.sin(a) * y
+        if t == "parallel3d":
+            return _det_index(lateral, su, nu), _det_index(z, sv, nv)
+        sod = float(f32(pg["DistanceOriginSource"]))
+        odd = float(f32(pg["DistanceOriginDetector"]))
+        depth = -np.sin(a) * x + np.cos(a) * y + sod
+        mag = (sod + odd) / depth
+        return _det_index(lateral * mag, su, nu), _det_index(z * mag, sv, nv)
+    if t in ("parallel3d_vec", "cone_vec"):
+        _, vec = vectors_3d(pg)
+        src, corner, u, v = vec[i, 0:3], vec[i, 3:6], vec[i, 6:9], vec[i, 9:12]
+        if t == "parallel3d_vec":
+            # the point moves along the ray onto corner + a u + b v
+            den = float(np.dot(src, np.cross(u, v)))
+            rel = p - corner
+            return (float(np.dot(np.cross(v, src), rel)) / den - 0.5,
+                    float(np.dot(np.cross(src, u), rel)) / den - 0.5)
+        w = corner - src
+        den = float(np.dot(np.cross(u, v), p - src))
+        return ((float(np.dot(np.cross(v, w), p)) + float(np.dot(src, np.cross(corner, v)))) / den - 0.5,
+                (float(np.dot(np.cross(w, u), p)) - float(np.dot(src, np.cross(corner, u)))) / den - 0.5)
+    if t == "cyl_cone_vec":
+        _, vec = vectors_3d(pg)
+        row = vec[i]
+        s, d, u, v, r = row[0:3], row[3:6], row[6:9], row[9:12], row[12]
+        c, a, b, axis = _cyl_frame(row, flip=False)
+        an, bn, cn = a / np.linalg.norm(a), b / np.linalg.norm(b), axis / np.linalg.norm(axis)
+        ray = p - s
+        ra, rb, rc = np.dot(ray, an), np.dot(ray, bn), np.dot(ray, cn)
+        sa, sb, sc = np.dot(s - c, an), np.dot(s - c, bn), np.dot(s - c, cn)
+        # |(t ra + sa, t rb + sb)| = r, the far root
+        qa, qb, qc = ra * ra + rb * rb, 2.0 * (ra * sa + rb * sb), sa * sa + sb * sb - r * r
+        disc = qb * qb - 4.0 * qa * qc
+        if disc >= 0:
+            tt = (-qb + np.sqrt(disc)) / (2.0 * qa)
+            theta = np.arcsin((tt * rb + sb) / r)
+            dth = np.linalg.norm(u) / r
+            return (float(theta / dth + 0.5 * nu - 0.5),
+                    float((tt * rc + sc) / np.linalg.norm(v) + 0.5 * nv - 0.5))
+        side = np.linalg.det(np.array([u, d - s, v])) * np.linalg.det(np.array([ray, d - s, v]))
+        return (float("-inf") if side < 0 else float("inf")), float("nan")
+    raise AstraError("projectPoint: unsupported geometry type '%s'" % t)
+
+
+def _bbox_cyl_single(vec, nu, nv, corners, corner_us):
+    """getProjectedBBoxSingleAngle of the cylindrical detector
+    (src/CylConeVecProjectionGeometry3D.cpp:201-340): u from the projected
+    corners and column index 0; v from the corners projected onto a near and
+    a far plane normal to the ray through column 0's centre."""
+    s = vec[0:3]
+    c, a, b, axis = _cyl_frame(vec, flip=True)
+    axis_n = axis / np.linalg.norm(axis)
+    dth = np.linalg.norm(vec[6:9]) / vec[12]
+    us = [0.0] + list(corner_us)
+    ray = None
+    near = far = 0.0
+    for uu in us:
+        theta = min(max((uu + 0.5 - 0.5 * nu) * dth, -np.pi / 2), np.pi / 2)
+        pt = c + a * np.cos(theta) + b * np.sin(theta)
+        if ray is None:
+            ray = (pt - c) / np.linalg.norm(pt - c)
+            near = far = float(np.dot(pt - s, ray))
+        else:
+            near = min(near, float(np.dot(pt - s, ray)))
+    vmin, vmax = 1e20, -1e20
+    for x in corners:
+        along = float(np.dot(x - s, ray))
+        lat = float(np.dot(x - s, axis_n))
+        for plane in (near, far):
+            vv = lat * plane / along + 0.5 * nv - 0.5
+            vmin, vmax = min(vmin, vv), max(vmax, vv)
+    return min(us), max(us), vmin, vmax
+
+
+def projected_bbox_3d(pg, xmin, xmax, ymin, ymax, zmin, zmax):
+    """(umin, umax, vmin, vmax) over all projections of the box's corners
+    (CProjectionGeometry3D::getProjectedBBox,
+    src/ProjectionGeometry3D.cpp:162-236)."""
+    corners = [np.array([cx, cy, cz], dtype=np.float64)
+               for cx in (xmin, xmax) for cy in (ymin, ymax) for cz in (zmin, zmax)]
+    nu, nv = int(pg["DetectorColCount"]), int(pg["DetectorRowCount"])
+    na = proj_shape(pg)[1]
+    vecs = vectors_3d(pg)[1] if pg["type"] == "cyl_cone_vec" else None
+    g = None
+    for i in range(na):
+        if vecs is not None:
+            b = _bbox_cyl_single(vecs[i], nu, nv, corners,
+                                 [project_point_3d(pg, *p, i)[0] for p in corners])
+        else:
+            uv = [project_point_3d(pg, *p, i) for p in corners]
+            us, vs = [q[0] for q in uv], [q[1] for q in uv]
+            b = (min(us), max(us), min(vs), max(vs))
+        g = b if g is None else (min(g[0], b[0]), max(g[1], b[1]), min(g[2], b[2]), max(g[3], b[3]))
+    return g
