@@ -16,7 +16,11 @@ output part, which is then written back.
 The split is exact: FP of a volume is the sum of the FPs of its z slabs
 (each slab samples the same world points and reads zeros outside its own
 voxels, and linear interpolation is linear in the data), and the BP of a
-voxel sums independent per-angle terms.  Data already on the GPU is used
+voxel sums independent per-angle terms.  As in the reference, the input of
+each part is first reduced to what can reach the output part: a BP slab
+uploads only the detector rows its window (grown by half a voxel, plus a
+row of margin) projects onto, and an FP skips the slices that project off
+the detector (:382-410, 526-625, 854-866).  Data already on the GPU is used
 in place and never split.  FDK filters the whole projection set once on
 the GPU and back projects it slab by slab, so the projections (and their
 padded copy) must fit.
@@ -137,6 +141,80 @@ def proj_angle_geom(pg, a0, a1):
     return v
 
 
+def proj_rows_geom(pg, r0, r1):
+    """Projection geometry (vector form) of detector rows [r0, r1): the same
+    pixels, with the detector centre moved along v."""
+    v = dict(G.geom_2vec(pg))
+    vec = np.array(v["Vectors"], dtype=np.float64)
+    nv = int(v["DetectorRowCount"])
+    vec[:, 3:6] += (r0 - 0.5 * nv + 0.5 * (r1 - r0)) * vec[:, 9:12]
+    v["Vectors"] = vec
+    v["DetectorRowCount"] = r1 - r0
+    return v
+
+
+# ------------------------------------------------- vertical reduction ----
+def vertical_range(vg, pg):
+    """Detector rows (fractional, clipped to [-1, rows]) that the volume
+    window grown by half a voxel projects onto over all projections; (0, 0)
+    when it misses the detector (reduceProjectionVertical,
+    src/CompositeGeometryManager.cpp:382-410)."""
+    px, py, pz = (float(x) for x in G.pixel_lengths(vg))
+    box = []
+    for ax, n, h in (("X", vg["GridColCount"], px), ("Y", vg["GridRowCount"], py),
+                     ("Z", vg["GridSliceCount"], pz)):
+        lo, hi = G._window(vg, ax, n)
+        box += [float(lo) - 0.5 * h, float(hi) + 0.5 * h]
+    _, _, vmin, vmax = G.projected_bbox_3d(pg, *box)
+    vmin = max(vmin, -1.0)
+    vmax = min(vmax, float(pg["DetectorRowCount"]))
+    return (0.0, 0.0) if vmin >= vmax else (vmin, vmax)
+
+
+def rows_needed(vg, pg):
+    """[r0, r1): the projection rows a volume part reads in a BP, one row of
+    margin each side (reduceProjectionPart, :854-866); (0, 0) when none."""
+    vmin, vmax = vertical_range(vg, pg)
+    if vmin == vmax:  # off the detector (the reference keeps row 0 here; it adds nothing)
+        return (0, 0)
+    nv = int(pg["DetectorRowCount"])
+    r0 = max(int(np.floor(vmin - 1.0)), 0)
+    r1 = min(int(np.ceil(vmax + 1.0)), nv)
+    return (0, 0) if r0 >= r1 else (r0, r1)
+
+
+def slices_needed(vg, pg):
+    """[z0, z1): the volume slices whose projection can reach the detector
+    rows, one slice of margin each side (reduceVolumePart, :526-625, with
+    its binary searches for the first and last slice); (0, 0) when the
+    whole volume projects off the detector."""
+    nz = int(vg["GridSliceCount"])
+    full = vertical_range(vg, pg)
+    if full[0] >= full[1]:
+        return (0, 0)
+
+    def off(z0, z1):  # slices [z0, z1) reach no row of `full`
+        lo, hi = vertical_range(vol_slab_geom(vg, z0, z1), pg)
+        return lo == hi or lo >= full[1] or hi <= full[0]
+
+    lo, hi = 0, nz - 1
+    while lo < hi:
+        mid = (lo + hi + 1) // 2
+        if off(0, mid):
+            lo = mid
+        else:
+            hi = mid - 1
+    top = lo
+    lo, hi = top + 1, nz
+    while lo < hi:
+        mid = (lo + hi) // 2
+        if off(mid, nz):
+            hi = mid
+        else:
+            lo = mid + 1
+    return max(top - 1, 0), min(hi + 1, nz)
+
+
 def _sub_projector(p, pg, vg):
     return _core.Projector("cuda3d", pg, vg, p.options)
 
@@ -164,6 +242,10 @@ def fp(p, dvol, dproj, mode):
     nv, na, nu = p.proj_shape
     for (a0, a1), slabs in plan:
         pg = proj_angle_geom(p.proj_geom, a0, a1)
+        # only the slices that can reach this part's rows (the rest add 0)
+        z0r, z1r = slices_needed(p.vol_geom, pg)
+        slabs = [(z0, z1) for z0, z1 in _chunks(z1r - z0r, slabs[0][1] - slabs[0][0])]
+        slabs = [(z0 + z0r, z1 + z0r) for z0, z1 in slabs]
         if proj_host is not None:
             s = (torch.from_numpy(np.ascontiguousarray(proj_host[:, a0:a1, :])).to(dev)
                  if mode == MODE_ADD else torch.zeros((nv, a1 - a0, nu), device=dev))
@@ -202,6 +284,8 @@ def bp(p, dproj, dvol, mode, kernel=None):
         _core.sync(dev)
         return
     nz, ny, nx = p.vol_shape
+    nv = p.proj_shape[0]
+    matched = (p.kernel if kernel is None else kernel) == _lib.KER3D_MATCHED_BP
     for (z0, z1), chunks in plan:
         vg = vol_slab_geom(p.vol_geom, z0, z1)
         if vol_host is not None:
@@ -211,11 +295,19 @@ def bp(p, dproj, dvol, mode, kernel=None):
             v = dvol.linked[z0:z1].clone() if mode == MODE_ADD else \
                 torch.zeros((z1 - z0, ny, nx), device=dev)
         for a0, a1 in chunks:
-            sp = _sub_projector(p, proj_angle_geom(p.proj_geom, a0, a1), vg)
-            if proj_host is not None:
-                s = torch.from_numpy(np.ascontiguousarray(proj_host[:, a0:a1, :])).to(dev)
+            pg = proj_angle_geom(p.proj_geom, a0, a1)
+            if matched:  # the transpose of the FP keeps the FP's full detector
+                r0, r1 = 0, nv
+                sp = _sub_projector(p, pg, vg)
             else:
-                s = dproj.linked[:, a0:a1, :].contiguous()
+                r0, r1 = rows_needed(vg, pg)  # rows outside add nothing to this slab
+                if r0 == r1:
+                    continue
+                sp = _sub_projector(p, proj_rows_geom(pg, r0, r1), vg)
+            if proj_host is not None:
+                s = torch.from_numpy(np.ascontiguousarray(proj_host[r0:r1, a0:a1, :])).to(dev)
+            else:
+                s = dproj.linked[r0:r1, a0:a1, :].contiguous()
             _core.bp(sp, s, v, MODE_ADD, kernel=kernel)
             sp.release()
         if vol_host is not None:

@@ -227,127 +227,132 @@ def geom_2vec(pg):
 def _det_index(offset, spacing, n):
     """detectorOffset{X,Y}To{Col,Row}IndexFloat
     (include/astra/ProjectionGeometry3D.h:442-458): float32 arithmetic."""
-    return float(f32(f32(offset) / f32(spacing)) + f32((f32(n) - f32(1.0)) / f32(2.0)))
+    return (np.asarray(offset, dtype=f32) / f32(spacing) + f32((f32(n) - f32(1.0)) / f32(2.0))).astype(np.float64)
 
 
-def _cyl_frame(vec, flip):
-    """Cylinder centre, radial (a), tangential (b) and axis vectors of one
-    cyl_cone_vec row; `flip` orients a away from the source as
+def _cyl_frame(vecs, flip):
+    """Cylinder centre c, radial a, tangential b and axis vectors of
+    cyl_cone_vec rows [N, 13]; `flip` orients a away from the source as
     getCylConeAxes does (src/GeometryUtil3D.cpp:561-583)."""
-    s, d, u, v, r = vec[0:3], vec[3:6], vec[6:9], vec[9:12], vec[12]
-    a = -np.cross(u, v) * (r / (np.linalg.norm(u) * np.linalg.norm(v)))
-    if flip and np.linalg.norm(d - a - s) > np.linalg.norm(d + a - s):
-        a = -a
-    return d - a, a, u * (r / np.linalg.norm(u)), v
+    s, d, u, v, r = vecs[:, 0:3], vecs[:, 3:6], vecs[:, 6:9], vecs[:, 9:12], vecs[:, 12:13]
+    a = -np.cross(u, v) * (r / (np.linalg.norm(u, axis=1, keepdims=True) *
+                                np.linalg.norm(v, axis=1, keepdims=True)))
+    if flip:
+        swap = np.linalg.norm(d - a - s, axis=1) > np.linalg.norm(d + a - s, axis=1)
+        a = np.where(swap[:, None], -a, a)
+    return d - a, a, u * (r / np.linalg.norm(u, axis=1, keepdims=True)), v
 
 
-def project_point_3d(pg, x, y, z, i):
-    """(u, v) detector index coordinates (0 = centre of the first column /
-    row) of world point (x, y, z) in projection i: projectPoint of the 3D
-    geometry classes (src/ParallelProjectionGeometry3D.cpp:153-167,
+def _dot(a, b):
+    return np.einsum("...k,...k->...", a, b)
+
+
+def project_points_3d(pg, pts):
+    """(u, v), each [projections, points]: detector index coordinates (0 =
+    centre of the first column / row) of world points pts [P, 3] in every
+    projection: projectPoint of the 3D geometry classes
+    (src/ParallelProjectionGeometry3D.cpp:153-167,
     src/ConeProjectionGeometry3D.cpp:176-200,
     src/ParallelVecProjectionGeometry3D.cpp:201-215,
     src/ConeVecProjectionGeometry3D.cpp:199-214,
     src/CylConeVecProjectionGeometry3D.cpp:343-448).  A cylindrical
-    detector that the ray misses gives u = -inf / +inf (the side it passes)
-    and v = NaN."""
+    detector that a ray misses gives u = -inf / +inf (the side the ray
+    passes) and v = NaN."""
     t = pg["type"]
-    p = np.array([x, y, z], dtype=np.float64)
+    p = np.atleast_2d(np.asarray(pts, dtype=np.float64))[None, :, :]  # [1, P, 3]
     nu, nv = int(pg["DetectorColCount"]), int(pg["DetectorRowCount"])
     if t in ("parallel3d", "cone"):
-        a = float(_angles(pg)[i])
-        su, sv = pg["DetectorSpacingX"], pg["DetectorSpacingY"]
+        a = _angles(pg)[:, None]
+        x, y, z = p[..., 0], p[..., 1], p[..., 2]
         lateral = np.cos(a) * x + np.sin(a) * y
+        su, sv = pg["DetectorSpacingX"], pg["DetectorSpacingY"]
         if t == "parallel3d":
-            return _det_index(lateral, su, nu), _det_index(z, sv, nv)
+            return _det_index(lateral, su, nu), _det_index(np.broadcast_to(z, lateral.shape), sv, nv)
         sod = float(f32(pg["DistanceOriginSource"]))
         odd = float(f32(pg["DistanceOriginDetector"]))
-        depth = -np.sin(a) * x + np.cos(a) * y + sod
-        mag = (sod + odd) / depth
+        mag = (sod + odd) / (-np.sin(a) * x + np.cos(a) * y + sod)
         return _det_index(lateral * mag, su, nu), _det_index(z * mag, sv, nv)
+    _, vec = vectors_3d(pg)
     if t in ("parallel3d_vec", "cone_vec"):
-        _, vec = vectors_3d(pg)
-        src, corner, u, v = vec[i, 0:3], vec[i, 3:6], vec[i, 6:9], vec[i, 9:12]
+        src, corner = vec[:, None, 0:3], vec[:, None, 3:6]
+        u, v = vec[:, None, 6:9], vec[:, None, 9:12]
         if t == "parallel3d_vec":
             # the point moves along the ray onto corner + a u + b v
-            den = float(np.dot(src, np.cross(u, v)))
+            den = _dot(src, np.cross(u, v))
             rel = p - corner
-            return (float(np.dot(np.cross(v, src), rel)) / den - 0.5,
-                    float(np.dot(np.cross(src, u), rel)) / den - 0.5)
+            return (_dot(np.cross(v, src), rel) / den - 0.5, _dot(np.cross(src, u), rel) / den - 0.5)
         w = corner - src
-        den = float(np.dot(np.cross(u, v), p - src))
-        return ((float(np.dot(np.cross(v, w), p)) + float(np.dot(src, np.cross(corner, v)))) / den - 0.5,
-                (float(np.dot(np.cross(w, u), p)) - float(np.dot(src, np.cross(corner, u)))) / den - 0.5)
+        den = _dot(np.cross(u, v), p - src)
+        return ((_dot(np.cross(v, w), p) + _dot(src, np.cross(corner, v))) / den - 0.5,
+                (_dot(np.cross(w, u), p) - _dot(src, np.cross(corner, u))) / den - 0.5)
     if t == "cyl_cone_vec":
-        _, vec = vectors_3d(pg)
-        row = vec[i]
-        s, d, u, v, r = row[0:3], row[3:6], row[6:9], row[9:12], row[12]
-        c, a, b, axis = _cyl_frame(row, flip=False)
-        an, bn, cn = a / np.linalg.norm(a), b / np.linalg.norm(b), axis / np.linalg.norm(axis)
+        c, a, b, axis = _cyl_frame(vec, flip=False)
+        s, d, u, v, r = vec[:, None, 0:3], vec[:, None, 3:6], vec[:, None, 6:9], vec[:, None, 9:12], \
+            vec[:, None, 12]
+        an = (a / np.linalg.norm(a, axis=1, keepdims=True))[:, None]
+        bn = (b / np.linalg.norm(b, axis=1, keepdims=True))[:, None]
+        cn = (axis / np.linalg.norm(axis, axis=1, keepdims=True))[:, None]
+        c = c[:, None]
         ray = p - s
-        ra, rb, rc = np.dot(ray, an), np.dot(ray, bn), np.dot(ray, cn)
-        sa, sb, sc = np.dot(s - c, an), np.dot(s - c, bn), np.dot(s - c, cn)
+        ra, rb, rc = _dot(ray, an), _dot(ray, bn), _dot(ray, cn)
+        sa, sb, sc = _dot(s - c, an), _dot(s - c, bn), _dot(s - c, cn)
         # |(t ra + sa, t rb + sb)| = r, the far root
         qa, qb, qc = ra * ra + rb * rb, 2.0 * (ra * sa + rb * sb), sa * sa + sb * sb - r * r
         disc = qb * qb - 4.0 * qa * qc
-        if disc >= 0:
-            tt = (-qb + np.sqrt(disc)) / (2.0 * qa)
+        hit = disc >= 0
+        tt = (-qb + np.sqrt(np.where(hit, disc, 0.0))) / (2.0 * qa)
+        with np.errstate(invalid="ignore"):
             theta = np.arcsin((tt * rb + sb) / r)
-            dth = np.linalg.norm(u) / r
-            return (float(theta / dth + 0.5 * nu - 0.5),
-                    float((tt * rc + sc) / np.linalg.norm(v) + 0.5 * nv - 0.5))
-        side = np.linalg.det(np.array([u, d - s, v])) * np.linalg.det(np.array([ray, d - s, v]))
-        return (float("-inf") if side < 0 else float("inf")), float("nan")
+        dth = np.linalg.norm(u, axis=-1) / r
+        uu = theta / dth + 0.5 * nu - 0.5
+        vv = (tt * rc + sc) / np.linalg.norm(v, axis=-1) + 0.5 * nv - 0.5
+        side = np.linalg.det(np.stack([u, d - s, v], axis=-2))
+        side = side * np.linalg.det(np.stack(np.broadcast_arrays(ray, d - s, v), axis=-2))
+        miss_u = np.where(side < 0, -np.inf, np.inf)
+        return np.where(hit, uu, miss_u), np.where(hit, vv, np.nan)
     raise AstraError("projectPoint: unsupported geometry type '%s'" % t)
 
 
-def _bbox_cyl_single(vec, nu, nv, corners, corner_us):
-    """getProjectedBBoxSingleAngle of the cylindrical detector
-    (src/CylConeVecProjectionGeometry3D.cpp:201-340): u from the projected
-    corners and column index 0; v from the corners projected onto a near and
-    a far plane normal to the ray through column 0's centre."""
-    s = vec[0:3]
-    c, a, b, axis = _cyl_frame(vec, flip=True)
-    axis_n = axis / np.linalg.norm(axis)
-    dth = np.linalg.norm(vec[6:9]) / vec[12]
-    us = [0.0] + list(corner_us)
-    ray = None
-    near = far = 0.0
-    for uu in us:
-        theta = min(max((uu + 0.5 - 0.5 * nu) * dth, -np.pi / 2), np.pi / 2)
-        pt = c + a * np.cos(theta) + b * np.sin(theta)
-        if ray is None:
-            ray = (pt - c) / np.linalg.norm(pt - c)
-            near = far = float(np.dot(pt - s, ray))
-        else:
-            near = min(near, float(np.dot(pt - s, ray)))
-    vmin, vmax = 1e20, -1e20
-    for x in corners:
-        along = float(np.dot(x - s, ray))
-        lat = float(np.dot(x - s, axis_n))
-        for plane in (near, far):
-            vv = lat * plane / along + 0.5 * nv - 0.5
-            vmin, vmax = min(vmin, vv), max(vmax, vv)
-    return min(us), max(us), vmin, vmax
+def project_point_3d(pg, x, y, z, i):
+    """(u, v) of one world point in projection i (see project_points_3d)."""
+    u, v = project_points_3d(pg, [[x, y, z]])
+    return float(u[i, 0]), float(v[i, 0])
+
+
+def _bbox_cyl(vecs, nu, nv, corners, corner_us):
+    """getProjectedBBoxSingleAngle of the cylindrical detector for every
+    projection (src/CylConeVecProjectionGeometry3D.cpp:201-340): u from the
+    projected corners and column index 0; v from the corners projected onto
+    a near and a far plane normal to the ray through column 0's centre."""
+    s = vecs[:, 0:3]
+    c, a, b, axis = _cyl_frame(vecs, flip=True)
+    axis_n = axis / np.linalg.norm(axis, axis=1, keepdims=True)
+    dth = np.linalg.norm(vecs[:, 6:9], axis=1) / vecs[:, 12]
+    us = np.concatenate([np.zeros((len(vecs), 1)), corner_us], axis=1)      # [N, 9]
+    theta = np.clip((us + 0.5 - 0.5 * nu) * dth[:, None], -np.pi / 2, np.pi / 2)
+    pts = c[:, None] + a[:, None] * np.cos(theta)[..., None] + b[:, None] * np.sin(theta)[..., None]
+    ray = pts[:, 0] - c
+    ray = ray / np.linalg.norm(ray, axis=1, keepdims=True)
+    depth = _dot(pts - s[:, None], ray[:, None])                           # [N, 9]
+    far = depth[:, 0]
+    near = depth.min(axis=1)
+    rel = corners[None] - s[:, None]                                       # [N, 8, 3]
+    along = _dot(rel, ray[:, None])
+    lat = _dot(rel, axis_n[:, None])
+    vs = np.concatenate([lat * near[:, None] / along, lat * far[:, None] / along], axis=1) + 0.5 * nv - 0.5
+    return us.min(axis=1), us.max(axis=1), vs.min(axis=1), vs.max(axis=1)
 
 
 def projected_bbox_3d(pg, xmin, xmax, ymin, ymax, zmin, zmax):
     """(umin, umax, vmin, vmax) over all projections of the box's corners
     (CProjectionGeometry3D::getProjectedBBox,
     src/ProjectionGeometry3D.cpp:162-236)."""
-    corners = [np.array([cx, cy, cz], dtype=np.float64)
-               for cx in (xmin, xmax) for cy in (ymin, ymax) for cz in (zmin, zmax)]
+    corners = np.array([[cx, cy, cz] for cx in (xmin, xmax) for cy in (ymin, ymax) for cz in (zmin, zmax)],
+                       dtype=np.float64)
     nu, nv = int(pg["DetectorColCount"]), int(pg["DetectorRowCount"])
-    na = proj_shape(pg)[1]
-    vecs = vectors_3d(pg)[1] if pg["type"] == "cyl_cone_vec" else None
-    g = None
-    for i in range(na):
-        if vecs is not None:
-            b = _bbox_cyl_single(vecs[i], nu, nv, corners,
-                                 [project_point_3d(pg, *p, i)[0] for p in corners])
-        else:
-            uv = [project_point_3d(pg, *p, i) for p in corners]
-            us, vs = [q[0] for q in uv], [q[1] for q in uv]
-            b = (min(us), max(us), min(vs), max(vs))
-        g = b if g is None else (min(g[0], b[0]), max(g[1], b[1]), min(g[2], b[2]), max(g[3], b[3]))
-    return g
+    u, v = project_points_3d(pg, corners)
+    if pg["type"] == "cyl_cone_vec":
+        umin, umax, vmin, vmax = _bbox_cyl(vectors_3d(pg)[1], nu, nv, corners, u)
+    else:
+        umin, umax, vmin, vmax = u.min(axis=1), u.max(axis=1), v.min(axis=1), v.max(axis=1)
+    return float(umin.min()), float(umax.max()), float(vmin.min()), float(vmax.max())

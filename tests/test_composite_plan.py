@@ -63,3 +63,35 @@ def test_set_gpu_index_memory_floor():
     with pytest.raises(astra.AstraError):
         astra.set_gpu_index(0, memory=1000)
     astra.set_gpu_index(0, memory=0)
+
+
+def test_vertical_reduction_closed_forms():
+    """parallel3d with unit pixels and voxels: detector row v sees height
+    z = v - (rows - 1) / 2, so the rows a slab reads and the slices that
+    reach the detector follow from the window, the half-voxel growth and
+    the one-row / one-slice margins (src/CompositeGeometryManager.cpp:
+    382-410, 526-625, 854-866)."""
+    pg = astra.create_proj_geom("parallel3d", 1.0, 1.0, 40, 64, np.linspace(0, np.pi, 30, endpoint=False))
+    vg = astra.create_vol_geom(32, 32, 100)                    # z window [-50, 50]
+    # slab [50, 60): z in [0, 10], grown to [-0.5, 10.5] -> v in [19, 30]
+    assert C.rows_needed(C.vol_slab_geom(vg, 50, 60), pg) == (18, 31)
+    assert C.rows_needed(C.vol_slab_geom(vg, 0, 10), pg) == (0, 0)     # z <= -40: off the detector
+    assert C.rows_needed(vg, pg) == (0, 40)
+    # rows [0, 40) see z in [-19.5, 19.5] (+-1 row: [-20.5, 20.5]); slice k
+    # (centre k - 49.5) reaches it from k = 29 to 70, plus one slice each side
+    z0, z1 = C.slices_needed(vg, pg)
+    assert (z0, z1) == (28, 72)
+    assert C.slices_needed(astra.create_vol_geom(32, 32, 20), pg) == (0, 20)
+
+
+def test_row_subgeometry_keeps_pixels():
+    """Rows [r0, r1) of a detector are the same pixels: a point projects to
+    row v in the full geometry and to v - r0 in the sub-geometry."""
+    from astra import geometry as G
+    pg = astra.create_proj_geom("cone", 1.0, 1.3, 30, 40, np.linspace(0, 2 * np.pi, 7), 150.0, 60.0)
+    sub = C.proj_rows_geom(pg, 6, 19)
+    assert sub["DetectorRowCount"] == 13
+    pts = np.random.default_rng(0).uniform(-10, 10, (20, 3))
+    u, v = G.project_points_3d(pg, pts)
+    su, sv = G.project_points_3d(sub, pts)
+    assert np.allclose(su, u, atol=1e-5) and np.allclose(sv, v - 6, atol=1e-5)

@@ -18,7 +18,8 @@ def small_memory():
     orig = C._sub_projector
 
     def spy(p, pg, vg):
-        calls.append((pg["Vectors"].shape[0] if "Vectors" in pg else None, vg["GridSliceCount"]))
+        calls.append((pg["Vectors"].shape[0] if "Vectors" in pg else None, vg["GridSliceCount"],
+                      pg["DetectorRowCount"]))
         return orig(p, pg, vg)
 
     C._sub_projector = spy
@@ -60,6 +61,45 @@ def test_split_fp_bp_equal_unsplit(small_memory, kind):
     n_fp = len(calls)
     _, gotb = astra.create_backprojection3d_gpu(ref, pg, vg)
     assert n_fp >= 2 and len(calls) >= n_fp + 2  # both jobs were split
+    assert rel_l2(got, ref) <= 1e-6
+    assert rel_l2(gotb, refb) <= 1e-6
+
+
+@pytest.mark.parametrize("kind", ["cone", "parallel3d", "cylcone"])
+def test_split_jobs_reduce_to_reachable_rows_and_slices(small_memory, kind):
+    """A volume taller than the detector's view: the split FP skips the
+    slices that project off the detector and the split BP uploads only the
+    rows each slab reads; both still equal the unsplit jobs."""
+    astra, calls = small_memory
+    from astra import phantom
+    import torch
+    from astra import composite as C
+    n, nz, na, nu, nv = 32, 128, 40, 48, 24
+    vg = astra.create_vol_geom(n, n, nz)
+    ang = np.linspace(0, 2 * np.pi, na, endpoint=False)
+    if kind == "cone":
+        pg = astra.create_proj_geom("cone", 1.0, 1.0, nv, nu, ang, 120.0, 60.0)
+    elif kind == "parallel3d":
+        pg = astra.create_proj_geom("parallel3d", 1.0, 1.0, nv, nu, ang)
+    else:
+        base = astra.geom_2vec(astra.create_proj_geom("cone", 1.0, 1.0, nv, nu, ang, 120.0, 60.0))
+        pg = astra.create_proj_geom("cyl_cone_vec", nv, nu,
+                                    np.hstack([base["Vectors"], np.full((na, 1), 170.0)]))
+    vol = np.concatenate([phantom.ellipsoids_3d(n, seed=104 + k) for k in range(nz // n)])
+    proj = np.random.default_rng(105).random((nv, na, nu), dtype=np.float32)
+    _, ref = astra.create_sino3d_gpu(vol, pg, vg)
+    _, refb = astra.create_backprojection3d_gpu(proj, pg, vg)
+    astra.set_gpu_index(0, memory=1024 * 1024)
+    _, got = astra.create_sino3d_gpu(vol, pg, vg)
+    fp_calls = list(calls)
+    _, gotb = astra.create_backprojection3d_gpu(proj, pg, vg)
+    bp_calls = calls[len(fp_calls):]
+    assert len(fp_calls) >= 2 and len(bp_calls) >= 2
+    plan = C.plan_fp((nz, n, n), (nv, na, nu), C.budget(torch.device("cuda", 0)))
+    assert sum(z for _, z, _ in fp_calls) < len(plan) * nz  # slices off the detector skipped
+    pairs = sum(len(ch) for _, ch in C.plan_bp((nz, n, n), (nv, na, nu), C.budget(torch.device("cuda", 0))))
+    # slabs read a subset of the rows, or none
+    assert min(r for _, _, r in bp_calls) < nv or len(bp_calls) < pairs
     assert rel_l2(got, ref) <= 1e-6
     assert rel_l2(gotb, refb) <= 1e-6
 
