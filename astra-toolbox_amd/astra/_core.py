@@ -75,6 +75,11 @@ class Projector:
     CCudaProjector2D/3D (ProjectionKernel, VoxelSuperSampling,
     DetectorSuperSampling, GPUindex)."""
 
+    def description(self):
+        """CCudaProjector2D / 3D::description (src/CudaProjector2D.cpp:109-112,
+        src/CudaProjector3D.cpp:148-151)."""
+        return "" if self.type == "cuda3d" else "cuda"
+
     def __init__(self, ptype, proj_geom, vol_geom, options=None):
         options = dict(options or {})
         self.type = ptype
@@ -239,6 +244,12 @@ class Data:
         self.shape = tuple(shape)
         self.host = host
         self.linked = linked
+
+    def description(self):
+        """CData2D / CData3D::description (src/Data2D.cpp:34-41,
+        src/Data3D.cpp:34-41): the dimensions fastest axis first."""
+        what = "volume" if self.kind == "-vol" else "sinogram"
+        return "%s %s data \t" % ("x".join(str(n) for n in reversed(self.shape)), what)
 
     # -- storage views
     def is_gpu(self):

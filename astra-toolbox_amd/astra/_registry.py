@@ -46,7 +46,18 @@ class Registry:
         self.delete(list(self.objects.keys()))
 
     def info(self):
-        return {k: type(v).__name__ for k, v in self.objects.items()}
+        """Print the object list as CAstraObjectManager::info does
+        (src/AstraObjectManager.cpp:161-191): a header, one line per object
+        (id, initialised flag, description), a closing rule."""
+        rule = "-" * 41
+        lines = ["id  init  description", rule]
+        with self.lock:
+            items = sorted(self.objects.items())
+        for k, obj in items:
+            desc = obj.description() if hasattr(obj, "description") else type(obj).__name__
+            lines.append("%d \tv     %s" % (k, desc))
+        lines.append(rule)
+        print("\n".join(lines) + "\n")
 
 
 data2d = Registry("Data object")

@@ -24,7 +24,7 @@ except ImportError:  # pragma: no cover
 
 
 def _opt(cfg, key, default=None):
-    for sect in ("option", "options"):
+    for sect in ("option", "options", "Option", "Options"):
         o = cfg.get(sect) or {}
         if key in o:
             return o[key]
@@ -72,6 +72,9 @@ class _Algo:
 
     def get_res_norm(self):
         raise AstraError("this algorithm has no residual norm")
+
+    def description(self):
+        return type(self).__name__
 
     def release(self):
         pass

@@ -120,8 +120,8 @@ def vol_slab_geom(vg, z0, z1):
     """Volume geometry of slices [z0, z1) (the window shrinks along z)."""
     nz = int(vg["GridSliceCount"])
     lo, hi = G._window(vg, "Z", nz)
-    out = dict(vg)
-    opt = dict(vg.get("option", {}))
+    out = {k: v for k, v in vg.items() if k not in G.OPTION_KEYS}
+    opt = G.options_of(vg)
     for ax, n in (("X", vg["GridColCount"]), ("Y", vg["GridRowCount"])):
         wlo, whi = G._window(vg, ax, n)
         opt["WindowMin" + ax], opt["WindowMax" + ax] = float(wlo), float(whi)

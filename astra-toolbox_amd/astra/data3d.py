@@ -46,7 +46,9 @@ def store(i, data):
 
 
 def get_geometry(i):
-    return _reg.get(i).geom
+    """The object's geometry in the reference's configuration form (a
+    volume's window under 'options')."""
+    return G.config_form(_reg.get(i).geom)
 
 
 def change_geometry(i, geometry):
@@ -73,4 +75,10 @@ def info():
 
 
 def shepp_logan(geometry, modified=True, returnData=True):
-    raise AstraError("shepp_logan is not part of this build (phantoms are generated by the caller)")
+    """A volume data object holding the Shepp-Logan phantom on the
+    geometry's grid (data3d.shepp_logan, python/astra/data3d.py); modified=False
+    gives the original densities.  Returns the id, or (id, data)."""
+    from .phantom import shepp_logan_3d
+    i = create("-vol", geometry)
+    _reg.get(i).store(shepp_logan_3d(*G.vol_shape(geometry), modified=bool(modified)))
+    return (i, get(i)) if returnData else i

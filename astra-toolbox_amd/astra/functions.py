@@ -57,7 +57,10 @@ def move_vol_geom(geom, pos, is_relative=False):
     """Translate a volume geometry window (functions.py:123-154)."""
     import copy
     g = copy.deepcopy(geom)
-    o = g["option"]
+    o = G.options_of(g)
+    for k in G.OPTION_KEYS:
+        g.pop(k, None)
+    g["option"] = o
     axes = ["X", "Y", "Z"][: len(pos)]
     for ax, p in zip(axes, pos):
         w = o["WindowMax" + ax] - o["WindowMin" + ax]

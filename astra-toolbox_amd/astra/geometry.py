@@ -28,9 +28,41 @@ def is_3d_vol(vg):
     return "GridSliceCount" in vg
 
 
+OPTION_KEYS = ("option", "options", "Option", "Options")
+
+
+def options_of(cfg):
+    """The options of a config or geometry dictionary: 'option', 'options',
+    'Option' and 'Options' are synonyms (python/astra/utils.pyx:121)."""
+    out = {}
+    for k in OPTION_KEYS:
+        out.update(cfg.get(k) or {})
+    return out
+
+
+def config_form(geom):
+    """A geometry as the reference's get_geometry returns it: the
+    geometry's configuration read back, where a volume's window is under
+    'options' (python/astra/utils.pyx:213-235, src/VolumeGeometry3D.cpp:
+    290-305); projection geometries come back with their own keys."""
+    if "GridColCount" not in geom:
+        return dict(geom)
+    out = {"GridColCount": int(geom["GridColCount"]), "GridRowCount": int(geom["GridRowCount"])}
+    axes = [("X", geom["GridColCount"]), ("Y", geom["GridRowCount"])]
+    if "GridSliceCount" in geom:
+        out["GridSliceCount"] = int(geom["GridSliceCount"])
+        axes.append(("Z", geom["GridSliceCount"]))
+    opts = {}
+    for ax, n in axes:
+        lo, hi = _window(geom, ax, n)
+        opts["WindowMin" + ax], opts["WindowMax" + ax] = float(lo), float(hi)
+    out["options"] = opts
+    return out
+
+
 # ------------------------------------------------------------- volumes ---
 def _window(vg, axis, n):
-    o = vg.get("option", {})
+    o = options_of(vg)
     lo = o.get("WindowMin" + axis, -n / 2.0)
     hi = o.get("WindowMax" + axis, n / 2.0)
     return f32(lo), f32(hi)

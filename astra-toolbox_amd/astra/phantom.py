@@ -101,3 +101,98 @@ def ellipsoids_3d(n, count=20, seed=0, chunk=64):
             slab[((u / ax[0]) ** 2 + (v / ax[1]) ** 2)[None] + (w / ax[2]) ** 2 <= 1.0] += rho
         vol[z0:z1] = np.clip(slab, 0.0, 1.0)
     return vol
+
+
+# ------------------------------------------------------------ Shepp-Logan --
+# The Shepp-Logan ellipses (Shepp & Logan 1974) in the reference's units:
+# centre (x, y[, z]) and semi-axes as fractions of the half extent, rotation
+# in degrees, density; the modified densities (Toft) replace the originals
+# (src/SheppLogan.cpp:187-246).  One table serves 2D (z ignored).
+#                x      y       z    ax      ay     az    rot  value  modified
+_SHEPP_LOGAN = ((0.0,   0.0,    0.0, 0.69,   0.92,  0.81,   0,  2.00,  1.0),
+                (0.0,  -0.0184, 0.0, 0.6624, 0.874, 0.78,   0, -0.98, -0.8),
+                (0.22,  0.0,    0.0, 0.11,   0.31,  0.22, -18, -0.02, -0.2),
+                (-0.22, 0.0,    0.0, 0.16,   0.41,  0.28,  18, -0.02, -0.2),
+                (0.0,   0.35,   0.0, 0.21,   0.25,  0.41,   0,  0.01,  0.1),
+                (0.0,   0.1,    0.0, 0.046,  0.046, 0.05,   0,  0.01,  0.1),
+                (0.0,  -0.1,    0.0, 0.046,  0.046, 0.05,   0,  0.01,  0.1),
+                (-0.08, -0.605, 0.0, 0.046,  0.023, 0.05,   0,  0.01,  0.1),
+                (0.0,  -0.605,  0.0, 0.023,  0.023, 0.02,   0,  0.01,  0.1),
+                (0.06, -0.605,  0.0, 0.023,  0.046, 0.02,   0,  0.01,  0.1))
+
+
+def _row_spans(ex, ey, ax, ay, rot, width, height):
+    """Per image row, the columns [lo, hi) inside one ellipse, and the
+    implicit-function coefficients (A, B(y), C(y)) of its boundary
+    A x^2 + B x + C = 0.  Pixel centres sit at integer (x, y); rows whose
+    span would start left of column 0 are left out, as the reference's
+    unsigned column arithmetic leaves them out (src/SheppLogan.cpp:56-116)."""
+    a = (0.5 + 0.5 * ex) * width - 0.5
+    b = (0.5 - 0.5 * ey) * height - 0.5
+    c2 = (0.5 * ax * width) ** 2
+    d2 = (0.5 * ay * height) ** 2
+    th = rot * np.pi / 180.0
+    co, si = np.cos(th), np.sin(th)
+    cc, ss, cs = co * co, si * si, co * si
+    # ((x-a)cos - (y-b)sin)^2 / c^2 + ((x-a)sin + (y-b)cos)^2 / d^2 - 1,
+    # expanded in x and y
+    qxx = cc / c2 + ss / d2
+    qx = -2 * a * cc / c2 + 2 * b * cs / c2 - 2 * b * cs / d2 - 2 * a * ss / d2
+    qxy = -2 * cs / c2 + 2 * cs / d2
+    q0 = (a * a * cc / c2 + b * b * cc / d2 - 2 * a * b * cs / c2 + 2 * a * b * cs / d2
+          + b * b * ss / c2 + a * a * ss / d2 - 1)
+    qy = -2 * b * cc / d2 + 2 * a * cs / c2 - 2 * a * cs / d2 - 2 * b * ss / c2
+    qyy = cc / d2 + ss / c2
+    y = np.arange(height, dtype=np.float64)
+    A = qxx
+    B = y * qxy + qx
+    C = y * y * qyy + y * qy + q0
+    disc = B * B - 4 * A * C
+    ok = disc >= 0
+    root = np.sqrt(np.where(ok, disc, 0.0))
+    lo = np.ceil(0.5 * (-B - root) / A)
+    hi = np.floor(0.5 * (-B + root) / A) + 1
+    ok &= (lo >= 0) & (lo < width) & (hi > 0)
+    lo = np.where(ok, lo, 0).astype(np.int64)
+    hi = np.where(ok, np.minimum(hi, width), 0).astype(np.int64)
+    return ok, lo, hi, A, B, C
+
+
+def shepp_logan_2d(rows, cols, modified=True):
+    """[rows][cols] float32 Shepp-Logan phantom on pixel centres, negative
+    sums clipped to 0 (generateSheppLogan, src/SheppLogan.cpp:187-222)."""
+    img = np.zeros((rows, cols), dtype=np.float32)
+    x = np.arange(cols)[None, :]
+    for ex, ey, _, ax, ay, _, rot, val, mval in _SHEPP_LOGAN:
+        ok, lo, hi, _, _, _ = _row_spans(ex, ey, ax, ay, rot, cols, rows)
+        inside = ok[:, None] & (x >= lo[:, None]) & (x < hi[:, None])
+        img[inside] += np.float32(mval if modified else val)
+    img[img < 0] = 0.0
+    return img
+
+
+def shepp_logan_3d(slices, rows, cols, modified=True):
+    """[slices][rows][cols] float32 Shepp-Logan phantom: ellipsoids whose
+    x-y section is the 2D ellipse; voxel (x, y) of a section spans the
+    slices within ((z - e) / f)^2 <= -q(x, y), q the section's implicit
+    function (generateSheppLogan3D, src/SheppLogan.cpp:118-184, 224-246)."""
+    vol = np.zeros((slices, rows, cols), dtype=np.float32)
+    x = np.arange(cols, dtype=np.float64)[None, :]
+    zz = np.arange(slices)[:, None, None]
+    for ex, ey, ez, ax, ay, az, rot, val, mval in _SHEPP_LOGAN:
+        ok, lo, hi, A, B, C = _row_spans(ex, ey, ax, ay, rot, cols, rows)
+        e = (0.5 - 0.5 * ez) * slices - 0.5
+        f = 0.5 * az * slices
+        inside = ok[:, None] & (x >= lo[:, None]) & (x < hi[:, None])
+        q = (A * x) * x + B[:, None] * x + C[:, None]
+        with np.errstate(invalid="ignore"):
+            half = f * np.sqrt(-q)
+        zlo = np.ceil(e - half)
+        zhi = np.floor(e + half) + 1
+        inside &= (zlo >= 0) & (zlo < slices) & (zhi > 0)   # NaN (q > 0) fails these
+        zlo = np.where(inside, zlo, 0).astype(np.int64)
+        zhi = np.where(inside, np.minimum(zhi, slices), 0).astype(np.int64)
+        mask = inside[None] & (zz >= zlo[None]) & (zz < zhi[None])
+        vol[mask] += np.float32(mval if modified else val)
+    vol[vol < 0] = 0.0
+    return vol

@@ -5,6 +5,7 @@ projector types of the reference ('line', 'linear', 'strip', ...) are
 outside this build's hot path and raise.
 """
 from . import _core
+from . import geometry as G
 from ._registry import projectors2d as _reg
 from .log import AstraError
 
@@ -18,8 +19,7 @@ def create(config):
         raise AstraError("CPU projector type '%s' is not part of this build; use 'cuda'" % t)
     if t != "cuda":
         raise AstraError("unknown 2D projector type '%s'" % t)
-    opts = dict(config.get("options", {}) or {})
-    opts.update(config.get("option", {}) or {})
+    opts = G.options_of(config)
     p = _core.Projector("cuda", config["ProjectionGeometry"], config["VolumeGeometry"], opts)
     return _reg.add(p)
 
@@ -45,7 +45,9 @@ def projection_geometry(i):
 
 
 def volume_geometry(i):
-    return _reg.get(i).vol_geom
+    """The projector's volume geometry in configuration form (window under
+    'options'), as the reference returns it."""
+    return G.config_form(_reg.get(i).vol_geom)
 
 
 def is_cuda(i):

@@ -1,6 +1,7 @@
 """3D projector objects (python/astra/projector3d.py:28-128); type 'cuda3d'
 (CCudaProjector3D, src/CudaProjector3D.cpp:83-124)."""
 from . import _core
+from . import geometry as G
 from ._registry import projectors3d as _reg
 from .log import AstraError
 
@@ -9,8 +10,7 @@ def create(config):
     t = config.get("type")
     if t != "cuda3d":
         raise AstraError("unknown 3D projector type '%s'" % t)
-    opts = dict(config.get("options", {}) or {})
-    opts.update(config.get("option", {}) or {})
+    opts = G.options_of(config)
     p = _core.Projector("cuda3d", config["ProjectionGeometry"], config["VolumeGeometry"], opts)
     return _reg.add(p)
 
@@ -36,7 +36,9 @@ def projection_geometry(i):
 
 
 def volume_geometry(i):
-    return _reg.get(i).vol_geom
+    """The projector's volume geometry in configuration form (window under
+    'options'), as the reference returns it."""
+    return G.config_form(_reg.get(i).vol_geom)
 
 
 def is_cuda(i):

@@ -80,3 +80,20 @@ def reference():
     if not os.path.exists(REF_LIB):
         return None
     return LinearCPU(REF_LIB, "ref")
+
+
+def reference_shepp_logan(shape, modified=True):
+    """The reference's generateSheppLogan / generateSheppLogan3D
+    (src/SheppLogan.cpp:187-246) through oracle/_ref, or None when it was
+    not built.  shape: (rows, cols) or (slices, rows, cols)."""
+    if not os.path.exists(REF_LIB):
+        return None
+    lib = ctypes.CDLL(REF_LIB)
+    out = np.zeros(shape, dtype=np.float32)
+    if len(shape) == 2:
+        r = lib.ref_shepp_logan_2d(int(shape[0]), int(shape[1]), int(bool(modified)), out.ctypes.data_as(_fp))
+    else:
+        r = lib.ref_shepp_logan_3d(int(shape[0]), int(shape[1]), int(shape[2]), int(bool(modified)),
+                                   out.ctypes.data_as(_fp))
+    assert r == 0
+    return out

@@ -1,6 +1,7 @@
 // C entry points over the REFERENCE CPU library (CParallelBeamLinearKernel-
-// Projector2D, CSirtAlgorithm, CForward/CBackProjectionAlgorithm), built from
-// the reference sources by oracle/Makefile into oracle/_ref/.
+// Projector2D, CSirtAlgorithm, CForward/CBackProjectionAlgorithm, the
+// Shepp-Logan generators), built from the reference sources by
+// oracle/Makefile into oracle/_ref/.
 //
 // TEST INFRASTRUCTURE ONLY: used by tests/ (golden checks of the C
 // restatement) and by bench.py's cpu_baseline leg.  Never part of the product.
@@ -11,9 +12,12 @@
 #include "astra/Data2D.h"
 #include "astra/ForwardProjectionAlgorithm.h"
 #include "astra/ParallelBeamLinearKernelProjector2D.h"
+#include "astra/Data3D.h"
 #include "astra/ParallelProjectionGeometry2D.h"
+#include "astra/SheppLogan.h"
 #include "astra/SirtAlgorithm.h"
 #include "astra/VolumeGeometry2D.h"
+#include "astra/VolumeGeometry3D.h"
 
 using namespace astra;
 
@@ -63,6 +67,24 @@ int ref_linear_sirt(int rows, int cols, int na, int nd, float w, const float *an
     CSirtAlgorithm alg;
     if (!alg.initialize(&s.proj, &s.sino, &s.vol) || !alg.run(iters)) return -1;
     std::memcpy(vol, s.vol.getFloat32Memory(), sizeof(float) * (size_t)rows * cols);
+    return 0;
+}
+
+int ref_shepp_logan_2d(int rows, int cols, int modified, float *out) {
+    CVolumeGeometry2D vg(cols, rows);
+    CFloat32VolumeData2D vol(vg, new CDataMemory<float32>((size_t)rows * cols));
+    generateSheppLogan(&vol, modified != 0);
+    std::memcpy(out, vol.getFloat32Memory(), sizeof(float) * (size_t)rows * cols);
+    return 0;
+}
+
+int ref_shepp_logan_3d(int slices, int rows, int cols, int modified, float *out) {
+    CVolumeGeometry3D vg(cols, rows, slices);
+    CFloat32VolumeData3D *vol = createCFloat32VolumeData3DMemory(vg);
+    if (!vol) return -1;
+    generateSheppLogan3D(vol, modified != 0);
+    std::memcpy(out, vol->getFloat32Memory(), sizeof(float) * (size_t)slices * rows * cols);
+    delete vol;
     return 0;
 }
 
