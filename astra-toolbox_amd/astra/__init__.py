@@ -18,6 +18,11 @@ from .functions import add_noise_to_sino, clear, data_op, geom_2vec, geom_size, 
 from .log import AstraError
 from .pythonutils import GPULink
 
+try:  # scipy LinearOperator view of a projector (python/astra/__init__.py)
+    from .optomo import OpTomo
+except ImportError:  # pragma: no cover - scipy absent
+    pass
+
 __version__ = "2.5.99+mi355x"
 
 
