@@ -10,11 +10,15 @@ import os
 
 from . import (algorithm, autograd, batched, composite, data2d, data3d, experimental, functions, log,
                projector, projector3d, pythonutils)
-from ._core import set_gpu_index
+from . import astra
+from .astra import get_gpu_info, has_feature, set_gpu_index, use_cuda
+from .extrautils import clipCircle
+from .tests import test, test_CUDA, test_noCUDA
 from .creators import (astra_dict, create_backprojection, create_backprojection3d_gpu,
                        create_proj_geom, create_projector, create_reconstruction, create_sino,
                        create_sino3d_gpu, create_vol_geom)
-from .functions import add_noise_to_sino, clear, data_op, geom_2vec, geom_size, move_vol_geom
+from .functions import (add_noise_to_sino, clear, data_op, geom_2vec, geom_postalignment, geom_size,
+                        move_vol_geom)
 from .log import AstraError
 from .pythonutils import GPULink
 
@@ -24,28 +28,6 @@ except ImportError:  # pragma: no cover - scipy absent
     pass
 
 __version__ = "2.5.99+mi355x"
-
-
-def use_cuda():
-    """True when a GPU and the native library are usable."""
-    try:
-        import torch
-        from . import _lib
-        _lib.load()
-        return torch.cuda.is_available()
-    except Exception:
-        return False
-
-
-def has_feature(name):
-    return name in ("cuda", "gpu", "dlpack", "unchunked_3d", "hip")
-
-
-def get_gpu_info(idx=-1):
-    import torch
-    i = 0 if idx < 0 else idx
-    p = torch.cuda.get_device_properties(i)
-    return "GPU #%d: %s, with %dMB" % (i, p.name, p.total_memory // (1024 * 1024))
 
 
 if "ASTRA_GPU_INDEX" in os.environ:
