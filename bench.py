@@ -91,8 +91,15 @@ def cpu_all_cores(iters, workers):
     img = phantom.ellipses_2d(256, seed=0)
     ctx = mp.get_context("spawn")
     t0 = time.perf_counter()
-    with ctx.Pool(workers) as pool:
-        secs = pool.map(_cpu_sirt_worker, [(img, iters)] * workers)
+    # the workers finish and exit on their own (close + join): a pool torn
+    # down with SIGTERM (the context manager's terminate) can leave workers
+    # that a profiler's signal handler keeps alive, and the join then waits
+    pool = ctx.Pool(workers)
+    try:
+        secs = pool.map_async(_cpu_sirt_worker, [(img, iters)] * workers).get(timeout=600)
+    finally:
+        pool.close()
+        pool.join()
     wall = time.perf_counter() - t0
     n, na, nd = 256, 180, 384
     updates = workers * iters * 2.0 * na * nd * n
