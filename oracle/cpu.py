@@ -97,3 +97,33 @@ def reference_shepp_logan(shape, modified=True):
                                    out.ctypes.data_as(_fp))
     assert r == 0
     return out
+
+
+def reference_strip():
+    """(fp, bp) of the reference's parallel 'strip' projector
+    (include/astra/ParallelBeamStripKernelProjector2D.inl) through
+    oracle/_ref, or None when it was not built."""
+    if not os.path.exists(REF_LIB):
+        return None
+    lib = ctypes.CDLL(REF_LIB)
+    i, f = ctypes.c_int, ctypes.c_float
+    for fn in (lib.ref_strip_fp, lib.ref_strip_bp):
+        fn.argtypes = [i, i, i, i, f, _fp, _fp, _fp]
+        fn.restype = ctypes.c_int
+
+    def fp(vol, angles, n_det, det_width=1.0):
+        vol = np.ascontiguousarray(vol, dtype=np.float32)
+        ang = np.ascontiguousarray(angles, dtype=np.float32)
+        out = np.zeros((len(ang), n_det), dtype=np.float32)
+        assert lib.ref_strip_fp(vol.shape[0], vol.shape[1], len(ang), n_det, det_width,
+                                ang.ctypes.data_as(_fp), vol.ctypes.data_as(_fp), out.ctypes.data_as(_fp)) == 0
+        return out
+
+    def bp(sino, angles, shape, det_width=1.0):
+        sino = np.ascontiguousarray(sino, dtype=np.float32)
+        ang = np.ascontiguousarray(angles, dtype=np.float32)
+        out = np.zeros(shape, dtype=np.float32)
+        assert lib.ref_strip_bp(shape[0], shape[1], len(ang), sino.shape[1], det_width,
+                                ang.ctypes.data_as(_fp), sino.ctypes.data_as(_fp), out.ctypes.data_as(_fp)) == 0
+        return out
+    return fp, bp

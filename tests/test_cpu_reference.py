@@ -55,3 +55,23 @@ def test_gpu_path_formula_vs_cpu_linear(gold):
     bp = O2.par_bp(gold["sino_in"].astype(np.float64), v, (64, 64), osc)
     e_bp = np.linalg.norm(bp - gold["bp"]) / np.linalg.norm(gold["bp"])
     assert e_bp < 3e-2
+
+
+def test_strip_fixture_is_the_reference_strip_projector(gold):
+    """tests/golden/strip_cpu_64.npz holds the reference strip projector's
+    outputs on the linear fixture's inputs (checked against the reference
+    build when it is present); the GPU-path formula differs from it by the
+    stated 1.6e-3 (FP) and 2.6e-3 (voxel-driven BP)."""
+    strip = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "strip_cpu_64.npz"))
+    ref = cpu.reference_strip()
+    if ref is not None:
+        fp, bp = ref
+        assert np.array_equal(fp(gold["vol"], gold["angles"], int(gold["n_det"])), strip["fp"])
+        assert np.array_equal(bp(gold["sino_in"], gold["angles"], (64, 64)), strip["bp"])
+    vg = OG.vol_geom_2d(64, 64)
+    v, osc = OG.normalise_2d(OG.gen_par_projections(gold["angles"], 96, 1.0), "par", vg)
+    e_fp = np.linalg.norm(O2.par_fp(gold["vol"].astype(np.float64), v, 96, osc) - strip["fp"]) / \
+        np.linalg.norm(strip["fp"])
+    e_bp = np.linalg.norm(O2.par_bp(gold["sino_in"].astype(np.float64), v, (64, 64), osc) - strip["bp"]) / \
+        np.linalg.norm(strip["bp"])
+    assert 1e-3 < e_fp < 5e-3 and 1e-3 < e_bp < 6e-3
