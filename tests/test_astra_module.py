@@ -83,3 +83,36 @@ def test_clip_circle(n):
 def test_self_check_without_gpu(capsys):
     astra.test_noCUDA()
     assert len(capsys.readouterr().out.split("\n")) - 1 == 2
+
+
+def test_log_module(tmp_path, capsys):
+    """astra.log (python/astra/log.py): levels, screen and file outputs,
+    formats with the reference's substitutions, enable / disable."""
+    log = astra.log
+    log.setOutputScreen(log.STDOUT, log.WARN)
+    log.setFormatScreen("%l|%m|%%\n")
+    log.info("hidden")
+    log.warn("shown")
+    assert capsys.readouterr().out == "WARN|shown|%\n"
+    path = tmp_path / "astra.log"
+    log.setOutputFile(str(path), log.DEBUG)
+    log.setFormatFile("%l %f:%n %m\n")
+    log.debug("to file only")
+    log.disableScreen()
+    log.error("file and no screen")
+    log.enableScreen()
+    log.disableFile()
+    log.error("screen only")
+    text = path.read_text()
+    assert "DEBUG " in text and "to file only" in text and "file and no screen" in text
+    assert "screen only" not in text and "test_astra_module.py:" in text
+    out = capsys.readouterr().out
+    assert out == "ERROR|screen only|%\n"
+    log.disable()
+    log.error("nowhere")
+    assert capsys.readouterr().out == ""
+    log.enable()
+    log.setOutputScreen(log.STDERR, log.ERROR)
+    log.setFormatScreen("%d %t %f(%n): %l: %m\n")
+    e = astra.AstraError("plain", append_log=False)
+    assert str(e) == "plain"
