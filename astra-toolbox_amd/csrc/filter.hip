@@ -1,6 +1,7 @@
 // Row filtering for FBP / FDK on gfx950: a hand-written batched radix-2 FFT
 // in LDS.  Two real rows share one complex transform (row a in the real
-// part, row a+1 in the imaginary part): with a real, even filter h the
+// part, row a+1 in the imaginary part; rows too long for a pair in LDS
+// take filter_row_real_kernel, one row per workgroup): with a real, even filter h the
 // filtered rows are Re and Im of IFFT(H . FFT(x + i y)).  Forward pass is
 // decimation-in-time (bit-reversed load, natural-order spectrum), the
 // inverse is decimation-in-frequency (natural input, bit-reversed output),
@@ -14,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdlib>
 
 #include "devutil.hpp"
 #include "internal.hpp"
@@ -142,6 +144,122 @@ __global__ __launch_bounds__(256) void filter_rows_kernel(FilterArgs A) {
     }
 }
 
+// One real row per workgroup, for transforms whose row pair does not fit
+// the CU's LDS (n_fft > 16384 on gfx950): the length-N real transform is a
+// length-M = N/2 complex transform of z[m] = x[2m] + i x[2m+1] followed by
+// the even / odd split
+//   E[k] = (Z[k] + conj Z[M-k]) / 2,  O[k] = (Z[k] - conj Z[M-k]) / 2i,
+//   X[k] = E[k] + W^k O[k],  X[k+M] = E[k] - W^k O[k],  W = exp(-2 pi i / N),
+// and back: with the filtered X' (Hermitian, so X'[k+M] = conj X'[M-k]),
+//   E'[k] = (X'[k] + conj X'[M-k]) / 2,  O'[k] = (X'[k] - conj X'[M-k]) / (2 W^k),
+// z' = IFFT_M(E' + i O') holds x'[2m] = Re z'[m] and x'[2m+1] = Im z'[m].
+// The filter multiplies X[k], k = 0..M (real even spectrum, or row r's
+// complex Hermitian spectrum r % n_spec).  M points of float2 in LDS.
+template <bool GENERAL>
+__global__ __launch_bounds__(256) void filter_row_real_kernel(FilterArgs A) {
+    extern __shared__ float2 Z[];
+    const unsigned N = A.n_fft, M = N >> 1, half_m = M >> 1;
+    const size_t r = blockIdx.x;
+    float *row = A.rows + r * A.n_det;
+    const float *w = A.weights ? A.weights + (r / A.rows_per_weight) * A.n_det : nullptr;
+    const float *v = A.weights2 ? A.weights2 + (r % A.w2_rows) * A.n_det : nullptr;
+    const unsigned shift = 32 - (A.log2n - 1);
+
+    // load pairs (x[2m], x[2m+1]) at bit-reversed positions of the M-point transform
+    for (unsigned m = threadIdx.x; m < M; m += blockDim.x) {
+        float a = 0.f, b = 0.f;
+        const unsigned j = 2 * m;
+        if (j < A.n_det) {
+            a = row[j];
+            if (w) a *= w[j];
+            if (v) a *= v[j];
+        }
+        if (j + 1 < A.n_det) {
+            b = row[j + 1];
+            if (w) b *= w[j + 1];
+            if (v) b *= v[j + 1];
+        }
+        Z[__brev(m) >> shift] = make_float2(a, b);
+    }
+    __syncthreads();
+
+    // forward DIT over M points; exp(-2 pi i k / len) = tw[k * N / len]
+    for (unsigned len = 2, step = M; len <= M; len <<= 1, step >>= 1) {
+        const unsigned h = len >> 1;
+        for (unsigned b = threadIdx.x; b < half_m; b += blockDim.x) {
+            unsigned k = b & (h - 1);
+            unsigned i = (b - k) * 2 + k;
+            float2 t = cmul(Z[i + h], A.tw[k * step]);
+            float2 u = Z[i];
+            Z[i] = make_float2(u.x + t.x, u.y + t.y);
+            Z[i + h] = make_float2(u.x - t.x, u.y - t.y);
+        }
+        __syncthreads();
+    }
+
+    // split, filter, merge: one thread per pair (k, M - k), k = 0 .. M/2;
+    // the conjugate of Z' is stored for the inverse (conj(FFT(conj .)))
+    const float2 *H = GENERAL ? A.cspec + (size_t)(r % A.n_spec) * N : nullptr;
+    for (unsigned k = threadIdx.x; k <= half_m; k += blockDim.x) {
+        const unsigned j = M - k;             // == M for k == 0
+        const float2 zk = Z[k], zj = Z[j & (M - 1)];
+        // E[k], O[k] and the spectrum at k and at j (for k == 0: X[0], X[M])
+        const float2 ek = make_float2(0.5f * (zk.x + zj.x), 0.5f * (zk.y - zj.y));
+        const float2 dk = make_float2(0.5f * (zk.x - zj.x), 0.5f * (zk.y + zj.y));
+        const float2 ok = make_float2(dk.y, -dk.x);                  // d / i
+        const float2 wk = A.tw[k];                                   // W^k, k < M
+        const float2 wok = cmul(wk, ok);
+        float2 xk = make_float2(ek.x + wok.x, ek.y + wok.y);
+        // X[j] = conj E[k] + W^j conj O[k], W^j = -conj W^k
+        const float2 wj = make_float2(-wk.x, wk.y);
+        const float2 woj = cmul(wj, conjf2(ok));
+        float2 xj = make_float2(ek.x + woj.x, -ek.y + woj.y);
+        if (GENERAL) {
+            xk = cmul(H[k], xk);
+            xj = cmul(H[j], xj);
+        } else {
+            const float hk = A.spec[k], hj = A.spec[j];
+            xk = make_float2(xk.x * hk, xk.y * hk);
+            xj = make_float2(xj.x * hj, xj.y * hj);
+        }
+        // E'[k] = (X'[k] + conj X'[j]) / 2, O'[k] = (X'[k] - conj X'[j]) conj(W^k) / 2
+        const float2 e2 = make_float2(0.5f * (xk.x + xj.x), 0.5f * (xk.y - xj.y));
+        const float2 o2 = cmul(make_float2(0.5f * (xk.x - xj.x), 0.5f * (xk.y + xj.y)), conjf2(wk));
+        // Z'[k] = E' + i O'; stored conjugated
+        Z[k] = make_float2(e2.x - o2.y, -(e2.y + o2.x));
+        if (k != 0 && j != k) {
+            // E'[j] = conj E'[k]; O'[j] = (X'[j] - conj X'[k]) conj(W^j) / 2
+            const float2 o3 = cmul(make_float2(0.5f * (xj.x - xk.x), 0.5f * (xj.y + xk.y)), conjf2(wj));
+            const float2 e3 = conjf2(e2);
+            Z[j] = make_float2(e3.x - o3.y, -(e3.y + o3.x));
+        }
+    }
+    __syncthreads();
+
+    // DIF over M points: bit-reversed output
+    for (unsigned len = M, step = 2; len >= 2; len >>= 1, step <<= 1) {
+        const unsigned h = len >> 1;
+        for (unsigned b = threadIdx.x; b < half_m; b += blockDim.x) {
+            unsigned k = b & (h - 1);
+            unsigned i = (b - k) * 2 + k;
+            float2 u = Z[i], q = Z[i + h];
+            Z[i] = make_float2(u.x + q.x, u.y + q.y);
+            Z[i + h] = cmul(make_float2(u.x - q.x, u.y - q.y), A.tw[k * step]);
+        }
+        __syncthreads();
+    }
+
+    // z' = conj(result) / M: x'[2m] = Re / M, x'[2m+1] = -Im / M
+    const float inv = 1.0f / (float)M;
+    for (unsigned m = threadIdx.x; m < M; m += blockDim.x) {
+        const unsigned j = 2 * m;
+        if (j >= A.n_det) break;
+        const float2 q = Z[__brev(m) >> shift];
+        row[j] = q.x * inv;
+        if (j + 1 < A.n_det) row[j + 1] = -q.y * inv;
+    }
+}
+
 // rows *= w[r / div] (optional) * w2[r % mod] (optional): the weights of an
 // identity ('none') filter
 __global__ void k_row_weights(float *rows, unsigned len, const float *w, size_t div,
@@ -191,13 +309,37 @@ void filter_rows_launch(const Filter &f, float *rows, size_t n_rows, const float
     A.w2_rows = w2_rows;
     // One row pair's transform lives in LDS: n_fft <= 8192 fits the default
     // 64 KiB; larger transforms (up to the CU's LDS, 160 KiB on gfx950)
-    // raise the kernel's dynamic LDS limit first.
-    const size_t lds = (size_t)f.n_fft * sizeof(float2);
+    // raise the kernel's dynamic LDS limit first.  Beyond that, one real
+    // row per workgroup as a half-length complex transform.
     int dev = 0, max_lds = 0;
     AMX_HIP(hipGetDevice(&dev));
     AMX_HIP(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
-    AMX_REQUIRE(lds <= (size_t)max_lds, "filter: n_fft %u needs %zu bytes of LDS, the device has %d",
-                f.n_fft, lds, max_lds);
+    const size_t pair_lds = (size_t)f.n_fft * sizeof(float2);
+    // ASTRA_AMD_FILTER_SINGLE_ROW=1 takes the single-row path at any size
+    // (read at every call; for testing it where the pair path also fits)
+    const char *single = std::getenv("ASTRA_AMD_FILTER_SINGLE_ROW");
+    if (pair_lds > (size_t)max_lds || (single && single[0] == '1')) {
+        const size_t lds = pair_lds / 2;
+        AMX_REQUIRE(f.n_fft >= 4 && lds <= (size_t)max_lds,
+                    "filter: n_fft %u needs %zu bytes of LDS for one row, the device has %d", f.n_fft,
+                    lds, max_lds);
+        if (f.general)
+            AMX_HIP(hipFuncSetAttribute((const void *)filter_row_real_kernel<true>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        else
+            AMX_HIP(hipFuncSetAttribute((const void *)filter_row_real_kernel<false>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        AMX_REQUIRE(n_rows < (1ull << 31), "filter: too many rows");
+        ProfToken pt = prof_begin("filter_rows", (double)n_rows, s);
+        if (f.general)
+            filter_row_real_kernel<true><<<(unsigned)n_rows, 256, lds, s>>>(A);
+        else
+            filter_row_real_kernel<false><<<(unsigned)n_rows, 256, lds, s>>>(A);
+        AMX_LAUNCHED();
+        prof_end(pt, s);
+        return;
+    }
+    const size_t lds = pair_lds;
     if (lds > 65536) {
         if (f.general)
             AMX_HIP(hipFuncSetAttribute((const void *)filter_rows_kernel<true>,

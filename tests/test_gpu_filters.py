@@ -231,3 +231,73 @@ def test_fbp_wide_detector_large_fft():
     astra.algorithm.run(astra.algorithm.create(cfg))
     ref = OF.fbp_par(sino, ovec, (n, n), osc, 1.0)
     assert rel_l2(astra.data2d.get(rid), ref) <= 5e-5
+
+
+@pytest.fixture
+def single_row():
+    """Forces the single-row filter path (a real transform of length N as
+    a complex one of length N/2; taken by itself above 16384 points) for
+    the calls inside the test."""
+    import os
+    key = "ASTRA_AMD_FILTER_SINGLE_ROW"
+    old = os.environ.get(key)
+    os.environ[key] = "1"
+    yield
+    if old is None:
+        os.environ.pop(key, None)
+    else:
+        os.environ[key] = old
+
+
+@pytest.mark.parametrize("name", ["ram-lak", "hann"])
+def test_single_row_path_fbp(par, single_row, name):
+    from oracle import fdk as OF
+    got = _fbp(par, FilterType=name)
+    assert rel_l2(got, _fbp_ref(par, H=OF.windowed_filter(name, ND))) <= 1e-5
+
+
+@pytest.mark.parametrize("kind", ["sinogram", "rsinogram"])
+def test_single_row_path_custom_filters(par, single_row, kind):
+    """Per-row (complex, Hermitian) spectra through the single-row path."""
+    from oracle import fdk as OF
+    astra = par["astra"]
+    base = _fourier_filter() if kind == "sinogram" else _real_filter()
+    base = np.outer(np.random.default_rng(36).random(NA), base)
+    fg = astra.create_proj_geom("parallel", 1.0, base.shape[1], np.zeros(base.shape[0]))
+    fid = astra.data2d.create("-sino", fg, base.astype(np.float32))
+    got = _fbp(par, FilterType=kind, FilterSinogramId=fid)
+    H = OF.custom_filter(base.astype(np.float32), ND, kind.startswith("r"))
+    assert rel_l2(got, _fbp_ref(par, H=H)) <= 1e-5
+    astra.data2d.delete(fid)
+
+
+def test_single_row_path_fdk_weights(single_row):
+    """The FDK pre-weights (and Parker weights) applied while loading a
+    single row."""
+    test_fdk_short_scan()
+
+
+def test_fbp_detector_wider_than_a_row_pair():
+    """10000 detector bins: the transform has 32768 points, whose row pair
+    (256 KiB) exceeds the CU's LDS, so each row is filtered alone (128 KiB).
+    Padded detector coordinates reach 10000, where an fp32 step is 9.8e-4
+    texel, so the BP of the filtered data is held to 1e-4."""
+    import astra
+    from oracle import fdk as OF, geometry as OG, proj2d as O2
+    from astra import phantom
+    n, na, nd = 48, 12, 10000
+    angles = np.linspace(0, np.pi, na, endpoint=False)
+    vg = astra.create_vol_geom(n, n)
+    pg = astra.create_proj_geom("parallel", 0.008, nd, angles)
+    ovec, osc = OG.normalise_2d(OG.gen_par_projections(angles, nd, 0.008), "par", vg)
+    sino = O2.par_fp(phantom.ellipses_2d(n, seed=39), ovec, nd, osc).astype(np.float32)
+    pid = astra.create_projector("cuda", pg, vg)
+    sid = astra.data2d.create("-sino", pg, sino)
+    rid = astra.data2d.create("-vol", vg, 0)
+    cfg = astra.astra_dict("FBP_CUDA")
+    cfg.update(ProjectorId=pid, ProjectionDataId=sid, ReconstructionDataId=rid)
+    astra.algorithm.run(astra.algorithm.create(cfg))
+    ref = OF.fbp_par(sino, ovec, (n, n), osc, 1.0)
+    e = rel_l2(astra.data2d.get(rid), ref)
+    print("FBP at 10000 bins rel-L2", e)
+    assert e <= 1e-4
