@@ -41,6 +41,83 @@ __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
 }
 
 __device__ __forceinline__ float2 conjf2(float2 a) { return make_float2(a.x, -a.y); }
+__device__ __forceinline__ float2 addf2(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 subf2(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+
+// In-place FFT over n points in LDS: decimation in time, bit-reversed input,
+// natural-order output.  The radix-2 stages are taken two at a time (a
+// radix-2^2 butterfly on 4 points held in registers: the same operations
+// as two radix-2 passes, with half the LDS round trips and barriers); an
+// odd stage count ends with one radix-2 stage.  The twiddle of position k
+// in a stage of length len is tw[k * (n / len) * ts], where the table has
+// length T = n * ts (tw[j] = exp(-2 pi i j / T), j < T / 2).
+__device__ __forceinline__ void lds_fft_dit(float2 *X, unsigned n, const float2 *__restrict__ tw,
+                                            unsigned ts) {
+    unsigned len = 2;
+    for (; 2 * len <= n; len <<= 2) {
+        const unsigned h = len >> 1, s1 = (n / len) * ts, s2 = s1 >> 1;
+        for (unsigned g = threadIdx.x; g < (n >> 2); g += blockDim.x) {
+            const unsigned k = g & (h - 1), a = (g - k) * 4 + k;
+            float2 xa = X[a], xb = X[a + h], xc = X[a + 2 * h], xd = X[a + 3 * h];
+            const float2 w1 = tw[k * s1];
+            float2 t = cmul(xb, w1);
+            xb = subf2(xa, t);
+            xa = addf2(xa, t);
+            t = cmul(xd, w1);
+            xd = subf2(xc, t);
+            xc = addf2(xc, t);
+            t = cmul(xc, tw[k * s2]);
+            X[a + 2 * h] = subf2(xa, t);
+            X[a] = addf2(xa, t);
+            t = cmul(xd, tw[(k + h) * s2]);
+            X[a + 3 * h] = subf2(xb, t);
+            X[a + h] = addf2(xb, t);
+        }
+        __syncthreads();
+    }
+    if (len <= n) {
+        const unsigned h = len >> 1, s1 = (n / len) * ts;
+        for (unsigned b = threadIdx.x; b < (n >> 1); b += blockDim.x) {
+            const unsigned k = b & (h - 1), i = (b - k) * 2 + k;
+            const float2 t = cmul(X[i + h], tw[k * s1]);
+            const float2 u = X[i];
+            X[i] = addf2(u, t);
+            X[i + h] = subf2(u, t);
+        }
+        __syncthreads();
+    }
+}
+
+// In-place FFT over n points in LDS: decimation in frequency, natural
+// input, bit-reversed output; stages two at a time as in lds_fft_dit.
+__device__ __forceinline__ void lds_fft_dif(float2 *X, unsigned n, const float2 *__restrict__ tw,
+                                            unsigned ts) {
+    unsigned len = n;
+    for (; len >= 4; len >>= 2) {
+        const unsigned q = len >> 2, s1 = (n / len) * ts, s2 = s1 << 1;
+        for (unsigned g = threadIdx.x; g < (n >> 2); g += blockDim.x) {
+            const unsigned k = g & (q - 1), a = (g - k) * 4 + k;
+            const float2 xa = X[a], xb = X[a + q], xc = X[a + 2 * q], xd = X[a + 3 * q];
+            const float2 ya = addf2(xa, xc), yc = cmul(subf2(xa, xc), tw[k * s1]);
+            const float2 yb = addf2(xb, xd), yd = cmul(subf2(xb, xd), tw[(k + q) * s1]);
+            const float2 w2 = tw[k * s2];
+            X[a] = addf2(ya, yb);
+            X[a + q] = cmul(subf2(ya, yb), w2);
+            X[a + 2 * q] = addf2(yc, yd);
+            X[a + 3 * q] = cmul(subf2(yc, yd), w2);
+        }
+        __syncthreads();
+    }
+    if (len == 2) {
+        for (unsigned b = threadIdx.x; b < (n >> 1); b += blockDim.x) {
+            const unsigned i = 2 * b;
+            const float2 u = X[i], v = X[i + 1];
+            X[i] = addf2(u, v);
+            X[i + 1] = cmul(subf2(u, v), tw[0]);
+        }
+        __syncthreads();
+    }
+}
 
 // GENERAL = false: one real, even spectrum for both rows (the pair's
 // transform is simply scaled).  GENERAL = true: each row has its own
@@ -82,18 +159,7 @@ __global__ __launch_bounds__(256) void filter_rows_kernel(FilterArgs A) {
     __syncthreads();
 
     // forward DIT: natural-order spectrum
-    for (unsigned len = 2, step = half_n; len <= N; len <<= 1, step >>= 1) {
-        const unsigned h = len >> 1;
-        for (unsigned b = threadIdx.x; b < half_n; b += blockDim.x) {
-            unsigned k = b & (h - 1);
-            unsigned i = (b - k) * 2 + k;
-            float2 t = cmul(X[i + h], A.tw[k * step]);
-            float2 u = X[i];
-            X[i] = make_float2(u.x + t.x, u.y + t.y);
-            X[i + h] = make_float2(u.x - t.x, u.y - t.y);
-        }
-        __syncthreads();
-    }
+    lds_fft_dit(X, N, A.tw, 1);
 
     // x filter, conjugate (inverse via conj(FFT(conj(.))))
     if (!GENERAL) {
@@ -123,17 +189,7 @@ __global__ __launch_bounds__(256) void filter_rows_kernel(FilterArgs A) {
     __syncthreads();
 
     // DIF: bit-reversed output
-    for (unsigned len = N, step = 1; len >= 2; len >>= 1, step <<= 1) {
-        const unsigned h = len >> 1;
-        for (unsigned b = threadIdx.x; b < half_n; b += blockDim.x) {
-            unsigned k = b & (h - 1);
-            unsigned i = (b - k) * 2 + k;
-            float2 u = X[i], v = X[i + h];
-            X[i] = make_float2(u.x + v.x, u.y + v.y);
-            X[i + h] = cmul(make_float2(u.x - v.x, u.y - v.y), A.tw[k * step]);
-        }
-        __syncthreads();
-    }
+    lds_fft_dif(X, N, A.tw, 1);
 
     // y = conj(F) / N: row0 = Re F / N, row1 = -Im F / N
     const float inv = 1.0f / (float)N;
@@ -184,18 +240,7 @@ __global__ __launch_bounds__(256) void filter_row_real_kernel(FilterArgs A) {
     __syncthreads();
 
     // forward DIT over M points; exp(-2 pi i k / len) = tw[k * N / len]
-    for (unsigned len = 2, step = M; len <= M; len <<= 1, step >>= 1) {
-        const unsigned h = len >> 1;
-        for (unsigned b = threadIdx.x; b < half_m; b += blockDim.x) {
-            unsigned k = b & (h - 1);
-            unsigned i = (b - k) * 2 + k;
-            float2 t = cmul(Z[i + h], A.tw[k * step]);
-            float2 u = Z[i];
-            Z[i] = make_float2(u.x + t.x, u.y + t.y);
-            Z[i + h] = make_float2(u.x - t.x, u.y - t.y);
-        }
-        __syncthreads();
-    }
+    lds_fft_dit(Z, M, A.tw, 2);
 
     // split, filter, merge: one thread per pair (k, M - k), k = 0 .. M/2;
     // the conjugate of Z' is stored for the inverse (conj(FFT(conj .)))
@@ -237,17 +282,7 @@ __global__ __launch_bounds__(256) void filter_row_real_kernel(FilterArgs A) {
     __syncthreads();
 
     // DIF over M points: bit-reversed output
-    for (unsigned len = M, step = 2; len >= 2; len >>= 1, step <<= 1) {
-        const unsigned h = len >> 1;
-        for (unsigned b = threadIdx.x; b < half_m; b += blockDim.x) {
-            unsigned k = b & (h - 1);
-            unsigned i = (b - k) * 2 + k;
-            float2 u = Z[i], q = Z[i + h];
-            Z[i] = make_float2(u.x + q.x, u.y + q.y);
-            Z[i + h] = cmul(make_float2(u.x - q.x, u.y - q.y), A.tw[k * step]);
-        }
-        __syncthreads();
-    }
+    lds_fft_dif(Z, M, A.tw, 2);
 
     // z' = conj(result) / M: x'[2m] = Re / M, x'[2m+1] = -Im / M
     const float inv = 1.0f / (float)M;
