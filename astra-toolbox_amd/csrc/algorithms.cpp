@@ -471,54 +471,61 @@ void fdk_preweight_only(const Geom3D &g, float *proj, hipStream_t s) {
 // (src/CudaFilteredBackProjectionAlgorithm.cpp:174-239, cuda/2d/fbp.cu:77-190)
 void fbp2d_run(const Geom2D &g, float *sino, float *vol, float out_scale, const Filter *f,
                bool short_scan, int mode, hipStream_t s) {
-    std::unique_ptr<Filter> own;
     if (!f) {
-        own.reset(ramlak_filter(g.nd));
-        f = own.get();
+        std::lock_guard<std::mutex> lk(g.fbp_mu);
+        if (!g.fbp_ramlak) g.fbp_ramlak.reset(ramlak_filter(g.nd));
+        f = g.fbp_ramlak.get();
     }
     AMX_REQUIRE(f->n_det == g.nd, "FBP: filter built for %u detectors, geometry has %u",
                 f->n_det, g.nd);
     AMX_REQUIRE(f->n_spec == 1 || f->n_spec == g.na,
                 "FBP: a per-angle filter needs %u rows, got %u", g.na, f->n_spec);
     if (g.kind == ASTRA_AMD_GEOM_FAN) {
-        // one-row cone pre-weighting with getFanParameters of the last angle
-        // (cuda/2d/fbp.cu:112-148, src/GeometryUtil2D.cpp:138-177)
-        const float *p = &g.vecs[6 * (size_t)(g.na - 1)];
-        double sx = p[0], sy = p[1], dsx = p[2], dsy = p[3], ux = p[4], uy = p[5];
-        double sod = std::sqrt(sx * sx + sy * sy);
-        double du = std::sqrt(ux * ux + uy * uy);
-        double t = (-dsx * ux - dsy * uy) / (ux * ux + uy * uy);
-        double odd = std::sqrt((dsx + t * ux) * (dsx + t * ux) + (dsy + t * uy) * (dsy + t * uy));
-        double D = sod + odd;
-        double W = D * (D / (du * sod)) * (M_PI / 2.0) / (double)g.na;
-        std::vector<float> w(g.nd);
-        for (unsigned i = 0; i < g.nd; ++i) {
-            double U = (i - 0.5 * g.nd + 0.5) * du;
-            w[i] = (float)(W / std::sqrt(D * D + U * U));
+        const float *parker = nullptr;
+        {
+            std::lock_guard<std::mutex> lk(g.fbp_mu);
+            // one-row cone pre-weighting with getFanParameters of the last
+            // angle (cuda/2d/fbp.cu:112-148, src/GeometryUtil2D.cpp:138-177)
+            const float *p = &g.vecs[6 * (size_t)(g.na - 1)];
+            double sx = p[0], sy = p[1], dsx = p[2], dsy = p[3], ux = p[4], uy = p[5];
+            double sod = std::sqrt(sx * sx + sy * sy);
+            double du = std::sqrt(ux * ux + uy * uy);
+            double t = (-dsx * ux - dsy * uy) / (ux * ux + uy * uy);
+            double odd = std::sqrt((dsx + t * ux) * (dsx + t * ux) + (dsy + t * uy) * (dsy + t * uy));
+            if (!g.fbp_w.bytes()) {
+                double D = sod + odd;
+                double W = D * (D / (du * sod)) * (M_PI / 2.0) / (double)g.na;
+                std::vector<float> w(g.nd);
+                for (unsigned i = 0; i < g.nd; ++i) {
+                    double U = (i - 0.5 * g.nd + 0.5) * du;
+                    w[i] = (float)(W / std::sqrt(D * D + U * U));
+                }
+                DevArray dw(sizeof(float) * g.nd);
+                dw.upload(w.data(), sizeof(float) * g.nd);
+                g.fbp_w = std::move(dw);
+            }
+            if (short_scan && g.na > 1) {
+                if (!g.fbp_parker.bytes()) {
+                    // Parker weights with each angle's getFanParameters angle
+                    // atan2(uy, ux) and the last angle's distances (cuda/2d/fbp.cu:117-144)
+                    std::vector<double> ang(g.na);
+                    for (unsigned a = 0; a < g.na; ++a)
+                        ang[a] = std::atan2((double)g.vecs[6 * a + 5], (double)g.vecs[6 * a + 4]);
+                    std::vector<float> pw;
+                    parker_weights(ang, g.nd, sod, odd, du, pw);
+                    DevArray dp(sizeof(float) * pw.size());
+                    dp.upload(pw.data(), sizeof(float) * pw.size());
+                    g.fbp_parker = std::move(dp);
+                }
+                parker = g.fbp_parker.as<float>();
+            }
         }
-        DevArray dw(sizeof(float) * g.nd);
-        dw.upload(w.data(), sizeof(float) * g.nd);
-        DevArray dp;
-        if (short_scan && g.na > 1) {
-            // Parker weights with each angle's getFanParameters angle
-            // atan2(uy, ux) and the last angle's distances (cuda/2d/fbp.cu:117-144)
-            std::vector<double> ang(g.na);
-            for (unsigned a = 0; a < g.na; ++a)
-                ang[a] = std::atan2((double)g.vecs[6 * a + 5], (double)g.vecs[6 * a + 4]);
-            std::vector<float> pw;
-            parker_weights(ang, g.nd, sod, odd, du, pw);
-            dp = DevArray(sizeof(float) * pw.size());
-            dp.upload(pw.data(), sizeof(float) * pw.size());
-        }
-        filter_rows_launch(*f, sino, g.na, dw.as<float>(), g.na, s,
-                           dp.bytes() ? dp.as<float>() : nullptr, g.na);
+        filter_rows_launch(*f, sino, g.na, g.fbp_w.as<float>(), g.na, s, parker, g.na);
         bp2d_launch(g, sino, vol, out_scale, true, mode, nullptr, -INFINITY, INFINITY, s);
-        AMX_HIP(hipStreamSynchronize(s));  // dw and own are released on return
     } else {
         filter_rows_launch(*f, sino, g.na, nullptr, 1, s);
         bp2d_launch(g, sino, vol, out_scale * (float)((M_PI / 2.0) / (double)g.na), false, mode,
                     nullptr, -INFINITY, INFINITY, s);
-        if (own) AMX_HIP(hipStreamSynchronize(s));
     }
 }
 

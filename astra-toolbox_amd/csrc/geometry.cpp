@@ -5,6 +5,7 @@
 // routines; they are computed once per geometry handle (not once per call as
 // in the reference) and kept in device memory.
 #include <cmath>
+#include <complex>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -639,26 +640,47 @@ void parker_weights(const std::vector<double> &angles, unsigned nu, double sod, 
     }
 }
 
+// In-place radix-2 FFT of a power-of-two length (FP64, host; forward sign).
+static void host_fft(std::vector<std::complex<double>> &a) {
+    const size_t n = a.size();
+    for (size_t i = 1, j = 0; i < n; ++i) {  // bit-reversal permutation
+        size_t bit = n >> 1;
+        for (; j & bit; bit >>= 1) j ^= bit;
+        j ^= bit;
+        if (i < j) std::swap(a[i], a[j]);
+    }
+    for (size_t len = 2; len <= n; len <<= 1) {
+        const double ang = -2.0 * M_PI / (double)len;
+        for (size_t i = 0; i < n; i += len) {
+            for (size_t k = 0; k < len / 2; ++k) {
+                const std::complex<double> w(std::cos(ang * (double)k), std::sin(ang * (double)k));
+                const std::complex<double> u = a[i + k], t = a[i + k + len / 2] * w;
+                a[i + k] = u + t;
+                a[i + k + len / 2] = u - t;
+            }
+        }
+    }
+}
+
 void ramlak_half_spectrum(unsigned n_det, double d, std::vector<double> &out) {
     // genFilter(FILTER_RAMLAK) (src/Filters.cpp:133-160, 473-485) with the
     // spatial kernel of generateRampFilter (src/Filters.cpp:41-68):
-    // h[0] = 1/4, h[j odd] = -1/(pi min(j, N-j))^2.  h is real and even, so
-    // its DFT is real: H[k] = 1/4 + sum_{odd j} h[j] cos(2 pi j k / N).
-    // Filter = 2 H[k].
+    // h[0] = 1/4, h[j odd] = -1/(pi min(j, N-j))^2, h[j even] = 0.  h is real
+    // and even, so its DFT is real; Filter = 2 Re DFT(h)[k], k = 0 .. N/2,
+    // zeroed where 2 pi k / N > pi d.  The DFT is an FP64 FFT.
     unsigned n = next_pow2(2 * n_det);
     unsigned nb = n / 2 + 1;
+    std::vector<std::complex<double>> h(n, 0.0);
+    h[0] = 0.25;
+    for (unsigned j = 1; j < n; j += 2) {
+        const double jj = (2 * j > n) ? (double)(n - j) : (double)j;
+        h[j] = -1.0 / ((M_PI * jj) * (M_PI * jj));
+    }
+    host_fft(h);
     out.assign(nb, 0.0);
-    std::vector<double> cs(n);
-    for (unsigned j = 0; j < n; ++j) cs[j] = std::cos(2.0 * M_PI * (double)j / (double)n);
     for (unsigned k = 0; k < nb; ++k) {
-        double acc = 0.25;
-        for (unsigned j = 1; j < n; j += 2) {
-            unsigned jj = (2 * j > n) ? n - j : j;
-            double h = -1.0 / ((M_PI * jj) * (M_PI * jj));
-            acc += h * cs[(size_t)((uint64_t)j * k % n)];
-        }
-        double wv = 2.0 * M_PI * (double)k / (double)n;
-        out[k] = (wv > M_PI * d) ? 0.0 : 2.0 * acc;
+        const double wv = 2.0 * M_PI * (double)k / (double)n;
+        out[k] = (wv > M_PI * d) ? 0.0 : 2.0 * h[k].real();
     }
 }
 

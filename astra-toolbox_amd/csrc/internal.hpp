@@ -180,6 +180,12 @@ struct Geom2D {
     std::vector<float> vecs;  // 6 per angle
     DevArray fp;              // FP2DAngle[na]
     DevArray bp, bp_fbp;      // BP2DAngle[na]
+    // FBP state built on first use: the default Ram-Lak filter, the fan
+    // pre-weights [nd] and short-scan Parker weights [na][nd]
+    mutable std::mutex fbp_mu;
+    mutable std::unique_ptr<Filter> fbp_ramlak;
+    mutable DevArray fbp_w;
+    mutable DevArray fbp_parker;
 };
 
 // ------------------------------------------------------ profiling --------
