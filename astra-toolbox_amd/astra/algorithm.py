@@ -193,26 +193,25 @@ class FBP_CUDA(_Algo):
         super().__init__(cfg)
         self.short_scan = 1 if _truthy(_opt(cfg, "ShortScan", False)) else 0
         self.filter = filter_config(cfg)
+        self._filters = {}
 
     def run(self, iterations=1):
         dev = self.proj.device()
-        nd = self.proj.proj_shape[1]
         px, py = G.pixel_lengths(self.vol.geom)
         scale = float(self.proj.out_scale) / float(px * py)
         s = self.sino.on_device(dev).clone()
         v = self.vol.on_device(dev, init=False)
         h = ctypes.c_void_p(self.proj.handle(dev))
         with torch.cuda.device(dev):
-            f = _filter_handle(nd, self.proj.proj_shape[0], self.filter)
-            try:
-                _lib.call("astra_amd_fbp2d", h, _core.ptr(s), _core.ptr(v), scale, self.short_scan,
-                          f, _lib.MODE_SET, _core.stream_ptr(dev))
-                _core.sync(dev)
-            finally:
-                if f is not None:
-                    _lib.call("astra_amd_filter_destroy", f)
+            f = _cached_filter(self._filters, dev, self.proj.proj_shape[1], self.proj.proj_shape[0],
+                               self.filter)
+            _lib.call("astra_amd_fbp2d", h, _core.ptr(s), _core.ptr(v), scale, self.short_scan,
+                      f, _lib.MODE_SET, _core.stream_ptr(dev))
         self.vol.write_back(v)
         _core.sync(dev)
+
+    def release(self):
+        _release_filters(self._filters)
 
 
 class FDK_CUDA(_Algo):
@@ -224,14 +223,41 @@ class FDK_CUDA(_Algo):
         super().__init__(cfg)
         self.short_scan = _truthy(_opt(cfg, "ShortScan", False))
         self.filter = filter_config(cfg)
+        self._filters = {}
         if self.proj.kind != "cone":
             raise AstraError("FDK_CUDA requires a cone geometry")
 
     def run(self, iterations=1, mode=_lib.MODE_SET):
         nv, na, nu = self.proj.proj_shape
-        flt = self.filter
-        composite.fdk(self.proj, self.sino, self.vol, mode,
-                      lambda: None if flt is None else _filter_handle(nu, na, flt), self.short_scan)
+        dev = self.proj.device()
+        with torch.cuda.device(dev):
+            f = _cached_filter(self._filters, dev, nu, na, self.filter)
+        composite.fdk(self.proj, self.sino, self.vol, mode, f, self.short_scan)
+
+    def release(self):
+        _release_filters(self._filters)
+
+
+def _cached_filter(cache, dev, n_det, n_angles, fc):
+    """The algorithm's native filter on `dev`, built on first use (None:
+    the library's default Ram-Lak)."""
+    if fc is None:
+        return None
+    if dev.index not in cache:
+        cache[dev.index] = _filter_handle(n_det, n_angles, fc)
+    return cache[dev.index]
+
+
+def _release_filters(cache):
+    for dev, f in list(cache.items()):
+        if f is not None:
+            try:
+                with torch.cuda.device(dev):
+                    _core.sync(torch.device("cuda", dev))
+                    _lib.call("astra_amd_filter_destroy", f)
+            except Exception:
+                pass
+    cache.clear()
 
 
 def fdk(proj, p, v, mode, filter=None, short_scan=False):

@@ -317,9 +317,11 @@ def bp(p, dproj, dvol, mode, kernel=None):
         _core.sync(dev)
 
 
-def fdk(p, dproj, dvol, mode, filter_handle_fn, short_scan):
+def fdk(p, dproj, dvol, mode, filter_handle, short_scan):
     """dvol (=|+=) FDK(dproj); dproj is left unchanged.  The volume is split
-    over z when it does not fit beside the filtered projections."""
+    over z when it does not fit beside the filtered projections.
+    `filter_handle` is a native filter owned by the caller (None: the
+    library's Ram-Lak)."""
     dev = p.device()
     vol_host = _host_array(dvol)
     pt = dproj.on_device(dev).clone()
@@ -333,14 +335,9 @@ def fdk(p, dproj, dvol, mode, filter_handle_fn, short_scan):
             raise AstraError("FDK job does not fit in %d bytes of GPU memory" % cap)
         slabs = _chunks(nz, (cap - fixed) // (nx * ny * 4))
     with torch.cuda.device(dev):
-        f = filter_handle_fn()
-        try:
-            h = ctypes.c_void_p(p.handle(dev))
-            _lib.call("astra_amd_fdk_filter", h, _core.ptr(pt), 1 if short_scan else 0, f,
-                      _core.stream_ptr(dev))
-        finally:
-            if f is not None:
-                _lib.call("astra_amd_filter_destroy", f)
+        h = ctypes.c_void_p(p.handle(dev))
+        _lib.call("astra_amd_fdk_filter", h, _core.ptr(pt), 1 if short_scan else 0, filter_handle,
+                  _core.stream_ptr(dev))
     for z0, z1 in slabs:
         whole = (z0, z1) == (0, nz)
         sp = p if whole else _sub_projector(p, p.proj_geom, vol_slab_geom(p.vol_geom, z0, z1))

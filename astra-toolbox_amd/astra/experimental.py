@@ -138,4 +138,4 @@ def accumulate_FDK(projector_id, vol_id, proj_id):
     MODE_ADD); the projection data object is left unchanged."""
     p = _p3.get(projector_id)
     dv, dp = _d3.get(vol_id), _d3.get(proj_id)
-    composite.fdk(p, dp, dv, MODE_ADD, lambda: None, False)
+    composite.fdk(p, dp, dv, MODE_ADD, None, False)
