@@ -15,7 +15,7 @@ LIB_PATH = os.environ.get(
     "ASTRA_AMD_LIB", os.path.join(os.path.dirname(_HERE), "lib", "libastra_amd.so"))
 
 OK = 0
-ERR_ARG, ERR_HIP, ERR_UNSUPPORTED, ERR_ALLOC, ERR_INTERNAL = 1, 2, 3, 4, 5
+ERR_ARG, ERR_HIP, ERR_UNSUPPORTED, ERR_ALLOC, ERR_INTERNAL, ERR_ABORTED = 1, 2, 3, 4, 5, 6
 MODE_ADD, MODE_SET = 0, 1
 GEOM_PAR, GEOM_FAN, GEOM_PAR3D, GEOM_CONE, GEOM_CYLCONE = 0, 1, 2, 3, 4
 KER3D_DEFAULT, KER3D_SUM_SQUARE_WEIGHTS, KER3D_FDK_WEIGHTING, KER3D_2D_WEIGHTING, KER3D_MATCHED_BP = range(5)
@@ -38,6 +38,9 @@ _i = ctypes.c_int
 _f = ctypes.c_float
 _d = ctypes.c_double
 _sz = ctypes.c_size_t
+
+# int (*)(void *user): non-zero stops an iterative run
+ABORT_HOOK = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p)
 
 # name -> argtypes (all functions return int status, except the two strings)
 SIGNATURES = {
@@ -74,6 +77,7 @@ SIGNATURES = {
     "astra_amd_algo_set_mask": [_vp, ctypes.c_char_p, _vp],
     "astra_amd_algo_set_order": [_vp, _vp, _u],
     "astra_amd_algo_run": [_vp, _vp, _vp, _i, _vp],
+    "astra_amd_set_abort_hook": [ABORT_HOOK, _vp],
     "astra_amd_algo_residual_norm": [_vp, _vp, _vp, ctypes.POINTER(_d), _vp],
     "astra_amd_algo_destroy": [_vp],
     "astra_amd_batch2d_create": [_u, _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(_vp)],
@@ -123,13 +127,18 @@ def load():
         return lib
 
 
+class Aborted(AstraError):
+    """An iterative run stopped by the abort hook (status ERR_ABORTED)."""
+
+
 def call(name, *args):
     """Call an entry point and turn a non-zero status into AstraError."""
     lib = load()
     st = getattr(lib, name)(*args)
     if st != OK:
         msg = lib.astra_amd_last_error().decode(errors="replace")
-        raise AstraError("%s failed (status %d): %s" % (name, st, msg))
+        cls = Aborted if st == ERR_ABORTED else AstraError
+        raise cls("%s failed (status %d): %s" % (name, st, msg))
     return st
 
 

@@ -5,8 +5,11 @@ BP_CUDA, FBP_CUDA, SIRT_CUDA, CGLS_CUDA, FP3D_CUDA, BP3D_CUDA, FDK_CUDA,
 SIRT3D_CUDA, CGLS3D_CUDA.  Each mirrors the configuration keys of the
 reference algorithm class and runs through libastra_amd on one GPU.
 """
+import contextlib
 import ctypes
 import math
+import signal
+import threading
 
 import numpy as np
 
@@ -21,6 +24,49 @@ try:
     import torch
 except ImportError:  # pragma: no cover
     torch = None
+
+
+# Ctrl-C during an iterative run.  The reference checks shouldAbort() before
+# every iteration (src/CudaSirtAlgorithm.cpp:347) and its MATLAB interface
+# points the hook at the interpreter's interrupt flag
+# (matlab/mex/astra_mex_algorithm_c.cpp:133).  Here a SIGINT handler,
+# installed for the run's duration on the main thread, sets a flag that the
+# native abort hook reads before each iteration; the run then stops, the
+# iterate reached is written back and KeyboardInterrupt is raised.
+_stop = threading.Event()
+
+
+def _should_abort():
+    return _stop.is_set()
+
+
+def _hook(user):
+    try:
+        return 1 if _should_abort() else 0
+    except BaseException:
+        return 1
+
+
+_native_hook = _lib.ABORT_HOOK(_hook)
+_no_hook = _lib.ABORT_HOOK()  # NULL
+
+
+@contextlib.contextmanager
+def _interruptible():
+    """Let SIGINT stop a native iterative run between iterations (main
+    thread only; a handler installed outside Python is left alone)."""
+    if threading.current_thread() is not threading.main_thread() or \
+            signal.getsignal(signal.SIGINT) is None:
+        yield
+        return
+    _stop.clear()
+    prev = signal.signal(signal.SIGINT, lambda signum, frame: _stop.set())
+    try:
+        _lib.call("astra_amd_set_abort_hook", _native_hook, None)
+        yield
+    finally:
+        _lib.call("astra_amd_set_abort_hook", _no_hook, None)
+        signal.signal(signal.SIGINT, prev)
 
 
 def _opt(cfg, key, default=None):
@@ -341,8 +387,14 @@ class _Iterative(_Algo):
             self._bind_masks(h, dev)
             b = self.sino.on_device(dev)
             x = self.vol.on_device(dev)
-            _lib.call("astra_amd_algo_run", h, _core.ptr(b), _core.ptr(x), int(iterations),
-                      _core.stream_ptr(dev))
+            try:
+                with _interruptible():
+                    _lib.call("astra_amd_algo_run", h, _core.ptr(b), _core.ptr(x),
+                              int(iterations), _core.stream_ptr(dev))
+            except _lib.Aborted as e:
+                self.vol.write_back(x)
+                _core.sync(dev)
+                raise KeyboardInterrupt(str(e)) from None
             self.vol.write_back(x)
             _core.sync(dev)
 

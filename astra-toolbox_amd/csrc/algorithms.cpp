@@ -8,6 +8,7 @@
 // SIRT update (pixel weights, relaxation, clamping) in its epilogue.  One
 // SIRT iteration is therefore two projector launches plus the forward
 // projector's input copy; no host synchronisation happens inside run().
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <limits>
@@ -60,6 +61,42 @@ struct Op {
             proj_to_padded_am(*g3, proj, pp, s);
         else
             sino_to_padded(*g2, proj, pp, s);
+    }
+};
+
+// The abort hook (setShouldAbortHook) and the per-run gate that asks it:
+// before iteration it, wait for iteration it - kDepth to finish on the GPU
+// (so a stop request lands within kDepth iterations of GPU work), then ask.
+std::atomic<astra_amd_abort_hook> g_abort_hook{nullptr};
+std::atomic<void *> g_abort_user{nullptr};
+
+struct AbortGate {
+    static constexpr int kDepth = 3;
+    hipStream_t s;
+    astra_amd_abort_hook fn;
+    void *user;
+    hipEvent_t ev[kDepth] = {};
+    explicit AbortGate(hipStream_t s_) : s(s_), fn(g_abort_hook.load()), user(g_abort_user.load()) {
+        if (fn)
+            for (auto &e : ev) AMX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+    ~AbortGate() {
+        for (auto &e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+    AbortGate(const AbortGate &) = delete;
+    AbortGate &operator=(const AbortGate &) = delete;
+    // true: stop before iteration it
+    bool stop(int it) {
+        if (!fn) return false;
+        if (it >= kDepth) AMX_HIP(hipEventSynchronize(ev[it % kDepth]));
+        return fn(user) != 0;
+    }
+    void done(int it) {
+        if (fn) AMX_HIP(hipEventRecord(ev[it % kDepth], s));
+    }
+    [[noreturn]] void aborted(int it) const {
+        fail(ASTRA_AMD_ERR_ABORTED, "run aborted by the abort hook after %d iterations", it);
     }
 };
 
@@ -153,7 +190,9 @@ struct Sirt : AlgoBase {
             weights_ready = true;
         }
         const size_t nv = op.nvol();
+        AbortGate gate(s);
         for (int it = 0; it < iters; ++it) {
+            if (gate.stop(it)) gate.aborted(it);
             const float *xin = x;
             if (recon_mask) {
                 copy_launch(t.as<float>(), x, nv, s);
@@ -166,6 +205,7 @@ struct Sirt : AlgoBase {
             op.bp(r.as<float>(), x, 1.f, OUT_UPDATE, pixel_w.as<float>(), lo, hi, s);
             if (min_mask || max_mask)
                 clamp_launch(x, nv, -INFINITY, INFINITY, min_mask, max_mask, s);
+            gate.done(it);
         }
     }
 };
@@ -189,7 +229,9 @@ struct Cgls : AlgoBase {
         op.bp(r.as<float>(), p.as<float>(), 1.f, OUT_SET, nullptr, lo, hi, s);
         if (recon_mask) mul_launch(p.as<float>(), recon_mask, nv, s);
         dot_launch(p.as<float>(), p.as<float>(), nv, g0, s);
+        AbortGate gate(s);
         for (int it = 0; it < iters; ++it) {
+            if (gate.stop(it)) gate.aborted(it);
             // w = A p ; alpha = gamma / <w, w>
             op.fp(p.as<float>(), w.as<float>(), 1.f, OUT_SET, nullptr, nullptr, s);
             dot_launch(w.as<float>(), w.as<float>(), np, ww, s);
@@ -201,6 +243,7 @@ struct Cgls : AlgoBase {
             dot_launch(z.as<float>(), z.as<float>(), nv, g1, s);
             xpby_ratio_launch(p.as<float>(), z.as<float>(), nv, g1, g0, s);
             std::swap(g0, g1);
+            gate.done(it);
         }
     }
 };
@@ -221,9 +264,12 @@ struct Em : AlgoBase {
             invert_launch(pixel_w.as<float>(), op.nvol(), 1.f, s);
             weights_ready = true;
         }
+        AbortGate gate(s);
         for (int it = 0; it < iters; ++it) {
+            if (gate.stop(it)) gate.aborted(it);
             op.fp(x, r.as<float>(), 1.f, OUT_RATIO, b, nullptr, s);
             op.bp(r.as<float>(), x, 1.f, OUT_EMUPD, pixel_w.as<float>(), lo, hi, s);
+            gate.done(it);
         }
     }
 };
@@ -258,7 +304,9 @@ struct Sart : AlgoBase {
         const Geom2D &g = *op.g2;
         const size_t nv = op.nvol();
         const size_t pitch = g.nd + 2 * 2;
+        AbortGate gate(s);
         for (int it = 0; it < iters; ++it, ++iteration) {
+            if (gate.stop(it)) gate.aborted(it);
             const int a = order.empty() ? (int)(iteration % g.na)
                                         : order[(size_t)(iteration % (long long)order.size())];
             const float *xin = x;
@@ -273,6 +321,7 @@ struct Sart : AlgoBase {
                              op.out_scale * (float)relaxation, recon_mask, lo, hi, s);
             if (min_mask || max_mask)
                 clamp_launch(x, nv, -INFINITY, INFINITY, min_mask, max_mask, s);
+            gate.done(it);
         }
     }
 };
@@ -339,6 +388,11 @@ void algo_set_mask(AlgoBase *a, const char *key, const float *m) {
     else if (k == "MaxMask") a->max_mask = m;
     else fail(ASTRA_AMD_ERR_ARG, "algorithm: unknown mask '%s'", k.c_str());
     a->weights_ready = false;
+}
+
+void set_abort_hook(astra_amd_abort_hook fn, void *user) {
+    g_abort_user.store(user);
+    g_abort_hook.store(fn);
 }
 
 void algo_run(AlgoBase *a, const float *b, float *x, int iters, hipStream_t s) {

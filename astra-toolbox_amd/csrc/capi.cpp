@@ -20,6 +20,7 @@ void algo_set_mask(AlgoBase *a, const char *key, const float *m);
 void algo_set_order(AlgoBase *a, const int *order, unsigned n);
 void algo_run(AlgoBase *a, const float *b, float *x, int iters, hipStream_t s);
 double algo_residual(AlgoBase *a, const float *b, const float *x, hipStream_t s);
+void set_abort_hook(astra_amd_abort_hook fn, void *user);
 Filter *filter_build(unsigned n_det, const float *half, unsigned n_bins);
 void fdk_run(const Geom3D &g, float *proj, float *vol, float out_scale, const Filter *f,
              bool short_scan, int mode, hipStream_t s);
@@ -544,6 +545,10 @@ int astra_amd_algo_run(astra_amd_algo a, const float *sino, float *vol, int iter
         check_device(a->device);
         amx::algo_run(a->a, sino, vol, iterations, as_stream(stream));
     });
+}
+
+int astra_amd_set_abort_hook(astra_amd_abort_hook fn, void *user) {
+    return guarded([&] { amx::set_abort_hook(fn, user); });
 }
 
 int astra_amd_algo_residual_norm(astra_amd_algo a, const float *sino, const float *vol,

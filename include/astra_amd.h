@@ -36,6 +36,7 @@ extern "C" {
 #define ASTRA_AMD_ERR_UNSUPPORTED 3 /* valid in ASTRA, not built here      */
 #define ASTRA_AMD_ERR_ALLOC 4       /* device allocation failed            */
 #define ASTRA_AMD_ERR_INTERNAL 5
+#define ASTRA_AMD_ERR_ABORTED 6     /* the abort hook stopped a run        */
 
 /* Write modes: CCompositeGeometryManager::EJobMode
  * (include/astra/CompositeGeometryManager.h:127-130). */
@@ -321,9 +322,19 @@ int astra_amd_algo_set_mask(astra_amd_algo a, const char *key, const float *mask
  * the angle of iteration k is order[k % n]; n = 0 restores 0, 1, 2, ...
  * Host array, copied. */
 int astra_amd_algo_set_order(astra_amd_algo a, const int *order, unsigned int n);
-/* run(iterations): vol is read as the initial iterate and overwritten. */
+/* run(iterations): vol is read as the initial iterate and overwritten.
+ * With an abort hook set, the hook is asked before every iteration and a
+ * non-zero answer stops the run with ASTRA_AMD_ERR_ABORTED, vol holding the
+ * iterate reached (at most 3 iterations are then queued on the GPU ahead
+ * of the host). */
 int astra_amd_algo_run(astra_amd_algo a, const float *sino, float *vol, int iterations,
                        void *stream);
+/* Process-wide abort hook of the iterative algorithms, replacing
+ * setShouldAbortHook / shouldAbort (include/astra/Globals.h:156-164; the
+ * reference asks it once per iteration, src/CudaSirtAlgorithm.cpp:347).
+ * fn = NULL removes it. */
+typedef int (*astra_amd_abort_hook)(void *user);
+int astra_amd_set_abort_hook(astra_amd_abort_hook fn, void *user);
 /* ||b - A x|| (getResidualNorm). Synchronous. */
 int astra_amd_algo_residual_norm(astra_amd_algo a, const float *sino, const float *vol,
                                  double *out, void *stream);

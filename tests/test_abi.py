@@ -68,3 +68,18 @@ def test_filter_spectra_match_the_oracle():
                                        err_msg=name)
     with pytest.raises(Exception):
         _lib.call("astra_amd_filter_spectrum", 99, 10, 1.0, -1.0, None, 0)
+
+
+def test_abort_hook_set_and_cleared():
+    """astra_amd_set_abort_hook is host-only: a hook and NULL are accepted;
+    the Python layer installs its hook only for the duration of a run."""
+    import signal
+    from astra import _lib, algorithm
+    _lib.call("astra_amd_set_abort_hook", algorithm._native_hook, None)
+    _lib.call("astra_amd_set_abort_hook", algorithm._no_hook, None)
+    before = signal.getsignal(signal.SIGINT)
+    with algorithm._interruptible():
+        assert signal.getsignal(signal.SIGINT) is not before
+        signal.raise_signal(signal.SIGINT)  # sets the stop flag, raises nothing
+        assert algorithm._hook(None) == 1
+    assert signal.getsignal(signal.SIGINT) is before
