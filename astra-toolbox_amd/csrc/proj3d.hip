@@ -388,6 +388,30 @@ struct BPArgs {
     const float *pre; // bp3d_kernel: BP of earlier angle chunks, added before the epilogue (or NULL)
 };
 
+// Epilogue of a column of ZS voxels: scale, add the earlier chunks, write
+// in the launch's mode.
+template <int ZS>
+__device__ __forceinline__ void bp_column_store(const BPArgs &A, int X, int Y, int Z0,
+                                                const float (&acc)[ZS]) {
+    const int nzr = min(ZS, (int)A.nz - Z0);
+    for (int i = 0; i < ZS; ++i) {
+        if (i >= nzr) break;
+        size_t idx = ((size_t)(Z0 + i) * A.ny + Y) * A.nx + X;
+        float val = acc[i] * A.osc;
+        if (A.pre) val += A.pre[idx];
+        if (A.mode == OUT_ADD) {
+            A.vol[idx] += val;
+        } else if (A.mode == OUT_SET) {
+            A.vol[idx] = val;
+        } else if (A.mode == OUT_EMUPD) {
+            A.vol[idx] *= val * A.uw[idx];
+        } else {  // OUT_UPDATE: clamp(x + w * val)
+            float w = A.uw ? A.uw[idx] : 1.f;
+            A.vol[idx] = fminf(fmaxf(fmaf(w, val, A.vol[idx]), A.lo), A.hi);
+        }
+    }
+}
+
 // KIND: 0 cone fr^2 (default / FDK), 1 cone fr (2D weighting), 2 par3d.
 // One thread per (x, y) column of ZS voxels; 16 x 16 threads per group.
 // The angle loop is uniform, so the per-angle coefficients are scalar loads
@@ -451,23 +475,174 @@ __global__ __launch_bounds__(256) void bp3d_kernel(BPArgs A) {
             }
         }
     }
-    const int nzr = min(ZS, (int)A.nz - Z0);
-    for (int i = 0; i < ZS; ++i) {
-        if (i >= nzr) break;
-        size_t idx = ((size_t)(Z0 + i) * A.ny + Y) * A.nx + X;
-        float val = acc[i] * A.osc;
-        if (A.pre) val += A.pre[idx];
-        if (A.mode == OUT_ADD) {
-            A.vol[idx] += val;
-        } else if (A.mode == OUT_SET) {
-            A.vol[idx] = val;
-        } else if (A.mode == OUT_EMUPD) {
-            A.vol[idx] *= val * A.uw[idx];
-        } else {  // OUT_UPDATE: clamp(x + w * val)
-            float w = A.uw ? A.uw[idx] : 1.f;
-            A.vol[idx] = fminf(fmaxf(fmaf(w, val, A.vol[idx]), A.lo), A.hi);
+    bp_column_store<ZS>(A, X, Y, Z0, acc);
+}
+
+// ---- BP with the detector footprint staged in LDS -----------------------
+// The z-constant BP above reads one 16-byte tap pair per voxel and angle
+// through the texture-address path, which bounds it.  Here each workgroup
+// (16 x 16 columns of ZS voxels) stages, per angle, the rectangle of the
+// pair plane that its block projects into (cooperative, coalesced 8-byte
+// loads issued one angle ahead) and takes the taps from LDS.  The footprint
+// is the bounding box of the projected corners of the block (u and v are
+// linear-fractional in the voxel position, so the extremes over the block
+// are at its corners; the clamp to the padded range is monotone), widened
+// by one texel for fp32 rounding.  Taps, weights and the order of the sums
+// are those of bp3d_kernel<KIND, ZS, true>, so both give the same floats.
+// An angle whose footprint exceeds the tile, or whose denominator changes
+// sign over the block, is sampled from global memory as before.
+constexpr int kTileCap = 2048;  // pairs per staged tile (16 KiB)
+constexpr int kFpGroup = 256;   // angles whose footprints are held at once
+constexpr int kTileSlots = kTileCap / 256;
+
+template <int KIND>
+__device__ int4 bp_footprint(const BP3DAngle &c, float x0, float x1, float y0, float y1, float z0,
+                             float z1, float hiu, float hiv, int pitch, int prow) {
+    float umin = INFINITY, umax = -INFINITY, vmin = INFINITY, vmax = -INFINITY;
+    float dmin = INFINITY, dmax = -INFINITY;
+    for (int j = 0; j < 4; ++j) {
+        const float x = (j & 1) ? x1 : x0, y = (j & 2) ? y1 : y0;
+        const float un = fmaf(y, c.nu[1], fmaf(x, c.nu[0], c.nu[3]));
+        const float dn = fmaf(y, c.dn[1], fmaf(x, c.dn[0], c.dn[3]));
+        dmin = fminf(dmin, dn);
+        dmax = fmaxf(dmax, dn);
+        const float fr = (KIND == 2) ? 1.f : 1.f / dn;
+        const float pu = clamp_pad(un * fr, hiu);
+        umin = fminf(umin, pu);
+        umax = fmaxf(umax, pu);
+        for (int k = 0; k < 2; ++k) {
+            const float z = k ? z1 : z0;
+            const float vn = fmaf(z, c.nv[2], fmaf(y, c.nv[1], fmaf(x, c.nv[0], c.nv[3])));
+            const float pv = clamp_pad(vn * fr, hiv);
+            vmin = fminf(vmin, pv);
+            vmax = fmaxf(vmax, pv);
         }
     }
+    const bool sign_ok = (KIND == 2) || dmin > 0.f || dmax < 0.f;
+    const bool finite = __builtin_isfinite(umin) && __builtin_isfinite(umax) &&
+                        __builtin_isfinite(vmin) && __builtin_isfinite(vmax);
+    if (!sign_ok || !finite) return make_int4(0, 0, 0, 0);
+    // taps: columns floor(pu) and floor(pu) + 1, pair rows floor(pv); one
+    // texel of margin on each side
+    const int u0 = max(0, (int)umin - 1), u1 = min(pitch - 1, (int)umax + 2);
+    const int v0 = max(0, (int)vmin - 1), v1 = min(prow - 1, (int)vmax + 1);
+    const int w = u1 - u0 + 1, h = v1 - v0 + 1;
+    if (w > 1024 || w * h > kTileCap) return make_int4(0, 0, 0, 0);
+    return make_int4(u0, v0, w, h);
+}
+
+template <int KIND, int ZS>
+__global__ __launch_bounds__(256) void bp3d_lds_kernel(BPArgs A) {
+    __shared__ float2 tile[2][kTileCap];
+    __shared__ int4 fps[kFpGroup];
+    const unsigned tid = threadIdx.x;
+    const unsigned tiles_x = (A.nx + 15) / 16;
+    const int X0 = (blockIdx.x % tiles_x) * 16, Y0 = (blockIdx.x / tiles_x) * 16;
+    const int X = X0 + (int)(tid & 15), Y = Y0 + (int)(tid >> 4);
+    const int Z0 = blockIdx.y * ZS;
+    const bool active = X < (int)A.nx && Y < (int)A.ny;
+    const float hx = 0.5f * A.nx - 0.5f, hy = 0.5f * A.ny - 0.5f, hz = 0.5f * A.nz - 0.5f;
+    const float fx = X - hx, fy = Y - hy, fz = Z0 - hz;
+    // corners of the block's voxel centres (the columns inside the volume,
+    // all ZS slices: the slices past nz are computed and discarded)
+    const float cx0 = X0 - hx, cx1 = min(X0 + 15, (int)A.nx - 1) - hx;
+    const float cy0 = Y0 - hy, cy1 = min(Y0 + 15, (int)A.ny - 1) - hy;
+    const float cz0 = fz, cz1 = (Z0 + ZS - 1) - hz;
+    const unsigned pitch = A.nu + 2 * kPad;                    // pairs per row
+    const unsigned prow = A.nv + 2 * kPad;                     // rows per angle
+    const size_t plane = (size_t)prow * pitch;                 // pairs per angle
+    const unsigned rowb = pitch * 8u;
+    const float hiu = (float)A.nu + 2.5f, hiv = (float)A.nv + 2.5f;
+    const float2 *pp2 = reinterpret_cast<const float2 *>(A.pp);
+    float acc[ZS];
+#pragma unroll
+    for (int i = 0; i < ZS; ++i) acc[i] = 0.f;
+    float2 pre[kTileSlots];
+
+    auto fetch = [&](unsigned a, int4 f) {
+        const int n = f.z * f.w;
+        const float invw = 1.f / (float)f.z;
+        const float2 *src = pp2 + (size_t)a * plane + (size_t)f.y * pitch + f.x;
+#pragma unroll
+        for (int k = 0; k < kTileSlots; ++k) {
+            const int i = (int)tid + 256 * k;
+            if (i < n) {
+                // (i + 0.5) / w is at least 0.5 / w from an integer, far
+                // beyond the fp32 error for w <= 1024, i < 2048
+                const int r = (int)(((float)i + 0.5f) * invw);
+                pre[k] = src[(size_t)r * pitch + (i - r * f.z)];
+            }
+        }
+    };
+
+    for (unsigned g0 = 0; g0 < A.na; g0 += kFpGroup) {
+        const unsigned gn = min((unsigned)kFpGroup, A.na - g0);
+        __syncthreads();  // the previous group's footprints are no longer read
+        if (tid < gn)
+            fps[tid] = bp_footprint<KIND>(A.ang[g0 + tid], cx0, cx1, cy0, cy1, cz0, cz1, hiu, hiv,
+                                          (int)pitch, (int)prow);
+        __syncthreads();
+        if (fps[0].z) fetch(g0, fps[0]);
+        for (unsigned k = 0; k < gn; ++k) {
+            const unsigned a = g0 + k;
+            const int4 f = fps[k];
+            float2 *t = tile[a & 1];
+            const int n = f.z * f.w;
+            if (f.z) {
+#pragma unroll
+                for (int j = 0; j < kTileSlots; ++j) {
+                    const int i = (int)tid + 256 * j;
+                    if (i < n) t[i] = pre[j];
+                }
+            }
+            // tile[a & 1] is complete; tile[(a + 1) & 1] was last read at
+            // angle a - 1, before every thread reached this barrier
+            __syncthreads();
+            if (k + 1 < gn) {
+                const int4 fn = fps[k + 1];
+                if (fn.z) fetch(a + 1, fn);
+            }
+            if (!active) continue;
+            const BP3DAngle c = A.ang[a];
+            const float un = fmaf(fz, c.nu[2], fmaf(fy, c.nu[1], fmaf(fx, c.nu[0], c.nu[3])));
+            float vn = fmaf(fz, c.nv[2], fmaf(fy, c.nv[1], fmaf(fx, c.nv[0], c.nv[3])));
+            const float dn = fmaf(fz, c.dn[2], fmaf(fy, c.dn[1], fmaf(fx, c.dn[0], c.dn[3])));
+            const float wa = (KIND == 2) ? A.aw[a] : 1.f;
+            const float fr = (KIND == 2) ? 1.f : __builtin_amdgcn_rcpf(dn);
+            const float pu = clamp_pad(un * fr, hiu);
+            const float wu = frac(pu);
+            const float wgt = (KIND == 0) ? fr * fr : (KIND == 1 ? fr : wa);
+            if (f.z) {
+                // all ZS tap reads in flight before the first use
+                const int cu = (int)(unsigned)pu - f.x;
+                float4 q[ZS];
+                float wv[ZS];
+#pragma unroll
+                for (int i = 0; i < ZS; ++i) {
+                    const float pv = clamp_pad(vn * fr, hiv);
+                    wv[i] = frac(pv);
+                    int off = ((int)(unsigned)pv - f.y) * f.z + cu;
+                    off = min(max(off, 0), n - 2);
+                    const float2 t0 = t[off], t1 = t[off + 1];
+                    q[i] = make_float4(t0.x, t0.y, t1.x, t1.y);
+                    vn += c.nv[2];
+                }
+#pragma unroll
+                for (int i = 0; i < ZS; ++i) acc[i] = fmaf(wgt, bilerp_pairs(q[i], wu, wv[i]), acc[i]);
+            } else {
+                const float *pl = A.pp + (size_t)a * plane * 2;
+                const unsigned ou = (unsigned)pu << 3;
+#pragma unroll
+                for (int i = 0; i < ZS; ++i) {
+                    const float pv = clamp_pad(vn * fr, hiv);
+                    const float val = bilerp_pairs(ld4b(pl, __umul24((unsigned)pv, rowb) + ou), wu, frac(pv));
+                    acc[i] = fmaf(wgt, val, acc[i]);
+                    vn += c.nv[2];
+                }
+            }
+        }
+    }
+    if (active) bp_column_store<ZS>(A, X, Y, Z0, acc);
 }
 
 // Supersampled BP (dev_cone_BP_SS / dev_par3D_BP_SS,
@@ -922,6 +1097,13 @@ bool zc_disabled() {
 }
 
 // z voxels per thread of bp3d_kernel: 8, or 4 / 16 via ASTRA_AMD_BP3D_ZS.
+// ASTRA_AMD_BP3D_LDS=1: the z-constant BP stages detector footprints in LDS
+// (read at every launch)
+bool bp_lds() {
+    const char *e = std::getenv("ASTRA_AMD_BP3D_LDS");
+    return e && e[0] == '1';
+}
+
 int bp_zs() {
     static const int v = [] {
         const char *e = std::getenv("ASTRA_AMD_BP3D_ZS");
@@ -1077,6 +1259,12 @@ namespace {
 template <int ZS>
 void bp3d_dispatch(int kind, bool zc, const BPArgs &A, hipStream_t s) {
     dim3 grid(((A.nx + 15) / 16) * ((A.ny + 15) / 16), (A.nz + ZS - 1) / ZS);
+    if (zc && bp_lds()) {
+        if (kind == 0) bp3d_lds_kernel<0, ZS><<<grid, 256, 0, s>>>(A);
+        else if (kind == 1) bp3d_lds_kernel<1, ZS><<<grid, 256, 0, s>>>(A);
+        else bp3d_lds_kernel<2, ZS><<<grid, 256, 0, s>>>(A);
+        return;
+    }
     if (kind == 0)
         zc ? bp3d_kernel<0, ZS, true><<<grid, 256, 0, s>>>(A) : bp3d_kernel<0, ZS, false><<<grid, 256, 0, s>>>(A);
     else if (kind == 1)
