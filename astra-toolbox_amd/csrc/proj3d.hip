@@ -538,7 +538,10 @@ __global__ __launch_bounds__(256) void bp3d_lds_kernel(BPArgs A) {
     const unsigned tid = threadIdx.x;
     const unsigned tiles_x = (A.nx + 15) / 16;
     const int X0 = (blockIdx.x % tiles_x) * 16, Y0 = (blockIdx.x / tiles_x) * 16;
-    const int X = X0 + (int)(tid & 15), Y = Y0 + (int)(tid >> 4);
+    // each 16 lanes take a 4 x 4 patch of columns, so the taps that one LDS
+    // cycle serves fall in a narrow window of the tile (bank conflicts)
+    const int X = X0 + (int)((tid & 3) | (((tid >> 4) & 3) << 2));
+    const int Y = Y0 + (int)(((tid >> 2) & 3) | ((tid >> 6) << 2));
     const int Z0 = blockIdx.y * ZS;
     const bool active = X < (int)A.nx && Y < (int)A.ny;
     const float hx = 0.5f * A.nx - 0.5f, hy = 0.5f * A.ny - 0.5f, hz = 0.5f * A.nz - 0.5f;
