@@ -26,6 +26,7 @@ sys.path.insert(0, os.path.join(ROOT, "astra-toolbox_amd"))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+ENGINE_CLOCK_GHZ = 2.4  # MI355X peak engine clock
 
 
 def log(*a):
@@ -507,6 +508,10 @@ def main():
     outputs_per_launch = units_per_launch / (n if dom.startswith("fp") else na)
     bytes_per_launch = 16.0 * units_per_launch + 8.0 * outputs_per_launch
     achieved = bytes_per_launch / (kd["avg_launch_ms"] * 1e-3) / 1e9
+    # the limiter of the gather kernels: each tap is one 16-byte load and
+    # the texture-address unit of a CU takes 4 such lanes per clock
+    cus = torch.cuda.get_device_properties(dev).multi_processor_count
+    taps_per_clk = units_per_launch / (kd["avg_launch_ms"] * 1e-3) / (cus * ENGINE_CLOCK_GHZ * 1e9)
     for k, v in kernels.items():
         u = v["units"] / max(v["launches"], 1)
         o = u / (n if k.startswith("fp") else na)
@@ -568,7 +573,12 @@ def main():
                      "traffic": measured_traffic(dom, n, na, nd),
                      "traffic_source": TRAFFIC_FILE if measured_traffic(dom, n, na, nd) else None,
                      "bytes_per_launch": bytes_per_launch,
-                     "avg_launch_ms": kd["avg_launch_ms"]},
+                     "avg_launch_ms": kd["avg_launch_ms"],
+                     "texture_path": {"taps_per_clk_per_cu": taps_per_clk, "limit": 4.0,
+                                      "frac": taps_per_clk / 4.0, "cus": cus,
+                                      "clock_GHz": ENGINE_CLOCK_GHZ,
+                                      "note": "16-byte tap loads, 4 lanes per clock per CU "
+                                              "at the peak engine clock"}},
         "fdk_seconds": fdk_s,
         "batched_sirt": batched_res,
         "differentiable": diff_res,
