@@ -9,7 +9,7 @@ runs in libastra_amd (hand-written HIP kernels for gfx950).
 import os
 
 from . import (algorithm, autograd, batched, composite, data2d, data3d, experimental, functions, log,
-               projector, projector3d, pythonutils)
+               plugin, plugins, projector, projector3d, pythonutils)
 from . import astra
 from .astra import get_gpu_info, has_feature, set_gpu_index, use_cuda
 from .extrautils import clipCircle

@@ -493,10 +493,35 @@ NOT_BUILT = ("FP", "BP", "FBP", "SIRT", "SART", "ART", "CGLS",
              "DART_SMOOTHING_CUDA3D", "ROI_SELECT_CUDA", "DATA_OPERATION_CUDA")
 
 
+class _Plugin:
+    """An algorithm object of a registered Python plugin
+    (python/astra/src/PythonPluginAlgorithm.cpp:98-125)."""
+
+    def __init__(self, cls, cfg):
+        self.instance = cls()
+        self.instance.astra_init(cfg)
+
+    def run(self, iterations=1):
+        self.instance.run(int(iterations))
+
+    def get_res_norm(self):
+        raise AstraError("this algorithm has no residual norm")
+
+    def description(self):
+        return "plugin " + type(self.instance).__name__
+
+    def release(self):
+        pass
+
+
 def create(config):
     t = config.get("type")
     if t in TYPES:
         return _reg.add(TYPES[t](config))
+    from . import plugin
+    cls = plugin.lookup(t)
+    if cls is not None:
+        return _reg.add(_Plugin(cls, config))
     if t in NOT_BUILT:
         raise AstraError("algorithm type '%s' is not part of this build" % t)
     raise AstraError("unknown algorithm type '%s'" % t)
@@ -523,4 +548,8 @@ def info():
 
 
 def get_plugin_object(i):
-    raise AstraError("plugins are not part of this build")
+    """The Python object of a plugin algorithm (python/astra/algorithm.py:69-77)."""
+    a = _reg.get(i)
+    if not isinstance(a, _Plugin):
+        raise AstraError("Not a plugin algorithm")
+    return a.instance
