@@ -11,6 +11,8 @@ import os
 from . import (algorithm, autograd, batched, composite, data2d, data3d, experimental, functions, log,
                plugin, plugins, projector, projector3d, pythonutils)
 from . import astra
+from . import matlab
+from . import matlab as m
 from .astra import get_gpu_info, has_feature, set_gpu_index, use_cuda
 from .extrautils import clipCircle
 from .tests import test, test_CUDA, test_noCUDA

@@ -87,3 +87,22 @@ def test_example_plugins_are_exposed():
     assert astra.plugins.CGLSPlugin.astra_name == "CGLS-PLUGIN"
     plugin.register(astra.plugins.SIRTPlugin)
     assert "'Relaxation'" in plugin.get_help("SIRT-PLUGIN")
+
+
+def test_matlab_style_calls():
+    """astra.m (python/astra/matlab.py): operation name first; 'iterate'
+    is 'run' (shown with a host-only plugin)."""
+    plugin.register(Scaler)
+    vg = astra.create_vol_geom(3, 3)
+    vid = astra.m.data2d("create", "-vol", vg, 2.0)
+    aid = astra.m.algorithm("create", {"type": "SCALER-TEST", "ReconstructionDataId": vid,
+                                       "options": {"Factor": 3.0}})
+    astra.m.algorithm("iterate", aid, 2)
+    assert np.array_equal(astra.m.data2d("get", vid), np.full((3, 3), 18.0, np.float32))
+    assert astra.m.astra("use_cuda") in (True, False)
+    with pytest.raises(astra.AstraError):
+        astra.m.data2d("no_such_command", vid)
+    with pytest.raises(astra.AstraError):
+        astra.m.matrix("delete", 0)
+    astra.m.algorithm("delete", aid)
+    astra.m.data2d("delete", vid)
