@@ -9,7 +9,7 @@ runs in libastra_amd (hand-written HIP kernels for gfx950).
 import os
 
 from . import (algorithm, autograd, batched, composite, data2d, data3d, experimental, functions, log,
-               plugin, plugins, projector, projector3d, pythonutils)
+               matrix, plugin, plugins, projector, projector3d, pythonutils)
 from . import astra
 from . import matlab
 from . import matlab as m

@@ -65,3 +65,4 @@ data3d = Registry("Data object")
 projectors2d = Registry("Projector")
 projectors3d = Registry("Projector")
 algorithms = Registry("Algorithm")
+matrices = Registry("Matrix")

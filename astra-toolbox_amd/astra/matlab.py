@@ -8,6 +8,7 @@ from . import algorithm as _algorithm
 from . import astra as _astra
 from . import data2d as _data2d
 from . import data3d as _data3d
+from . import matrix as _matrix
 from . import projector as _projector
 from . import projector3d as _projector3d
 from .log import AstraError
@@ -48,9 +49,8 @@ def projector3d(command, *args):
 
 
 def matrix(command, *args):
-    """The sparse-matrix store belongs to the CPU projectors, which this
-    library does not build."""
-    raise AstraError("astra.matrix is not part of this build")
+    """``astra.m.matrix('delete', i)`` etc."""
+    return _call(_matrix, command, args)
 
 
 def algorithm(command, *args):

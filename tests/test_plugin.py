@@ -102,7 +102,30 @@ def test_matlab_style_calls():
     assert astra.m.astra("use_cuda") in (True, False)
     with pytest.raises(astra.AstraError):
         astra.m.data2d("no_such_command", vid)
-    with pytest.raises(astra.AstraError):
-        astra.m.matrix("delete", 0)
+    mid = astra.m.matrix("create", __import__("scipy.sparse").sparse.eye(3, format="csr"))
+    assert astra.m.matrix("get_size", mid) == (3, 3)
+    astra.m.matrix("delete", mid)
     astra.m.algorithm("delete", aid)
     astra.m.data2d("delete", vid)
+
+
+def test_sparse_matrix_store():
+    """astra.matrix (python/astra/matrix_c.pyx:54-113): round trip, store
+    within capacity, errors past it."""
+    import scipy.sparse as ss
+    rng = np.random.default_rng(0)
+    m = ss.random(7, 11, density=0.3, random_state=1, format="csr", dtype=np.float32)
+    mid = astra.matrix.create(m)
+    assert astra.matrix.get_size(mid) == (7, 11)
+    got = astra.matrix.get(mid)
+    assert (got != m).nnz == 0 and got.dtype == np.float32
+    smaller = ss.random(7, 11, density=0.1, random_state=2, format="csr", dtype=np.float32)
+    astra.matrix.store(mid, smaller)
+    assert (astra.matrix.get(mid) != smaller).nnz == 0
+    with pytest.raises(astra.AstraError, match="too large"):
+        astra.matrix.store(mid, ss.csr_matrix(rng.random((7, 11), dtype=np.float32)))
+    with pytest.raises(astra.AstraError):
+        astra.matrix.create(np.eye(3))
+    astra.matrix.delete(mid)
+    with pytest.raises(astra.AstraError):
+        astra.matrix.get(mid)
