@@ -63,3 +63,57 @@ def test_example_plugins_match_native(case):
     e = rel_l2(got, want)
     print("%s: plugin vs native rel-L2 %.3g" % (case, e))
     assert np.abs(want).max() > 0 and e <= 1e-5
+
+
+def test_reconstruction_algorithm_bases():
+    """plugin.ReconstructionAlgorithm2D / 3D set pid, s, v, vg, pg before
+    initialize; 2D data that does not match the projector is refused."""
+    import astra
+
+    class Probe2D(astra.plugin.ReconstructionAlgorithm2D):
+        astra_name = "PROBE-2D-TEST"
+
+        def initialize(self, cfg, Step=0.5):
+            self.step = Step
+
+        def run(self, its):
+            W = astra.OpTomo(self.pid)
+            for _ in range(its):
+                self.v += np.float32(self.step) * W.BP(self.s - W.FP(self.v)) / np.float32(1e4)
+
+    class Probe3D(astra.plugin.ReconstructionAlgorithm3D):
+        astra_name = "PROBE-3D-TEST"
+
+        def initialize(self, cfg):
+            pass
+
+        def run(self, its):
+            self.v[...] = 1.0
+
+    astra.plugin.register(Probe2D)
+    astra.plugin.register(Probe3D)
+    a, mod, vg, pid, sid = _problem2d()
+    rid = mod.create("-vol", vg, 0.0)
+    aid = a.algorithm.create({"type": "PROBE-2D-TEST", "ProjectorId": pid, "ProjectionDataId": sid,
+                              "ReconstructionDataId": rid})
+    obj = a.algorithm.get_plugin_object(aid)
+    assert obj.s.shape == (60, 72) and obj.v.shape == (48, 40) and obj.step == 0.5
+    assert obj.vg["GridRowCount"] == 48 and obj.pg["type"] == "fanflat"
+    a.algorithm.run(aid, 2)
+    assert np.abs(mod.get(rid)).max() > 0
+    bad = mod.create("-vol", a.create_vol_geom(10, 10), 0.0)
+    with pytest.raises(a.AstraError, match="not compatible"):
+        a.algorithm.create({"type": "PROBE-2D-TEST", "ProjectorId": pid, "ProjectionDataId": sid,
+                            "ReconstructionDataId": bad})
+    a.algorithm.delete(aid)
+    mod.delete([rid, bad])
+    a3, mod3, vg3, pid3, sid3 = _problem3d()
+    rid3 = mod3.create("-vol", vg3, 0.0)
+    aid3 = a3.algorithm.create({"type": "PROBE-3D-TEST", "ProjectorId": pid3,
+                                "ProjectionDataId": sid3, "ReconstructionDataId": rid3})
+    o3 = a3.algorithm.get_plugin_object(aid3)
+    assert o3.s.shape == (20, 36, 32) and o3.vg["GridSliceCount"] == 16
+    a3.algorithm.run(aid3, 1)
+    assert np.array_equal(mod3.get(rid3), np.ones((16, 24, 20), np.float32))
+    a3.algorithm.delete(aid3)
+    mod3.delete(rid3)
