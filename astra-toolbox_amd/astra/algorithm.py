@@ -54,9 +54,10 @@ _no_hook = _lib.ABORT_HOOK()  # NULL
 @contextlib.contextmanager
 def _interruptible():
     """Let SIGINT stop a native iterative run between iterations (main
-    thread only; a handler installed outside Python is left alone)."""
+    thread only; an ignored SIGINT or a handler installed outside Python is
+    left alone)."""
     if threading.current_thread() is not threading.main_thread() or \
-            signal.getsignal(signal.SIGINT) is None:
+            signal.getsignal(signal.SIGINT) in (None, signal.SIG_IGN):
         yield
         return
     _stop.clear()

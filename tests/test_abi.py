@@ -83,3 +83,14 @@ def test_abort_hook_set_and_cleared():
         signal.raise_signal(signal.SIGINT)  # sets the stop flag, raises nothing
         assert algorithm._hook(None) == 1
     assert signal.getsignal(signal.SIGINT) is before
+
+
+def test_ignored_sigint_is_left_alone():
+    import signal
+    from astra import algorithm
+    prev = signal.signal(signal.SIGINT, signal.SIG_IGN)
+    try:
+        with algorithm._interruptible():
+            assert signal.getsignal(signal.SIGINT) is signal.SIG_IGN
+    finally:
+        signal.signal(signal.SIGINT, prev)
