@@ -88,6 +88,8 @@ SIGNATURES = {
     "astra_amd_profile_enable": [_i],
     "astra_amd_profile_read": [ctypes.c_char_p, ctypes.POINTER(_d), ctypes.POINTER(_d),
                                ctypes.POINTER(ctypes.c_ulonglong)],
+    "astra_amd_calibrate_gather": [_i, _u, ctypes.POINTER(_d), _vp],
+    "astra_amd_scratch_usage": [ctypes.POINTER(_sz), ctypes.POINTER(_sz), _i],
 }
 STRINGS = ["astra_amd_last_error", "astra_amd_version"]
 
@@ -157,3 +159,21 @@ def profile_read(kernel):
     call("astra_amd_profile_read", kernel.encode(), ctypes.byref(ms), ctypes.byref(units),
          ctypes.byref(n))
     return ms.value, units.value, n.value
+
+
+def calibrate_gather(mode, iterations=4096):
+    """Lane-loads per second of the gather calibration kernel on the
+    current device (mode 0: 16-byte loads with the taps' lane pattern,
+    1: 16-byte coalesced, 2: 8-byte taps, 3: 4-byte taps)."""
+    out = ctypes.c_double()
+    call("astra_amd_calibrate_gather", int(mode), int(iterations), ctypes.byref(out), None)
+    return out.value
+
+
+def scratch_usage(reset_peak=False):
+    """(bytes in use, high-water mark) of the library's stream-ordered device
+    scratch pool on the current device; reset_peak resets the mark after
+    reading."""
+    cur, peak = ctypes.c_size_t(), ctypes.c_size_t()
+    call("astra_amd_scratch_usage", ctypes.byref(cur), ctypes.byref(peak), 1 if reset_peak else 0)
+    return cur.value, peak.value

@@ -47,8 +47,9 @@ def _check(t, shape, what):
     return t.contiguous()
 
 
-def _fp(P, vol):
-    out = torch.empty(P.proj_shape, dtype=torch.float32, device=vol.device)
+def _fp(P, vol, out=None):
+    if out is None:
+        out = torch.empty(P.proj_shape, dtype=torch.float32, device=vol.device)
     _core.fp(P, vol, out, _lib.MODE_SET)
     return out
 
@@ -108,11 +109,20 @@ def sirt_weights(projector, relaxation=1.0, device=None):
     return R, C
 
 
-def _step(P, R, C, b, x, lo, hi):
-    """One SIRT iteration; returns (x_next, z) with z the unclamped value."""
-    r = (b - _fp(P, x)) * R
-    z = x + C * _bp_matched(P, r)
-    return z.clamp(lo, hi), z
+def _constrained(lo, hi):
+    return lo != -math.inf or hi != math.inf
+
+
+def _step(P, R, C, b, x, lo, hi, pbuf=None):
+    """One SIRT iteration; returns (x_next, z) with z the unclamped value
+    (the same tensor when there are no constraints).  pbuf: optional
+    projection-shaped work buffer for the residual (overwritten)."""
+    r = _fp(P, x, pbuf)
+    torch.sub(b, r, out=r)      # b - A x
+    r.mul_(R)
+    z = _bp_matched(P, r)
+    z.mul_(C).add_(x)           # x + C A^T R (b - A x)
+    return (z.clamp(lo, hi) if _constrained(lo, hi) else z), z
 
 
 class _UnrolledSIRT(torch.autograd.Function):
@@ -120,10 +130,12 @@ class _UnrolledSIRT(torch.autograd.Function):
     def forward(ctx, b, x0, P, iterations, R, C, lo, hi, every):
         x = x0
         ckpts = [x0]
+        pbuf = torch.empty_like(b)  # one residual buffer for every iteration
         for k in range(iterations):
             if k > 0 and k % every == 0:
                 ckpts.append(x)
-            x, _ = _step(P, R, C, b, x, lo, hi)
+            x, _ = _step(P, R, C, b, x, lo, hi, pbuf)
+        del pbuf
         ctx.P, ctx.n, ctx.lo, ctx.hi, ctx.every = P, iterations, lo, hi, every
         ctx.save_for_backward(b, R, C, *ckpts)
         return x
@@ -132,20 +144,26 @@ class _UnrolledSIRT(torch.autograd.Function):
     def backward(ctx, g):
         b, R, C, *ckpts = ctx.saved_tensors
         P, n, lo, hi, every = ctx.P, ctx.n, ctx.lo, ctx.hi, ctx.every
-        g = _check(g, P.vol_shape, "gradient")
+        g = _check(g, P.vol_shape, "gradient").clone()
         gb = torch.zeros_like(b)
+        pbuf = torch.empty_like(b)  # the only projection-shaped work buffer
+        cons = _constrained(lo, hi)
         for seg in range(len(ckpts) - 1, -1, -1):
             start, stop = seg * every, min(n, (seg + 1) * every)
             xs = [ckpts[seg]]  # recompute x_start .. x_{stop-1}
             for _ in range(start, stop - 1):
-                xs.append(_step(P, R, C, b, xs[-1], lo, hi)[0])
+                xs.append(_step(P, R, C, b, xs[-1], lo, hi, pbuf)[0])
             for k in range(stop - 1, start - 1, -1):
                 # x_{k+1} = clamp(z), z = x_k + C A^T R (b - A x_k)
-                _, z = _step(P, R, C, b, xs[k - start], lo, hi)
-                h = g * ((z >= lo) & (z <= hi)).to(g.dtype)  # torch.clamp's gradient
-                t = R * _fp(P, C * h)          # dz/db^T h = R A C h
-                g = h - _bp_matched(P, t)      # dz/dx^T h = h - A^T R A C h
+                if cons:  # torch.clamp's gradient: h = g where lo <= z <= hi
+                    _, z = _step(P, R, C, b, xs[k - start], lo, hi, pbuf)
+                    g.mul_(((z >= lo) & (z <= hi)).to(g.dtype))
+                    del z
+                t = _fp(P, C * g, pbuf)        # dz/db^T h = R A C h
+                t.mul_(R)
                 gb += t
+                g.sub_(_bp_matched(P, t))      # dz/dx^T h = h - A^T R A C h
+                xs[k - start] = None           # x_k is no longer needed
             del xs
         return gb, g, None, None, None, None, None, None, None
 

@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "lib", "obj")
 OUT = os.path.join(HERE, "lib", "libastra_amd.so")
 SOURCES = ["geometry.cpp", "proj3d.hip", "proj2d.hip", "filter.hip", "arith.hip",
-           "algorithms.cpp", "batch2d.hip", "profiling.cpp", "capi.cpp"]
+           "algorithms.cpp", "batch2d.hip", "profiling.cpp", "calib.hip", "capi.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("ASTRA_AMD_ARCH", "gfx950")
 

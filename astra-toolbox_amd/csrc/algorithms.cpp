@@ -409,14 +409,14 @@ double algo_residual(AlgoBase *a, const float *b, const float *x, hipStream_t s)
 namespace {
 void upload_twiddles(Filter &f) {
     const unsigned n = f.n_fft;
-    std::vector<float> tw(n);  // n/2 complex
-    for (unsigned k = 0; k < n / 2; ++k) {
+    std::vector<float> tw(2 * (size_t)n);  // n complex: exp(-2 pi i k / n), k < n
+    for (unsigned k = 0; k < n; ++k) {
         double ang = -2.0 * M_PI * (double)k / (double)n;
         tw[2 * k] = (float)std::cos(ang);
         tw[2 * k + 1] = (float)std::sin(ang);
     }
-    f.twiddle = DevArray(sizeof(float) * n);
-    f.twiddle.upload(tw.data(), sizeof(float) * n);
+    f.twiddle = DevArray(sizeof(float) * tw.size());
+    f.twiddle.upload(tw.data(), sizeof(float) * tw.size());
 }
 }  // namespace
 

@@ -464,6 +464,21 @@ int astra_amd_profile_read(const char *kernel, double *total_ms, double *units,
     });
 }
 
+int astra_amd_scratch_usage(size_t *in_use, size_t *peak, int reset_peak) {
+    return guarded([&] {
+        AMX_REQUIRE(in_use && peak, "scratch_usage: NULL argument");
+        amx::scratch_usage(*in_use, *peak, reset_peak != 0);
+    });
+}
+
+int astra_amd_calibrate_gather(int mode, unsigned int iterations, double *lane_loads_per_second,
+                               void *stream) {
+    return guarded([&] {
+        AMX_REQUIRE(lane_loads_per_second != nullptr, "calibrate_gather: out is NULL");
+        *lane_loads_per_second = amx::gather_calibrate(mode, iterations, as_stream(stream));
+    });
+}
+
 // -------------------------------------------------------- elementwise ---
 int astra_amd_dot(const float *a, const float *b, size_t n, double *out, void *stream) {
     return guarded([&] {

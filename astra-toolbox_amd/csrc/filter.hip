@@ -200,6 +200,275 @@ __global__ __launch_bounds__(256) void filter_rows_kernel(FilterArgs A) {
     }
 }
 
+// ---- Stockham FFT in registers (the real, even spectrum of every FDK /
+// FBP window) ----------------------------------------------------------
+// Each transform of N = 2^LOGN points is held by T = N / 16 threads, 16
+// complex values per thread: thread t holds elements t + T m (m = 0..15).
+// The passes are radix 16 (the last one radix 16, 8, 4 or 2), each a
+// register-resident DFT of its butterflies; between passes the values go
+// through LDS once (Stockham ordering: natural-order input and output, no
+// bit reversal).  After the forward transform every thread holds its 16
+// spectrum values in natural order, so the filter multiply and the first
+// pass of the inverse need no exchange.  Pass with radix R after earlier
+// radices of product Ns: butterfly j (of N / R) takes in[j + k N / R],
+// k < R, multiplies element k by exp(-2 pi i (j mod Ns) k / (Ns R)), and
+// its DFT output k goes to (j / Ns) Ns R + (j mod Ns) + k Ns.
+// A thread with 16 / R = B butterflies takes j = t + T b (b < B), element k
+// of butterfly b being its value m = b + B k, so the thread's inputs are
+// always in[t + T m] and the last pass leaves out[t + T m] in value m.
+template <int LOGN>
+struct FftPlan {
+    static constexpr unsigned N = 1u << LOGN;
+    static constexpr unsigned T = N / 16;                       // threads per transform
+    static constexpr unsigned P = (LOGN + 3) / 4;               // passes
+    static constexpr unsigned LAST = 1u << (LOGN - 4 * (P - 1));
+    static constexpr unsigned F = T >= 256 ? 1 : 256 / T;       // transforms per workgroup
+    static constexpr unsigned WG = T * F;
+    static constexpr unsigned PADN = N + N / 16;                // LDS float2 per transform
+};
+
+// LDS index with one float2 of padding per 16: the 16 lanes that write a
+// stride-16 pattern (first pass) land in distinct banks.
+__device__ __forceinline__ unsigned padi(unsigned a) { return a + (a >> 4); }
+
+// x * exp(-2 pi i e / 16), e constant after unrolling
+template <int E>
+__device__ __forceinline__ float2 w16(float2 x) {
+    constexpr int e = E & 15;
+    if constexpr (e == 0) return x;
+    else if constexpr (e == 4) return make_float2(x.y, -x.x);   // -i
+    else if constexpr (e == 8) return make_float2(-x.x, -x.y);
+    else if constexpr (e == 12) return make_float2(-x.y, x.x);  // +i
+    else {
+        // cos / sin of 2 pi e / 16 from the first quadrant's values
+        constexpr double q[5] = {1.0, 0.92387953251128675613, 0.70710678118654752440,
+                                 0.38268343236508977173, 0.0};
+        constexpr int qe = e & 3, quad = e >> 2;
+        constexpr double c0 = q[qe], s0 = q[4 - qe];
+        constexpr float c = (float)(quad == 0 ? c0 : quad == 1 ? -s0 : quad == 2 ? -c0 : s0);
+        constexpr float sn = (float)(quad == 0 ? s0 : quad == 1 ? c0 : quad == 2 ? -s0 : -c0);
+        // (x.x + i x.y)(c - i sn)
+        return make_float2(fmaf(x.x, c, x.y * sn), fmaf(x.y, c, -x.x * sn));
+    }
+}
+
+__device__ __forceinline__ void dft2(float2 &a, float2 &b) {
+    const float2 t = a;
+    a = addf2(t, b);
+    b = subf2(t, b);
+}
+
+__device__ __forceinline__ void dft4(float2 &x0, float2 &x1, float2 &x2, float2 &x3) {
+    const float2 a0 = addf2(x0, x2), a1 = subf2(x0, x2), a2 = addf2(x1, x3), a3 = subf2(x1, x3);
+    const float2 m3 = make_float2(a3.y, -a3.x);  // -i a3
+    x0 = addf2(a0, a2);
+    x2 = subf2(a0, a2);
+    x1 = addf2(a1, m3);
+    x3 = subf2(a1, m3);
+}
+
+// In-register forward DFT of R points, natural order in and out:
+// X[k] = sum_n x[n] exp(-2 pi i n k / R).  R = 8, 16 as 2 x 4 and 4 x 4
+// (inner DFTs over n1 of x[(R / N1')...], twiddles, outer DFTs).
+template <unsigned R>
+__device__ __forceinline__ void dft(float2 (&x)[R]) {
+    if constexpr (R == 2) {
+        dft2(x[0], x[1]);
+    } else if constexpr (R == 4) {
+        dft4(x[0], x[1], x[2], x[3]);
+    } else if constexpr (R == 8) {
+        // n = 4 n1 + n2, k = k1 + 2 k2: inner radix 2 over n1, twiddle
+        // W8^(n2 k1) = W16^(2 n2 k1), radix 4 over n2
+#pragma unroll
+        for (int n2 = 0; n2 < 4; ++n2) dft2(x[n2], x[4 + n2]);
+        x[5] = w16<2>(x[5]);
+        x[6] = w16<4>(x[6]);
+        x[7] = w16<6>(x[7]);
+        dft4(x[0], x[1], x[2], x[3]);  // k1 = 0 -> X[2 k2]
+        dft4(x[4], x[5], x[6], x[7]);  // k1 = 1 -> X[1 + 2 k2]
+        const float2 y[8] = {x[0], x[4], x[1], x[5], x[2], x[6], x[3], x[7]};
+#pragma unroll
+        for (int k = 0; k < 8; ++k) x[k] = y[k];
+    } else {
+        static_assert(R == 16, "radix");
+        // n = 4 n1 + n2, k = k1 + 4 k2: inner radix 4 over n1 (x[n2], x[4 +
+        // n2], x[8 + n2], x[12 + n2] -> k1 = 0..3), twiddle W16^(n2 k1),
+        // radix 4 over n2
+#pragma unroll
+        for (int n2 = 0; n2 < 4; ++n2) dft4(x[n2], x[4 + n2], x[8 + n2], x[12 + n2]);
+        x[5] = w16<1>(x[5]);
+        x[6] = w16<2>(x[6]);
+        x[7] = w16<3>(x[7]);
+        x[9] = w16<2>(x[9]);
+        x[10] = w16<4>(x[10]);
+        x[11] = w16<6>(x[11]);
+        x[13] = w16<3>(x[13]);
+        x[14] = w16<6>(x[14]);
+        x[15] = w16<9>(x[15]);
+        // now x[4 k1 + n2]: radix 4 over n2 for each k1 -> X[k1 + 4 k2]
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1) dft4(x[4 * k1], x[4 * k1 + 1], x[4 * k1 + 2], x[4 * k1 + 3]);
+        float2 y[16];
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1)
+#pragma unroll
+            for (int k2 = 0; k2 < 4; ++k2) y[k1 + 4 * k2] = x[4 * k1 + k2];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) x[k] = y[k];
+    }
+}
+
+// One pass (radix R, earlier radices of product NS).  Not the last pass:
+// the outputs go through LDS (lds = this transform's region) and come back
+// as the next pass's inputs in[t + T m].
+template <int LOGN, unsigned R, unsigned NS, bool LASTP>
+__device__ __forceinline__ void fft_pass(float2 (&v)[16], unsigned t, const float2 *__restrict__ tw,
+                                         float2 *lds) {
+    using PL = FftPlan<LOGN>;
+    constexpr unsigned N = PL::N, T = PL::T, B = 16 / R;
+#pragma unroll
+    for (unsigned b = 0; b < B; ++b) {
+        const unsigned j = t + T * b;
+        float2 x[R];
+#pragma unroll
+        for (unsigned k = 0; k < R; ++k) x[k] = v[b + B * k];
+        if constexpr (NS > 1) {
+            // w^k for k < R from the table's w, w^2, w^4, w^8 (exact to
+            // fp32), the others as products of at most four of them
+            const unsigned step = (j & (NS - 1)) * (N / (NS * R));
+            float2 w[R];
+            w[1] = tw[step];
+            if constexpr (R > 2) w[2] = tw[2 * step];
+            if constexpr (R > 4) w[4] = tw[4 * step];
+            if constexpr (R > 8) w[8] = tw[8 * step];
+#pragma unroll
+            for (unsigned k = 3; k < R; ++k) {
+                const unsigned hb = 1u << (31 - __builtin_clz(k));  // highest power of two in k
+                if (k != hb) w[k] = cmul(w[k - hb], w[hb]);
+            }
+#pragma unroll
+            for (unsigned k = 1; k < R; ++k) x[k] = cmul(x[k], w[k]);
+        }
+        dft<R>(x);
+#pragma unroll
+        for (unsigned k = 0; k < R; ++k) v[b + B * k] = x[k];
+    }
+    if constexpr (!LASTP) {
+#pragma unroll
+        for (unsigned b = 0; b < B; ++b) {
+            const unsigned j = t + T * b;
+            const unsigned base = (j / NS) * NS * R + (j & (NS - 1));
+#pragma unroll
+            for (unsigned k = 0; k < R; ++k) lds[padi(base + k * NS)] = v[b + B * k];
+        }
+        __syncthreads();
+#pragma unroll
+        for (unsigned m = 0; m < 16; ++m) v[m] = lds[padi(t + T * m)];
+        __syncthreads();  // the next pass writes the same region
+    }
+}
+
+template <int LOGN, unsigned p = 0>
+__device__ __forceinline__ void fft_passes(float2 (&v)[16], unsigned t, const float2 *__restrict__ tw,
+                                           float2 *lds) {
+    using PL = FftPlan<LOGN>;
+    if constexpr (p < PL::P) {
+        constexpr bool last = p + 1 == PL::P;
+        constexpr unsigned R = last ? PL::LAST : 16u;
+        fft_pass<LOGN, R, (1u << (4 * p)), last>(v, t, tw, lds);
+        fft_passes<LOGN, p + 1>(v, t, tw, lds);
+    }
+}
+
+// Rows filtered with one real, even spectrum: two rows per complex
+// transform, F transforms per workgroup.  FDK / Parker pre-weights are
+// applied on load; the output is conj(FFT(conj(H . FFT(x + i y)))) / N,
+// i.e. row a = Re, row a + 1 = -Im of FFT(conj(H . Z)) / N.
+template <int LOGN>
+__global__ __launch_bounds__(FftPlan<LOGN>::WG) void filter_rows_stockham(FilterArgs A) {
+    using PL = FftPlan<LOGN>;
+    constexpr unsigned N = PL::N, T = PL::T;
+    __shared__ float2 lds_all[PL::F * PL::PADN];
+    const unsigned f = threadIdx.x / T, t = threadIdx.x % T;
+    float2 *lds = lds_all + f * PL::PADN;
+    const size_t r0 = ((size_t)blockIdx.x * PL::F + f) * 2;
+    const bool has0 = r0 < A.n_rows, has1 = r0 + 1 < A.n_rows;
+    const unsigned nd = A.n_det;
+    float2 v[16];
+    {
+        const float *row0 = A.rows + r0 * nd;
+        const float *row1 = row0 + nd;
+        const float *w0 = (A.weights && has0) ? A.weights + (r0 / A.rows_per_weight) * nd : nullptr;
+        const float *w1 = (A.weights && has1) ? A.weights + ((r0 + 1) / A.rows_per_weight) * nd : nullptr;
+        const float *p0 = (A.weights2 && has0) ? A.weights2 + (r0 % A.w2_rows) * nd : nullptr;
+        const float *p1 = (A.weights2 && has1) ? A.weights2 + ((r0 + 1) % A.w2_rows) * nd : nullptr;
+#pragma unroll
+        for (unsigned m = 0; m < 16; ++m) {
+            const unsigned i = t + T * m;
+            float a = 0.f, b = 0.f;
+            if (i < nd) {
+                if (has0) {
+                    a = row0[i];
+                    if (w0) a *= w0[i];
+                    if (p0) a *= p0[i];
+                }
+                if (has1) {
+                    b = row1[i];
+                    if (w1) b *= w1[i];
+                    if (p1) b *= p1[i];
+                }
+            }
+            v[m] = make_float2(a, b);
+        }
+    }
+    fft_passes<LOGN>(v, t, A.tw, lds);
+    // x H, conjugated (inverse as conj(FFT(conj .)))
+#pragma unroll
+    for (unsigned m = 0; m < 16; ++m) {
+        const float h = A.spec[t + T * m];
+        v[m] = make_float2(v[m].x * h, -v[m].y * h);
+    }
+    fft_passes<LOGN>(v, t, A.tw, lds);
+    const float inv = 1.0f / (float)N;
+    float *row0 = A.rows + r0 * nd;
+    float *row1 = row0 + nd;
+#pragma unroll
+    for (unsigned m = 0; m < 16; ++m) {
+        const unsigned i = t + T * m;
+        if (i < nd) {
+            if (has0) row0[i] = v[m].x * inv;
+            if (has1) row1[i] = -v[m].y * inv;
+        }
+    }
+}
+
+template <int LOGN>
+void launch_stockham(const FilterArgs &A, hipStream_t s) {
+    using PL = FftPlan<LOGN>;
+    const size_t per = 2 * (size_t)PL::F;  // rows per workgroup
+    const size_t groups = (A.n_rows + per - 1) / per;
+    AMX_REQUIRE(groups < (1ull << 31), "filter: too many rows");
+    filter_rows_stockham<LOGN><<<(unsigned)groups, PL::WG, 0, s>>>(A);
+    AMX_LAUNCHED();
+}
+
+// transforms of 16 .. 4096 points (detectors up to 2048 bins; at most
+// 34 KiB of static LDS per workgroup)
+bool stockham_launch(const FilterArgs &A, unsigned log2n, hipStream_t s) {
+    switch (log2n) {
+    case 4: launch_stockham<4>(A, s); return true;
+    case 5: launch_stockham<5>(A, s); return true;
+    case 6: launch_stockham<6>(A, s); return true;
+    case 7: launch_stockham<7>(A, s); return true;
+    case 8: launch_stockham<8>(A, s); return true;
+    case 9: launch_stockham<9>(A, s); return true;
+    case 10: launch_stockham<10>(A, s); return true;
+    case 11: launch_stockham<11>(A, s); return true;
+    case 12: launch_stockham<12>(A, s); return true;
+    default: return false;  // 8192 points and up: the LDS kernels
+    }
+}
+
 // One real row per workgroup, for transforms whose row pair does not fit
 // the CU's LDS (n_fft > 16384 on gfx950): the length-N real transform is a
 // length-M = N/2 complex transform of z[m] = x[2m] + i x[2m+1] followed by
@@ -373,6 +642,15 @@ void filter_rows_launch(const Filter &f, float *rows, size_t n_rows, const float
         AMX_LAUNCHED();
         prof_end(pt, s);
         return;
+    }
+    // ASTRA_AMD_FILTER_LDS_RADIX4=1 keeps the LDS radix-2^2 kernel for the
+    // real-spectrum case (read at every call; for comparing the two)
+    const char *old_path = std::getenv("ASTRA_AMD_FILTER_LDS_RADIX4");
+    if (!f.general && !(old_path && old_path[0] == '1')) {
+        ProfToken pt = prof_begin("filter_rows", (double)n_rows, s);
+        const bool done = stockham_launch(A, l, s);
+        prof_end(pt, s);
+        if (done) return;
     }
     const size_t lds = pair_lds;
     if (lds > 65536) {

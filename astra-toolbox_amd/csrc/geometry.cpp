@@ -42,6 +42,22 @@ void configure_pool_once() {
     done.push_back(dev);
 }
 
+void scratch_usage(size_t &in_use, size_t &peak, bool reset_peak) {
+    int dev = 0;
+    AMX_HIP(hipGetDevice(&dev));
+    hipMemPool_t pool;
+    AMX_HIP(hipDeviceGetDefaultMemPool(&pool, dev));
+    uint64_t cur = 0, high = 0;
+    AMX_HIP(hipMemPoolGetAttribute(pool, hipMemPoolAttrUsedMemCurrent, &cur));
+    AMX_HIP(hipMemPoolGetAttribute(pool, hipMemPoolAttrUsedMemHigh, &high));
+    in_use = (size_t)cur;
+    peak = (size_t)high;
+    if (reset_peak) {
+        uint64_t zero = 0;
+        AMX_HIP(hipMemPoolSetAttribute(pool, hipMemPoolAttrUsedMemHigh, &zero));
+    }
+}
+
 Scratch::Scratch(size_t bytes, hipStream_t s) : s_(s) {
     configure_pool_once();
     if (bytes == 0) bytes = 256;

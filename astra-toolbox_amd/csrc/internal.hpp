@@ -59,6 +59,9 @@ class Scratch {
 };
 
 void configure_pool_once();
+// Bytes of the current device's stream-ordered pool in use now and at the
+// high-water mark (optionally reset to the current use).
+void scratch_usage(size_t &in_use, size_t &peak, bool reset_peak);
 
 // Persistent device allocation owned by a handle (hipMalloc / hipFree).
 class DevArray {
@@ -146,7 +149,7 @@ struct Filter {
     bool general = false;
     unsigned n_spec = 1;
     DevArray spec;
-    DevArray twiddle;  // float2[n_fft/2]: exp(-2 pi i k / n_fft)
+    DevArray twiddle;  // float2[n_fft]: exp(-2 pi i k / n_fft)
 };
 
 struct Geom3D {
@@ -198,6 +201,8 @@ ProfToken prof_begin(const char *name, double units, hipStream_t s);
 void prof_end(const ProfToken &t, hipStream_t s);
 void prof_enable(bool on);
 bool prof_read(const char *name, double &ms, double &units, unsigned long long &launches);
+// 16-byte gather calibration (calib.hip): lane-loads per second.
+double gather_calibrate(int mode, unsigned iters, hipStream_t s);
 
 // ------------------------------------------------------ launchers --------
 // 3D

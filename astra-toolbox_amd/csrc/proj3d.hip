@@ -1432,10 +1432,12 @@ void bp3d_matched_launch(const Geom3D &g, const float *proj, float *vol, float o
     // 32-bit byte offsets into one angle's ray table
     AMX_REQUIRE(NU * NV * sizeof(float4) < (1ull << 32),
                 "bp3d: detector too large for the matched adjoint");
-    // ray tables are built per chunk of angles, at most ~1 GiB at a time
+    // ray tables are built per chunk of angles, at most 128 MiB at a time
+    // (one angle at least): the tables are scratch of every gradient step
+    // of the differentiable projector, so they stay small beside the data
     const size_t per_angle = NU * NV * (sizeof(float4) + sizeof(float));
     const unsigned chunk = (unsigned)std::max<size_t>(
-        1, std::min<size_t>(g.na, (size_t(1) << 30) / per_angle));
+        1, std::min<size_t>(g.na, (size_t(128) << 20) / per_angle));
     const bool cyl = g.kind == ASTRA_AMD_GEOM_CYLCONE;
     Scratch rays(per_angle * chunk, s), geo((size_t)chunk * 3 * sizeof(float4), s);
     float4 *rp = rays.as<float4>();

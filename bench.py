@@ -26,16 +26,37 @@ sys.path.insert(0, os.path.join(ROOT, "astra-toolbox_amd"))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
-ENGINE_CLOCK_GHZ = 2.4  # MI355X peak engine clock
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(iters):
+def host_cpu():
+    """nproc, the CPUs this process may run on, and the CPU model (lscpu)."""
+    import subprocess
+
+    model = None
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except (OSError, subprocess.SubprocessError):
+        pass
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except AttributeError:
+        allowed = os.cpu_count() or 1
+    return {"nproc": os.cpu_count(), "affinity_cpus": allowed, "model": model}
+
+
+def cpu_baseline(iters, runs=5):
     """Reference CPU SIRT (256^2, 180 angles, 384 detectors, 'linear'),
-    single thread, on a bounded number of iterations."""
+    single thread: one warm-up run, then the median of `runs` timed runs of
+    `iters` iterations (phantom, sinogram and geometry set up outside the
+    timed region)."""
     from astra import phantom
     from oracle import cpu
 
@@ -47,28 +68,34 @@ def cpu_baseline(iters):
     ang = np.linspace(0, np.pi, na, endpoint=False)
     vol = phantom.ellipses_2d(n, seed=0)
     sino = lib.fp(vol, ang, nd)
-    lib.sirt(sino, ang, (n, n), 1)  # warm-up
-    t0 = time.perf_counter()
-    lib.sirt(sino, ang, (n, n), iters)
-    dt = time.perf_counter() - t0
+    lib.sirt(sino, ang, (n, n), 2)  # warm-up
+    secs = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        lib.sirt(sino, ang, (n, n), iters)
+        secs.append(time.perf_counter() - t0)
+    dt = float(np.median(secs))
     updates = iters * 2.0 * na * nd * n  # FP + BP ray-slice samples per iteration
     return {
         "value": updates / dt / 1e9,
         "unit": "Gupdates/s",
         "cores": 1,
         "kind": kind,
-        "sample": "CPU SIRT, linear projector, 256x256, 180 angles x 384 dets, %d iterations "
-                  "in %.2f s (%.1f ms/iteration); updates = 2 x angles x dets x 256 per "
-                  "iteration" % (iters, dt, 1e3 * dt / iters),
+        "sample": "CPU SIRT, linear projector, 256x256, 180 angles x 384 dets: median of %d runs "
+                  "of %d iterations after a warm-up, %.2f s per run (%.1f ms/iteration; runs "
+                  "%s s); updates = 2 x angles x dets x 256 per iteration"
+                  % (runs, iters, dt, 1e3 * dt / iters, ", ".join("%.2f" % x for x in secs)),
         "seconds": dt,
+        "host": host_cpu(),
     }
 
 
 def _cpu_sirt_worker(job):
-    """One independent CPU SIRT run (a process of the all-cores baseline) on
-    the phantom image `img`; returns its seconds.  Imports numpy and the
-    CPU library only."""
-    img, iters = job
+    """Independent CPU SIRT runs (one process of the all-cores baseline) on
+    the phantom image `img`: a warm-up, then `runs` timed runs of `iters`
+    iterations; returns their seconds.  Imports numpy and the CPU library
+    only."""
+    img, iters, runs = job
     from oracle import cpu
 
     lib = cpu.reference() or cpu.port()
@@ -76,15 +103,20 @@ def _cpu_sirt_worker(job):
     ang = np.linspace(0, np.pi, na, endpoint=False)
     sino = lib.fp(img, ang, nd)
     lib.sirt(sino, ang, (n, n), 1)
-    t0 = time.perf_counter()
-    lib.sirt(sino, ang, (n, n), iters)
-    return time.perf_counter() - t0
+    out = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        lib.sirt(sino, ang, (n, n), iters)
+        out.append(time.perf_counter() - t0)
+    return out
 
 
-def cpu_all_cores(iters, workers):
-    """Throughput of `workers` concurrent independent CPU SIRT runs (the
-    reference CPU path is single-threaded, so the host's cores are used as
-    independent jobs).  Runs before the GPU is touched."""
+def cpu_all_cores(iters, workers, runs=5):
+    """Throughput of `workers` concurrent independent CPU SIRT processes
+    (the reference CPU path is single-threaded, so the host's cores are
+    used as independent jobs), each timing `runs` runs of `iters`
+    iterations; the rate uses the median over all runs.  Runs before the
+    GPU is touched."""
     import multiprocessing as mp
 
     from astra import phantom
@@ -97,21 +129,26 @@ def cpu_all_cores(iters, workers):
     # that a profiler's signal handler keeps alive, and the join then waits
     pool = ctx.Pool(workers)
     try:
-        secs = pool.map_async(_cpu_sirt_worker, [(img, iters)] * workers).get(timeout=600)
+        secs = pool.map_async(_cpu_sirt_worker, [(img, iters, runs)] * workers).get(timeout=900)
     finally:
         pool.close()
         pool.join()
     wall = time.perf_counter() - t0
     n, na, nd = 256, 180, 384
+    per_run = float(np.median([x for w in secs for x in w]))
     updates = workers * iters * 2.0 * na * nd * n
-    slowest = max(secs)
     from oracle import cpu
 
     kind = "reference" if cpu.reference() is not None else "port"
-    return {"value": updates / slowest / 1e9, "unit": "Gupdates/s", "cores": workers, "kind": kind,
-            "sample": "%d concurrent CPU SIRT runs (256x256, 180 x 384, %d iterations each); "
-                      "slowest run %.2f s, pool wall time incl. start-up %.2f s"
-                      % (workers, iters, slowest, wall)}
+    host = host_cpu()
+    return {"value": updates / per_run / 1e9, "unit": "Gupdates/s", "cores": workers, "kind": kind,
+            "host": host,
+            "sample": "%d concurrent CPU SIRT processes (256x256, 180 x 384), each %d runs of %d "
+                      "iterations after a warm-up; median run %.2f s, slowest %.2f s; pool wall "
+                      "time incl. start-up %.2f s; host nproc %s, %s CPUs in this process's "
+                      "affinity, %s"
+                      % (workers, runs, iters, per_run, max(x for w in secs for x in w), wall,
+                         host["nproc"], host["affinity_cpus"], host["model"])}
 
 
 def copy_calibration(dev, nbytes=1 << 30):
@@ -227,24 +264,32 @@ def differentiable_bench(n=256, na=360, nd=384, iters=20):
 
     W = AG.sirt_weights(pid)
     res = {}
+    from astra import _lib
     for name, fn in (("recompute", AG.sirt_unrolled), ("stored", AG.sirt_unrolled_stored)):
         b = y.clone().requires_grad_(True)
         torch.cuda.synchronize(dev)
         torch.cuda.reset_peak_memory_stats(dev)
         base = torch.cuda.memory_allocated(dev)
+        sbase, _ = _lib.scratch_usage(reset_peak=True)
         t = time.perf_counter()
         x = fn(b, pid, iters, weights=W)
         loss = ((x - target) ** 2).sum()
         loss.backward()
         torch.cuda.synchronize(dev)
-        res[name] = {"seconds": time.perf_counter() - t,
-                     "peak_extra_MiB": (torch.cuda.max_memory_allocated(dev) - base) / 2 ** 20}
+        secs = time.perf_counter() - t
+        _, speak = _lib.scratch_usage()
+        torch_extra = (torch.cuda.max_memory_allocated(dev) - base) / 2 ** 20
+        scratch_extra = (speak - sbase) / 2 ** 20
+        # the two peaks need not coincide in time: their sum bounds the device peak
+        res[name] = {"seconds": secs, "peak_extra_MiB": torch_extra,
+                     "library_scratch_peak_MiB": scratch_extra,
+                     "device_peak_extra_MiB_bound": torch_extra + scratch_extra}
         del b, x, loss
     log("bench: differentiable %d^3: grad step %.3f s; unrolled SIRT %d its fwd+bwd recompute %.2f s "
-        "(+%.0f MiB) vs stored %.2f s (+%.0f MiB)" % (n, grad_s, iters, res["recompute"]["seconds"],
-                                                      res["recompute"]["peak_extra_MiB"],
-                                                      res["stored"]["seconds"],
-                                                      res["stored"]["peak_extra_MiB"]))
+        "(+%.0f MiB torch, +%.0f MiB scratch) vs stored %.2f s (+%.0f MiB torch, +%.0f MiB scratch)"
+        % (n, grad_s, iters, res["recompute"]["seconds"], res["recompute"]["peak_extra_MiB"],
+           res["recompute"]["library_scratch_peak_MiB"], res["stored"]["seconds"],
+           res["stored"]["peak_extra_MiB"], res["stored"]["library_scratch_peak_MiB"]))
     astra.projector3d.delete(pid)
     return {"workload": "cone %d^3, %d angles, %dx%d detector, SOD 1000, ODD 500" % (n, na, nd, nd),
             "fp_plus_matched_grad_seconds": grad_s, "unrolled_sirt_iterations": iters,
@@ -380,26 +425,33 @@ def large_cone_bench(n=1024, na=1440, nd=1536, steps=2):
     fdk(P, pcopy, bpv, _lib.MODE_SET)
     torch.cuda.synchronize(dev)
     fdk_s = time.perf_counter() - t
+    pcopy.copy_(proj)
+    _lib.profile_enable(True)
+    fdk(P, pcopy, bpv, _lib.MODE_SET)
+    fms, _, fl = _lib.profile_read("filter_rows")
+    _lib.profile_enable(False)
     peak = torch.cuda.max_memory_allocated(dev) / 2 ** 30
     ups = (float(na) * nd * nd * n + float(n) ** 3 * na) / dt / 1e9
-    log("bench: cone 1024^3 / 1440 x 1536^2: FP+BP %.3f s/step (%.0f Gupdates/s), FDK %.3f s"
-        % (dt, ups, fdk_s))
+    log("bench: cone 1024^3 / 1440 x 1536^2: FP+BP %.3f s/step (%.0f Gupdates/s), FDK %.3f s "
+        "(row filter %.1f ms)" % (dt, ups, fdk_s, fms / max(fl, 1)))
     del vol, proj, bpv, pcopy
     astra.projector3d.delete(pid)
     torch.cuda.empty_cache()
     return {"workload": "cone %d^3, %d angles, %dx%d detector, SOD 1000, ODD 500" % (n, na, nd, nd),
             "fpbp_seconds_per_step": dt, "Gupdates_per_s": ups, "fdk_seconds": fdk_s,
+            "fdk_row_filter_ms": fms / max(fl, 1),
             "torch_peak_GiB": peak}
 
 
 TRAFFIC_FILE = "profiles/cone512_fpbp_counters.md"
 
 
-def measured_traffic(kernel, n, na, nd):
-    """HBM bytes per launch of `kernel` at the 512^3 / 720 x 768^2 workload,
-    from the calibrated FETCH_SIZE + WRITE_SIZE profiler passes recorded in
-    TRAFFIC_FILE (profiler counters cannot be read from inside this run);
-    None for any other workload."""
+def profile_value(key, kernel, n, na, nd):
+    """A per-launch counter figure of `kernel` at the 512^3 / 720 x 768^2
+    workload recorded in TRAFFIC_FILE as a line `<key> <kernel> <value>`:
+    `traffic_bytes_per_launch` (calibrated FETCH_SIZE + WRITE_SIZE) or
+    `vmem_rd_insts_per_launch` (SQ_INSTS_VMEM_RD).  Profiler counters
+    cannot be read from inside this run; None for any other workload."""
     if (n, na, nd) != (512, 720, 768):
         return None
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), TRAFFIC_FILE)
@@ -407,11 +459,20 @@ def measured_traffic(kernel, n, na, nd):
         with open(path) as f:
             for line in f:
                 parts = line.split()
-                if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
+                if len(parts) == 3 and parts[0] == key and parts[1] == kernel:
                     return float(parts[2])
     except OSError:
         return None
     return None
+
+
+def gather_limits():
+    """Lane-loads per second of the 16-byte gather calibration kernel (the
+    taps' lane pattern, and coalesced), measured on this device now."""
+    from astra import _lib
+
+    return {"taps": _lib.calibrate_gather(0), "coalesced": _lib.calibrate_gather(1),
+            "taps_2d_8byte": _lib.calibrate_gather(2)}
 
 
 def main():
@@ -424,7 +485,8 @@ def main():
     ap.add_argument("--det", type=int, default=768)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fdk", action="store_true")
-    ap.add_argument("--cpu-iters", type=int, default=100)
+    ap.add_argument("--cpu-iters", type=int, default=50,
+                    help="SIRT iterations per timed CPU run (median of 5 runs)")
     ap.add_argument("--batched-slices", type=int, default=4096)
     ap.add_argument("--plain-slices", type=int, default=64)
     ap.add_argument("--batched-iters", type=int, default=50)
@@ -435,7 +497,8 @@ def main():
     ap.add_argument("--no-sirt3d", action="store_true")
     ap.add_argument("--sirt3d-iters", type=int, default=50)
     ap.add_argument("--cpu-workers", type=int, default=0,
-                    help="concurrent CPU SIRT runs for the all-cores figure (0: min(16, cores))")
+                    help="concurrent CPU SIRT processes for the all-cores figure (0: the CPUs "
+                         "this process may use, at most 16: a one-GPU box's CPU share)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 1 GiB device copy first (profiler counter scaling)")
     args = ap.parse_args()
@@ -446,9 +509,9 @@ def main():
 
     cpu_all = None
     if not args.no_cpu:
-        workers = args.cpu_workers or min(16, os.cpu_count() or 1)
-        log("bench: CPU baseline, %d concurrent runs" % workers)
-        cpu_all = cpu_all_cores(max(args.cpu_iters // 5, 1), workers)
+        workers = args.cpu_workers or min(16, host_cpu()["affinity_cpus"])
+        log("bench: CPU baseline, %d concurrent processes" % workers)
+        cpu_all = cpu_all_cores(max(args.cpu_iters // 2, 1), workers)
 
     import torch
 
@@ -500,23 +563,60 @@ def main():
     elapsed = t1 - t0
     value = args.steps * (fp_samples + bp_updates) / elapsed / 1e9
 
-    # roofline of the dominant kernel: algorithmic bytes per launch
-    # (16 B per bilinear sample + 8 B output read-modify-write per ray / voxel)
+    # roofline of the dominant kernel.  Its algorithmic bytes (16 B per
+    # bilinear sample, 8 B output read-modify-write per ray / voxel) are
+    # served mostly by L1 / L2, so they are reported beside the HBM figure,
+    # not as it: `achieved` is the measured HBM traffic per launch
+    # (calibrated profiler counters) over the live launch time.
     dom = max(kernels, key=lambda k: kernels[k]["total_ms"])
-    kd = kernels[dom]
-    units_per_launch = kd["units"] / max(kd["launches"], 1)
-    outputs_per_launch = units_per_launch / (n if dom.startswith("fp") else na)
-    bytes_per_launch = 16.0 * units_per_launch + 8.0 * outputs_per_launch
-    achieved = bytes_per_launch / (kd["avg_launch_ms"] * 1e-3) / 1e9
-    # the limiter of the gather kernels: each tap is one 16-byte load and
-    # the texture-address unit of a CU takes 4 such lanes per clock
-    cus = torch.cuda.get_device_properties(dev).multi_processor_count
-    taps_per_clk = units_per_launch / (kd["avg_launch_ms"] * 1e-3) / (cus * ENGINE_CLOCK_GHZ * 1e9)
     for k, v in kernels.items():
         u = v["units"] / max(v["launches"], 1)
         o = u / (n if k.startswith("fp") else na)
-        v["algorithmic_GBps"] = (16.0 * u + 8.0 * o) / (v["avg_launch_ms"] * 1e-3) / 1e9
-        v["Gupdates_per_s"] = u / (v["avg_launch_ms"] * 1e-3) / 1e9
+        t = v["avg_launch_ms"] * 1e-3
+        v["algorithmic_bytes_per_launch"] = 16.0 * u + 8.0 * o
+        v["algorithmic_GBps"] = v["algorithmic_bytes_per_launch"] / t / 1e9
+        v["Gupdates_per_s"] = u / t / 1e9
+        tr = profile_value("traffic_bytes_per_launch", k, n, na, nd)
+        v["hbm_traffic_bytes_per_launch"] = tr
+        v["hbm_GBps"] = tr / t / 1e9 if tr else None
+        v["hbm_frac"] = tr / t / 1e9 / HBM_PEAK_GBS if tr else None
+        vm = profile_value("vmem_rd_insts_per_launch", k, n, na, nd)
+        v["issued_lane_loads_per_launch"] = 64.0 * vm if vm else None
+        v["issued_lane_loads_per_s"] = 64.0 * vm / t if vm else None
+    kd = kernels[dom]
+    limits = gather_limits()
+    log("bench: gather calibration %.3e lane-loads/s (taps), %.3e (coalesced)"
+        % (limits["taps"], limits["coalesced"]))
+    issued = kd["issued_lane_loads_per_s"]
+    if kd["hbm_GBps"] is not None:
+        achieved, achieved_kind = kd["hbm_GBps"], "measured HBM traffic"
+    else:
+        achieved, achieved_kind = kd["algorithmic_GBps"], "algorithmic bytes (no counters for this workload)"
+    roofline = {
+        "bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBS, "traffic": kd["hbm_traffic_bytes_per_launch"],
+        "achieved_kind": achieved_kind,
+        "traffic_source": TRAFFIC_FILE if kd["hbm_traffic_bytes_per_launch"] else None,
+        "avg_launch_ms": kd["avg_launch_ms"],
+        "algorithmic": {"bytes_per_launch": kd["algorithmic_bytes_per_launch"],
+                        "GBps": kd["algorithmic_GBps"],
+                        "note": "16 B per bilinear sample + 8 B per output; served by L1/L2, "
+                                "so above the HBM peak"},
+        "per_kernel_hbm_frac": {k: v["hbm_frac"] for k, v in kernels.items()},
+        "texture_path": {
+            "issued_lane_loads_per_launch": kd["issued_lane_loads_per_launch"],
+            "issued_lane_loads_per_s": issued,
+            "limit_lane_loads_per_s": limits["taps"],
+            "coalesced_lane_loads_per_s": limits["coalesced"],
+            "taps_2d_8byte_lane_loads_per_s": limits["taps_2d_8byte"],
+            "ta_busy_frac": profile_value("ta_busy_frac", dom, n, na, nd),
+            "frac": issued / limits["taps"] if issued else None,
+            "note": "issued = SQ_INSTS_VMEM_RD x 64 per launch (profile) over the live launch "
+                    "time; limit = astra_amd_calibrate_gather(mode 0) measured in this run: "
+                    "16-byte loads of a 16 x 4 ray bundle two thirds of a texel apart, from "
+                    "L1-resident data; ta_busy_frac = TA_BUSY_avr over the kernel's cycles "
+                    "(profile)"},
+    }
 
     fdk_s = None
     if not args.no_fdk:
@@ -529,7 +629,24 @@ def main():
         fdk(P, pcopy, out, _lib.MODE_SET)
         torch.cuda.synchronize(dev)
         fdk_s = time.perf_counter() - t
-        log("bench: FDK %.3f s" % fdk_s)
+        # the row filter's own time, from a second (event-timed) run
+        pcopy.copy_(proj)
+        _lib.profile_enable(True)
+        fdk(P, pcopy, out, _lib.MODE_SET)
+        ms, rows, launches = _lib.profile_read("filter_rows")
+        _lib.profile_enable(False)
+        n_fft = 1
+        while n_fft < 2 * nd:
+            n_fft *= 2
+        kernels["filter_rows"] = {
+            "total_ms": ms, "units": rows, "launches": launches,
+            "avg_launch_ms": ms / max(launches, 1),
+            # rows read and written once (fp32), pre-weights read
+            "hbm_bytes_per_launch": rows * nd * 8.0 / max(launches, 1),
+            "GBps": rows * nd * 8.0 / max(ms * 1e-3, 1e-12) / 1e9,
+            # 5 N log2 N per complex transform, two per row pair
+            "GFLOPs": rows / 2.0 * 2 * 5.0 * n_fft * np.log2(n_fft) / max(ms * 1e-3, 1e-12) / 1e9}
+        log("bench: FDK %.3f s (row filter %.2f ms)" % (fdk_s, ms / max(launches, 1)))
         del pcopy, out
 
     del proj, bpv
@@ -568,17 +685,7 @@ def main():
                                "detector, pixel 1.0, SOD %g, ODD %g" % (n, na, nd, nd, sod, odd),
                    "volume": [n, n, n], "angles": na, "detector": [nd, nd],
                    "source_origin": sod, "origin_detector": odd},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(dom, n, na, nd),
-                     "traffic_source": TRAFFIC_FILE if measured_traffic(dom, n, na, nd) else None,
-                     "bytes_per_launch": bytes_per_launch,
-                     "avg_launch_ms": kd["avg_launch_ms"],
-                     "texture_path": {"taps_per_clk_per_cu": taps_per_clk, "limit": 4.0,
-                                      "frac": taps_per_clk / 4.0, "cus": cus,
-                                      "clock_GHz": ENGINE_CLOCK_GHZ,
-                                      "note": "16-byte tap loads, 4 lanes per clock per CU "
-                                              "at the peak engine clock"}},
+        "roofline": roofline,
         "fdk_seconds": fdk_s,
         "batched_sirt": batched_res,
         "differentiable": diff_res,

@@ -291,6 +291,27 @@ int astra_amd_profile_enable(int on);
 int astra_amd_profile_read(const char *kernel, double *total_ms, double *units,
                            unsigned long long *launches);
 
+/* Device memory of the library's stream-ordered scratch (the current
+ * device's default memory pool, which also serves any other
+ * hipMallocAsync user in the process): bytes in use now and the
+ * high-water mark since the last reset; reset_peak != 0 resets the mark
+ * to the current use after reading.  No reference counterpart (a
+ * measurement utility). */
+int astra_amd_scratch_usage(size_t *in_use, size_t *peak, int reset_peak);
+/* Ceiling of the CU's vector memory path for the projectors' tap loads
+ * (no reference counterpart; a measurement utility): every wave of a full
+ * grid issues only global loads from a 4 KiB window per workgroup that
+ * stays in L1.  mode 0: 16-byte loads (global_load_dwordx4) with the 3D
+ * taps' lane pattern (16 columns x 4 rows of 8-byte pairs, one texel of
+ * per-lane jitter); mode 1: 16-byte loads, coalesced (1 KiB per wave);
+ * mode 2: 8-byte loads at 4-byte alignment with the same lane pattern
+ * (the 2D projectors' linear fetch); mode 3: 4-byte loads, same pattern.
+ * Returns lane-loads per second (each lane of each load instruction
+ * counts once), best of 3 timed launches after a warm-up.  Synchronous;
+ * allocates and frees its own 1 MiB buffer. */
+int astra_amd_calibrate_gather(int mode, unsigned int iterations, double *lane_loads_per_second,
+                               void *stream);
+
 /* ---------------------------------------------------- elementwise ---- */
 /* Sum of a[i]*b[i] accumulated in float64 (dotProduct2D / dotProduct3D,
  * cuda/2d/util.cu:304-357, cuda/3d/mem3d.cu:581-590). Synchronous. */
