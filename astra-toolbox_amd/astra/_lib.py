@@ -51,6 +51,7 @@ SIGNATURES = {
     "astra_amd_geom3d_create": [_i, _u, _u, _u, _u, _u, _u, _vp, _vp, ctypes.POINTER(_vp)],
     "astra_amd_geom3d_destroy": [_vp],
     "astra_amd_fp2d": [_vp, _vp, _vp, ctypes.POINTER(Params2D), _i, _vp],
+    "astra_amd_fp2d_noalloc": [_vp, _vp, _vp, ctypes.POINTER(Params2D), _i, _vp],
     "astra_amd_bp2d": [_vp, _vp, _vp, ctypes.POINTER(Params2D), _i, _vp],
     "astra_amd_bp2d_fbp_weighted": [_vp, _vp, _vp, ctypes.POINTER(Params2D), _i, _vp],
     "astra_amd_bp2d_matched": [_vp, _vp, _vp, ctypes.POINTER(Params2D), _i, _vp],

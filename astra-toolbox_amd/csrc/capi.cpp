@@ -183,6 +183,17 @@ int astra_amd_fp2d(astra_amd_geom2d g, const float *vol, float *sino,
     });
 }
 
+int astra_amd_fp2d_noalloc(astra_amd_geom2d g, const float *vol, float *sino,
+                           const astra_amd_params2d *p, int mode, void *stream) {
+    return guarded([&] {
+        check2d(g, p);
+        AMX_REQUIRE(vol && sino, "fp2d_noalloc: NULL data pointer");
+        unsigned rays = p->rays_per_det ? p->rays_per_det : 1;
+        amx::fp2d_launch_noalloc(*g->g, vol, sino, p->output_scale, out_mode(mode), rays,
+                                 as_stream(stream));
+    });
+}
+
 int astra_amd_bp2d(astra_amd_geom2d g, const float *sino, float *vol,
                    const astra_amd_params2d *p, int mode, void *stream) {
     return guarded([&] {

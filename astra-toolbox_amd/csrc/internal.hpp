@@ -235,6 +235,14 @@ void fp2d_launch(const Geom2D &g, const float *vol, float *sino, float out_scale
 void fp2d_launch_range(const Geom2D &g, const float *vol, float *sino, float out_scale, int mode,
                        const float *resid_b, const float *resid_w, bool padded_out, unsigned rays,
                        unsigned a0, unsigned na_sub, hipStream_t s);
+size_t fp2d_copies_elems(const Geom2D &g);
+void fp2d_run(const Geom2D &g, const float *vol, float *sino, float out_scale, int mode,
+              const float *resid_b, const float *resid_w, bool padded_out, unsigned rays,
+              unsigned a0, unsigned na_sub, float *copies, bool prof, hipStream_t s);
+// FP with the volume copies in a fixed per-GPU array (no allocation)
+void fp2d_launch_noalloc(const Geom2D &g, const float *vol, float *sino, float out_scale,
+                         int mode, unsigned rays, hipStream_t s);
+size_t fp2d_noalloc_max_floats();
 // SART's unweighted BP of one padded sinogram row: vol = clamp(vol + w * BP)
 void bp2d_sart_launch(const Geom2D &g, const float *sp_row, float *vol, unsigned angle,
                       float out_scale, const float *upd_w, float lo, float hi, hipStream_t s);

@@ -14,6 +14,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
+#include <vector>
 
 #include "proj2d_dev.hpp"
 
@@ -256,8 +258,22 @@ void fp2d_launch_range(const Geom2D &g, const float *vol, float *sino, float out
                        unsigned a0, unsigned na_sub, hipStream_t s) {
     AMX_REQUIRE(rays >= 1 && rays <= 64, "fp2d: rays per detector must be in [1, 64]");
     AMX_REQUIRE(a0 + na_sub <= g.na && na_sub > 0, "fp2d: angle range out of bounds");
-    Scratch copies((rows_elems(g) + cols_elems(g)) * sizeof(float), s);
-    float *rows = copies.as<float>();
+    Scratch copies(fp2d_copies_elems(g) * sizeof(float), s);
+    fp2d_run(g, vol, sino, out_scale, mode, resid_b, resid_w, padded_out, rays, a0, na_sub,
+             copies.as<float>(), true, s);
+}
+
+size_t fp2d_copies_elems(const Geom2D &g) { return rows_elems(g) + cols_elems(g); }
+
+// FP through the caller's copy buffer `copies` (fp2d_copies_elems(g) floats,
+// every element written here before it is read).  `prof` = false records no
+// timing events.
+void fp2d_run(const Geom2D &g, const float *vol, float *sino, float out_scale, int mode,
+              const float *resid_b, const float *resid_w, bool padded_out, unsigned rays,
+              unsigned a0, unsigned na_sub, float *copies, bool prof, hipStream_t s) {
+    AMX_REQUIRE(rays >= 1 && rays <= 64, "fp2d: rays per detector must be in [1, 64]");
+    AMX_REQUIRE(a0 + na_sub <= g.na && na_sub > 0, "fp2d: angle range out of bounds");
+    float *rows = copies;
     float *cols = rows + rows_elems(g);
     dim3 pg((g.w + 2 * kPad + 63) / 64, (g.h + 2 * kPad + 63) / 64);
     vol2d_copies<<<pg, 256, 0, s>>>(vol, rows, cols, g.w, g.h);
@@ -278,14 +294,60 @@ void fp2d_launch_range(const Geom2D &g, const float *vol, float *sino, float out
     A.rw = resid_w ? resid_w + (size_t)a0 * out_pitch : nullptr;
     A.padded_out = padded_out ? 1 : 0;
     dim3 grid((g.nd + 63) / 64, (na_sub + 3) / 4);
-    ProfToken pt = prof_begin(g.kind == ASTRA_AMD_GEOM_FAN ? "fp2d_fan" : "fp2d_par",
-                              (double)na_sub * g.nd * rays * std::max(g.w, g.h), s);
+    ProfToken pt{};
+    if (prof)
+        pt = prof_begin(g.kind == ASTRA_AMD_GEOM_FAN ? "fp2d_fan" : "fp2d_par",
+                        (double)na_sub * g.nd * rays * std::max(g.w, g.h), s);
     if (g.kind == ASTRA_AMD_GEOM_FAN)
         fp2d_kernel<true><<<grid, 256, 0, s>>>(A);
     else
         fp2d_kernel<false><<<grid, 256, 0, s>>>(A);
     AMX_LAUNCHED();
-    prof_end(pt, s);
+    if (prof) prof_end(pt, s);
+}
+
+// ---- FP that allocates nothing --------------------------------------------
+// For callers that may not allocate device memory (code under stream
+// capture, serving loops, harnesses that forbid hipMalloc): the volume
+// copies live in this fixed array, one per GPU, held until the process
+// ends.  Every call on a GPU shares that GPU's array, so calls that overlap
+// on one GPU (two streams, two host threads) race; nothing in it needs to
+// start at zero (fp2d_run writes every element it reads).
+constexpr size_t kNoAllocFloats = size_t(16) << 20;  // 64 MiB
+static __device__ __align__(256) float scratch[kNoAllocFloats];
+
+namespace {
+std::mutex g_scratch_mu;
+std::vector<float *> g_scratch_ptr;  // per device: address of that GPU's array
+
+// The array's device address on the current device.  The host-side symbol
+// `scratch` is only a handle: it is never passed to a kernel or a copy.
+float *noalloc_scratch() {
+    int dev = -1;
+    AMX_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if ((size_t)dev >= g_scratch_ptr.size()) g_scratch_ptr.resize(dev + 1, nullptr);
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        AMX_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        AMX_REQUIRE(ptr != nullptr, "fp2d_noalloc: no device address for the scratch array");
+        g_scratch_ptr[dev] = static_cast<float *>(ptr);
+    }
+    return g_scratch_ptr[dev];
+}
+}  // namespace
+
+size_t fp2d_noalloc_max_floats() { return kNoAllocFloats; }
+
+void fp2d_launch_noalloc(const Geom2D &g, const float *vol, float *sino, float out_scale,
+                         int mode, unsigned rays, hipStream_t s) {
+    const size_t need = fp2d_copies_elems(g);
+    AMX_REQUIRE(need <= kNoAllocFloats,
+                "fp2d_noalloc: a %u x %u volume needs %zu floats of scratch, more than the "
+                "fixed %zu (64 MiB); use astra_amd_fp2d",
+                g.w, g.h, need, kNoAllocFloats);
+    fp2d_run(g, vol, sino, out_scale, mode, nullptr, nullptr, false, rays, 0, g.na,
+             noalloc_scratch(), false, s);
 }
 
 void bp2d_launch_padded(const Geom2D &g, const float *sp, float *vol, float out_scale, bool fbp,

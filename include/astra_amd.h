@@ -147,6 +147,17 @@ int astra_amd_geom3d_destroy(astra_amd_geom3d g);
  * cuda/2d/mem2d.cu:224-261). */
 int astra_amd_fp2d(astra_amd_geom2d g, const float *vol, float *sino,
                    const astra_amd_params2d *p, int mode, void *stream);
+/* 2D forward projection that allocates no device memory.  No counterpart
+ * in the reference: it is for callers that cannot call hipMalloc (code
+ * under stream capture, tight serving loops, harnesses that forbid
+ * allocation).  Same arguments and result as astra_amd_fp2d; the volume's
+ * two padded copies live in a fixed 64 MiB array per GPU instead of
+ * stream-ordered scratch, so h * (w + 4) + w * (h + 4) must not exceed
+ * 16 Mi floats (about 2040 x 2040), else ASTRA_AMD_ERR_ARG.  The array is
+ * shared by every call on that GPU: calls that overlap on one GPU (other
+ * streams, other host threads) race. */
+int astra_amd_fp2d_noalloc(astra_amd_geom2d g, const float *vol, float *sino,
+                           const astra_amd_params2d *p, int mode, void *stream);
 /* 2D back projection: astraCUDA::BP (include/astra/cuda/2d/mem2d.h:81-83,
  * cuda/2d/mem2d.cu:278-315). */
 int astra_amd_bp2d(astra_amd_geom2d g, const float *sino, float *vol,
