@@ -16,6 +16,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <limits>
 #include <memory>
 #include <mutex>
@@ -125,6 +126,43 @@ __global__ __launch_bounds__(256) void batch_fp(BatchArgs B, const float *__rest
     }
 }
 
+// The same over 64 detectors x 16 angles per workgroup with the slice rows
+// staged in LDS (fp2d_tile_lds).
+__global__ __launch_bounds__(256) void batch_fp_lds(BatchArgs B, const float *__restrict__ rows,
+                                                    const float *__restrict__ cols,
+                                                    const float *__restrict__ sinos,
+                                                    const float *__restrict__ lw,
+                                                    float *__restrict__ out, int mode) {
+    __shared__ Fp2Lds L;
+    const unsigned item = blockIdx.x;
+    const unsigned s = find_slice(B.prefix, B.n, item);
+    const SliceDesc d = B.desc[s];
+    const unsigned local = item - B.prefix[s];
+    const unsigned dblocks = (d.nd + 63) / 64;
+    const int det = (local % dblocks) * 64 + (threadIdx.x & 63);
+    const int a0 = (local / dblocks) * 16 + (threadIdx.x >> 6) * kFp2Rays;
+    const FP2DAngle *fpa = B.fpa + d.ang_off;
+    const float *r = rows + d.rows_off, *c = cols + d.cols_off;
+    float val[kFp2Rays];
+    if (d.kind == ASTRA_AMD_GEOM_FAN)
+        fp2d_tile_lds<true>(det, a0, d.na, d.nd, d.w, d.h, fpa, r, c, val, L);
+    else
+        fp2d_tile_lds<false>(det, a0, d.na, d.nd, d.w, d.h, fpa, r, c, val, L);
+    if (det >= (int)d.nd) return;
+#pragma unroll
+    for (int j = 0; j < kFp2Rays; ++j) {
+        const int angle = a0 + j;
+        if (angle >= (int)d.na) break;
+        const size_t oidx = d.psino_off + (size_t)angle * (d.nd + 2 * kPad) + det + kPad;
+        if (mode == OUT_SET) {
+            out[oidx] = val[j] * d.osc;
+        } else {
+            const size_t idx = d.sino_off + (size_t)angle * d.nd + det;
+            out[oidx] = (sinos[idx] - val[j] * d.osc) * lw[oidx];
+        }
+    }
+}
+
 // ------------------------------------------------------------------ BP ---
 // 64 columns x (4 * PY) rows per workgroup.  mode OUT_SET writes
 // osc * A^T r; OUT_UPDATE writes clamp(x + pw * osc * A^T r).
@@ -156,15 +194,47 @@ __global__ __launch_bounds__(256) void batch_bp(BatchArgs B, const float *__rest
     }
 }
 
+// The same with the sinogram footprints staged in LDS (bp2d_tile_lds).
+template <int PY>
+__global__ __launch_bounds__(256) void batch_bp_lds(BatchArgs B, const float *__restrict__ psino,
+                                                    const float *__restrict__ pw,
+                                                    float *__restrict__ vols, int mode, float lo,
+                                                    float hi) {
+    __shared__ Bp2Lds L;
+    const unsigned item = blockIdx.x;
+    const unsigned s = find_slice(B.prefix, B.n, item);
+    const SliceDesc d = B.desc[s];
+    const unsigned local = item - B.prefix[s];
+    const unsigned xblocks = (d.w + 63) / 64;
+    const int X0 = (local % xblocks) * 64;
+    const int Yt0 = (local / xblocks) * 4 * PY;
+    float acc[PY];
+    const float *sp = psino + d.psino_off;
+    const BP2DAngle *ang = B.bpa + d.ang_off;
+    if (d.kind == ASTRA_AMD_GEOM_FAN)
+        bp2d_tile_lds<1, PY>(X0, Yt0, d.w, d.h, d.na, d.nd, sp, ang, acc, L);
+    else
+        bp2d_tile_lds<0, PY>(X0, Yt0, d.w, d.h, d.na, d.nd, sp, ang, acc, L);
+    const int X = X0 + (threadIdx.x & 63);
+    const int Y0 = Yt0 + (threadIdx.x >> 6) * PY;
+    if (X >= (int)d.w || Y0 >= (int)d.h) return;
+    for (int i = 0; i < PY; ++i) {
+        const int Y = Y0 + i;
+        if (Y >= (int)d.h) break;
+        const size_t idx = d.vol_off + (size_t)Y * d.w + X;
+        store_bp(vols, idx, acc[i] * d.osc, mode, pw, lo, hi);
+    }
+}
+
 // ------------------------------------------------------------ handle -----
 struct Batch2D {
     unsigned n = 0;
     int device = 0;
     std::vector<SliceDesc> host;
     size_t vol_total = 0, sino_total = 0, psino_total = 0, rows_total = 0, cols_total = 0;
-    std::vector<unsigned> pre_prep, pre_pad, pre_fp[3], pre_bp[3];
+    std::vector<unsigned> pre_prep, pre_pad, pre_fp[4], pre_bp[3];
     DevArray desc, fpa, bpa;
-    DevArray d_prep, d_pad, d_fp[3], d_bp[3];
+    DevArray d_prep, d_pad, d_fp[4], d_bp[3];
     // work buffers (built on first use)
     DevArray line_w, pixel_w, resid, rows, cols, ones;
     bool weights_ready = false;
@@ -221,6 +291,7 @@ Batch2D *batch2d_build(unsigned n, const int *kinds, const unsigned *ws, const u
         b->pre_fp[k].assign(n + 1, 0);
         b->pre_bp[k].assign(n + 1, 0);
     }
+    b->pre_fp[3].assign(n + 1, 0);
     size_t ang = 0;
     for (unsigned i = 0; i < n; ++i) {
         SliceDesc &d = b->host[i];
@@ -255,6 +326,7 @@ Batch2D *batch2d_build(unsigned n, const int *kinds, const unsigned *ws, const u
             const unsigned rows_per_group = 4 * kBPPY[k];
             acc(b->pre_bp[k], (size_t)((d.w + 63) / 64) * ((d.h + rows_per_group - 1) / rows_per_group));
         }
+        acc(b->pre_fp[3], (size_t)((d.nd + 63) / 64) * ((d.na + 15) / 16));
     }
     b->desc = DevArray(sizeof(SliceDesc) * n);
     b->desc.upload(b->host.data(), sizeof(SliceDesc) * n);
@@ -272,6 +344,7 @@ Batch2D *batch2d_build(unsigned n, const int *kinds, const unsigned *ws, const u
         up(b->d_fp[k], b->pre_fp[k]);
         up(b->d_bp[k], b->pre_bp[k]);
     }
+    up(b->d_fp[3], b->pre_fp[3]);
     return b.release();
 }
 
@@ -326,22 +399,41 @@ void launch_fp(const Batch2D &b, int cfg, const float *sinos, float *out, int mo
         batch_fp<64><<<g, 256, 0, s>>>(a, rows, cols, sinos, lw, out, mode);
     else if (cfg == 1)
         batch_fp<128><<<g, 256, 0, s>>>(a, rows, cols, sinos, lw, out, mode);
-    else
+    else if (cfg == 2)
         batch_fp<32><<<g, 256, 0, s>>>(a, rows, cols, sinos, lw, out, mode);
+    else
+        batch_fp_lds<<<g, 256, 0, s>>>(a, rows, cols, sinos, lw, out, mode);
     AMX_LAUNCHED();
 }
 
+// FP candidates: kFPDB tilings (0..2) and the LDS-staged one (3); the
+// autotuner times the first kFPTimed (the others by request only)
+constexpr int kFPCandidates = 4;
+constexpr int kFPTimed = 3;
+
+// BP candidates: cfg % 3 picks the rows per thread (kBPPY), cfg >= 3 the
+// LDS-staged form of the same tiling
+constexpr int kBPCandidates = 6;
+constexpr int kBPTimed = 3;
+
 void launch_bp(const Batch2D &b, int cfg, const float *psino, float *vols, int mode, float lo,
                float hi, hipStream_t s) {
-    BatchArgs a = args(b, b.d_bp[cfg]);
-    const unsigned g = b.pre_bp[cfg][b.n];
+    const int t = cfg % 3;
+    BatchArgs a = args(b, b.d_bp[t]);
+    const unsigned g = b.pre_bp[t][b.n];
     const float *pw = b.pixel_w.as<float>();
     if (cfg == 0)
         batch_bp<4><<<g, 256, 0, s>>>(a, psino, pw, vols, mode, lo, hi);
     else if (cfg == 1)
         batch_bp<2><<<g, 256, 0, s>>>(a, psino, pw, vols, mode, lo, hi);
-    else
+    else if (cfg == 2)
         batch_bp<8><<<g, 256, 0, s>>>(a, psino, pw, vols, mode, lo, hi);
+    else if (cfg == 3)
+        batch_bp_lds<4><<<g, 256, 0, s>>>(a, psino, pw, vols, mode, lo, hi);
+    else if (cfg == 4)
+        batch_bp_lds<2><<<g, 256, 0, s>>>(a, psino, pw, vols, mode, lo, hi);
+    else
+        batch_bp_lds<8><<<g, 256, 0, s>>>(a, psino, pw, vols, mode, lo, hi);
     AMX_LAUNCHED();
 }
 
@@ -367,7 +459,7 @@ void autotune(Batch2D &b, hipStream_t s) {
     float best_fp = 1e30f, best_bp = 1e30f;
     float *scratch = b.resid.as<float>();
     launch_prep(b, b.ones.as<float>(), s);
-    for (int k = 0; k < 3; ++k) {
+    for (int k = 0; k < kFPTimed; ++k) {
         float ms = 0.f;
         launch_fp(b, k, nullptr, scratch, OUT_SET, s);  // warm
         AMX_HIP(hipEventRecord(e0, s));
@@ -381,7 +473,7 @@ void autotune(Batch2D &b, hipStream_t s) {
         }
     }
     float *vtmp = b.ones.as<float>();
-    for (int k = 0; k < 3; ++k) {
+    for (int k = 0; k < kBPTimed; ++k) {
         float ms = 0.f;
         launch_bp(b, k, scratch, vtmp, OUT_SET, 0.f, 0.f, s);
         AMX_HIP(hipEventRecord(e0, s));
@@ -396,6 +488,12 @@ void autotune(Batch2D &b, hipStream_t s) {
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    // ASTRA_AMD_BATCH_BP_CFG=<0..5> / ASTRA_AMD_BATCH_FP_CFG=<0..3> replace
+    // the timed choice (read when a batch is first run)
+    const char *eb = std::getenv("ASTRA_AMD_BATCH_BP_CFG");
+    if (eb && *eb) b.bp_cfg = std::min(std::max(std::atoi(eb), 0), kBPCandidates - 1);
+    const char *ef = std::getenv("ASTRA_AMD_BATCH_FP_CFG");
+    if (ef && *ef) b.fp_cfg = std::min(std::max(std::atoi(ef), 0), kFPCandidates - 1);
 }
 
 void iterate(const Batch2D &b, const float *sinos, float *vols, int iters, float lo, float hi,

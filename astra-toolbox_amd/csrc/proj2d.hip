@@ -14,6 +14,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <mutex>
 #include <vector>
 
@@ -92,6 +93,25 @@ __global__ __launch_bounds__(256) void bp2d_kernel(BP2DArgs A) {
     if (X >= (int)A.w || Y0 >= (int)A.h) return;
     float acc[PY];
     bp2d_column<KIND, PY>(X, Y0, A.w, A.h, A.na, A.nd, A.sp, A.ang, acc);
+    for (int i = 0; i < PY; ++i) {
+        const int Y = Y0 + i;
+        if (Y >= (int)A.h) break;
+        store_bp(A.vol, (size_t)Y * A.w + X, acc[i] * A.osc, A.mode, A.uw, A.lo, A.hi);
+    }
+}
+
+// The same with the sinogram footprints staged in LDS (bp2d_tile_lds):
+// equal floats, taps from LDS instead of the texture-address path.
+template <int KIND>
+__global__ __launch_bounds__(256) void bp2d_lds_kernel(BP2DArgs A) {
+    constexpr int PY = 4;
+    __shared__ Bp2Lds L;
+    const int X0 = blockIdx.x * 64, Yt0 = blockIdx.y * 4 * PY;
+    float acc[PY];
+    bp2d_tile_lds<KIND, PY>(X0, Yt0, A.w, A.h, A.na, A.nd, A.sp, A.ang, acc, L);
+    const int X = X0 + (threadIdx.x & 63);
+    const int Y0 = Yt0 + (threadIdx.x >> 6) * PY;
+    if (X >= (int)A.w || Y0 >= (int)A.h) return;
     for (int i = 0; i < PY; ++i) {
         const int Y = Y0 + i;
         if (Y >= (int)A.h) break;
@@ -350,6 +370,13 @@ void fp2d_launch_noalloc(const Geom2D &g, const float *vol, float *sino, float o
              noalloc_scratch(), false, s);
 }
 
+// ASTRA_AMD_BP2D_LDS=1 takes the taps of the 2D BP from LDS-staged
+// footprints instead of global memory (read at every launch)
+bool bp2d_lds() {
+    const char *e = std::getenv("ASTRA_AMD_BP2D_LDS");
+    return e && e[0] == '1';
+}
+
 void bp2d_launch_padded(const Geom2D &g, const float *sp, float *vol, float out_scale, bool fbp,
                         int mode, const float *upd_w, float lo, float hi, hipStream_t s,
                         unsigned ss) {
@@ -375,13 +402,16 @@ void bp2d_launch_padded(const Geom2D &g, const float *sp, float *vol, float out_
             bp2d_ss_kernel<1><<<sgrid, 256, 0, s>>>(A, (int)ss);
     } else if (g.kind == ASTRA_AMD_GEOM_PAR) {
         A.ang = g.bp.as<BP2DAngle>();
-        bp2d_kernel<0><<<grid, 256, 0, s>>>(A);
+        if (bp2d_lds()) bp2d_lds_kernel<0><<<grid, 256, 0, s>>>(A);
+        else bp2d_kernel<0><<<grid, 256, 0, s>>>(A);
     } else if (!fbp) {
         A.ang = g.bp.as<BP2DAngle>();
-        bp2d_kernel<1><<<grid, 256, 0, s>>>(A);
+        if (bp2d_lds()) bp2d_lds_kernel<1><<<grid, 256, 0, s>>>(A);
+        else bp2d_kernel<1><<<grid, 256, 0, s>>>(A);
     } else {
         A.ang = g.bp_fbp.as<BP2DAngle>();
-        bp2d_kernel<2><<<grid, 256, 0, s>>>(A);
+        if (bp2d_lds()) bp2d_lds_kernel<2><<<grid, 256, 0, s>>>(A);
+        else bp2d_kernel<2><<<grid, 256, 0, s>>>(A);
     }
     AMX_LAUNCHED();
     prof_end(pt, s);

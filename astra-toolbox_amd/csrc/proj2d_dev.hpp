@@ -73,12 +73,31 @@ __device__ __forceinline__ Ray2D ray2d(const FP2DAngle &G, double fdet, bool ver
 
 // Sum of linear samples along a ray over slices [lo, hi] of a padded
 // slice-major copy (slice pitch `pitch`, coordinate clamp `hi_t`).
+// Samples at even offsets from lo go to acc0, odd ones to acc1; four loads
+// are issued before the first of them is used (the loop is bound by the
+// loads' latency).
 __device__ __forceinline__ float march2d(const float *__restrict__ base, unsigned pitch,
                                          float hi_t, const Ray2D &R) {
     float acc0 = 0.f, acc1 = 0.f;
     const float *p = base + (size_t)(R.lo > 0 ? R.lo : 0) * pitch;
     int s = R.lo;
-    for (; s + 1 <= R.hi; s += 2) {
+    for (; s + 3 <= R.hi; s += 4) {
+        float2 q[4];
+        float wt[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const float t = clamp_pad(fmaf(R.a, (float)(s + k), R.c), hi_t);
+            const float f = floorf(t);
+            q[k] = ld2b(p + k * pitch, (unsigned)f << 2);
+            wt[k] = t - f;
+        }
+        acc0 += lerp(q[0].x, q[0].y, wt[0]);
+        acc1 += lerp(q[1].x, q[1].y, wt[1]);
+        acc0 += lerp(q[2].x, q[2].y, wt[2]);
+        acc1 += lerp(q[3].x, q[3].y, wt[3]);
+        p += 4 * pitch;
+    }
+    if (s + 1 <= R.hi) {
         const float sf = (float)s;
         float t0 = clamp_pad(fmaf(R.a, sf, R.c), hi_t);
         float t1 = clamp_pad(fmaf(R.a, sf + 1.f, R.c), hi_t);
@@ -87,6 +106,7 @@ __device__ __forceinline__ float march2d(const float *__restrict__ base, unsigne
         acc0 += lerp(q0.x, q0.y, t0 - f0);
         acc1 += lerp(q1.x, q1.y, t1 - f1);
         p += 2 * pitch;
+        s += 2;
     }
     if (s <= R.hi) {
         float t0 = clamp_pad(fmaf(R.a, (float)s, R.c), hi_t);
@@ -156,17 +176,298 @@ __device__ __forceinline__ void bp2d_column(int X, int Y0, unsigned w, unsigned 
         const BP2DAngle c = ang[a];
         float num = fmaf(fy0, c.num[2], fmaf(fx, c.num[1], c.num[0]));
         float den = fmaf(fy0, c.den[2], fmaf(fx, c.den[1], c.den[0]));
+        // every tap load of the PY rows issued before the first use (the
+        // loop is bound by the loads' latency)
+        float2 q[PY];
+        float wt[PY], wg[PY];
 #pragma unroll
         for (int i = 0; i < PY; ++i) {
             float fr = (KIND == 0 || KIND == 3) ? 1.f : __builtin_amdgcn_rcpf(den);
             float t = clamp_pad(num * fr, hi);
             float f = floorf(t);
-            float2 q = ld2b(row, (unsigned)f << 2);
-            float val = lerp(q.x, q.y, t - f);
-            float wgt = (KIND == 0) ? c.w : (KIND == 1 ? fr : (KIND == 2 ? fr * fr : 1.f));
-            acc[i] = fmaf(wgt, val, acc[i]);
+            q[i] = ld2b(row, (unsigned)f << 2);
+            wt[i] = t - f;
+            wg[i] = (KIND == 0) ? c.w : (KIND == 1 ? fr : (KIND == 2 ? fr * fr : 1.f));
             num -= c.num[2];  // next row: world y decreases by 1
             den -= c.den[2];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int i = 0; i < PY; ++i) acc[i] = fmaf(wg[i], lerp(q[i].x, q[i].y, wt[i]), acc[i]);
+    }
+}
+
+// ---- forward projection with slice rows staged in LDS -------------------
+// A workgroup of 256 threads covers 64 detectors x 16 angles: thread t takes
+// detector d0 + (t & 63) at the 4 angles a0 + 4 (t >> 6) + j, j < 4.  When
+// every ray of the group runs along the same axis (rows or columns), the
+// group marches the slices in lockstep, kFp2SC at a time: the interval of
+// padded texels that its rays sample in those slices (bounding interval of
+// each ray's coordinate at the chunk's ends, reduced over the group,
+// widened by one texel) is copied into LDS with coalesced loads, and the
+// samples read LDS.  A chunk whose interval exceeds kFp2Wcap texels, and a
+// group with rays along both axes, sample global memory.  Each ray sums its
+// samples exactly as march2d does (alternating accumulators from its own
+// first slice), so the values equal fp2d_value's.
+constexpr int kFp2SC = 16;     // slices per staged chunk
+constexpr int kFp2Wcap = 256;  // texels per staged slice row
+constexpr int kFp2Rays = 4;    // angles per thread
+
+struct Fp2Lds {
+    float buf[kFp2SC * kFp2Wcap];
+    int red[2][2][4];  // [chunk parity][min, max][wave]
+};
+
+template <bool FAN>
+__device__ __forceinline__ void fp2d_tile_lds(int det, int a0, unsigned na, unsigned nd, unsigned w,
+                                              unsigned h, const FP2DAngle *__restrict__ fpa,
+                                              const float *__restrict__ rows,
+                                              const float *__restrict__ cols, float (&val)[kFp2Rays],
+                                              Fp2Lds &L) {
+    const unsigned tid = threadIdx.x, wave = tid >> 6;
+    Ray2D R[kFp2Rays];
+    bool on[kFp2Rays];
+    int vert = 0, horiz = 0;
+#pragma unroll
+    for (int j = 0; j < kFp2Rays; ++j) {
+        const int angle = a0 + j;
+        on[j] = det < (int)nd && angle < (int)na;
+        R[j].a = 0.f;
+        R[j].c = 0.5f;
+        R[j].scale = 0.f;
+        R[j].lo = 1;
+        R[j].hi = 0;
+        val[j] = 0.f;
+        if (on[j]) {
+            const FP2DAngle &G = fpa[angle];
+            const bool v = ray_vertical<FAN>(G, det);
+            R[j] = ray2d<FAN>(G, det + 0.5, v, w, h);
+            vert |= v ? 1 : 0;
+            horiz |= v ? 0 : 1;
+        }
+    }
+    const bool any_v = __syncthreads_or(vert), any_h = __syncthreads_or(horiz);
+    if (any_v && any_h) {
+        // rays along both axes: each ray from global memory
+#pragma unroll
+        for (int j = 0; j < kFp2Rays; ++j) {
+            if (!on[j]) continue;
+            const bool v = ray_vertical<FAN>(fpa[a0 + j], det);
+            const float s = v ? march2d(rows, w + 2 * kPad, (float)w + 2.5f, R[j])
+                              : march2d(cols, h + 2 * kPad, (float)h + 2.5f, R[j]);
+            val[j] = s * R[j].scale;
+        }
+        return;
+    }
+    if (!any_v && !any_h) return;  // no ray in the group
+    const bool V = any_v;
+    const float *base = V ? rows : cols;
+    const unsigned pitch = V ? w + 2 * kPad : h + 2 * kPad;
+    const float hi_t = V ? (float)w + 2.5f : (float)h + 2.5f;
+    // slice range of the group
+    int lo = 1 << 30, hi = -1;
+#pragma unroll
+    for (int j = 0; j < kFp2Rays; ++j)
+        if (on[j] && R[j].lo <= R[j].hi) {
+            lo = min(lo, R[j].lo);
+            hi = max(hi, R[j].hi);
+        }
+    lo = wave_min(lo);
+    hi = wave_max(hi);
+    if ((tid & 63) == 0) {
+        L.red[1][0][wave] = lo;
+        L.red[1][1][wave] = hi;
+    }
+    __syncthreads();
+    int glo = L.red[1][0][0], ghi = L.red[1][1][0];
+#pragma unroll
+    for (int k = 1; k < 4; ++k) {
+        glo = min(glo, L.red[1][0][k]);
+        ghi = max(ghi, L.red[1][1][k]);
+    }
+    float acc0[kFp2Rays], acc1[kFp2Rays];
+#pragma unroll
+    for (int j = 0; j < kFp2Rays; ++j) acc0[j] = acc1[j] = 0.f;
+    int parity = 0;
+    for (int cs = glo; cs <= ghi; cs += kFp2SC, parity ^= 1) {
+        const int ce = min(cs + kFp2SC - 1, ghi);
+        // interval of texels sampled in slices [cs, ce]
+        int tlo = 1 << 30, thi = -1;
+#pragma unroll
+        for (int j = 0; j < kFp2Rays; ++j) {
+            const int sb = max(cs, R[j].lo), se = min(ce, R[j].hi);
+            if (on[j] && sb <= se) {
+                const float tb = clamp_pad(fmaf(R[j].a, (float)sb, R[j].c), hi_t);
+                const float te = clamp_pad(fmaf(R[j].a, (float)se, R[j].c), hi_t);
+                tlo = min(tlo, (int)fminf(tb, te));
+                thi = max(thi, (int)fmaxf(tb, te));
+            }
+        }
+        tlo = wave_min(tlo);
+        thi = wave_max(thi);
+        // L.red[parity] was last read before the previous chunk's second
+        // barrier
+        if ((tid & 63) == 0) {
+            L.red[parity][0][wave] = tlo;
+            L.red[parity][1][wave] = thi;
+        }
+        __syncthreads();  // also: every thread is done with the previous chunk's rows
+        int t0 = L.red[parity][0][0], t1 = L.red[parity][1][0];
+#pragma unroll
+        for (int k = 1; k < 4; ++k) {
+            t0 = min(t0, L.red[parity][0][k]);
+            t1 = max(t1, L.red[parity][1][k]);
+        }
+        t0 = max(0, t0 - 1);
+        t1 = min((int)pitch - 1, t1 + 2);
+        const int wd = t1 - t0 + 1;
+        const bool staged = t1 >= t0 && wd <= kFp2Wcap;
+        if (staged) {
+            const int nrows = ce - cs + 1;
+            for (int i = (int)tid; i < nrows * kFp2Wcap; i += 256) {
+                const int r = i / kFp2Wcap, c = i & (kFp2Wcap - 1);
+                if (c < wd) L.buf[i] = base[(size_t)(cs + r) * pitch + t0 + c];
+            }
+        }
+        __syncthreads();
+        for (int sl = cs; sl <= ce; ++sl) {
+            const float sf = (float)sl;
+            const float *lrow = L.buf + (sl - cs) * kFp2Wcap;
+            const float *grow = base + (size_t)sl * pitch;
+#pragma unroll
+            for (int j = 0; j < kFp2Rays; ++j) {
+                if (!on[j] || sl < R[j].lo || sl > R[j].hi) continue;
+                const float t = clamp_pad(fmaf(R[j].a, sf, R[j].c), hi_t);
+                const float f = floorf(t);
+                float2 q;
+                if (staged) {
+                    // inside the staged interval by its margin; the clamp
+                    // only keeps the LDS address inside the row
+                    const int o = min(max((int)f - t0, 0), kFp2Wcap - 2);
+                    q = make_float2(lrow[o], lrow[o + 1]);
+                } else {
+                    q = ld2b(grow, (unsigned)f << 2);
+                }
+                const float x = lerp(q.x, q.y, t - f);
+                if ((sl - R[j].lo) & 1)
+                    acc1[j] += x;
+                else
+                    acc0[j] += x;
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < kFp2Rays; ++j)
+        if (on[j]) val[j] = (acc0[j] + acc1[j]) * R[j].scale;
+}
+
+// ---- back projection with sinogram footprints staged in LDS -------------
+// A workgroup of 256 threads covers 64 columns x (4 PY) rows (thread: column
+// X0 + (tid & 63), rows Y0 + (tid >> 6) PY ..).  For every block of kBp2KA
+// angles, the texels of each angle's sinogram row that the tile's taps can
+// reach are copied into LDS with coalesced loads, and the taps read LDS.
+// The reach is the bounding interval of the tap coordinate at the tile's
+// four corner pixels (t is linear-fractional in the pixel position, so its
+// extremes over the tile are at the corners, and the clamp is monotone),
+// widened by one texel on each side, far beyond the fp32 rounding of the
+// taps' incremental coordinates.  An angle whose interval exceeds
+// kBp2Wcap texels, or whose denominator changes sign over the tile, is
+// sampled from global memory.  Taps, weights and the order of the sums are
+// those of bp2d_column, so both give the same floats.
+constexpr int kBp2KA = 16;     // angles per staged block
+constexpr int kBp2Wcap = 256;  // texels per staged angle row (power of two)
+
+struct Bp2Lds {
+    float buf[kBp2KA * kBp2Wcap];
+    int2 fp[kBp2KA];  // (first texel, width); width 0: from global memory
+};
+
+template <int KIND>
+__device__ __forceinline__ int2 bp2d_footprint(const BP2DAngle &c, float fx0, float fx1, float fy0,
+                                               float fy1, float hi, int pitch) {
+    float tmin = INFINITY, tmax = -INFINITY, dmin = INFINITY, dmax = -INFINITY;
+    for (int j = 0; j < 4; ++j) {
+        const float fx = (j & 1) ? fx1 : fx0, fy = (j & 2) ? fy1 : fy0;
+        const float num = fmaf(fy, c.num[2], fmaf(fx, c.num[1], c.num[0]));
+        const float den = fmaf(fy, c.den[2], fmaf(fx, c.den[1], c.den[0]));
+        dmin = fminf(dmin, den);
+        dmax = fmaxf(dmax, den);
+        const float fr = (KIND == 0 || KIND == 3) ? 1.f : 1.f / den;
+        const float t = clamp_pad(num * fr, hi);
+        tmin = fminf(tmin, t);
+        tmax = fmaxf(tmax, t);
+    }
+    const bool sign_ok = KIND == 0 || KIND == 3 || dmin > 0.f || dmax < 0.f;
+    if (!sign_ok || !__builtin_isfinite(tmin) || !__builtin_isfinite(tmax)) return make_int2(0, 0);
+    // taps read texels floor(t) and floor(t) + 1 (t >= 0.5 after the clamp)
+    const int t0 = max(0, (int)tmin - 1), t1 = min(pitch - 1, (int)tmax + 2);
+    const int wd = t1 - t0 + 1;
+    return wd <= kBp2Wcap ? make_int2(t0, wd) : make_int2(0, 0);
+}
+
+template <int KIND, int PY>
+__device__ __forceinline__ void bp2d_tile_lds(int X0, int Yt0, unsigned w, unsigned h, unsigned na,
+                                              unsigned nd, const float *__restrict__ sp,
+                                              const BP2DAngle *__restrict__ ang, float *acc,
+                                              Bp2Lds &L) {
+    const unsigned tid = threadIdx.x;
+    const int X = X0 + (int)(tid & 63);
+    const int Y0 = Yt0 + (int)(tid >> 6) * PY;
+    const bool active = X < (int)w && Y0 < (int)h;
+    const float fx = X - 0.5f * w + 0.5f;
+    const float fy0 = -(Y0 - 0.5f * h + 0.5f);  // world y of row Y0
+    // tile corners (pixels inside the slice)
+    const float cx0 = X0 - 0.5f * w + 0.5f;
+    const float cx1 = min(X0 + 63, (int)w - 1) - 0.5f * w + 0.5f;
+    const float cy0 = -(Yt0 - 0.5f * h + 0.5f);
+    const float cy1 = -(min(Yt0 + 4 * PY - 1, (int)h - 1) - 0.5f * h + 0.5f);
+    const unsigned pitch = nd + 2 * kPad;
+    const float hi = (float)nd + 2.5f;
+#pragma unroll
+    for (int i = 0; i < PY; ++i) acc[i] = 0.f;
+    for (unsigned a0 = 0; a0 < na; a0 += kBp2KA) {
+        __syncthreads();  // the previous block's rows are no longer read
+        if (tid < (unsigned)kBp2KA)
+            L.fp[tid] = (a0 + tid < na) ? bp2d_footprint<KIND>(ang[a0 + tid], cx0, cx1, cy0, cy1,
+                                                               hi, (int)pitch)
+                                        : make_int2(0, 0);
+        __syncthreads();
+#pragma unroll 4
+        for (int i = (int)tid; i < kBp2KA * kBp2Wcap; i += 256) {
+            const int k = i / kBp2Wcap, j = i & (kBp2Wcap - 1);
+            const int2 f = L.fp[k];
+            if (j < f.y) L.buf[i] = sp[(size_t)(a0 + k) * pitch + f.x + j];
+        }
+        __syncthreads();
+        if (!active) continue;
+        const unsigned nk = min((unsigned)kBp2KA, na - a0);
+        for (unsigned k = 0; k < nk; ++k) {
+            const BP2DAngle c = ang[a0 + k];
+            const int2 f = L.fp[k];
+            float num = fmaf(fy0, c.num[2], fmaf(fx, c.num[1], c.num[0]));
+            float den = fmaf(fy0, c.den[2], fmaf(fx, c.den[1], c.den[0]));
+            const float *row = sp + (size_t)(a0 + k) * pitch;
+            const float *lrow = L.buf + k * kBp2Wcap;
+#pragma unroll
+            for (int i = 0; i < PY; ++i) {
+                float fr = (KIND == 0 || KIND == 3) ? 1.f : __builtin_amdgcn_rcpf(den);
+                float t = clamp_pad(num * fr, hi);
+                float fl = floorf(t);
+                float2 q;
+                if (f.y) {
+                    // inside the staged interval by its margin; the clamp
+                    // only keeps the LDS address inside the row
+                    const int o = min(max((int)fl - f.x, 0), kBp2Wcap - 2);
+                    q = make_float2(lrow[o], lrow[o + 1]);
+                } else {
+                    q = ld2b(row, (unsigned)fl << 2);
+                }
+                float val = lerp(q.x, q.y, t - fl);
+                float wgt = (KIND == 0) ? c.w : (KIND == 1 ? fr : (KIND == 2 ? fr * fr : 1.f));
+                acc[i] = fmaf(wgt, val, acc[i]);
+                num -= c.num[2];
+                den -= c.den[2];
+            }
         }
     }
 }

@@ -237,7 +237,10 @@ __device__ __forceinline__ void fp3d_ray(const FPArgs &A, const FP3DAngle &G, do
 }
 
 // Sum of the bilinear samples of slices [slo, shi] (wave-uniform) along the
-// ray (c1, c2) = a s + c through the padded group copy.
+// ray (c1, c2) = a s + c through the padded group copy.  Samples at even
+// offsets from slo go to acc0, odd ones to acc1; four loads are issued
+// before the first of them is used (the loop is bound by the loads'
+// latency).
 __device__ __forceinline__ float march3d(const FPArgs &A, float a1f, float a2f, float c1f,
                                          float c2f, int slo, int shi) {
     const unsigned pitch = A.n1 + 2 * kPad;                        // pairs per row
@@ -246,7 +249,25 @@ __device__ __forceinline__ float march3d(const FPArgs &A, float a1f, float a2f, 
     float acc0 = 0.f, acc1 = 0.f;
     int s = slo;
     const float *pl = A.volp + (size_t)(slo > 0 ? slo : 0) * plane;
-    for (; s + 1 <= shi; s += 2) {
+    for (; s + 3 <= shi; s += 4) {
+        float4 q[4];
+        float w1[4], w2[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const float sf = (float)(s + k);
+            const float p1 = clamp_pad(fmaf(a1f, sf, c1f), hi1);
+            const float p2 = clamp_pad(fmaf(a2f, sf, c2f), hi2);
+            w1[k] = frac(p1);
+            w2[k] = frac(p2);
+            q[k] = ld4b(pl + k * plane, pair_off(p1, p2, pitch));
+        }
+        acc0 += bilerp_pairs(q[0], w1[0], w2[0]);
+        acc1 += bilerp_pairs(q[1], w1[1], w2[1]);
+        acc0 += bilerp_pairs(q[2], w1[2], w2[2]);
+        acc1 += bilerp_pairs(q[3], w1[3], w2[3]);
+        pl += 4 * plane;
+    }
+    if (s + 1 <= shi) {
         const float sf = (float)s;
         float p1 = clamp_pad(fmaf(a1f, sf, c1f), hi1);
         float p2 = clamp_pad(fmaf(a2f, sf, c2f), hi2);
@@ -257,6 +278,7 @@ __device__ __forceinline__ float march3d(const FPArgs &A, float a1f, float a2f, 
         acc0 += bilerp_pairs(r, frac(p1), frac(p2));
         acc1 += bilerp_pairs(t, frac(q1), frac(q2));
         pl += 2 * plane;
+        s += 2;
     }
     if (s <= shi) {
         const float sf = (float)s;
@@ -326,6 +348,276 @@ __global__ __launch_bounds__(256) void fp3d_kernel(FPArgs A) {
         float r = A.out[oidx] + val;
         if (A.rw) r *= A.rw[oidx];
         A.out[oidx] = r;
+    }
+}
+
+// ---- FP with volume slices staged in LDS ---------------------------------
+// A workgroup takes a 16 x 16 detector tile at kFpAngles consecutive angles
+// of its direction group (each thread: one detector pixel at every one of
+// those angles; each wave: the 16 u x 4 v rays of fp3d_kernel's waves).
+// Neighbouring angles sample nearly the same voxels, so the workgroup
+// marches the slices in lockstep, kFpSlices at a time: for every slice of
+// a chunk it stages the rectangle of 16-byte quads (pair (r, c) and pair
+// (r, c + 1), one whole bilinear tap) that its rays can sample -- the
+// bounding box of the clamped sample positions of the corner rays of every
+// angle (within a slice the sample position is linear-fractional in the
+// detector position, so its extremes over the tile are at the corners),
+// widened by one texel -- and every tap is one aligned 16-byte LDS read.
+// The next chunk's quads are loaded into registers while the current one
+// is sampled.  Each ray sums exactly the samples of march3d in the same
+// order (the wave's slice range, alternating accumulators), so the values
+// equal fp3d_kernel's.  A chunk whose rectangle exceeds the tile samples
+// global memory.
+constexpr int kFpAngles = 4;          // angles per workgroup
+constexpr int kFpSlices = 4;          // slices per staged chunk
+constexpr int kFpQuadCap = 1536;      // LDS quads per chunk (24 KiB)
+constexpr int kFpSlots = kFpQuadCap / 256;
+
+struct FpBox {
+    int u0[kFpSlices], v0[kFpSlices];  // per slice: first quad column, first row
+    int w, h, lp;                      // common width, height, LDS row pitch (quads)
+};
+
+// workgroup -> (detector tile, first of kFpAngles angles), XCD-aware as
+// fp_map: the workgroups resident on one XCD take neighbouring angle
+// groups of the same tiles
+__device__ __forceinline__ bool fp_lds_map(const FPArgs &A, unsigned &tile, unsigned &li0) {
+    const unsigned L = blockIdx.x;
+    const unsigned per = gridDim.x >> 3;
+    const unsigned logical = (L & 7u) * per + (L >> 3);
+    constexpr unsigned sub = 4;  // angle groups per block
+    const unsigned a_in = logical % sub;
+    const unsigned rest = logical / sub;
+    tile = rest % A.tiles;
+    li0 = ((rest / A.tiles) * sub + a_in) * kFpAngles;
+    return li0 < A.lcount;
+}
+
+template <int GK>
+__global__ __launch_bounds__(256) void fp3d_lds_kernel(FPArgs A) {
+    __shared__ float4 tile[kFpQuadCap];
+    __shared__ float cpar[kFpAngles][4][4];  // corner rays: a1, c1, a2, c2
+    __shared__ int crng[kFpAngles][4];       // corner rays: 1 if the angle exists
+    __shared__ FpBox box[2];
+    __shared__ int wr[4][2];                 // per wave: slice range of the group
+    unsigned tile_id, li0;
+    if (!fp_lds_map(A, tile_id, li0)) return;  // uniform over the workgroup
+    const unsigned tid = threadIdx.x;
+    const unsigned tiles_u = (A.nu + 15) / 16;
+    const unsigned tu = tile_id % tiles_u, tv = tile_id / tiles_u;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int u = tu * 16 + (lane & 15);
+    const int v = tv * 16 + wave * 4 + (lane >> 4);
+    const bool pix = u < (int)A.nu && v < (int)A.nv;
+    // last pixel of the tile inside the detector (the corner rays)
+    const int ulast = min((int)tu * 16 + 15, (int)A.nu - 1), vlast = min((int)tv * 16 + 15, (int)A.nv - 1);
+    // corner q of the tile: u = (q & 1 ? ulast : first), v = (q & 2 ? vlast : first)
+    unsigned corners = 0;
+    for (int q = 0; q < 4; ++q)
+        if (pix && u == ((q & 1) ? ulast : (int)tu * 16) && v == ((q & 2) ? vlast : (int)tv * 16))
+            corners |= 1u << q;
+
+    float a1[kFpAngles], a2[kFpAngles], c1[kFpAngles], c2[kFpAngles], sc[kFpAngles];
+    int slo[kFpAngles], shi[kFpAngles], angle[kFpAngles];
+    int glo = (int)A.n0, ghi = -1;
+#pragma unroll
+    for (int j = 0; j < kFpAngles; ++j) {
+        const unsigned li = li0 + j;
+        const bool on = li < A.lcount;
+        angle[j] = on ? A.alist[A.lfirst + li] : -1;
+        a1[j] = a2[j] = 0.f;
+        c1[j] = c2[j] = 0.5f;
+        sc[j] = 0.f;
+        int lo = (int)A.n0, hi = -1;
+        if (on && pix) fp3d_ray<GK>(A, A.ang[angle[j]], u, v, a1[j], a2[j], c1[j], c2[j], sc[j], lo, hi);
+        slo[j] = __builtin_amdgcn_readfirstlane(wave_min(lo));
+        shi[j] = __builtin_amdgcn_readfirstlane(wave_max(hi));
+        glo = min(glo, slo[j]);
+        ghi = max(ghi, shi[j]);
+        // (a tile of one pixel column / row has coinciding corners: one
+        // lane then fills several slots)
+        for (int q = 0; q < 4; ++q) {
+            if (!(corners & (1u << q))) continue;
+            cpar[j][q][0] = a1[j];
+            cpar[j][q][1] = c1[j];
+            cpar[j][q][2] = a2[j];
+            cpar[j][q][3] = c2[j];
+            // the clamped positions bound every ray of the tile at every
+            // slice, whether or not the corner ray itself meets the volume
+            crng[j][q] = on ? 1 : 0;
+        }
+    }
+    if (lane == 0) {
+        wr[wave][0] = glo;
+        wr[wave][1] = ghi;
+    }
+    __syncthreads();
+    glo = min(min(wr[0][0], wr[1][0]), min(wr[2][0], wr[3][0]));
+    ghi = max(max(wr[0][1], wr[1][1]), max(wr[2][1], wr[3][1]));
+    if (glo > ghi) return;  // no ray of the group meets the volume (uniform)
+
+    const unsigned pitch = A.n1 + 2 * kPad;                  // pairs per row
+    const unsigned prow = A.n2 + 2 * kPad;                   // rows per slice
+    const size_t plane = (size_t)prow * pitch * 2;           // floats per slice
+    const float hi1 = (float)A.n1 + 2.5f, hi2 = (float)A.n2 + 2.5f;
+
+    // boxes of chunk [cs, cs + kFpSlices) into box[slot] (threads < 64)
+    auto make_box = [&](int cs, int slot) {
+        if (tid >= 64) return;
+        const int k = tid & (kFpSlices - 1);
+        const int sl = cs + k;
+        float mn1 = INFINITY, mx1 = -INFINITY, mn2 = INFINITY, mx2 = -INFINITY;
+        if (sl <= ghi) {
+            const float sf = (float)sl;
+            for (int j = 0; j < kFpAngles; ++j)
+                for (int q = 0; q < 4; ++q) {
+                    if (!crng[j][q]) continue;
+                    const float p1 = clamp_pad(fmaf(cpar[j][q][0], sf, cpar[j][q][1]), hi1);
+                    const float p2 = clamp_pad(fmaf(cpar[j][q][2], sf, cpar[j][q][3]), hi2);
+                    mn1 = fminf(mn1, p1);
+                    mx1 = fmaxf(mx1, p1);
+                    mn2 = fminf(mn2, p2);
+                    mx2 = fmaxf(mx2, p2);
+                }
+        }
+        int u0 = 0, v0 = 0, w = 0, h = 0;
+        if (mn1 <= mx1) {
+            u0 = max(0, (int)mn1 - 1);
+            const int u1 = min((int)pitch - 2, (int)mx1 + 1);
+            v0 = max(0, (int)mn2 - 1);
+            const int v1 = min((int)prow - 1, (int)mx2 + 1);
+            w = u1 - u0 + 1;
+            h = v1 - v0 + 1;
+        }
+        // common width / height over the chunk's slices (lanes k < kFpSlices)
+        int W = w, H = h;
+        for (int o = 1; o < kFpSlices; o <<= 1) {
+            W = max(W, __shfl_xor(W, o, 64));
+            H = max(H, __shfl_xor(H, o, 64));
+        }
+        if (tid < kFpSlices) {
+            box[slot].u0[k] = u0;
+            box[slot].v0[k] = v0;
+            if (tid == 0) {
+                const int lp = W | 1;
+                const bool fits = W > 0 && lp * H * kFpSlices <= kFpQuadCap;
+                box[slot].w = fits ? W : 0;
+                box[slot].h = H;
+                box[slot].lp = lp;
+            }
+        }
+    };
+
+    float4 pre[kFpSlots];
+    auto fetch = [&](int cs, const FpBox &b) {
+        const int per_slice = b.w * b.h;
+        const int n = per_slice * kFpSlices;
+        const float inv_ps = 1.f / (float)per_slice, inv_w = 1.f / (float)b.w;
+#pragma unroll
+        for (int m = 0; m < kFpSlots; ++m) {
+            const int i = (int)tid + 256 * m;
+            if (i < n) {
+                // exact for i < 2^12 (see the BP footprint loads)
+                const int k = (int)(((float)i + 0.5f) * inv_ps);
+                const int e = i - k * per_slice;
+                const int r = (int)(((float)e + 0.5f) * inv_w);
+                const int c = e - r * b.w;
+                // a slice whose own box is smaller than the common one
+                // reads clamped (unused) quads past its box
+                const int sl = min(cs + k, (int)A.n0 - 1);
+                const int rr = min(b.v0[k] + r, (int)prow - 1);
+                const int cc = min(b.u0[k] + c, (int)pitch - 2);
+                const float *pl = A.volp + (size_t)sl * plane;
+                pre[m] = ld4b(pl, (unsigned)(rr * (int)pitch + cc) * 8u);
+            }
+        }
+    };
+
+    float acc0[kFpAngles], acc1[kFpAngles];
+#pragma unroll
+    for (int j = 0; j < kFpAngles; ++j) acc0[j] = acc1[j] = 0.f;
+
+    make_box(glo, 0);
+    __syncthreads();
+    if (box[0].w) fetch(glo, box[0]);
+    int slot = 0;
+    for (int cs = glo; cs <= ghi; cs += kFpSlices, slot ^= 1) {
+        const FpBox &b = box[slot];
+        make_box(cs + kFpSlices, slot ^ 1);
+        if (b.w) {
+            const int per_slice = b.w * b.h;
+            const int n = per_slice * kFpSlices;
+            const float inv_ps = 1.f / (float)per_slice, inv_w = 1.f / (float)b.w;
+#pragma unroll
+            for (int m = 0; m < kFpSlots; ++m) {
+                const int i = (int)tid + 256 * m;
+                if (i < n) {
+                    const int k = (int)(((float)i + 0.5f) * inv_ps);
+                    const int e = i - k * per_slice;
+                    const int r = (int)(((float)e + 0.5f) * inv_w);
+                    tile[(k * b.h + r) * b.lp + (e - r * b.w)] = pre[m];
+                }
+            }
+        }
+        __syncthreads();  // tile and box[slot ^ 1] complete
+        const FpBox &nb = box[slot ^ 1];
+        if (cs + kFpSlices <= ghi && nb.w) fetch(cs + kFpSlices, nb);
+        const int ce = min(cs + kFpSlices - 1, ghi);
+        for (int sl = cs; sl <= ce; ++sl) {
+            const int k = sl - cs;
+            const float sf = (float)sl;
+            const float *pl = A.volp + (size_t)sl * plane;
+            const int rb = k * b.h, u0 = b.u0[k], v0 = b.v0[k];
+#pragma unroll
+            for (int j = 0; j < kFpAngles; ++j) {
+                if (sl < slo[j] || sl > shi[j]) continue;  // wave-uniform
+                const float p1 = clamp_pad(fmaf(a1[j], sf, c1[j]), hi1);
+                const float p2 = clamp_pad(fmaf(a2[j], sf, c2[j]), hi2);
+                float4 q;
+                if (b.w) {
+                    int off = ((int)(unsigned)p2 - v0 + rb) * b.lp + (int)(unsigned)p1 - u0;
+                    // in range for every ray of the group by the box's
+                    // margin; the clamp keeps the idle lanes (pixels past
+                    // the detector) inside the tile
+                    off = min(max(off, 0), kFpQuadCap - 1);
+                    q = tile[off];
+                } else {
+                    q = ld4b(pl, pair_off(p1, p2, pitch));
+                }
+                const float x = bilerp_pairs(q, frac(p1), frac(p2));
+                if ((sl - slo[j]) & 1)
+                    acc1[j] += x;
+                else
+                    acc0[j] += x;
+            }
+        }
+        __syncthreads();  // the tile is read; box[slot] may be rewritten
+    }
+    if (!pix) return;
+#pragma unroll
+    for (int j = 0; j < kFpAngles; ++j) {
+        if (angle[j] < 0) continue;
+        const float val = (acc0[j] + acc1[j]) * sc[j];
+        const int an = angle[j];
+        size_t idx = ((size_t)v * A.na + an) * A.nu + u;
+        size_t oidx = A.padded_out
+                          ? ((size_t)an * (A.nv + 2 * kPad) + (v + kPad)) * (A.nu + 2 * kPad) + u + kPad
+                          : idx;
+        if (A.mode == OUT_ADD) {
+            A.out[oidx] += val;
+        } else if (A.mode == OUT_SET) {
+            A.out[oidx] = val;
+        } else if (A.mode == OUT_RESID) {
+            float r = A.rb[idx] + val;
+            if (A.rw) r *= A.rw[oidx];
+            A.out[oidx] = r;
+        } else if (A.mode == OUT_RATIO) {
+            A.out[oidx] = em_ratio(A.rb[idx], val);
+        } else {
+            float r = A.out[oidx] + val;
+            if (A.rw) r *= A.rw[oidx];
+            A.out[oidx] = r;
+        }
     }
 }
 
@@ -452,14 +744,22 @@ __global__ __launch_bounds__(256) void bp3d_kernel(BPArgs A) {
             const float wu = frac(pu);
             const unsigned ou = (unsigned)pu << 3;
             const float wgt = (KIND == 0) ? fr * fr : (KIND == 1 ? fr : wa);
+            // every tap load of the column issued before the first use: the
+            // loads' latency, not their issue rate, bounds this loop
+            float4 q[ZS];
+            float wv[ZS];
 #pragma unroll
             for (int i = 0; i < ZS; ++i) {
                 // pv in [0.5, nv + 2.5]: truncation is the floor
-                float pv = clamp_pad(vn * fr, hiv);
-                float val = bilerp_pairs(ld4b(pl, __umul24((unsigned)pv, rowb) + ou), wu, frac(pv));
-                acc[i] = fmaf(wgt, val, acc[i]);
+                const float pv = clamp_pad(vn * fr, hiv);
+                wv[i] = frac(pv);
+                q[i] = ld4b(pl, __umul24((unsigned)pv, rowb) + ou);
                 vn += c.nv[2];
             }
+            // keep the scheduler from sinking the loads into their uses
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int i = 0; i < ZS; ++i) acc[i] = fmaf(wgt, bilerp_pairs(q[i], wu, wv[i]), acc[i]);
         } else {
 #pragma unroll
             for (int i = 0; i < ZS; ++i) {
@@ -479,25 +779,39 @@ __global__ __launch_bounds__(256) void bp3d_kernel(BPArgs A) {
 }
 
 // ---- BP with the detector footprint staged in LDS -----------------------
-// The z-constant BP above reads one 16-byte tap pair per voxel and angle
-// through the texture-address path, which bounds it.  Here each workgroup
-// (16 x 16 columns of ZS voxels) stages, per angle, the rectangle of the
-// pair plane that its block projects into (cooperative, coalesced 8-byte
-// loads issued one angle ahead) and takes the taps from LDS.  The footprint
-// is the bounding box of the projected corners of the block (u and v are
+// The z-constant BP above reads one 16-byte tap (two difference pairs) per
+// voxel and angle through the texture-address path, which bounds it.  Here
+// each workgroup (16 x 16 columns of ZS voxels) stages, per angle, the
+// rectangle of the pair plane that its block projects into as 16-byte
+// quads -- quad (r, c) = pairs (r, c) and (r, c + 1), the whole tap -- so
+// that every tap is one aligned 16-byte LDS read (the LDS serves those at
+// several times the rate of the texture path, and of two 8-byte reads).
+// The quads are loaded one angle ahead with coalesced 16-byte loads and held
+// in registers while the current angle is sampled.  The footprint is the
+// bounding box of the projected corners of the block (u and v are
 // linear-fractional in the voxel position, so the extremes over the block
 // are at its corners; the clamp to the padded range is monotone), widened
-// by one texel for fp32 rounding.  Taps, weights and the order of the sums
-// are those of bp3d_kernel<KIND, ZS, true>, so both give the same floats.
-// An angle whose footprint exceeds the tile, or whose denominator changes
-// sign over the block, is sampled from global memory as before.
-constexpr int kTileCap = 2048;  // pairs per staged tile (16 KiB)
+// by one texel on each side, far beyond the fp32 rounding of the taps.
+// Taps, weights and the order of the sums are those of
+// bp3d_kernel<KIND, ZS, true>, so both give the same floats.  An angle
+// whose footprint exceeds the tile, or whose denominator changes sign over
+// the block, is sampled from global memory as in bp3d_kernel.
+// A staged row is `lp` quads long: lp = w | 1 (pm < 0) or
+// w + ((pm - w) mod 16) (pm in [0, 16)), so that rows start at different
+// offsets modulo the 16 quads (256 bytes) that one LDS cycle serves.
+// LM = 0: 16 lanes = one row of 16 columns; LM = 1: 16 lanes = a 4 x 4
+// patch of columns.
+constexpr int kTileCap = 1024;  // LDS quads per staged tile (16 KiB)
 constexpr int kFpGroup = 256;   // angles whose footprints are held at once
 constexpr int kTileSlots = kTileCap / 256;
 
+__device__ __forceinline__ int lds_row_pitch(int w, int pm) {
+    return pm < 0 ? (w | 1) : w + ((pm - w) & 15);
+}
+
 template <int KIND>
 __device__ int4 bp_footprint(const BP3DAngle &c, float x0, float x1, float y0, float y1, float z0,
-                             float z1, float hiu, float hiv, int pitch, int prow) {
+                             float z1, float hiu, float hiv, int pitch, int prow, int pm) {
     float umin = INFINITY, umax = -INFINITY, vmin = INFINITY, vmax = -INFINITY;
     float dmin = INFINITY, dmax = -INFINITY;
     for (int j = 0; j < 4; ++j) {
@@ -522,58 +836,59 @@ __device__ int4 bp_footprint(const BP3DAngle &c, float x0, float x1, float y0, f
     const bool finite = __builtin_isfinite(umin) && __builtin_isfinite(umax) &&
                         __builtin_isfinite(vmin) && __builtin_isfinite(vmax);
     if (!sign_ok || !finite) return make_int4(0, 0, 0, 0);
-    // taps: columns floor(pu) and floor(pu) + 1, pair rows floor(pv); one
-    // texel of margin on each side
-    const int u0 = max(0, (int)umin - 1), u1 = min(pitch - 1, (int)umax + 2);
+    // taps: quad floor(pu) (pairs floor(pu), floor(pu) + 1) of pair row
+    // floor(pv); one texel of margin on each side.  Quad c reads pairs c and
+    // c + 1, so the last quad column is at most pitch - 2.
+    const int u0 = max(0, (int)umin - 1), u1 = min(pitch - 2, (int)umax + 1);
     const int v0 = max(0, (int)vmin - 1), v1 = min(prow - 1, (int)vmax + 1);
     const int w = u1 - u0 + 1, h = v1 - v0 + 1;
-    if (w > 1024 || w * h > kTileCap) return make_int4(0, 0, 0, 0);
+    if (w < 1 || w > 1024 || lds_row_pitch(w, pm) * h > kTileCap) return make_int4(0, 0, 0, 0);
     return make_int4(u0, v0, w, h);
 }
 
-template <int KIND, int ZS>
-__global__ __launch_bounds__(256) void bp3d_lds_kernel(BPArgs A) {
-    __shared__ float2 tile[2][kTileCap];
+template <int KIND, int ZS, int LM>
+__global__ __launch_bounds__(256) void bp3d_lds_kernel(BPArgs A, int pm) {
+    __shared__ float4 tile[2][kTileCap];
     __shared__ int4 fps[kFpGroup];
     const unsigned tid = threadIdx.x;
     const unsigned tiles_x = (A.nx + 15) / 16;
     const int X0 = (blockIdx.x % tiles_x) * 16, Y0 = (blockIdx.x / tiles_x) * 16;
-    // each 16 lanes take a 4 x 4 patch of columns, so the taps that one LDS
-    // cycle serves fall in a narrow window of the tile (bank conflicts)
-    const int X = X0 + (int)((tid & 3) | (((tid >> 4) & 3) << 2));
-    const int Y = Y0 + (int)(((tid >> 2) & 3) | ((tid >> 6) << 2));
+    const int X = X0 + (LM == 0 ? (int)(tid & 15) : (int)((tid & 3) | (((tid >> 4) & 3) << 2)));
+    const int Y = Y0 + (LM == 0 ? (int)(tid >> 4) : (int)(((tid >> 2) & 3) | ((tid >> 6) << 2)));
     const int Z0 = blockIdx.y * ZS;
     const bool active = X < (int)A.nx && Y < (int)A.ny;
     const float hx = 0.5f * A.nx - 0.5f, hy = 0.5f * A.ny - 0.5f, hz = 0.5f * A.nz - 0.5f;
-    const float fx = X - hx, fy = Y - hy, fz = Z0 - hz;
+    const float fx = X - 0.5f * A.nx + 0.5f;
+    const float fy = Y - 0.5f * A.ny + 0.5f;
+    const float fz = Z0 - 0.5f * A.nz + 0.5f;
     // corners of the block's voxel centres (the columns inside the volume,
     // all ZS slices: the slices past nz are computed and discarded)
     const float cx0 = X0 - hx, cx1 = min(X0 + 15, (int)A.nx - 1) - hx;
     const float cy0 = Y0 - hy, cy1 = min(Y0 + 15, (int)A.ny - 1) - hy;
-    const float cz0 = fz, cz1 = (Z0 + ZS - 1) - hz;
+    const float cz0 = Z0 - hz, cz1 = (Z0 + ZS - 1) - hz;
     const unsigned pitch = A.nu + 2 * kPad;                    // pairs per row
     const unsigned prow = A.nv + 2 * kPad;                     // rows per angle
     const size_t plane = (size_t)prow * pitch;                 // pairs per angle
     const unsigned rowb = pitch * 8u;
     const float hiu = (float)A.nu + 2.5f, hiv = (float)A.nv + 2.5f;
-    const float2 *pp2 = reinterpret_cast<const float2 *>(A.pp);
     float acc[ZS];
 #pragma unroll
     for (int i = 0; i < ZS; ++i) acc[i] = 0.f;
-    float2 pre[kTileSlots];
+    float4 pre[kTileSlots];
 
+    // quads i < w h of the footprint, row-major, into registers
     auto fetch = [&](unsigned a, int4 f) {
         const int n = f.z * f.w;
         const float invw = 1.f / (float)f.z;
-        const float2 *src = pp2 + (size_t)a * plane + (size_t)f.y * pitch + f.x;
+        const float *src = A.pp + ((size_t)a * plane + (size_t)f.y * pitch + f.x) * 2;
 #pragma unroll
         for (int k = 0; k < kTileSlots; ++k) {
             const int i = (int)tid + 256 * k;
             if (i < n) {
                 // (i + 0.5) / w is at least 0.5 / w from an integer, far
-                // beyond the fp32 error for w <= 1024, i < 2048
+                // beyond the fp32 error for w <= 1024, i < 1024
                 const int r = (int)(((float)i + 0.5f) * invw);
-                pre[k] = src[(size_t)r * pitch + (i - r * f.z)];
+                pre[k] = ld4b(src, (unsigned)(r * (int)pitch + (i - r * f.z)) * 8u);
             }
         }
     };
@@ -583,19 +898,24 @@ __global__ __launch_bounds__(256) void bp3d_lds_kernel(BPArgs A) {
         __syncthreads();  // the previous group's footprints are no longer read
         if (tid < gn)
             fps[tid] = bp_footprint<KIND>(A.ang[g0 + tid], cx0, cx1, cy0, cy1, cz0, cz1, hiu, hiv,
-                                          (int)pitch, (int)prow);
+                                          (int)pitch, (int)prow, pm);
         __syncthreads();
         if (fps[0].z) fetch(g0, fps[0]);
         for (unsigned k = 0; k < gn; ++k) {
             const unsigned a = g0 + k;
             const int4 f = fps[k];
-            float2 *t = tile[a & 1];
-            const int n = f.z * f.w;
+            const int lp = lds_row_pitch(f.z, pm);
+            float4 *t = tile[a & 1];
             if (f.z) {
+                const int n = f.z * f.w;
+                const float invw = 1.f / (float)f.z;
 #pragma unroll
                 for (int j = 0; j < kTileSlots; ++j) {
                     const int i = (int)tid + 256 * j;
-                    if (i < n) t[i] = pre[j];
+                    if (i < n) {
+                        const int r = (int)(((float)i + 0.5f) * invw);
+                        t[r * lp + (i - r * f.z)] = pre[j];
+                    }
                 }
             }
             // tile[a & 1] is complete; tile[(a + 1) & 1] was last read at
@@ -618,16 +938,18 @@ __global__ __launch_bounds__(256) void bp3d_lds_kernel(BPArgs A) {
             if (f.z) {
                 // all ZS tap reads in flight before the first use
                 const int cu = (int)(unsigned)pu - f.x;
+                const int last = lp * f.w - 1;
                 float4 q[ZS];
                 float wv[ZS];
 #pragma unroll
                 for (int i = 0; i < ZS; ++i) {
                     const float pv = clamp_pad(vn * fr, hiv);
                     wv[i] = frac(pv);
-                    int off = ((int)(unsigned)pv - f.y) * f.z + cu;
-                    off = min(max(off, 0), n - 2);
-                    const float2 t0 = t[off], t1 = t[off + 1];
-                    q[i] = make_float4(t0.x, t0.y, t1.x, t1.y);
+                    int off = ((int)(unsigned)pv - f.y) * lp + cu;
+                    // in range by the footprint's margin; the clamp only
+                    // keeps the LDS address inside the tile
+                    off = min(max(off, 0), last);
+                    q[i] = t[off];
                     vn += c.nv[2];
                 }
 #pragma unroll
@@ -1107,6 +1429,21 @@ bool bp_lds() {
     return e && e[0] == '1';
 }
 
+// LDS row pitch of the staged BP: ASTRA_AMD_BP3D_LDS_PITCH=<0..15> pads a
+// staged row to that residue modulo 16 quads, -1 (default) to the next odd
+// length; ASTRA_AMD_BP3D_LDS_PATCH=1 maps 16 lanes to 4 x 4 columns instead
+// of a row of 16 (both read at every launch)
+int bp_lds_pitch_mod() {
+    const char *e = std::getenv("ASTRA_AMD_BP3D_LDS_PITCH");
+    const int x = e ? std::atoi(e) : -1;
+    return x < 0 ? -1 : (x & 15);
+}
+
+bool bp_lds_patch() {
+    const char *e = std::getenv("ASTRA_AMD_BP3D_LDS_PATCH");
+    return e && e[0] == '1';
+}
+
 int bp_zs() {
     static const int v = [] {
         const char *e = std::getenv("ASTRA_AMD_BP3D_ZS");
@@ -1169,6 +1506,35 @@ void fp_group_launches(FPArgs A, unsigned count, K kern, hipStream_t s) {
     }
 }
 
+// ASTRA_AMD_FP3D_LDS=1: the FP stages volume slices in LDS for groups of
+// kFpAngles angles (read at every call)
+bool fp_lds() {
+    const char *e = std::getenv("ASTRA_AMD_FP3D_LDS");
+    return e && e[0] == '1';
+}
+
+// fp3d_lds_kernel over the angles of one direction group: one workgroup per
+// (detector tile, kFpAngles angles), blocks of 16 angles (fp_lds_map)
+void fp_lds_launches(FPArgs A, unsigned count, bool cone, hipStream_t s) {
+    A.tiles = ((A.nu + 15) / 16) * ((A.nv + 15) / 16);
+    A.ablk = 16;
+    const size_t per_block = (size_t)A.tiles * (16 / kFpAngles);
+    const size_t max_groups = (1u << 24) - 8;
+    AMX_REQUIRE(per_block <= max_groups, "fp3d: detector too large for one launch");
+    const unsigned blocks_per_launch = (unsigned)(max_groups / per_block);
+    for (unsigned first = 0; first < count; first += blocks_per_launch * 16) {
+        A.lfirst = first;
+        A.lcount = std::min(count - first, blocks_per_launch * 16);
+        const unsigned nblk = (A.lcount + 15) / 16;
+        const unsigned groups = (unsigned)(((size_t)nblk * per_block + 7) & ~(size_t)7);
+        if (cone)
+            fp3d_lds_kernel<GEO_CONE><<<groups, 256, 0, s>>>(A);
+        else
+            fp3d_lds_kernel<GEO_PAR3D><<<groups, 256, 0, s>>>(A);
+        AMX_LAUNCHED();
+    }
+}
+
 FPArgs fp_args(const Geom3D &g, int grp, float out_scale) {
     FPArgs A{};
     group_dims(g, grp, A.n0, A.n1, A.n2);
@@ -1213,6 +1579,11 @@ void fp3d_launch(const Geom3D &g, const float *vol, float *proj, float out_scale
                                                                      : "fp3d_par3d",
                                   (double)g.dir_count[k] * g.nu * g.nv * A.n0 * ss * ss, s);
         A.ss = ss;
+        if (ss == 1 && g.kind != ASTRA_AMD_GEOM_CYLCONE && fp_lds()) {
+            fp_lds_launches(A, g.dir_count[k], g.kind == ASTRA_AMD_GEOM_CONE, s);
+            prof_end(pt, s);
+            continue;
+        }
         fp_group_launches(A, g.dir_count[k], [&](const FPArgs &a, unsigned groups) {
             if (g.kind == ASTRA_AMD_GEOM_CYLCONE)
                 fp3d_kernel<GEO_CYL, false><<<groups, 256, 0, s>>>(a);
@@ -1263,9 +1634,16 @@ template <int ZS>
 void bp3d_dispatch(int kind, bool zc, const BPArgs &A, hipStream_t s) {
     dim3 grid(((A.nx + 15) / 16) * ((A.ny + 15) / 16), (A.nz + ZS - 1) / ZS);
     if (zc && bp_lds()) {
-        if (kind == 0) bp3d_lds_kernel<0, ZS><<<grid, 256, 0, s>>>(A);
-        else if (kind == 1) bp3d_lds_kernel<1, ZS><<<grid, 256, 0, s>>>(A);
-        else bp3d_lds_kernel<2, ZS><<<grid, 256, 0, s>>>(A);
+        const int pm = bp_lds_pitch_mod();
+        if (bp_lds_patch()) {
+            if (kind == 0) bp3d_lds_kernel<0, ZS, 1><<<grid, 256, 0, s>>>(A, pm);
+            else if (kind == 1) bp3d_lds_kernel<1, ZS, 1><<<grid, 256, 0, s>>>(A, pm);
+            else bp3d_lds_kernel<2, ZS, 1><<<grid, 256, 0, s>>>(A, pm);
+        } else {
+            if (kind == 0) bp3d_lds_kernel<0, ZS, 0><<<grid, 256, 0, s>>>(A, pm);
+            else if (kind == 1) bp3d_lds_kernel<1, ZS, 0><<<grid, 256, 0, s>>>(A, pm);
+            else bp3d_lds_kernel<2, ZS, 0><<<grid, 256, 0, s>>>(A, pm);
+        }
         return;
     }
     if (kind == 0)
