@@ -125,18 +125,30 @@ def _step(P, R, C, b, x, lo, hi, pbuf=None):
     return (z.clamp(lo, hi) if _constrained(lo, hi) else z), z
 
 
+def _step_z(P, R, C, b, x, pbuf):
+    """z = x + C A^T R (b - A x), the unclamped SIRT update (a new volume)."""
+    r = _fp(P, x, pbuf)
+    torch.sub(b, r, out=r)
+    r.mul_(R)
+    z = _bp_matched(P, r)
+    return z.mul_(C).add_(x)
+
+
 class _UnrolledSIRT(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, b, x0, P, iterations, R, C, lo, hi, every):
+    def forward(ctx, b, x0, P, iterations, R, C, lo, hi, every, keep_segment):
         x = x0
-        ckpts = [x0]
+        # without constraints the backward pass needs no iterate: keep none
+        cons = _constrained(lo, hi)
+        ckpts = [x0] if cons else []
         pbuf = torch.empty_like(b)  # one residual buffer for every iteration
         for k in range(iterations):
-            if k > 0 and k % every == 0:
+            if cons and k > 0 and k % every == 0:
                 ckpts.append(x)
             x, _ = _step(P, R, C, b, x, lo, hi, pbuf)
         del pbuf
         ctx.P, ctx.n, ctx.lo, ctx.hi, ctx.every = P, iterations, lo, hi, every
+        ctx.keep_segment = keep_segment
         ctx.save_for_backward(b, R, C, *ckpts)
         return x
 
@@ -148,32 +160,60 @@ class _UnrolledSIRT(torch.autograd.Function):
         gb = torch.zeros_like(b)
         pbuf = torch.empty_like(b)  # the only projection-shaped work buffer
         cons = _constrained(lo, hi)
-        for seg in range(len(ckpts) - 1, -1, -1):
-            start, stop = seg * every, min(n, (seg + 1) * every)
-            xs = [ckpts[seg]]  # recompute x_start .. x_{stop-1}
-            for _ in range(start, stop - 1):
-                xs.append(_step(P, R, C, b, xs[-1], lo, hi, pbuf)[0])
+        cgbuf = None if cons else torch.empty_like(g)  # C h (constrained: in z's memory)
+
+        def iterate_from(x, steps):
+            # x_{j+1} = clamp(z_j), clamped in place: two volumes at a time
+            for _ in range(steps):
+                x = _step_z(P, R, C, b, x, pbuf)
+                if cons:
+                    x.clamp_(lo, hi)
+            return x
+
+        nseg = len(ckpts) if cons else 1
+        for seg in range(nseg - 1, -1, -1):
+            start, stop = (seg * every, min(n, (seg + 1) * every)) if cons else (0, n)
+            # keep_segment: recompute x_start .. x_{stop-1} once and hold
+            # them; else recompute each x_k from the checkpoint when needed
+            # (one volume instead of `every`, for about every / 2 times the
+            # recomputation)
+            xs = [ckpts[seg]] if cons else []
+            if cons and ctx.keep_segment:
+                for _ in range(start, stop - 1):
+                    xs.append(_step(P, R, C, b, xs[-1], lo, hi, pbuf)[0])
             for k in range(stop - 1, start - 1, -1):
                 # x_{k+1} = clamp(z), z = x_k + C A^T R (b - A x_k)
                 if cons:  # torch.clamp's gradient: h = g where lo <= z <= hi
-                    _, z = _step(P, R, C, b, xs[k - start], lo, hi, pbuf)
-                    g.mul_(((z >= lo) & (z <= hi)).to(g.dtype))
+                    xk = xs[k - start] if ctx.keep_segment else iterate_from(ckpts[seg], k - start)
+                    z = _step_z(P, R, C, b, xk, pbuf)
+                    del xk
+                    g.masked_fill_(~((z >= lo) & (z <= hi)), 0.0)
+                    cg = torch.mul(C, g, out=z)  # z's memory is free again
                     del z
-                t = _fp(P, C * g, pbuf)        # dz/db^T h = R A C h
+                else:
+                    cg = torch.mul(C, g, out=cgbuf)
+                t = _fp(P, cg, pbuf)           # dz/db^T h = R A C h
                 t.mul_(R)
                 gb += t
+                del cg
                 g.sub_(_bp_matched(P, t))      # dz/dx^T h = h - A^T R A C h
-                xs[k - start] = None           # x_k is no longer needed
+                if cons and ctx.keep_segment:
+                    xs[k - start] = None       # x_k is no longer needed
             del xs
-        return gb, g, None, None, None, None, None, None, None
+        return gb, g, None, None, None, None, None, None, None, None
 
 
 def sirt_unrolled(b, projector, iterations, x0=None, relaxation=1.0, min_constraint=None,
-                  max_constraint=None, checkpoint_every=None, weights=None):
+                  max_constraint=None, checkpoint_every=None, weights=None, keep_segment=False):
     """x_N of `iterations` SIRT steps x <- clamp(x + C A^T R (b - A x)) on the
     matched projector pair, differentiable in b and x0.  The backward pass
     keeps only every `checkpoint_every`-th iterate (default ~sqrt(N)) and
-    recomputes the others."""
+    recomputes the others: each from its checkpoint when it is needed
+    (default; the least memory), or once per segment, holding the segment's
+    iterates (keep_segment=True; less recomputation).  Only the
+    constrained iteration needs the iterates in the backward pass (for the
+    clamp masks): without constraints the gradient is the same linear
+    recursion for every k and nothing is recomputed."""
     P = _resolve(projector)
     b = _check(b, P.proj_shape, "projection data")
     if x0 is None:
@@ -186,7 +226,7 @@ def sirt_unrolled(b, projector, iterations, x0=None, relaxation=1.0, min_constra
     R, C = weights if weights is not None else sirt_weights(P, relaxation, b.device)
     lo = -math.inf if min_constraint is None else float(min_constraint)
     hi = math.inf if max_constraint is None else float(max_constraint)
-    return _UnrolledSIRT.apply(b, x0, P, iterations, R, C, lo, hi, every)
+    return _UnrolledSIRT.apply(b, x0, P, iterations, R, C, lo, hi, every, bool(keep_segment))
 
 
 def sirt_unrolled_stored(b, projector, iterations, x0=None, relaxation=1.0, min_constraint=None,

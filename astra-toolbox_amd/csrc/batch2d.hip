@@ -126,6 +126,41 @@ __global__ __launch_bounds__(256) void batch_fp(BatchArgs B, const float *__rest
     }
 }
 
+// The same with two neighbouring detectors per thread (fp2d_value_pair):
+// DB detectors x (512 / DB) angles per workgroup.
+template <int DB>
+__global__ __launch_bounds__(256) void batch_fp_pair(BatchArgs B, const float *__restrict__ rows,
+                                                     const float *__restrict__ cols,
+                                                     const float *__restrict__ sinos,
+                                                     const float *__restrict__ lw,
+                                                     float *__restrict__ out, int mode) {
+    constexpr int TD = DB / 2, AB = 256 / TD;
+    const unsigned item = blockIdx.x;
+    const unsigned s = find_slice(B.prefix, B.n, item);
+    const SliceDesc d = B.desc[s];
+    const unsigned local = item - B.prefix[s];
+    const unsigned dblocks = (d.nd + DB - 1) / DB;
+    const int det = (local % dblocks) * DB + 2 * (threadIdx.x % TD);
+    const int angle = (local / dblocks) * AB + (threadIdx.x / TD);
+    if (det >= (int)d.nd || angle >= (int)d.na) return;
+    const FP2DAngle &G = B.fpa[d.ang_off + angle];
+    const float *r = rows + d.rows_off, *c = cols + d.cols_off;
+    const float2 v = (d.kind == ASTRA_AMD_GEOM_FAN) ? fp2d_value_pair<true>(G, det, d.w, d.h, r, c)
+                                                     : fp2d_value_pair<false>(G, det, d.w, d.h, r, c);
+    for (int k = 0; k < 2; ++k) {
+        const int dd = det + k;
+        if (dd >= (int)d.nd) break;
+        const float val = k ? v.y : v.x;
+        const size_t oidx = d.psino_off + (size_t)angle * (d.nd + 2 * kPad) + dd + kPad;
+        if (mode == OUT_SET) {
+            out[oidx] = val * d.osc;
+        } else {
+            const size_t idx = d.sino_off + (size_t)angle * d.nd + dd;
+            out[oidx] = (sinos[idx] - val * d.osc) * lw[oidx];
+        }
+    }
+}
+
 // The same over 64 detectors x 16 angles per workgroup with the slice rows
 // staged in LDS (fp2d_tile_lds).
 __global__ __launch_bounds__(256) void batch_fp_lds(BatchArgs B, const float *__restrict__ rows,
@@ -166,7 +201,7 @@ __global__ __launch_bounds__(256) void batch_fp_lds(BatchArgs B, const float *__
 // ------------------------------------------------------------------ BP ---
 // 64 columns x (4 * PY) rows per workgroup.  mode OUT_SET writes
 // osc * A^T r; OUT_UPDATE writes clamp(x + pw * osc * A^T r).
-template <int PY>
+template <int PY, bool PR = false>
 __global__ __launch_bounds__(256) void batch_bp(BatchArgs B, const float *__restrict__ psino,
                                                 const float *__restrict__ pw,
                                                 float *__restrict__ vols, int mode, float lo,
@@ -183,9 +218,9 @@ __global__ __launch_bounds__(256) void batch_bp(BatchArgs B, const float *__rest
     const float *sp = psino + d.psino_off;
     const BP2DAngle *ang = B.bpa + d.ang_off;
     if (d.kind == ASTRA_AMD_GEOM_FAN)
-        bp2d_column<1, PY>(X, Y0, d.w, d.h, d.na, d.nd, sp, ang, acc);
+        bp2d_column<1, PY, PR>(X, Y0, d.w, d.h, d.na, d.nd, sp, ang, acc);
     else
-        bp2d_column<0, PY>(X, Y0, d.w, d.h, d.na, d.nd, sp, ang, acc);
+        bp2d_column<0, PY, PR>(X, Y0, d.w, d.h, d.na, d.nd, sp, ang, acc);
     for (int i = 0; i < PY; ++i) {
         const int Y = Y0 + i;
         if (Y >= (int)d.h) break;
@@ -232,9 +267,9 @@ struct Batch2D {
     int device = 0;
     std::vector<SliceDesc> host;
     size_t vol_total = 0, sino_total = 0, psino_total = 0, rows_total = 0, cols_total = 0;
-    std::vector<unsigned> pre_prep, pre_pad, pre_fp[4], pre_bp[3];
+    std::vector<unsigned> pre_prep, pre_pad, pre_fp[7], pre_bp[3];
     DevArray desc, fpa, bpa;
-    DevArray d_prep, d_pad, d_fp[4], d_bp[3];
+    DevArray d_prep, d_pad, d_fp[7], d_bp[3];
     // work buffers (built on first use)
     DevArray line_w, pixel_w, resid, rows, cols, ones;
     bool weights_ready = false;
@@ -291,7 +326,7 @@ Batch2D *batch2d_build(unsigned n, const int *kinds, const unsigned *ws, const u
         b->pre_fp[k].assign(n + 1, 0);
         b->pre_bp[k].assign(n + 1, 0);
     }
-    b->pre_fp[3].assign(n + 1, 0);
+    for (int k = 3; k < 7; ++k) b->pre_fp[k].assign(n + 1, 0);
     size_t ang = 0;
     for (unsigned i = 0; i < n; ++i) {
         SliceDesc &d = b->host[i];
@@ -327,6 +362,10 @@ Batch2D *batch2d_build(unsigned n, const int *kinds, const unsigned *ws, const u
             acc(b->pre_bp[k], (size_t)((d.w + 63) / 64) * ((d.h + rows_per_group - 1) / rows_per_group));
         }
         acc(b->pre_fp[3], (size_t)((d.nd + 63) / 64) * ((d.na + 15) / 16));
+        for (int k = 0; k < 3; ++k) {  // pair tilings: kFPDB[k] dets x 512 / kFPDB[k] angles
+            const unsigned db = kFPDB[k], ab = 512 / kFPDB[k];
+            acc(b->pre_fp[4 + k], (size_t)((d.nd + db - 1) / db) * ((d.na + ab - 1) / ab));
+        }
     }
     b->desc = DevArray(sizeof(SliceDesc) * n);
     b->desc.upload(b->host.data(), sizeof(SliceDesc) * n);
@@ -344,7 +383,7 @@ Batch2D *batch2d_build(unsigned n, const int *kinds, const unsigned *ws, const u
         up(b->d_fp[k], b->pre_fp[k]);
         up(b->d_bp[k], b->pre_bp[k]);
     }
-    up(b->d_fp[3], b->pre_fp[3]);
+    for (int k = 3; k < 7; ++k) up(b->d_fp[k], b->pre_fp[k]);
     return b.release();
 }
 
@@ -401,20 +440,30 @@ void launch_fp(const Batch2D &b, int cfg, const float *sinos, float *out, int mo
         batch_fp<128><<<g, 256, 0, s>>>(a, rows, cols, sinos, lw, out, mode);
     else if (cfg == 2)
         batch_fp<32><<<g, 256, 0, s>>>(a, rows, cols, sinos, lw, out, mode);
-    else
+    else if (cfg == 3)
         batch_fp_lds<<<g, 256, 0, s>>>(a, rows, cols, sinos, lw, out, mode);
+    else if (cfg == 4)
+        batch_fp_pair<64><<<g, 256, 0, s>>>(a, rows, cols, sinos, lw, out, mode);
+    else if (cfg == 5)
+        batch_fp_pair<128><<<g, 256, 0, s>>>(a, rows, cols, sinos, lw, out, mode);
+    else
+        batch_fp_pair<32><<<g, 256, 0, s>>>(a, rows, cols, sinos, lw, out, mode);
     AMX_LAUNCHED();
 }
 
-// FP candidates: kFPDB tilings (0..2) and the LDS-staged one (3); the
-// autotuner times the first kFPTimed (the others by request only)
-constexpr int kFPCandidates = 4;
+// FP candidates: kFPDB tilings (0..2), the LDS-staged one (3) and the
+// kFPDB tilings with two detectors per thread (4..6).  The autotuner times
+// the first three; the others never won on the 4096-slice workload
+// (LDS: no faster; pairs: 160 vs 173 slices/s) and run by request only.
+constexpr int kFPCandidates = 7;
 constexpr int kFPTimed = 3;
 
-// BP candidates: cfg % 3 picks the rows per thread (kBPPY), cfg >= 3 the
-// LDS-staged form of the same tiling
-constexpr int kBPCandidates = 6;
-constexpr int kBPTimed = 3;
+// BP candidates: cfg % 3 picks the rows per thread (kBPPY); cfg 0..2 read
+// every tap from global memory, 3..5 from LDS-staged footprints, 6..8 from
+// global memory two rows per load.  The autotuner times 0..5 (the row
+// pairs never won: 175 vs 173 slices/s at best).
+constexpr int kBPCandidates = 9;
+constexpr int kBPTimed = 6;
 
 void launch_bp(const Batch2D &b, int cfg, const float *psino, float *vols, int mode, float lo,
                float hi, hipStream_t s) {
@@ -432,8 +481,14 @@ void launch_bp(const Batch2D &b, int cfg, const float *psino, float *vols, int m
         batch_bp_lds<4><<<g, 256, 0, s>>>(a, psino, pw, vols, mode, lo, hi);
     else if (cfg == 4)
         batch_bp_lds<2><<<g, 256, 0, s>>>(a, psino, pw, vols, mode, lo, hi);
-    else
+    else if (cfg == 5)
         batch_bp_lds<8><<<g, 256, 0, s>>>(a, psino, pw, vols, mode, lo, hi);
+    else if (cfg == 6)
+        batch_bp<4, true><<<g, 256, 0, s>>>(a, psino, pw, vols, mode, lo, hi);
+    else if (cfg == 7)
+        batch_bp<2, true><<<g, 256, 0, s>>>(a, psino, pw, vols, mode, lo, hi);
+    else
+        batch_bp<8, true><<<g, 256, 0, s>>>(a, psino, pw, vols, mode, lo, hi);
     AMX_LAUNCHED();
 }
 
@@ -488,7 +543,7 @@ void autotune(Batch2D &b, hipStream_t s) {
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    // ASTRA_AMD_BATCH_BP_CFG=<0..5> / ASTRA_AMD_BATCH_FP_CFG=<0..3> replace
+    // ASTRA_AMD_BATCH_BP_CFG=<0..8> / ASTRA_AMD_BATCH_FP_CFG=<0..6> replace
     // the timed choice (read when a batch is first run)
     const char *eb = std::getenv("ASTRA_AMD_BATCH_BP_CFG");
     if (eb && *eb) b.bp_cfg = std::min(std::max(std::atoi(eb), 0), kBPCandidates - 1);

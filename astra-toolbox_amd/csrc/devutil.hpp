@@ -34,6 +34,22 @@ __device__ __forceinline__ float4 ld4b(const float *base, unsigned byte_off) {
     return make_float4(v.x, v.y, v.z, v.w);
 }
 
+// Four floats at base + byte offset, 4-byte aligned (one
+// global_load_dwordx4): two linear taps of one row when their texels lie
+// within four.
+__device__ __forceinline__ float4 ld4u(const float *base, unsigned byte_off) {
+    typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));
+    f4u v = *reinterpret_cast<const f4u *>(reinterpret_cast<const char *>(base) + byte_off);
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+
+// Texels (g + d, g + d + 1) out of four loaded from texel g, d in {0, 1, 2}
+// (a float holding an integer).
+__device__ __forceinline__ float2 pick2(float4 x, float d) {
+    const bool ge1 = d >= 0.5f, ge2 = d >= 1.5f;
+    return make_float2(ge1 ? (ge2 ? x.z : x.y) : x.x, ge1 ? (ge2 ? x.w : x.z) : x.y);
+}
+
 // 16-byte aligned float4 / one float at base + byte offset (scalar base,
 // 32-bit vector offset)
 __device__ __forceinline__ float4 ld4a(const float4 *base, unsigned byte_off) {

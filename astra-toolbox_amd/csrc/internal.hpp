@@ -236,6 +236,8 @@ void fp2d_launch_range(const Geom2D &g, const float *vol, float *sino, float out
                        const float *resid_b, const float *resid_w, bool padded_out, unsigned rays,
                        unsigned a0, unsigned na_sub, hipStream_t s);
 bool bp2d_lds();
+bool fp2d_pairs();
+bool bp2d_pairs();
 size_t fp2d_copies_elems(const Geom2D &g);
 void fp2d_run(const Geom2D &g, const float *vol, float *sino, float out_scale, int mode,
               const float *resid_b, const float *resid_w, bool padded_out, unsigned rays,

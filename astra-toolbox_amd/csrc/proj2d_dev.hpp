@@ -117,6 +117,47 @@ __device__ __forceinline__ float march2d(const float *__restrict__ base, unsigne
     return acc0 + acc1;
 }
 
+// Sums of two rays along the same slice axis of one padded copy, over the
+// union of their slice ranges: outside its own range a ray's coordinate is
+// clamped into the zero border, so the extra samples add exact zeros, and
+// one alternation of accumulators over the union splits each ray's
+// samples into the same two sums as march2d (possibly with the two roles
+// swapped), so each result equals march2d's.  One 16-byte load from the
+// lower of the two left texels serves both rays when their left texels
+// are at most two apart (the load starts at most pitch - 4, so it stays
+// inside the row); otherwise the other ray takes its own 8-byte load.
+__device__ __forceinline__ float2 march2d_pair(const float *__restrict__ base, unsigned pitch,
+                                               float hi_t, const Ray2D &A, const Ray2D &B) {
+    const bool ea = A.lo > A.hi, eb = B.lo > B.hi;
+    if (ea && eb) return make_float2(0.f, 0.f);
+    const int lo = ea ? B.lo : (eb ? A.lo : min(A.lo, B.lo));
+    const int hi = ea ? B.hi : (eb ? A.hi : max(A.hi, B.hi));
+    const float gmax = (float)pitch - 4.f;
+    float a0 = 0.f, a1 = 0.f, b0 = 0.f, b1 = 0.f;
+    const float *p = base + (size_t)(lo > 0 ? lo : 0) * pitch;
+    for (int s = lo; s <= hi; ++s, p += pitch) {
+        const float sf = (float)s;
+        const float ta = clamp_pad(fmaf(A.a, sf, A.c), hi_t);
+        const float tb = clamp_pad(fmaf(B.a, sf, B.c), hi_t);
+        const float fa = floorf(ta), fb = floorf(tb);
+        const float g = fminf(fminf(fa, fb), gmax);
+        const float4 x = ld4u(p, (unsigned)g << 2);
+        const float da = fa - g, db = fb - g;
+        float2 qa = pick2(x, da), qb = pick2(x, db);
+        if (da > 2.5f) qa = ld2b(p, (unsigned)fa << 2);
+        if (db > 2.5f) qb = ld2b(p, (unsigned)fb << 2);
+        const float va = lerp(qa.x, qa.y, ta - fa), vb = lerp(qb.x, qb.y, tb - fb);
+        if ((s - lo) & 1) {
+            a1 += va;
+            b1 += vb;
+        } else {
+            a0 += va;
+            b0 += vb;
+        }
+    }
+    return make_float2(a0 + a1, b0 + b1);
+}
+
 // Forward projection value of detector `det` (before the output scale):
 // mean over `rays` sub-rays of (ray sum x ray-length weight).
 template <bool FAN>
@@ -133,6 +174,25 @@ __device__ __forceinline__ float fp2d_value(const FP2DAngle &G, int det, unsigne
         sum += v * R.scale;
     }
     return sum / (float)rays;
+}
+
+// Values of detectors det and det + 1 (one sub-ray each), the two rays
+// marched together when they run along the same slice axis; equal to two
+// fp2d_value calls.  det + 1 may be past the detector (its value is unused).
+template <bool FAN>
+__device__ __forceinline__ float2 fp2d_value_pair(const FP2DAngle &G, int det, unsigned w,
+                                                  unsigned h, const float *rows,
+                                                  const float *cols) {
+    const bool va = ray_vertical<FAN>(G, det), vb = ray_vertical<FAN>(G, det + 1);
+    if (va != vb)
+        return make_float2(fp2d_value<FAN>(G, det, 1, w, h, rows, cols),
+                           fp2d_value<FAN>(G, det + 1, 1, w, h, rows, cols));
+    const Ray2D A = ray2d<FAN>(G, det + 0.5, va, w, h);
+    const Ray2D B = ray2d<FAN>(G, det + 1.5, va, w, h);
+    const float2 s = va ? march2d_pair(rows, w + 2 * kPad, (float)w + 2.5f, A, B)
+                        : march2d_pair(cols, h + 2 * kPad, (float)h + 2.5f, A, B);
+    // fp2d_value: (0 + sum * scale) / 1
+    return make_float2(s.x * A.scale, s.y * B.scale);
 }
 
 // out[oidx] per mode (OUT_ADD / OUT_SET / OUT_RESID / OUT_ADDMUL)
@@ -161,7 +221,11 @@ __device__ __forceinline__ void store_fp(float *out, size_t oidx, float val, int
 // 3 parallel unweighted, 4 fan unweighted (SART's BP, whose per-angle and
 // fr weights cancel against its line weights: cuda/2d/par_bp.cu:139-159,
 // cuda/2d/fan_bp.cu:167-199).
-template <int KIND, int PY>
+// PR: rows taken in pairs, one 16-byte load from the lower of the two left
+// texels serving both rows when those are at most two apart (the load
+// starts at most pitch - 4, inside the row), else an 8-byte load for the
+// far one; the same taps and values as PR = false.
+template <int KIND, int PY, bool PR = false>
 __device__ __forceinline__ void bp2d_column(int X, int Y0, unsigned w, unsigned h, unsigned na,
                                             unsigned nd, const float *__restrict__ sp,
                                             const BP2DAngle *__restrict__ ang, float *acc) {
@@ -179,17 +243,32 @@ __device__ __forceinline__ void bp2d_column(int X, int Y0, unsigned w, unsigned 
         // every tap load of the PY rows issued before the first use (the
         // loop is bound by the loads' latency)
         float2 q[PY];
-        float wt[PY], wg[PY];
+        float wt[PY], wg[PY], fl[PY];
 #pragma unroll
         for (int i = 0; i < PY; ++i) {
             float fr = (KIND == 0 || KIND == 3) ? 1.f : __builtin_amdgcn_rcpf(den);
             float t = clamp_pad(num * fr, hi);
             float f = floorf(t);
-            q[i] = ld2b(row, (unsigned)f << 2);
+            fl[i] = f;
+            if (!PR) q[i] = ld2b(row, (unsigned)f << 2);
             wt[i] = t - f;
             wg[i] = (KIND == 0) ? c.w : (KIND == 1 ? fr : (KIND == 2 ? fr * fr : 1.f));
             num -= c.num[2];  // next row: world y decreases by 1
             den -= c.den[2];
+        }
+        if (PR) {
+            static_assert(!PR || PY % 2 == 0, "row pairs");
+            const float gmax = (float)pitch - 4.f;
+#pragma unroll
+            for (int i = 0; i < PY; i += 2) {
+                const float g = fminf(fminf(fl[i], fl[i + 1]), gmax);
+                const float4 x = ld4u(row, (unsigned)g << 2);
+                const float d0 = fl[i] - g, d1 = fl[i + 1] - g;
+                q[i] = pick2(x, d0);
+                q[i + 1] = pick2(x, d1);
+                if (d0 > 2.5f) q[i] = ld2b(row, (unsigned)fl[i] << 2);
+                if (d1 > 2.5f) q[i + 1] = ld2b(row, (unsigned)fl[i + 1] << 2);
+            }
         }
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll

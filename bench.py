@@ -167,7 +167,7 @@ def copy_calibration(dev, nbytes=1 << 30):
     torch.cuda.empty_cache()
 
 
-def batched_sirt_bench(n_slices, n_plain, iters):
+def batched_sirt_bench(n_slices, n_plain, iters, host_phantoms=False):
     """Batched 2D SIRT slices/s (one hipGraph launch for all iterations of all
     slices) vs the plain per-slice SIRT_CUDA drop-in (one algorithm object
     per slice, run(iters), host data) on slices of the same seeded
@@ -183,8 +183,15 @@ def batched_sirt_bench(n_slices, n_plain, iters):
     pgs, vgs = batched.geometries(slices)
     B = batched.BatchedSIRT2D(pgs, vgs)
     truth = B.empty_volumes()
-    for i, s in enumerate(slices):
-        B.volume(truth, i).copy_(phantom.ellipses_2d_device(s["n"], seed=1000 + i, device=dev))
+    if host_phantoms:
+        # built on the host and uploaded once (a profiled run then sees a
+        # few hundred kernel dispatches instead of tens of thousands)
+        host = np.concatenate([phantom.ellipses_2d(s["n"], seed=1000 + i).astype(np.float32).ravel()
+                               for i, s in enumerate(slices)])
+        truth.copy_(torch.from_numpy(host))
+    else:
+        for i, s in enumerate(slices):
+            B.volume(truth, i).copy_(phantom.ellipses_2d_device(s["n"], seed=1000 + i, device=dev))
     sinos = B.forward(truth)
     work = [iters * 2.5 * s["na"] * s["n"] * s["n"] for s in slices]  # ~FP 1.5 + BP 1 per pixel-angle
     x = B.empty_volumes()
@@ -231,9 +238,10 @@ def batched_sirt_bench(n_slices, n_plain, iters):
 
 def differentiable_bench(n=256, na=360, nd=384, iters=20):
     """Differentiable cone projector: one forward projection + matched-adjoint
-    gradient of 1/2 ||A x - y||^2, and a 20-iteration unrolled SIRT
-    (forward + backward w.r.t. the projections) with recomputed iterates vs
-    autograd through stored iterates (time and peak torch memory)."""
+    gradient of 1/2 ||A x - y||^2, and a 20-iteration unrolled SIRT with
+    the non-negativity constraint (forward + backward w.r.t. the
+    projections) with recomputed iterates vs autograd through stored
+    iterates (time and peak torch memory)."""
     import torch
 
     import astra
@@ -265,14 +273,16 @@ def differentiable_bench(n=256, na=360, nd=384, iters=20):
     W = AG.sirt_weights(pid)
     res = {}
     from astra import _lib
-    for name, fn in (("recompute", AG.sirt_unrolled), ("stored", AG.sirt_unrolled_stored)):
+    keep = lambda *a, **k: AG.sirt_unrolled(*a, keep_segment=True, **k)
+    for name, fn in (("recompute", AG.sirt_unrolled), ("recompute_keep_segment", keep),
+                     ("stored", AG.sirt_unrolled_stored)):
         b = y.clone().requires_grad_(True)
         torch.cuda.synchronize(dev)
         torch.cuda.reset_peak_memory_stats(dev)
         base = torch.cuda.memory_allocated(dev)
         sbase, _ = _lib.scratch_usage(reset_peak=True)
         t = time.perf_counter()
-        x = fn(b, pid, iters, weights=W)
+        x = fn(b, pid, iters, weights=W, min_constraint=0.0)
         loss = ((x - target) ** 2).sum()
         loss.backward()
         torch.cuda.synchronize(dev)
@@ -286,9 +296,11 @@ def differentiable_bench(n=256, na=360, nd=384, iters=20):
                      "device_peak_extra_MiB_bound": torch_extra + scratch_extra}
         del b, x, loss
     log("bench: differentiable %d^3: grad step %.3f s; unrolled SIRT %d its fwd+bwd recompute %.2f s "
-        "(+%.0f MiB torch, +%.0f MiB scratch) vs stored %.2f s (+%.0f MiB torch, +%.0f MiB scratch)"
+        "(+%.0f MiB torch, +%.0f MiB scratch), recompute keeping segments %.2f s (+%.0f MiB torch) "
+        "vs stored %.2f s (+%.0f MiB torch, +%.0f MiB scratch)"
         % (n, grad_s, iters, res["recompute"]["seconds"], res["recompute"]["peak_extra_MiB"],
-           res["recompute"]["library_scratch_peak_MiB"], res["stored"]["seconds"],
+           res["recompute"]["library_scratch_peak_MiB"], res["recompute_keep_segment"]["seconds"],
+           res["recompute_keep_segment"]["peak_extra_MiB"], res["stored"]["seconds"],
            res["stored"]["peak_extra_MiB"], res["stored"]["library_scratch_peak_MiB"]))
     astra.projector3d.delete(pid)
     return {"workload": "cone %d^3, %d angles, %dx%d detector, SOD 1000, ODD 500" % (n, na, nd, nd),
@@ -491,6 +503,8 @@ def main():
     ap.add_argument("--plain-slices", type=int, default=64)
     ap.add_argument("--batched-iters", type=int, default=50)
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batched-host-phantoms", action="store_true",
+                    help="build the batched workload's phantoms on the host (for profiled runs)")
     ap.add_argument("--no-diff", action="store_true")
     ap.add_argument("--no-fan2d", action="store_true")
     ap.add_argument("--no-large", action="store_true")
@@ -653,7 +667,8 @@ def main():
     torch.cuda.empty_cache()
     batched_res = None
     if not args.no_batched:
-        batched_res = batched_sirt_bench(args.batched_slices, args.plain_slices, args.batched_iters)
+        batched_res = batched_sirt_bench(args.batched_slices, args.plain_slices, args.batched_iters,
+                                         args.batched_host_phantoms)
 
     diff_res = None
     if not args.no_diff:

@@ -199,9 +199,10 @@ def test_diff256_dot_product_identity():
 
 
 def test_diff256_unrolled_sirt_recompute_equals_stored():
-    """20-iteration unrolled SIRT at 256^3 / 360 x 384^2, loss = squared L2
-    to a seeded target: the recomputing backward equals autograd through
-    the stored iterates (<= 1e-4 after 20 iterations)."""
+    """20-iteration unrolled SIRT at 256^3 / 360 x 384^2 with the
+    non-negativity constraint, loss = squared L2 to a seeded target: the
+    recomputing backward equals autograd through the stored iterates
+    (<= 1e-4 after 20 iterations)."""
     import torch
     from astra import autograd as AG, phantom
     pid = _cone256()
@@ -212,7 +213,7 @@ def test_diff256_unrolled_sirt_recompute_equals_stored():
     out = []
     for fn in (AG.sirt_unrolled, AG.sirt_unrolled_stored):
         b = y.clone().requires_grad_(True)
-        x = fn(b, pid, 20, weights=W)
+        x = fn(b, pid, 20, weights=W, min_constraint=0.0)
         loss = ((x.double() - target.double()) ** 2).sum()
         (gb,) = torch.autograd.grad(loss, (b,))
         out.append((x.detach(), gb))

@@ -5,9 +5,10 @@ sums, so the outputs are equal bit for bit.
 * single-slice BP (parallel, fan, FBP-weighted fan), ragged tiles, angle
   counts that are not a multiple of the staged block, and a source close
   enough that footprints overflow the staged row (those angles fall back);
-* batched SIRT with every FP tiling (three global-memory ones and the
-  LDS-staged one) and every BP tiling (three rows-per-thread choices, each
-  global or LDS-staged).  The FP value of a ray does not depend on the
+* batched SIRT with every FP tiling (three global-memory ones with one or
+  two detectors per thread, and the LDS-staged one) and every BP tiling
+  (three rows-per-thread choices, each global, LDS-staged or with two rows
+  per load).  The FP value of a ray does not depend on the
   tiling, so every FP tiling gives the same volumes.  The BP steps a
   pixel's coordinate from its thread's first row (as the reference steps
   it along the row loop), so the last bits depend on the rows per thread:
@@ -109,9 +110,10 @@ def _batched_sirt(fp_cfg, bp_cfg, monkeypatch, iters=5):
 def test_batched_tilings(rows, monkeypatch):
     ref = _batched_sirt(0, rows, monkeypatch)
     assert np.abs(ref).max() > 0
-    # every FP tiling, with the global and the LDS-staged BP of these rows
-    for fp_cfg in (0, 1, 2, 3):
-        for bp_cfg in (rows, rows + 3):
+    # every FP tiling (one or two detectors per thread, global or LDS), with
+    # the global, LDS-staged and row-pair BP of these rows per thread
+    for fp_cfg in (0, 1, 2, 3, 4, 5, 6):
+        for bp_cfg in (rows, rows + 3, rows + 6):
             if (fp_cfg, bp_cfg) == (0, rows):
                 continue
             got = _batched_sirt(fp_cfg, bp_cfg, monkeypatch)
