@@ -207,10 +207,11 @@ def sirt_unrolled(b, projector, iterations, x0=None, relaxation=1.0, min_constra
                   max_constraint=None, checkpoint_every=None, weights=None, keep_segment=False):
     """x_N of `iterations` SIRT steps x <- clamp(x + C A^T R (b - A x)) on the
     matched projector pair, differentiable in b and x0.  The backward pass
-    keeps only every `checkpoint_every`-th iterate (default ~sqrt(N)) and
-    recomputes the others: each from its checkpoint when it is needed
-    (default; the least memory), or once per segment, holding the segment's
-    iterates (keep_segment=True; less recomputation).  Only the
+    keeps only every `checkpoint_every`-th iterate and recomputes the
+    others: each from its checkpoint when it is needed (default, with
+    three checkpoints; the least memory), or once per segment, holding the
+    segment's iterates (keep_segment=True, ~sqrt(N) checkpoints; less
+    recomputation).  Only the
     constrained iteration needs the iterates in the backward pass (for the
     clamp masks): without constraints the gradient is the same linear
     recursion for every k and nothing is recomputed."""
@@ -222,7 +223,15 @@ def sirt_unrolled(b, projector, iterations, x0=None, relaxation=1.0, min_constra
     iterations = int(iterations)
     if iterations < 1:
         raise AstraError("sirt_unrolled: iterations must be >= 1")
-    every = int(checkpoint_every or max(1, math.ceil(math.sqrt(iterations))))
+    # default checkpoints: ~sqrt(N) segments when each is held at once,
+    # else three (x_0 and two more), the least memory for a bounded
+    # recomputation (about N^2 / 18 extra steps)
+    if checkpoint_every:
+        every = int(checkpoint_every)
+    elif keep_segment:
+        every = max(1, math.ceil(math.sqrt(iterations)))
+    else:
+        every = max(1, math.ceil(iterations / 3))
     R, C = weights if weights is not None else sirt_weights(P, relaxation, b.device)
     lo = -math.inf if min_constraint is None else float(min_constraint)
     hi = math.inf if max_constraint is None else float(max_constraint)

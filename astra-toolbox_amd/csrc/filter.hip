@@ -216,13 +216,13 @@ __global__ __launch_bounds__(256) void filter_rows_kernel(FilterArgs A) {
 // A thread with 16 / R = B butterflies takes j = t + T b (b < B), element k
 // of butterfly b being its value m = b + B k, so the thread's inputs are
 // always in[t + T m] and the last pass leaves out[t + T m] in value m.
-template <int LOGN>
+template <int LOGN, unsigned WGT = 256>
 struct FftPlan {
     static constexpr unsigned N = 1u << LOGN;
     static constexpr unsigned T = N / 16;                       // threads per transform
     static constexpr unsigned P = (LOGN + 3) / 4;               // passes
     static constexpr unsigned LAST = 1u << (LOGN - 4 * (P - 1));
-    static constexpr unsigned F = T >= 256 ? 1 : 256 / T;       // transforms per workgroup
+    static constexpr unsigned F = T >= WGT ? 1 : WGT / T;       // transforms per workgroup
     static constexpr unsigned WG = T * F;
     static constexpr unsigned PADN = N + N / 16;                // LDS float2 per transform
 };
@@ -384,9 +384,9 @@ __device__ __forceinline__ void fft_passes(float2 (&v)[16], unsigned t, const fl
 // transform, F transforms per workgroup.  FDK / Parker pre-weights are
 // applied on load; the output is conj(FFT(conj(H . FFT(x + i y)))) / N,
 // i.e. row a = Re, row a + 1 = -Im of FFT(conj(H . Z)) / N.
-template <int LOGN>
-__global__ __launch_bounds__(FftPlan<LOGN>::WG) void filter_rows_stockham(FilterArgs A) {
-    using PL = FftPlan<LOGN>;
+template <int LOGN, unsigned WGT>
+__global__ __launch_bounds__((FftPlan<LOGN, WGT>::WG)) void filter_rows_stockham(FilterArgs A) {
+    using PL = FftPlan<LOGN, WGT>;
     constexpr unsigned N = PL::N, T = PL::T;
     __shared__ float2 lds_all[PL::F * PL::PADN];
     const unsigned f = threadIdx.x / T, t = threadIdx.x % T;
@@ -442,14 +442,29 @@ __global__ __launch_bounds__(FftPlan<LOGN>::WG) void filter_rows_stockham(Filter
     }
 }
 
-template <int LOGN>
-void launch_stockham(const FilterArgs &A, hipStream_t s) {
-    using PL = FftPlan<LOGN>;
+template <int LOGN, unsigned WGT>
+void launch_stockham_wg(const FilterArgs &A, hipStream_t s) {
+    using PL = FftPlan<LOGN, WGT>;
     const size_t per = 2 * (size_t)PL::F;  // rows per workgroup
     const size_t groups = (A.n_rows + per - 1) / per;
     AMX_REQUIRE(groups < (1ull << 31), "filter: too many rows");
-    filter_rows_stockham<LOGN><<<(unsigned)groups, PL::WG, 0, s>>>(A);
+    filter_rows_stockham<LOGN, WGT><<<(unsigned)groups, PL::WG, 0, s>>>(A);
     AMX_LAUNCHED();
+}
+
+// Workgroup size of the Stockham filter (threads, several transforms per
+// group when one takes fewer): ASTRA_AMD_FILTER_WG = 64 (default), 128,
+// 256 or 512, read at every call.  Measured at 768 x 720 rows x 2048
+// points: 2.08 / 2.08 / 2.20 / 2.38 ms; at 1536 x 1440 x 4096: 17.2 /
+// 17.3 / 17.3 / 19.7 ms.
+template <int LOGN>
+void launch_stockham(const FilterArgs &A, hipStream_t s) {
+    const char *e = std::getenv("ASTRA_AMD_FILTER_WG");
+    const int wg = e ? std::atoi(e) : 64;
+    if (wg == 64) launch_stockham_wg<LOGN, 64>(A, s);
+    else if (wg == 128) launch_stockham_wg<LOGN, 128>(A, s);
+    else if (wg == 512) launch_stockham_wg<LOGN, 512>(A, s);
+    else launch_stockham_wg<LOGN, 256>(A, s);
 }
 
 // transforms of 16 .. 4096 points (detectors up to 2048 bins; at most

@@ -784,26 +784,26 @@ __global__ __launch_bounds__(256) void bp3d_kernel(BPArgs A) {
 // each workgroup (16 x 16 columns of ZS voxels) stages, per angle, the
 // rectangle of the pair plane that its block projects into as 16-byte
 // quads -- quad (r, c) = pairs (r, c) and (r, c + 1), the whole tap -- so
-// that every tap is one aligned 16-byte LDS read (the LDS serves those at
-// several times the rate of the texture path, and of two 8-byte reads).
-// The quads are loaded one angle ahead with coalesced 16-byte loads and held
-// in registers while the current angle is sampled.  The footprint is the
-// bounding box of the projected corners of the block (u and v are
-// linear-fractional in the voxel position, so the extremes over the block
-// are at its corners; the clamp to the padded range is monotone), widened
-// by one texel on each side, far beyond the fp32 rounding of the taps.
-// Taps, weights and the order of the sums are those of
-// bp3d_kernel<KIND, ZS, true>, so both give the same floats.  An angle
-// whose footprint exceeds the tile, or whose denominator changes sign over
-// the block, is sampled from global memory as in bp3d_kernel.
-// A staged row is `lp` quads long: lp = w | 1 (pm < 0) or
-// w + ((pm - w) mod 16) (pm in [0, 16)), so that rows start at different
-// offsets modulo the 16 quads (256 bytes) that one LDS cycle serves.
-// LM = 0: 16 lanes = one row of 16 columns; LM = 1: 16 lanes = a 4 x 4
-// patch of columns.
-constexpr int kTileCap = 1024;  // LDS quads per staged tile (16 KiB)
-constexpr int kFpGroup = 256;   // angles whose footprints are held at once
-constexpr int kTileSlots = kTileCap / 256;
+// that every tap is one aligned 16-byte LDS read.  Thread t stages column
+// t % 64 of rows t / 64 + 4 k (k < 8) with 16-byte loads issued one angle
+// ahead and held in registers while the current angle is sampled, so the
+// staging needs no index division.  The footprint is the bounding box of
+// the projected corners of the block (u and v are linear-fractional in the
+// voxel position, so the extremes over the block are at its corners; the
+// clamp to the padded range is monotone), widened by one texel on each
+// side, far beyond the fp32 rounding of the taps (replayed on the CPU over
+// whole geometries by the tests).  Taps, weights and the order of the sums
+// are those of bp3d_kernel<KIND, ZS, true>, so both give the same floats.
+// An angle whose footprint exceeds 64 quads x 32 rows or the tile, or whose
+// denominator changes sign over the block, is sampled from global memory
+// as in bp3d_kernel.  A staged row is `lp` quads long: lp = w | 1 (pm < 0)
+// or w + ((pm - w) mod 16) (pm in [0, 16)), so that rows start at
+// different offsets modulo the 16 quads (256 bytes) of one LDS cycle.
+constexpr int kTileCap = 2048;   // LDS quads of the staged tile (32 KiB)
+constexpr int kFpGroup = 256;    // angles whose footprints are held at once
+constexpr int kStageCols = 64;   // widest staged footprint (quads)
+constexpr int kStageRows = 32;   // tallest staged footprint (rows): 4 per thread slot
+constexpr int kStageSlots = kStageRows / 4;
 
 __device__ __forceinline__ int lds_row_pitch(int w, int pm) {
     return pm < 0 ? (w | 1) : w + ((pm - w) & 15);
@@ -842,19 +842,20 @@ __device__ int4 bp_footprint(const BP3DAngle &c, float x0, float x1, float y0, f
     const int u0 = max(0, (int)umin - 1), u1 = min(pitch - 2, (int)umax + 1);
     const int v0 = max(0, (int)vmin - 1), v1 = min(prow - 1, (int)vmax + 1);
     const int w = u1 - u0 + 1, h = v1 - v0 + 1;
-    if (w < 1 || w > 1024 || lds_row_pitch(w, pm) * h > kTileCap) return make_int4(0, 0, 0, 0);
+    if (w < 1 || w > kStageCols || h > kStageRows || lds_row_pitch(w, pm) * h > kTileCap)
+        return make_int4(0, 0, 0, 0);
     return make_int4(u0, v0, w, h);
 }
 
-template <int KIND, int ZS, int LM>
+template <int KIND, int ZS>
 __global__ __launch_bounds__(256) void bp3d_lds_kernel(BPArgs A, int pm) {
-    __shared__ float4 tile[2][kTileCap];
+    __shared__ float4 tile[kTileCap];
     __shared__ int4 fps[kFpGroup];
     const unsigned tid = threadIdx.x;
     const unsigned tiles_x = (A.nx + 15) / 16;
     const int X0 = (blockIdx.x % tiles_x) * 16, Y0 = (blockIdx.x / tiles_x) * 16;
-    const int X = X0 + (LM == 0 ? (int)(tid & 15) : (int)((tid & 3) | (((tid >> 4) & 3) << 2)));
-    const int Y = Y0 + (LM == 0 ? (int)(tid >> 4) : (int)(((tid >> 2) & 3) | ((tid >> 6) << 2)));
+    const int X = X0 + (int)(tid & 15);
+    const int Y = Y0 + (int)(tid >> 4);
     const int Z0 = blockIdx.y * ZS;
     const bool active = X < (int)A.nx && Y < (int)A.ny;
     const float hx = 0.5f * A.nx - 0.5f, hy = 0.5f * A.ny - 0.5f, hz = 0.5f * A.nz - 0.5f;
@@ -871,31 +872,26 @@ __global__ __launch_bounds__(256) void bp3d_lds_kernel(BPArgs A, int pm) {
     const size_t plane = (size_t)prow * pitch;                 // pairs per angle
     const unsigned rowb = pitch * 8u;
     const float hiu = (float)A.nu + 2.5f, hiv = (float)A.nv + 2.5f;
+    const int sc = (int)(tid & 63), sr = (int)(tid >> 6);       // staging column, first row
+    const char *tile_b = reinterpret_cast<const char *>(tile);
     float acc[ZS];
 #pragma unroll
     for (int i = 0; i < ZS; ++i) acc[i] = 0.f;
-    float4 pre[kTileSlots];
+    float4 pre[kStageSlots];
 
-    // quads i < w h of the footprint, row-major, into registers
+    // this thread's quads of the footprint: column sc of rows sr + 4 k
     auto fetch = [&](unsigned a, int4 f) {
-        const int n = f.z * f.w;
-        const float invw = 1.f / (float)f.z;
-        const float *src = A.pp + ((size_t)a * plane + (size_t)f.y * pitch + f.x) * 2;
+        const float *pl = A.pp + (size_t)a * plane * 2;
+        unsigned off = ((unsigned)(f.y + sr) * pitch + (unsigned)(f.x + sc)) * 8u;
+        const unsigned step = 4u * rowb;
 #pragma unroll
-        for (int k = 0; k < kTileSlots; ++k) {
-            const int i = (int)tid + 256 * k;
-            if (i < n) {
-                // (i + 0.5) / w is at least 0.5 / w from an integer, far
-                // beyond the fp32 error for w <= 1024, i < 1024
-                const int r = (int)(((float)i + 0.5f) * invw);
-                pre[k] = ld4b(src, (unsigned)(r * (int)pitch + (i - r * f.z)) * 8u);
-            }
-        }
+        for (int k = 0; k < kStageSlots; ++k, off += step)
+            if (sc < f.z && sr + 4 * k < f.w) pre[k] = ld4b(pl, off);
     };
 
     for (unsigned g0 = 0; g0 < A.na; g0 += kFpGroup) {
         const unsigned gn = min((unsigned)kFpGroup, A.na - g0);
-        __syncthreads();  // the previous group's footprints are no longer read
+        __syncthreads();  // the previous group's footprints and tile are no longer read
         if (tid < gn)
             fps[tid] = bp_footprint<KIND>(A.ang[g0 + tid], cx0, cx1, cy0, cy1, cz0, cz1, hiu, hiv,
                                           (int)pitch, (int)prow, pm);
@@ -905,22 +901,14 @@ __global__ __launch_bounds__(256) void bp3d_lds_kernel(BPArgs A, int pm) {
             const unsigned a = g0 + k;
             const int4 f = fps[k];
             const int lp = lds_row_pitch(f.z, pm);
-            float4 *t = tile[a & 1];
+            if (k) __syncthreads();  // every thread is done with angle a - 1's tile
             if (f.z) {
-                const int n = f.z * f.w;
-                const float invw = 1.f / (float)f.z;
+                int idx = sr * lp + sc;
 #pragma unroll
-                for (int j = 0; j < kTileSlots; ++j) {
-                    const int i = (int)tid + 256 * j;
-                    if (i < n) {
-                        const int r = (int)(((float)i + 0.5f) * invw);
-                        t[r * lp + (i - r * f.z)] = pre[j];
-                    }
-                }
+                for (int j = 0; j < kStageSlots; ++j, idx += 4 * lp)
+                    if (sc < f.z && sr + 4 * j < f.w) tile[idx] = pre[j];
             }
-            // tile[a & 1] is complete; tile[(a + 1) & 1] was last read at
-            // angle a - 1, before every thread reached this barrier
-            __syncthreads();
+            __syncthreads();  // the tile holds angle a
             if (k + 1 < gn) {
                 const int4 fn = fps[k + 1];
                 if (fn.z) fetch(a + 1, fn);
@@ -936,20 +924,18 @@ __global__ __launch_bounds__(256) void bp3d_lds_kernel(BPArgs A, int pm) {
             const float wu = frac(pu);
             const float wgt = (KIND == 0) ? fr * fr : (KIND == 1 ? fr : wa);
             if (f.z) {
-                // all ZS tap reads in flight before the first use
-                const int cu = (int)(unsigned)pu - f.x;
-                const int last = lp * f.w - 1;
+                // byte offset of tap row r: r * lp * 16 + (column - u0 - v0 lp) * 16;
+                // inside the tile by the footprint's margin
+                const int lp16 = lp * 16;
+                const int base = ((int)(unsigned)pu - f.x - f.y * lp) * 16;
                 float4 q[ZS];
                 float wv[ZS];
 #pragma unroll
                 for (int i = 0; i < ZS; ++i) {
                     const float pv = clamp_pad(vn * fr, hiv);
                     wv[i] = frac(pv);
-                    int off = ((int)(unsigned)pv - f.y) * lp + cu;
-                    // in range by the footprint's margin; the clamp only
-                    // keeps the LDS address inside the tile
-                    off = min(max(off, 0), last);
-                    q[i] = t[off];
+                    const int off = __mul24((int)(unsigned)pv, lp16) + base;
+                    q[i] = *reinterpret_cast<const float4 *>(tile_b + off);
                     vn += c.nv[2];
                 }
 #pragma unroll
@@ -957,13 +943,17 @@ __global__ __launch_bounds__(256) void bp3d_lds_kernel(BPArgs A, int pm) {
             } else {
                 const float *pl = A.pp + (size_t)a * plane * 2;
                 const unsigned ou = (unsigned)pu << 3;
+                float4 q[ZS];
+                float wv[ZS];
 #pragma unroll
                 for (int i = 0; i < ZS; ++i) {
                     const float pv = clamp_pad(vn * fr, hiv);
-                    const float val = bilerp_pairs(ld4b(pl, __umul24((unsigned)pv, rowb) + ou), wu, frac(pv));
-                    acc[i] = fmaf(wgt, val, acc[i]);
+                    wv[i] = frac(pv);
+                    q[i] = ld4b(pl, __umul24((unsigned)pv, rowb) + ou);
                     vn += c.nv[2];
                 }
+#pragma unroll
+                for (int i = 0; i < ZS; ++i) acc[i] = fmaf(wgt, bilerp_pairs(q[i], wu, wv[i]), acc[i]);
             }
         }
     }
@@ -1431,17 +1421,11 @@ bool bp_lds() {
 
 // LDS row pitch of the staged BP: ASTRA_AMD_BP3D_LDS_PITCH=<0..15> pads a
 // staged row to that residue modulo 16 quads, -1 (default) to the next odd
-// length; ASTRA_AMD_BP3D_LDS_PATCH=1 maps 16 lanes to 4 x 4 columns instead
-// of a row of 16 (both read at every launch)
+// length (read at every launch)
 int bp_lds_pitch_mod() {
     const char *e = std::getenv("ASTRA_AMD_BP3D_LDS_PITCH");
     const int x = e ? std::atoi(e) : -1;
     return x < 0 ? -1 : (x & 15);
-}
-
-bool bp_lds_patch() {
-    const char *e = std::getenv("ASTRA_AMD_BP3D_LDS_PATCH");
-    return e && e[0] == '1';
 }
 
 int bp_zs() {
@@ -1635,15 +1619,9 @@ void bp3d_dispatch(int kind, bool zc, const BPArgs &A, hipStream_t s) {
     dim3 grid(((A.nx + 15) / 16) * ((A.ny + 15) / 16), (A.nz + ZS - 1) / ZS);
     if (zc && bp_lds()) {
         const int pm = bp_lds_pitch_mod();
-        if (bp_lds_patch()) {
-            if (kind == 0) bp3d_lds_kernel<0, ZS, 1><<<grid, 256, 0, s>>>(A, pm);
-            else if (kind == 1) bp3d_lds_kernel<1, ZS, 1><<<grid, 256, 0, s>>>(A, pm);
-            else bp3d_lds_kernel<2, ZS, 1><<<grid, 256, 0, s>>>(A, pm);
-        } else {
-            if (kind == 0) bp3d_lds_kernel<0, ZS, 0><<<grid, 256, 0, s>>>(A, pm);
-            else if (kind == 1) bp3d_lds_kernel<1, ZS, 0><<<grid, 256, 0, s>>>(A, pm);
-            else bp3d_lds_kernel<2, ZS, 0><<<grid, 256, 0, s>>>(A, pm);
-        }
+        if (kind == 0) bp3d_lds_kernel<0, ZS><<<grid, 256, 0, s>>>(A, pm);
+        else if (kind == 1) bp3d_lds_kernel<1, ZS><<<grid, 256, 0, s>>>(A, pm);
+        else bp3d_lds_kernel<2, ZS><<<grid, 256, 0, s>>>(A, pm);
         return;
     }
     if (kind == 0)

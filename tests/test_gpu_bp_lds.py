@@ -2,8 +2,8 @@
 (ASTRA_AMD_BP3D_LDS=1) against the kernel that reads the taps from global
 memory: same taps, weights and order of sums, so the volumes are equal bit
 for bit.  Cases cover the three kernel kinds (cone fr^2 and FDK weighting,
-cone 2d_weighting, par3d), LDS row layouts (odd or padded row lengths,
-rows or 4 x 4 patches of lanes), SET / ADD and the SIRT update, more than 256
+cone 2d_weighting, par3d), LDS row layouts (odd or padded row lengths),
+SET / ADD and the SIRT update, more than 256
 angles per launch (two footprint groups), volumes whose edge blocks are
 partly outside, and a source close enough that footprints overflow the
 tile (those angles fall back to global memory).
@@ -35,14 +35,12 @@ CASES = [
 ]
 
 
-@pytest.mark.parametrize("layout", [("-1", "0"), ("0", "0"), ("8", "1"), ("3", "1")],
-                         ids=["odd_rows", "pm0_rows", "pm8_patch", "pm3_patch"])
+@pytest.mark.parametrize("pitch", ["-1", "0", "8", "3"], ids=["odd_rows", "pm0", "pm8", "pm3"])
 @pytest.mark.parametrize("case", CASES, ids=["cone", "cone_groups", "par3d", "cone_overflow"])
-def test_lds_bp_equals_global_bp(case, layout, monkeypatch):
+def test_lds_bp_equals_global_bp(case, pitch, monkeypatch):
     kind, nx, ny, nz, na, nu, nv, sod, odd = case
     astra, vg, pg, proj = _case(kind, nx, ny, nz, na, nu, nv, sod, odd)
-    monkeypatch.setenv("ASTRA_AMD_BP3D_LDS_PITCH", layout[0])
-    monkeypatch.setenv("ASTRA_AMD_BP3D_LDS_PATCH", layout[1])
+    monkeypatch.setenv("ASTRA_AMD_BP3D_LDS_PITCH", pitch)
     out = {}
     for lds in (False, True):
         monkeypatch.setenv("ASTRA_AMD_BP3D_LDS", "1" if lds else "0")

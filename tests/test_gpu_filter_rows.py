@@ -95,3 +95,16 @@ def test_filter_rows_many_rows_full_size():
     got2 = _filter_rows(n_det, H, 2 * rows)
     assert np.array_equal(got2, 2 * got)
     del torch
+
+
+@pytest.mark.parametrize("wg", ["128", "256", "512"])
+@pytest.mark.parametrize("n_det", [20, 384, 768, 1536])
+def test_filter_rows_workgroup_sizes_equal(n_det, wg, monkeypatch):
+    """Every workgroup size of the register FFT (transforms per group)
+    gives the default's floats."""
+    half = _spectrum(n_det, n_det + 7)
+    rows = np.random.default_rng(n_det).random((37, n_det), dtype=np.float32)
+    ref = _filter_rows(n_det, half, rows)
+    monkeypatch.setenv("ASTRA_AMD_FILTER_WG", wg)
+    got = _filter_rows(n_det, half, rows)
+    assert np.array_equal(got, ref)
