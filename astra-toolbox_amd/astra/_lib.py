@@ -164,8 +164,10 @@ def profile_read(kernel):
 
 def calibrate_gather(mode, iterations=4096):
     """Lane-loads per second of the gather calibration kernel on the
-    current device (mode 0: 16-byte loads with the taps' lane pattern,
-    1: 16-byte coalesced, 2: 8-byte taps, 3: 4-byte taps)."""
+    current device (mode 0: 16-byte loads with the 3D taps' lane pattern,
+    1: 16-byte coalesced, 2: 8-byte 2D taps at 4-byte alignment, 3: 16-byte
+    loads of two 2D taps at 4-byte alignment, 4: the same at 8-byte
+    alignment)."""
     out = ctypes.c_double()
     call("astra_amd_calibrate_gather", int(mode), int(iterations), ctypes.byref(out), None)
     return out.value

@@ -42,6 +42,10 @@ __device__ __forceinline__ unsigned mix(unsigned x) {
 // MODE 2 (8-byte at 4-byte alignment, the 2D projectors' linear fetch):
 //   the 64 lanes of a wave read one row, 1.2 texels apart: floats
 //   floor(1.2 lane + f) + fb, <= 75 + 1 + 48 = 124 (+1) of the row's 128.
+// MODE 3 (16-byte at 4-byte alignment, two 2D taps per load): one row,
+//   1.8 texels apart: floats floor(1.8 lane + f) + fb', fb' in {0, 8},
+//   <= 113 + 1 + 8 = 122 (+3) of 128.
+// MODE 4: MODE 3's addresses rounded down to 8-byte alignment.
 template <int MODE>
 __global__ __launch_bounds__(256) void gather_calib_kernel(const float2 *__restrict__ buf,
                                                            unsigned iters, float *__restrict__ out) {
@@ -70,12 +74,21 @@ __global__ __launch_bounds__(256) void gather_calib_kernel(const float2 *__restr
             a1 += q.y;
             a2 += q.z;
             a3 += q.w;
-        } else {
+        } else if (MODE == 2) {
             const unsigned row = k & 7u, fb = ((k >> 3) & 1u) * 48u;
             const unsigned fl = fb + (unsigned)(1.2f * (float)lane + f);
             const float2 q = ld2b(win, (row * 128u + fl) << 2);
             a0 += q.x;
             a1 += q.y;
+        } else {
+            const unsigned row = k & 7u, fb = ((k >> 3) & 1u) * 8u;
+            unsigned fl = fb + (unsigned)(1.8f * (float)lane + f);
+            if (MODE == 4) fl &= ~1u;
+            const float4 q = ld4u(win, (row * 128u + fl) << 2);
+            a0 += q.x;
+            a1 += q.y;
+            a2 += q.z;
+            a3 += q.w;
         }
     }
     out[blockIdx.x * 256u + threadIdx.x] = (a0 + a1) + (a2 + a3);
@@ -84,9 +97,10 @@ __global__ __launch_bounds__(256) void gather_calib_kernel(const float2 *__restr
 }  // namespace
 
 double gather_calibrate(int mode, unsigned iters, hipStream_t s) {
-    AMX_REQUIRE(mode >= 0 && mode <= 2,
+    AMX_REQUIRE(mode >= 0 && mode <= 4,
                 "calibrate_gather: mode must be 0 (16-byte ray-bundle taps), 1 (16-byte "
-                "coalesced) or 2 (8-byte 2D taps)");
+                "coalesced), 2 (8-byte 2D taps), 3 (16-byte 2D tap pairs, 4-byte aligned) or "
+                "4 (the same, 8-byte aligned)");
     AMX_REQUIRE(iters >= 64 && iters <= (1u << 20), "calibrate_gather: iterations in [64, 2^20]");
     int dev = 0, cus = 0;
     AMX_HIP(hipGetDevice(&dev));
@@ -105,8 +119,12 @@ double gather_calibrate(int mode, unsigned iters, hipStream_t s) {
             gather_calib_kernel<0><<<grid, 256, 0, s>>>(buf.as<float2>(), iters, out.as<float>());
         else if (mode == 1)
             gather_calib_kernel<1><<<grid, 256, 0, s>>>(buf.as<float2>(), iters, out.as<float>());
-        else
+        else if (mode == 2)
             gather_calib_kernel<2><<<grid, 256, 0, s>>>(buf.as<float2>(), iters, out.as<float>());
+        else if (mode == 3)
+            gather_calib_kernel<3><<<grid, 256, 0, s>>>(buf.as<float2>(), iters, out.as<float>());
+        else
+            gather_calib_kernel<4><<<grid, 256, 0, s>>>(buf.as<float2>(), iters, out.as<float>());
         AMX_LAUNCHED();
         AMX_HIP(hipEventRecord(e1, s));
         AMX_HIP(hipEventSynchronize(e1));

@@ -206,11 +206,15 @@ def batched_sirt_bench(n_slices, n_plain, iters, host_phantoms=False):
     log("bench: batched SIRT %d slices x %d iterations in %.3f s (first run incl. capture %.3f s)"
         % (n_slices, iters, batched_s, setup_s))
 
-    # plain path on the first n_plain slices of the same list
-    host_sinos = [B.sinogram(sinos, i).cpu().numpy() for i in range(n_plain)]
+    # plain path on n_plain slices spread evenly over the same list (every
+    # (n_slices / n_plain)-th: the same size and angle distribution)
+    stride = max(1, n_slices // max(n_plain, 1))
+    picks = list(range(0, n_slices, stride))[:n_plain]
+    n_plain = len(picks)
+    host_sinos = {i: B.sinogram(sinos, i).cpu().numpy() for i in picks}
     torch.cuda.synchronize(dev)
     t = time.perf_counter()
-    for i in range(n_plain):
+    for i in picks:
         pid = astra.create_projector("cuda", pgs[i], vgs[i])
         sid = astra.data2d.create("-sino", pgs[i], host_sinos[i])
         rid = astra.data2d.create("-vol", vgs[i], 0)
@@ -227,11 +231,12 @@ def batched_sirt_bench(n_slices, n_plain, iters, host_phantoms=False):
     b_rate = n_slices / batched_s
     p_rate = n_plain / plain_s
     b_upd = sum(work) / batched_s
-    p_upd = sum(work[:n_plain]) / plain_s
+    p_upd = sum(work[i] for i in picks) / plain_s
     del B, truth, sinos, x
     return {"slices_per_s": b_rate, "plain_slices_per_s": p_rate, "speedup": b_rate / p_rate,
             "work_normalised_speedup": b_upd / p_upd, "batched_Gupdates_per_s": b_upd / 1e9,
             "plain_Gupdates_per_s": p_upd / 1e9, "slices": n_slices, "plain_slices": n_plain,
+            "plain_sample": "every %d-th slice of the batch" % stride,
             "iterations": iters, "batched_seconds": batched_s, "plain_seconds": plain_s,
             "first_run_seconds": setup_s}
 
@@ -484,7 +489,9 @@ def gather_limits():
     from astra import _lib
 
     return {"taps": _lib.calibrate_gather(0), "coalesced": _lib.calibrate_gather(1),
-            "taps_2d_8byte": _lib.calibrate_gather(2)}
+            "taps_2d_8byte": _lib.calibrate_gather(2),
+            "tap_pairs_2d_16byte_align4": _lib.calibrate_gather(3),
+            "tap_pairs_2d_16byte_align8": _lib.calibrate_gather(4)}
 
 
 def main():
@@ -500,7 +507,7 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=50,
                     help="SIRT iterations per timed CPU run (median of 5 runs)")
     ap.add_argument("--batched-slices", type=int, default=4096)
-    ap.add_argument("--plain-slices", type=int, default=64)
+    ap.add_argument("--plain-slices", type=int, default=128)
     ap.add_argument("--batched-iters", type=int, default=50)
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-host-phantoms", action="store_true",
@@ -623,6 +630,8 @@ def main():
             "limit_lane_loads_per_s": limits["taps"],
             "coalesced_lane_loads_per_s": limits["coalesced"],
             "taps_2d_8byte_lane_loads_per_s": limits["taps_2d_8byte"],
+            "tap_pairs_2d_16byte_align4_lane_loads_per_s": limits["tap_pairs_2d_16byte_align4"],
+            "tap_pairs_2d_16byte_align8_lane_loads_per_s": limits["tap_pairs_2d_16byte_align8"],
             "ta_busy_frac": profile_value("ta_busy_frac", dom, n, na, nd),
             "frac": issued / limits["taps"] if issued else None,
             "note": "issued = SQ_INSTS_VMEM_RD x 64 per launch (profile) over the live launch "

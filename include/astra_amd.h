@@ -315,8 +315,10 @@ int astra_amd_scratch_usage(size_t *in_use, size_t *peak, int reset_peak);
  * stays in L1.  mode 0: 16-byte loads (global_load_dwordx4) with the 3D
  * taps' lane pattern (16 columns x 4 rows of 8-byte pairs, one texel of
  * per-lane jitter); mode 1: 16-byte loads, coalesced (1 KiB per wave);
- * mode 2: 8-byte loads at 4-byte alignment with the same lane pattern
- * (the 2D projectors' linear fetch); mode 3: 4-byte loads, same pattern.
+ * mode 2: 8-byte loads at 4-byte alignment, the 64 lanes 1.2 floats apart
+ * along one row (the 2D projectors' linear fetch); mode 3: 16-byte loads
+ * at 4-byte alignment, lanes 1.8 floats apart (two 2D taps per load);
+ * mode 4: mode 3 with addresses rounded down to 8-byte alignment.
  * Returns lane-loads per second (each lane of each load instruction
  * counts once), best of 3 timed launches after a warm-up.  Synchronous;
  * allocates and frees its own 1 MiB buffer. */
