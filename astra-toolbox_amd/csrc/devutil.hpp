@@ -93,6 +93,14 @@ __device__ __forceinline__ int wave_max(int v) {
 // Slice range [lo, hi] of s in [0, n0) for which -1 < a*s + c < n can hold
 // (widened by one slice on each side; exactness is not required because
 // samples outside the array read the zero border).
+// Tent weight (1 - |d|)+ of the linear / bilinear interpolation.  Written
+// as a clamp to [0, 1] (1 - |d| <= 1, so the upper bound changes nothing and
+// NaN still gives 0), which folds into the subtraction's clamp bit: one
+// instruction instead of a subtraction and a maximum.
+__device__ __forceinline__ float tent(float d) {
+    return fminf(fmaxf(1.f - fabsf(d), 0.f), 1.f);
+}
+
 __device__ __forceinline__ void clip_range(double a, double c, double n, int n0, int &lo, int &hi) {
     if (fabs(a) < 1e-12) {
         if (!(c > -1.0 && c < n)) { lo = 1; hi = 0; }

@@ -226,7 +226,7 @@ __device__ __forceinline__ float gather2d_class(const float4 *__restrict__ ra, f
     float acc = 0.f;
     for (int L = l0; L <= l1; ++L) {
         const float4 r = ra[L];
-        const float e = fmaxf(1.f - fabsf(fmaf(r.x, sf, r.y) - J), 0.f);
+        const float e = tent(fmaf(r.x, sf, r.y) - J);
         acc = fmaf(r.w == want ? r.z : 0.f, e, acc);
     }
     return acc;

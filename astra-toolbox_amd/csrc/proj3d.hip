@@ -165,31 +165,43 @@ struct FPArgs {
     const float *rb;  // OUT_RESID: b in the output layout
     const float *rw;  // OUT_RESID: weights in the output layout (may be NULL)
     int padded_out;   // write [a][v+2][u+2] instead of [v][a][u]
-    // workgroup map (fp_map): detector tiles, angles per block, and the
-    // range [lfirst, lfirst + lcount) of alist covered by this launch
+    // workgroup map (fp_map): detector tiles, angles per block, the range
+    // [lfirst, lfirst + lcount) of alist covered by this launch, and the
+    // part [xfirst[x], xfirst[x + 1]) of that range taken by XCD x
     unsigned tiles, ablk, lfirst, lcount;
+    unsigned xfirst[9];
     unsigned ss;  // rays per detector pixel and dimension (DetectorSuperSampling)
 };
 
 // Workgroup -> (detector tile, angle) map.  Workgroups are handed out
 // round-robin over the 8 XCDs (workgroup L runs on XCD L % 8), and each XCD
-// has its own L2.  The grid is a multiple of 8; XCD x takes the contiguous
-// logical range [x * per, (x + 1) * per), in the order (angle block, tile,
-// angle within block).  The workgroups resident on one XCD at a time are
-// then a few neighbouring tiles, each over `ablk` consecutive angles, which
-// start together and march through the same slices at the same time: rays
-// of neighbouring angles sample nearly the same voxels, so they share the
-// XCD's L2 instead of each fetching the volume from HBM.
+// has its own L2.  XCD x takes the angles [xfirst[x], xfirst[x + 1]) of the
+// launch's range (an eighth of them), cut into equal blocks of at most
+// `ablk` angles, in the order (angle block, tile, angle within block); no
+// block spans two XCDs.  The workgroups resident on one XCD at a time are
+// then a few neighbouring tiles, each over a block of consecutive angles,
+// which start together and march through the same slices at the same
+// time: rays of neighbouring angles sample nearly the same voxels, so they
+// share the XCD's L2 instead of each fetching the volume from HBM.
+// Workgroups beyond their XCD's range return at once.  (A single range cut
+// into blocks across the XCDs left the last XCD with the partial block and
+// the others with 48 of every 45 angles' work: 48.7 vs 42.7 ms per group
+// launch at 512^3 / 720 x 768^2.)
 __device__ __forceinline__ bool fp_map(const FPArgs &A, unsigned &tile, int &angle) {
-    const unsigned L = blockIdx.x;
-    const unsigned per = gridDim.x >> 3;
-    const unsigned logical = (L & 7u) * per + (L >> 3);
-    const unsigned a_in = logical % A.ablk;
-    const unsigned rest = logical / A.ablk;
-    tile = rest % A.tiles;
-    const unsigned li = (rest / A.tiles) * A.ablk + a_in;
-    if (li >= A.lcount) return false;
-    angle = A.alist[A.lfirst + li];
+    const unsigned x = blockIdx.x & 7u, q = blockIdx.x >> 3;
+    const unsigned first = A.xfirst[x], count = A.xfirst[x + 1] - first;
+    if (count == 0) return false;
+    const unsigned nblk = (count + A.ablk - 1) / A.ablk;
+    const unsigned bsz = (count + nblk - 1) / nblk;  // angles per block
+    const unsigned span = bsz * A.tiles;             // items per full block
+    const unsigned blk = q / span;
+    const unsigned b0 = blk * bsz;
+    if (b0 >= count) return false;
+    const unsigned bs = min(bsz, count - b0);
+    const unsigned r = q - blk * span;
+    if (r >= bs * A.tiles) return false;
+    tile = r / bs;
+    angle = A.alist[A.lfirst + first + b0 + r % bs];
     return true;
 }
 
@@ -197,22 +209,26 @@ __device__ __forceinline__ bool fp_map(const FPArgs &A, unsigned &tile, int &ang
 // of one angle, each wave a 16 u x 4 v sub-tile.  Ray setup runs in FP64;
 // the slice loop is wave-uniform over the union of the lanes' clipped slice
 // ranges, so the slice base address is scalar and every fetch is one
-// dwordx2 load with a 32-bit vector offset.
+// 16-byte load with a 32-bit vector offset.
 // (u, v): detector coordinates of the ray relative to pixel (0, 0)'s centre
 // (fractional for sub-rays).
 // GK: GEO_PAR3D, GEO_CONE (flat detector) or GEO_CYL (cylindrical detector:
 // pixel centre on the cylinder, cuda/3d/cone_cyl.cu:243-258).
 enum { GEO_PAR3D = 0, GEO_CONE = 1, GEO_CYL = 2 };
 
+// The ray of detector coordinates (u, v) in the group's slice frame:
+// texel coordinate - 0.5 along axes 1, 2 at slice s = a_k s + c_k, and the
+// slices [lo, hi] where it can touch the volume.
 template <int GK>
-__device__ __forceinline__ void fp3d_ray(const FPArgs &A, const FP3DAngle &G, double u, double v,
-                                         float &a1f, float &a2f, float &c1f, float &c2f,
-                                         float &scale, int &lo, int &hi) {
+__device__ __forceinline__ void fp3d_ray_line(const FP3DAngle &G, unsigned nu, unsigned nv,
+                                              unsigned n0, unsigned n1, unsigned n2, double u,
+                                              double v, double &a1, double &a2, double &c1,
+                                              double &c2, int &lo, int &hi) {
     constexpr bool CONE = GK != GEO_PAR3D;
     double P[3], d[3];
     if (GK == GEO_CYL) {
-        const double th = (u - 0.5 * A.nu + 0.5) * G.dth;
-        const double fv = v - 0.5 * A.nv + 0.5;
+        const double th = (u - 0.5 * nu + 0.5) * G.dth;
+        const double fv = v - 0.5 * nv + 0.5;
         const double ct = cos(th), st = sin(th);
         for (int k = 0; k < 3; ++k) P[k] = G.p0[k] + ct * G.u[k] + st * G.b[k] + fv * G.v[k];
     } else {
@@ -220,15 +236,24 @@ __device__ __forceinline__ void fp3d_ray(const FPArgs &A, const FP3DAngle &G, do
     }
     for (int k = 0; k < 3; ++k) d[k] = CONE ? (G.s[k] - P[k]) : G.s[k];
     const double *B = CONE ? G.s : P;
-    double a1 = d[1] / d[0], a2 = d[2] / d[0];
+    a1 = d[1] / d[0];
+    a2 = d[2] / d[0];
     // texel coordinate - 0.5 at slice s: a_k (s - n0/2 + 0.5 - B0) + B_k + n_k/2 - 0.5
-    double base = -0.5 * A.n0 + 0.5 - B[0];
-    double c1 = a1 * base + B[1] + 0.5 * A.n1 - 0.5;
-    double c2 = a2 * base + B[2] + 0.5 * A.n2 - 0.5;
+    const double base = -0.5 * n0 + 0.5 - B[0];
+    c1 = a1 * base + B[1] + 0.5 * n1 - 0.5;
+    c2 = a2 * base + B[2] + 0.5 * n2 - 0.5;
     lo = 0;
-    hi = (int)A.n0 - 1;
-    clip_range(a1, c1, (double)A.n1, (int)A.n0, lo, hi);
-    clip_range(a2, c2, (double)A.n2, (int)A.n0, lo, hi);
+    hi = (int)n0 - 1;
+    clip_range(a1, c1, (double)n1, (int)n0, lo, hi);
+    clip_range(a2, c2, (double)n2, (int)n0, lo, hi);
+}
+
+template <int GK>
+__device__ __forceinline__ void fp3d_ray(const FPArgs &A, const FP3DAngle &G, double u, double v,
+                                         float &a1f, float &a2f, float &c1f, float &c2f,
+                                         float &scale, int &lo, int &hi) {
+    double a1, a2, c1, c2;
+    fp3d_ray_line<GK>(G, A.nu, A.nv, A.n0, A.n1, A.n2, u, v, a1, a2, c1, c2, lo, hi);
     a1f = (float)a1;
     a2f = (float)a2;
     c1f = (float)(c1 + kPad);  // padded texel coordinates
@@ -1330,8 +1355,8 @@ __global__ __launch_bounds__(256) void bp3d_matched_gather(MatchedArgs M) {
 #pragma unroll
                 for (int k = 0; k < S; ++k) {
                     const float sk = sf + (float)k;
-                    const float e1 = fmaxf(1.f - fabsf(fmaf(r.x, sk, r.y) - J1), 0.f);
-                    const float e2 = fmaxf(1.f - fabsf(fmaf(r.z, sk, r.w) - J2), 0.f);
+                    const float e1 = tent(fmaf(r.x, sk, r.y) - J1);
+                    const float e2 = tent(fmaf(r.z, sk, r.w) - J2);
                     acc[k] = fmaf(y * e1, e2, acc[k]);
                 }
             }
@@ -1459,16 +1484,26 @@ unsigned bp_achunk(size_t pair_bytes, unsigned na) {
     return pair_bytes >= (8ull << 30) ? 40u : std::max(na, 1u);
 }
 
-// Angles per block of the FP workgroup map (fp_map); ASTRA_AMD_FP3D_ANGLE_BLOCK
-// overrides the default (1..64).  Measured at 512^3 / 720 x 768^2 (both
-// direction groups): 4: 123.9 ms, 8: 109.8, 16: 105.3, 32: 101.8, 64: 104.5.
+// Most angles per block of the FP workgroup map (fp_map): an XCD's range
+// is cut into equal blocks of at most this many angles.
+// ASTRA_AMD_FP3D_ANGLE_BLOCK overrides the default 64 (1..64; read at every
+// call).  Measured at 512^3 / 720 x 768^2 (45 angles per XCD and group),
+// ms per group launch: 16: 43.5, 23: 42.9, 32 (blocks of 32 and 13): 44.0,
+// 45 or 64 (one block): 42.8, 42.7.
 unsigned fp_angle_block() {
-    static const unsigned v = [] {
-        const char *e = std::getenv("ASTRA_AMD_FP3D_ANGLE_BLOCK");
-        long x = e ? std::strtol(e, nullptr, 10) : 0;
-        return (x >= 1 && x <= 64) ? (unsigned)x : 32u;
-    }();
-    return v;
+    const char *e = std::getenv("ASTRA_AMD_FP3D_ANGLE_BLOCK");
+    const long x = e ? std::strtol(e, nullptr, 10) : 0;
+    return (x >= 1 && x <= 64) ? (unsigned)x : 64u;
+}
+
+// Split the angles [first, first + count) of a group's list over the 8
+// XCDs in equal counts: xfirst[x] is the first angle of XCD x (relative to
+// `first`), xfirst[8] = count.  (Splitting by an estimated cost per angle
+// -- samples, or cache lines fetched -- measured slower at 512^3 / 720 x
+// 768^2: 44.5-45.9 against 43.8-44.3 ms per group launch; the per-angle time
+// varies less with the angle than either estimate.)
+void xcd_split(unsigned count, unsigned (&xfirst)[9]) {
+    for (unsigned x = 0; x <= 8; ++x) xfirst[x] = (unsigned)((size_t)count * x / 8);
 }
 
 // Launch `kern` over the angles of one direction group, in as many
@@ -1476,17 +1511,18 @@ unsigned fp_angle_block() {
 template <typename K>
 void fp_group_launches(FPArgs A, unsigned count, K kern, hipStream_t s) {
     A.tiles = ((A.nu + 15) / 16) * ((A.nv + 15) / 16);
-    A.ablk = std::min(fp_angle_block(), count);
-    const size_t per_block = (size_t)A.tiles * A.ablk;
+    A.ablk = fp_angle_block();
     const size_t max_groups = (1u << 24) - 8;  // 256 threads each
-    AMX_REQUIRE(per_block <= max_groups, "fp3d: detector too large for one launch");
-    const unsigned blocks_per_launch = (unsigned)(max_groups / per_block);
-    for (unsigned first = 0; first < count; first += blocks_per_launch * A.ablk) {
+    AMX_REQUIRE((size_t)A.tiles * 16 <= max_groups, "fp3d: detector too large for one launch");
+    // at most max_groups / tiles - 8 angles per launch: 8 XCD ranges of
+    // ceil(count / 8) angles fit
+    const unsigned per_launch = (unsigned)(max_groups / A.tiles - 8);
+    for (unsigned first = 0; first < count; first += per_launch) {
         A.lfirst = first;
-        A.lcount = std::min(count - first, blocks_per_launch * A.ablk);
-        const unsigned nblk = (A.lcount + A.ablk - 1) / A.ablk;
-        const unsigned groups = (unsigned)(((size_t)nblk * per_block + 7) & ~(size_t)7);
-        kern(A, groups);
+        A.lcount = std::min(count - first, per_launch);
+        xcd_split(A.lcount, A.xfirst);
+        const size_t groups = (size_t)8 * ((A.lcount + 7) / 8) * A.tiles;
+        kern(A, (unsigned)groups);
         AMX_LAUNCHED();
     }
 }

@@ -513,6 +513,8 @@ def main():
     ap.add_argument("--batched-host-phantoms", action="store_true",
                     help="build the batched workload's phantoms on the host (for profiled runs)")
     ap.add_argument("--no-diff", action="store_true")
+    ap.add_argument("--diff-iters", type=int, default=20,
+                    help="unrolled SIRT iterations of the differentiable leg (fewer for profiled runs)")
     ap.add_argument("--no-fan2d", action="store_true")
     ap.add_argument("--no-large", action="store_true")
     ap.add_argument("--no-sirt3d", action="store_true")
@@ -681,7 +683,7 @@ def main():
 
     diff_res = None
     if not args.no_diff:
-        diff_res = differentiable_bench()
+        diff_res = differentiable_bench(iters=args.diff_iters)
     fan_res = None if args.no_fan2d else fan2d_bench()
     sirt3d_res = None if args.no_sirt3d else sirt3d_bench(n, na, nd, args.sirt3d_iters)
     large_res = None if args.no_large else large_cone_bench()
