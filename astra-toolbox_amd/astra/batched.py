@@ -98,15 +98,22 @@ class BatchedSIRT2D:
         return sinos
 
     def run(self, sinos, vols, iterations, relaxation=1.0, min_constraint=None,
-            max_constraint=None, use_graph=True, sync=True):
-        """`iterations` SIRT iterations on every slice; vols is updated in place."""
+            max_constraint=None, use_graph=True, sync=True, persistent=False):
+        """`iterations` SIRT iterations on every slice; vols is updated in place.
+
+        use_graph: the per-iteration launches (slice copies, FP, BP) are
+        captured once into a hipGraph and replayed as one graph launch.
+        persistent: the whole run is ONE persistent kernel launch over device
+        work queues (it returns when the run has ended).  Every mode gives
+        the same floats."""
         self._check(sinos, int(self.sino_offsets[-1]), "sinograms")
         self._check(vols, int(self.vol_offsets[-1]), "volumes")
         lo = -np.inf if min_constraint is None else float(min_constraint)
         hi = np.inf if max_constraint is None else float(max_constraint)
         with torch.cuda.device(self.device):
             _lib.call("astra_amd_batch2d_sirt", self._h, _core.ptr(sinos), _core.ptr(vols),
-                      int(iterations), float(relaxation), lo, hi, 1 if use_graph else 0,
+                      int(iterations), float(relaxation), lo, hi,
+                      2 if persistent else (1 if use_graph else 0),
                       _core.stream_ptr(self.device))
             if sync:
                 _core.sync(self.device)

@@ -7,8 +7,9 @@
 // work items (per-slice prefix sums) maps every workgroup to a (slice, tile)
 // pair, so a 128^2 slice and a 1024^2 slice share the GPU in the same
 // launch.  N iterations are captured once into a hipGraph (3N kernel nodes)
-// and replayed as a single graph launch.  FP and BP tile shapes are chosen
-// per batch by timing the candidates on the batch itself.
+// and replayed as a single graph launch -- or run as one persistent launch
+// over device work queues (batch_sirt_persistent).  FP and BP tile shapes
+// are chosen per batch by timing the candidates on the batch itself.
 //
 // The per-ray / per-pixel arithmetic is proj2d_dev.hpp, shared with the
 // single-slice projectors, so a batch reproduces per-slice SIRT_CUDA runs.
@@ -261,6 +262,201 @@ __global__ __launch_bounds__(256) void batch_bp_lds(BatchArgs B, const float *__
     }
 }
 
+// ---------------------------------------------------------- persistent ---
+// The whole SIRT run as ONE launch: a grid of workgroups takes work items
+// from device queues until every iteration of every slice is done.  An
+// item is one unit of one stage of one slice in one iteration:
+//   stage 0 (copy): one 64-row band of the padded slice copies (every
+//                   64 x 64 tile of the band),
+//   stage 1 (FP):   one block of DB detectors of the residual FP (every
+//                   angle tile: 256 / DB angles at a time),
+//   stage 2 (BP):   one band of 4 PY rows of the BP with the SIRT update
+//                   (every 64-column tile).
+// Per slice and stage a counter records completed items.  An item waits
+// until the stage before it is complete for its slice: copy(it) after
+// BP(it - 1), FP(it) after copy(it), BP(it) after FP(it).  Per slice this
+// is the order of the per-stage launches, and every item runs the same
+// per-tile code, so the results are the same floats.
+//
+// Each queue holds its items in dependency order (one iteration: every
+// copy item, then every FP item, then every BP item of the queue's slices;
+// iteration after iteration).  An item waits only for items of its own
+// queue that were dequeued before it; the workgroups that took those are
+// running and finish them without waiting on later items, so the grid
+// cannot deadlock, whether or not all of it is resident.  The wait is
+// bounded all the same (spin_limit polls): past it the workgroup raises
+// status[0], and every workgroup stops at its next dequeue or poll, so the
+// kernel always drains.
+//
+// There are 8 queues, one per XCD (workgroup L runs on XCD L % 8 and
+// starts with queue L % 8), each with a share of the slices balanced by
+// work; a workgroup whose queue is exhausted takes items from the next
+// ones.  Placement only affects locality: every thread publishes its
+// stores with an agent-scope release fence before the item's completion
+// count is raised (release), and the waiting thread acquires at agent
+// scope, so results cross XCDs (and their separate L2s) correctly.
+struct PersistArgs {
+    const SliceDesc *desc;
+    const FP2DAngle *fpa;
+    const BP2DAngle *bpa;
+    const uint2 *items;  // one iteration of every queue: (slice, stage << 30 | unit)
+    unsigned qoff[9];    // queue x: items[qoff[x], qoff[x + 1])
+    unsigned iters, n, spin_limit;
+    unsigned *heads;   // [8] dequeue counters (item index over all iterations)
+    unsigned *done;    // [3][n] completed items per stage and slice
+    unsigned *status;  // [0] != 0: a wait exceeded spin_limit
+    const float *sinos, *lw, *pw;
+    float *vols, *resid, *rows, *cols;
+    float lo, hi;
+};
+
+// units of one stage of a slice with DB detectors per FP tile and PY rows
+// per BP thread (host and device)
+__host__ __device__ __forceinline__ unsigned persist_units(unsigned h, unsigned nd, unsigned stage,
+                                                           unsigned db, unsigned py) {
+    if (stage == 0) return (h + 2 * kPad + 63) / 64;
+    if (stage == 1) return (nd + db - 1) / db;
+    return (h + 4 * py - 1) / (4 * py);
+}
+
+union PersistLds {
+    float tile[64][65];
+    Bp2Lds bp;
+};
+
+// WPE: the fewest waves per SIMD the register allocation must allow (1: no
+// limit; 6 caps the kernel at 80 VGPRs, with a few spills outside the
+// sample loops).
+template <int DB, int PY, bool LDS, int WPE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
+batch_sirt_persistent(PersistArgs P) {
+    __shared__ PersistLds S;
+    __shared__ unsigned sh[2];
+    const unsigned tid = threadIdx.x;
+    const unsigned home = blockIdx.x & 7u;
+    unsigned q = 0;  // queues exhausted so far (thread 0)
+    for (;;) {
+        if (tid == 0) {
+            unsigned ok = 0, s = 0, tag = 0, it = 0;
+            while (q < 8) {
+                if (__hip_atomic_load(P.status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                    q = 8;
+                    break;
+                }
+                const unsigned x = (home + q) & 7u;
+                const unsigned len = P.qoff[x + 1] - P.qoff[x];
+                const unsigned total = len * P.iters;
+                const unsigned i =
+                    total ? __hip_atomic_fetch_add(P.heads + x, 1u, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT)
+                          : 0u;
+                if (i < total) {
+                    const uint2 e = P.items[P.qoff[x] + i % len];
+                    s = e.x;
+                    tag = e.y;
+                    it = i / len;
+                    ok = 1;
+                    break;
+                }
+                ++q;
+            }
+            if (ok) {
+                // wait until the slice's previous stage is complete
+                const SliceDesc &d = P.desc[s];
+                const unsigned stage = tag >> 30, prev = stage == 0 ? 2u : stage - 1;
+                const unsigned need =
+                    (stage == 0 ? it : it + 1) * persist_units(d.h, d.nd, prev, DB, PY);
+                const unsigned *cnt = P.done + (size_t)prev * P.n + s;
+                unsigned k = 0;
+                while (__hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
+                    if (++k > P.spin_limit ||
+                        __hip_atomic_load(P.status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                        __hip_atomic_fetch_or(P.status, 1u, __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_AGENT);
+                        ok = 0;
+                        q = 8;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(4);
+                }
+                // the producers' stores are visible from here on
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            }
+            sh[0] = ok ? s : 0xffffffffu;
+            sh[1] = tag;
+        }
+        __syncthreads();
+        // wave-uniform: the slice descriptor and its offsets go to scalar registers
+        const unsigned s = __builtin_amdgcn_readfirstlane(sh[0]);
+        const unsigned tag = __builtin_amdgcn_readfirstlane(sh[1]);
+        if (s == 0xffffffffu) break;
+        const unsigned stage = tag >> 30, unit = tag & 0x3fffffffu;
+        const SliceDesc d = P.desc[s];
+        if (stage == 0) {
+            const unsigned tx = (d.w + 2 * kPad + 63) / 64;
+#pragma unroll 1
+            for (unsigned c = 0; c < tx; ++c) {
+                if (c) __syncthreads();  // the previous tile's LDS is read
+                vol2d_copy_tile(P.vols + d.vol_off, P.rows + d.rows_off, P.cols + d.cols_off, d.w,
+                                d.h, c * 64, unit * 64, S.tile);
+            }
+        } else if (stage == 1) {
+            constexpr int AB = 256 / DB;
+            const int det = unit * DB + (tid % DB);
+            const float *r = P.rows + d.rows_off, *c = P.cols + d.cols_off;
+#pragma unroll 1
+            for (unsigned a0 = 0; a0 < d.na; a0 += AB) {
+                const int angle = a0 + (tid / DB);
+                if (det >= (int)d.nd || angle >= (int)d.na) continue;
+                const FP2DAngle &G = P.fpa[d.ang_off + angle];
+                const float v = (d.kind == ASTRA_AMD_GEOM_FAN)
+                                    ? fp2d_value<true>(G, det, 1, d.w, d.h, r, c)
+                                    : fp2d_value<false>(G, det, 1, d.w, d.h, r, c);
+                const size_t oidx = d.psino_off + (size_t)angle * (d.nd + 2 * kPad) + det + kPad;
+                const size_t idx = d.sino_off + (size_t)angle * d.nd + det;
+                P.resid[oidx] = (P.sinos[idx] - v * d.osc) * P.lw[oidx];
+            }
+        } else {
+            const unsigned xblocks = (d.w + 63) / 64;
+            const int Yt0 = unit * 4 * PY;
+            const float *sp = P.resid + d.psino_off;
+            const BP2DAngle *ang = P.bpa + d.ang_off;
+#pragma unroll 1
+            for (unsigned b = 0; b < xblocks; ++b) {
+                const int X0 = b * 64;
+                const int X = X0 + (tid & 63);
+                const int Y0 = Yt0 + (tid >> 6) * PY;
+                float acc[PY];
+                if (LDS) {
+                    if (d.kind == ASTRA_AMD_GEOM_FAN)
+                        bp2d_tile_lds<1, PY>(X0, Yt0, d.w, d.h, d.na, d.nd, sp, ang, acc, S.bp);
+                    else
+                        bp2d_tile_lds<0, PY>(X0, Yt0, d.w, d.h, d.na, d.nd, sp, ang, acc, S.bp);
+                }
+                if (X >= (int)d.w || Y0 >= (int)d.h) continue;
+                if (!LDS) {
+                    if (d.kind == ASTRA_AMD_GEOM_FAN)
+                        bp2d_column<1, PY>(X, Y0, d.w, d.h, d.na, d.nd, sp, ang, acc);
+                    else
+                        bp2d_column<0, PY>(X, Y0, d.w, d.h, d.na, d.nd, sp, ang, acc);
+                }
+                for (int i = 0; i < PY; ++i) {
+                    const int Y = Y0 + i;
+                    if (Y >= (int)d.h) break;
+                    const size_t idx = d.vol_off + (size_t)Y * d.w + X;
+                    store_bp(P.vols, idx, acc[i] * d.osc, OUT_UPDATE, P.pw, P.lo, P.hi);
+                }
+            }
+        }
+        // publish this thread's stores device-wide, then count the item
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __syncthreads();  // also: every thread has read sh
+        if (tid == 0)
+            __hip_atomic_fetch_add(P.done + (size_t)stage * P.n + s, 1u, __ATOMIC_RELEASE,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
 // ------------------------------------------------------------ handle -----
 struct Batch2D {
     unsigned n = 0;
@@ -286,9 +482,18 @@ struct Batch2D {
     // after / before the caller's stream
     hipStream_t cap = nullptr;
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
+    int g_mode = -1, g_wpe = 0;
+    // persistent run: the queues (built for the tilings p_fp / p_bp), the
+    // sync words (heads[8], done[3][n], status) and a pinned status copy
+    DevArray p_items, p_sync;
+    unsigned p_qoff[9] = {};
+    int p_fp = -1, p_bp = -1, p_wpe = 1;
+    unsigned p_grid = 0;
+    unsigned *p_status_host = nullptr;
     std::mutex mu;
 
     ~Batch2D() {
+        if (p_status_host) (void)hipHostFree(p_status_host);
         if (exec) (void)hipGraphExecDestroy(exec);
         if (ev_in) (void)hipEventDestroy(ev_in);
         if (ev_out) (void)hipEventDestroy(ev_out);
@@ -560,13 +765,160 @@ void iterate(const Batch2D &b, const float *sinos, float *vols, int iters, float
     }
 }
 
+// Queues of the persistent run for the chosen tilings: slices dealt to the
+// 8 queues by decreasing work, each to the least loaded queue (work = FP
+// samples + BP updates per iteration); per queue one iteration's items,
+// stage after stage, slices in that order.
+void build_persistent(Batch2D &b) {
+    AMX_REQUIRE(b.fp_cfg >= 0 && b.fp_cfg < 3 && b.bp_cfg >= 0 && b.bp_cfg < 6,
+                "batch2d_sirt: the persistent run supports FP tilings 0-2 and BP tilings 0-5 "
+                "(forced: FP %d, BP %d)", b.fp_cfg, b.bp_cfg);
+    if (b.p_items.bytes() && b.p_fp == b.fp_cfg && b.p_bp == b.bp_cfg) return;
+    const unsigned db = kFPDB[b.fp_cfg], py = kBPPY[b.bp_cfg % 3];
+    std::vector<unsigned> order(b.n);
+    std::vector<double> work(b.n);
+    for (unsigned i = 0; i < b.n; ++i) {
+        const SliceDesc &d = b.host[i];
+        order[i] = i;
+        work[i] = (double)d.na * ((double)d.nd * std::max(d.w, d.h) + (double)d.w * d.h);
+    }
+    std::stable_sort(order.begin(), order.end(),
+                     [&](unsigned x, unsigned y) { return work[x] > work[y]; });
+    std::vector<std::vector<unsigned>> qs(8);
+    double load[8] = {};
+    for (unsigned i : order) {
+        const int x = (int)(std::min_element(load, load + 8) - load);
+        qs[x].push_back(i);
+        load[x] += work[i];
+    }
+    std::vector<uint2> items;
+    for (int x = 0; x < 8; ++x) {
+        b.p_qoff[x] = (unsigned)items.size();
+        for (unsigned stage = 0; stage < 3; ++stage)
+            for (unsigned i : qs[x]) {
+                const SliceDesc &d = b.host[i];
+                const unsigned nu = persist_units(d.h, d.nd, stage, db, py);
+                for (unsigned u = 0; u < nu; ++u) items.push_back(make_uint2(i, (stage << 30) | u));
+            }
+        AMX_REQUIRE(items.size() < (1ull << 31), "batch2d: too many persistent work items");
+    }
+    b.p_qoff[8] = (unsigned)items.size();
+    b.p_items = DevArray(sizeof(uint2) * items.size());
+    b.p_items.upload(items.data(), sizeof(uint2) * items.size());
+    if (!b.p_sync.bytes()) b.p_sync = DevArray(sizeof(unsigned) * (8 + 3 * (size_t)b.n + 1));
+    if (!b.p_status_host)
+        AMX_HIP(hipHostMalloc((void **)&b.p_status_host, sizeof(unsigned), hipHostMallocDefault));
+    b.p_fp = b.fp_cfg;
+    b.p_bp = b.bp_cfg;
+}
+
+// Most polls of a dependency wait (about 2 us each) before a persistent run
+// gives up; ASTRA_AMD_BATCH_SPIN_LIMIT overrides it (read at every run).
+unsigned persist_spin_limit() {
+    const char *e = std::getenv("ASTRA_AMD_BATCH_SPIN_LIMIT");
+    if (e && *e) return (unsigned)std::min<unsigned long>(std::strtoul(e, nullptr, 10), 0xffffffffu);
+    return 1u << 23;
+}
+
+// Waves per SIMD asked of the persistent kernel's register allocation
+// (ASTRA_AMD_BATCH_PERSIST_WPE: 6, or 1 for no limit; read at every run).
+int persist_wpe() {
+    const char *e = std::getenv("ASTRA_AMD_BATCH_PERSIST_WPE");
+    return (e && std::atoi(e) == 6) ? 6 : 1;
+}
+
+// P == NULL: size the grid of the kernel for (tilings, wpe) -- as many
+// workgroups as are resident at once, a multiple of 8 -- into *grid
+// (before a capture: the occupancy query is not a stream operation, but
+// it is kept out of the captured region all the same).  Else launch it
+// with *grid workgroups.
+template <int DB, int PY, bool LDS>
+void persist_t(const PersistArgs *P, int wpe, unsigned *grid, hipStream_t s) {
+    auto kern = wpe == 6 ? batch_sirt_persistent<DB, PY, LDS, 6>
+                         : batch_sirt_persistent<DB, PY, LDS, 1>;
+    if (!P) {
+        int per_cu = 0, cus = 0, dev = 0;
+        AMX_HIP(hipGetDevice(&dev));
+        AMX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, 0));
+        AMX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        *grid = std::max(8u, (unsigned)(std::max(per_cu, 1) * std::max(cus, 1)) & ~7u);
+        return;
+    }
+    kern<<<*grid, 256, 0, s>>>(*P);
+    AMX_LAUNCHED();
+}
+
+void persist_dispatch(const Batch2D &b, const PersistArgs *P, int wpe, unsigned *grid,
+                      hipStream_t s) {
+    switch (b.p_fp * 6 + b.p_bp) {
+#define AMX_PERSIST_CASE(F, B, DB, PY, LDS) \
+    case F * 6 + B: persist_t<DB, PY, LDS>(P, wpe, grid, s); break;
+        AMX_PERSIST_CASE(0, 0, 64, 4, false)
+        AMX_PERSIST_CASE(0, 1, 64, 2, false)
+        AMX_PERSIST_CASE(0, 2, 64, 8, false)
+        AMX_PERSIST_CASE(0, 3, 64, 4, true)
+        AMX_PERSIST_CASE(0, 4, 64, 2, true)
+        AMX_PERSIST_CASE(0, 5, 64, 8, true)
+        AMX_PERSIST_CASE(1, 0, 128, 4, false)
+        AMX_PERSIST_CASE(1, 1, 128, 2, false)
+        AMX_PERSIST_CASE(1, 2, 128, 8, false)
+        AMX_PERSIST_CASE(1, 3, 128, 4, true)
+        AMX_PERSIST_CASE(1, 4, 128, 2, true)
+        AMX_PERSIST_CASE(1, 5, 128, 8, true)
+        AMX_PERSIST_CASE(2, 0, 32, 4, false)
+        AMX_PERSIST_CASE(2, 1, 32, 2, false)
+        AMX_PERSIST_CASE(2, 2, 32, 8, false)
+        AMX_PERSIST_CASE(2, 3, 32, 4, true)
+        AMX_PERSIST_CASE(2, 4, 32, 2, true)
+        AMX_PERSIST_CASE(2, 5, 32, 8, true)
+#undef AMX_PERSIST_CASE
+        default: fail(ASTRA_AMD_ERR_INTERNAL, "batch2d_sirt: no persistent kernel for the tilings");
+    }
+}
+
+// One persistent run: clear the sync words (a fill kernel, so the run can
+// be captured), then the persistent kernel (b.p_wpe, b.p_grid).
+void iterate_persistent(const Batch2D &b, const float *sinos, float *vols, int iters, float lo,
+                        float hi, hipStream_t s) {
+    unsigned *sync = b.p_sync.as<unsigned>();
+    fill_launch(reinterpret_cast<float *>(sync), 8 + 3 * (size_t)b.n + 1, 0.f, s);
+    PersistArgs P;
+    P.desc = b.desc.as<SliceDesc>();
+    P.fpa = b.fpa.as<FP2DAngle>();
+    P.bpa = b.bpa.as<BP2DAngle>();
+    P.items = b.p_items.as<uint2>();
+    for (int x = 0; x <= 8; ++x) P.qoff[x] = b.p_qoff[x];
+    for (int x = 0; x < 8; ++x)
+        AMX_REQUIRE((size_t)(b.p_qoff[x + 1] - b.p_qoff[x]) * iters < (1ull << 31),
+                    "batch2d_sirt: too many work items for one persistent run (%d iterations)",
+                    iters);
+    P.iters = (unsigned)iters;
+    P.n = b.n;
+    P.spin_limit = persist_spin_limit();
+    P.heads = sync;
+    P.done = sync + 8;
+    P.status = sync + 8 + 3 * (size_t)b.n;
+    P.sinos = sinos;
+    P.lw = b.line_w.as<float>();
+    P.pw = b.pixel_w.as<float>();
+    P.vols = vols;
+    P.resid = b.resid.as<float>();
+    P.rows = b.rows.as<float>();
+    P.cols = b.cols.as<float>();
+    P.lo = lo;
+    P.hi = hi;
+    unsigned grid = b.p_grid;
+    persist_dispatch(b, &P, b.p_wpe, &grid, s);
+}
+
 }  // namespace
 
 void batch2d_sirt(Batch2D &b, const float *sinos, float *vols, int iters, double relaxation,
-                  float lo, float hi, bool use_graph, hipStream_t s) {
+                  float lo, float hi, int run_mode, hipStream_t s) {
     std::lock_guard<std::mutex> lk(b.mu);
     AMX_REQUIRE(sinos && vols, "batch2d_sirt: NULL data pointer");
     AMX_REQUIRE(iters >= 0, "batch2d_sirt: negative iteration count");
+    AMX_REQUIRE(run_mode >= 0 && run_mode <= 2, "batch2d_sirt: run mode must be 0, 1 or 2");
     ensure_buffers(b);
     if (!b.weights_ready || b.weights_relax != relaxation) {
         precompute(b, relaxation, s);
@@ -575,7 +927,14 @@ void batch2d_sirt(Batch2D &b, const float *sinos, float *vols, int iters, double
     }
     if (b.fp_cfg < 0) autotune(b, s);
     if (iters == 0) return;
-    if (!use_graph) {
+    const bool persistent = run_mode == 2;
+    const int wpe = persistent ? persist_wpe() : 1;
+    if (persistent) {
+        build_persistent(b);
+        persist_dispatch(b, nullptr, wpe, &b.p_grid, s);
+        b.p_wpe = wpe;
+    }
+    if (run_mode == 0) {
         iterate(b, sinos, vols, iters, lo, hi, s);
         return;
     }
@@ -588,7 +947,7 @@ void batch2d_sirt(Batch2D &b, const float *sinos, float *vols, int iters, double
     AMX_HIP(hipEventRecord(b.ev_in, s));
     AMX_HIP(hipStreamWaitEvent(b.cap, b.ev_in, 0));
     const bool same = b.exec && b.g_iters == iters && b.g_sinos == sinos && b.g_vols == vols &&
-                      b.g_lo == lo && b.g_hi == hi;
+                      b.g_lo == lo && b.g_hi == hi && b.g_mode == run_mode && b.g_wpe == wpe;
     if (!same) {
         if (b.exec) {
             (void)hipGraphExecDestroy(b.exec);
@@ -597,7 +956,10 @@ void batch2d_sirt(Batch2D &b, const float *sinos, float *vols, int iters, double
         hipGraph_t graph = nullptr;
         AMX_HIP(hipStreamBeginCapture(b.cap, hipStreamCaptureModeThreadLocal));
         try {
-            iterate(b, sinos, vols, iters, lo, hi, b.cap);
+            if (persistent)
+                iterate_persistent(b, sinos, vols, iters, lo, hi, b.cap);
+            else
+                iterate(b, sinos, vols, iters, lo, hi, b.cap);
         } catch (...) {
             (void)hipStreamEndCapture(b.cap, &graph);
             if (graph) (void)hipGraphDestroy(graph);
@@ -616,10 +978,24 @@ void batch2d_sirt(Batch2D &b, const float *sinos, float *vols, int iters, double
         b.g_vols = vols;
         b.g_lo = lo;
         b.g_hi = hi;
+        b.g_mode = run_mode;
+        b.g_wpe = wpe;
     }
-    ProfToken pt = prof_begin("batch2d_sirt", (double)iters, b.cap);
+    ProfToken pt = prof_begin(persistent ? "batch2d_sirt_persistent" : "batch2d_sirt",
+                              (double)iters, b.cap);
     AMX_HIP(hipGraphLaunch(b.exec, b.cap));
     prof_end(pt, b.cap);
+    if (persistent) {
+        // the run ends before the call returns; a wait past its limit
+        // stopped it (the sync words are cleared by the next run)
+        AMX_HIP(hipMemcpyAsync(b.p_status_host, b.p_sync.as<unsigned>() + 8 + 3 * (size_t)b.n,
+                               sizeof(unsigned), hipMemcpyDeviceToHost, b.cap));
+        AMX_HIP(hipStreamSynchronize(b.cap));
+        AMX_REQUIRE(*b.p_status_host == 0,
+                    "batch2d_sirt: persistent run stopped: a work item waited past the spin "
+                    "limit (%u polls); the volumes hold a partial result",
+                    persist_spin_limit());
+    }
     // the caller's stream continues after the graph
     AMX_HIP(hipEventRecord(b.ev_out, b.cap));
     AMX_HIP(hipStreamWaitEvent(s, b.ev_out, 0));

@@ -36,7 +36,7 @@ Batch2D *batch2d_build(unsigned n, const int *kinds, const unsigned *ws, const u
 void batch2d_destroy(Batch2D *b);
 void batch2d_offsets(const Batch2D &b, size_t *vol_off, size_t *sino_off);
 void batch2d_sirt(Batch2D &b, const float *sinos, float *vols, int iters, double relaxation,
-                  float lo, float hi, bool use_graph, hipStream_t s);
+                  float lo, float hi, int run_mode, hipStream_t s);
 void batch2d_fp(Batch2D &b, const float *vols, float *sinos, hipStream_t s);
 }  // namespace amx
 
@@ -450,7 +450,7 @@ int astra_amd_batch2d_sirt(astra_amd_batch2d b, const float *sinos, float *vols,
         AMX_REQUIRE(b && b->b, "batch handle is NULL");
         check_device(b->device);
         amx::batch2d_sirt(*b->b, sinos, vols, iterations, relaxation, min_constraint,
-                          max_constraint, use_graph != 0, as_stream(stream));
+                          max_constraint, use_graph, as_stream(stream));
     });
 }
 

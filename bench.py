@@ -167,7 +167,7 @@ def copy_calibration(dev, nbytes=1 << 30):
     torch.cuda.empty_cache()
 
 
-def batched_sirt_bench(n_slices, n_plain, iters, host_phantoms=False):
+def batched_sirt_bench(n_slices, n_plain, iters, host_phantoms=False, persist_iters=10):
     """Batched 2D SIRT slices/s (one hipGraph launch for all iterations of all
     slices) vs the plain per-slice SIRT_CUDA drop-in (one algorithm object
     per slice, run(iters), host data) on slices of the same seeded
@@ -205,6 +205,30 @@ def batched_sirt_bench(n_slices, n_plain, iters, host_phantoms=False):
     batched_s = time.perf_counter() - t
     log("bench: batched SIRT %d slices x %d iterations in %.3f s (first run incl. capture %.3f s)"
         % (n_slices, iters, batched_s, setup_s))
+    # the same batch as ONE persistent launch per run (device work queues):
+    # one warm-up iteration (queues + capture), then a timed run of
+    # persist_iters iterations; the result must equal the graph's
+    persist = None
+    if persist_iters > 0:
+        y = B.empty_volumes()
+        B.run(sinos, y, 1, persistent=True)
+        y.zero_()
+        torch.cuda.synchronize(dev)
+        t = time.perf_counter()
+        B.run(sinos, y, persist_iters, persistent=True)
+        persist_s = time.perf_counter() - t
+        same = None
+        if persist_iters == iters:
+            same = bool(torch.equal(x, y))
+        per_it = persist_s / persist_iters
+        persist = {"seconds_per_iteration": per_it,
+                   "slices_per_s": n_slices / (per_it * iters),
+                   "graph_seconds_per_iteration": batched_s / iters,
+                   "timed_iterations": persist_iters,
+                   "equal_to_graph": same}
+        log("bench: persistent batched SIRT %.1f ms per iteration (graph %.1f)"
+            % (1e3 * per_it, 1e3 * batched_s / iters))
+        del y
 
     # plain path on n_plain slices spread evenly over the same list (every
     # (n_slices / n_plain)-th: the same size and angle distribution)
@@ -238,7 +262,7 @@ def batched_sirt_bench(n_slices, n_plain, iters, host_phantoms=False):
             "plain_Gupdates_per_s": p_upd / 1e9, "slices": n_slices, "plain_slices": n_plain,
             "plain_sample": "every %d-th slice of the batch" % stride,
             "iterations": iters, "batched_seconds": batched_s, "plain_seconds": plain_s,
-            "first_run_seconds": setup_s}
+            "first_run_seconds": setup_s, "persistent": persist}
 
 
 def differentiable_bench(n=256, na=360, nd=384, iters=20):
@@ -510,6 +534,8 @@ def main():
     ap.add_argument("--plain-slices", type=int, default=128)
     ap.add_argument("--batched-iters", type=int, default=50)
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--persist-iters", type=int, default=10,
+                    help="timed iterations of the persistent batched run (0: skip)")
     ap.add_argument("--batched-host-phantoms", action="store_true",
                     help="build the batched workload's phantoms on the host (for profiled runs)")
     ap.add_argument("--no-diff", action="store_true")
@@ -679,7 +705,7 @@ def main():
     batched_res = None
     if not args.no_batched:
         batched_res = batched_sirt_bench(args.batched_slices, args.plain_slices, args.batched_iters,
-                                         args.batched_host_phantoms)
+                                         args.batched_host_phantoms, args.persist_iters)
 
     diff_res = None
     if not args.no_diff:

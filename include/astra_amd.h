@@ -282,8 +282,14 @@ int astra_amd_batch2d_offsets(astra_amd_batch2d b, size_t *vol_offsets, size_t *
 int astra_amd_batch2d_fp(astra_amd_batch2d b, const float *vols, float *sinos, void *stream);
 /* `iterations` SIRT iterations on every slice (vols = initial iterates,
  * overwritten), with relaxation and min/max constraints (+-INFINITY = off).
- * use_graph != 0: the iterations are captured once into a hipGraph and
- * replayed as one graph launch (re-captured when arguments change). */
+ * use_graph: 0 = three launches per iteration (slice copies, FP, BP) on
+ * `stream`; 1 = the same launches captured once into a hipGraph and
+ * replayed as one graph launch (re-captured when arguments change); 2 = the
+ * whole run as ONE persistent kernel launch over device work queues
+ * (captured in a graph with the reset of its queue counters); it returns
+ * when the run has ended, with ASTRA_AMD_ERR_ARG if a work item waited past
+ * the spin limit (ASTRA_AMD_BATCH_SPIN_LIMIT polls, default 2^23).  All
+ * modes give the same floats. */
 int astra_amd_batch2d_sirt(astra_amd_batch2d b, const float *sinos, float *vols, int iterations,
                            float relaxation, float min_constraint, float max_constraint,
                            int use_graph, void *stream);
