@@ -34,6 +34,36 @@ __device__ __forceinline__ float4 ld4b(const float *base, unsigned byte_off) {
     return make_float4(v.x, v.y, v.z, v.w);
 }
 
+// Wave-uniform copy of a pointer (from the first lane): keeps pointer
+// arithmetic on it in scalar registers.
+template <class T>
+__device__ __forceinline__ T *first_lane_ptr(T *p) {
+    const unsigned long long u = (unsigned long long)p;
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)u);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(u >> 32));
+    return (T *)(((unsigned long long)hi << 32) | lo);
+}
+
+// Structured buffer of 8-byte records (row-difference pairs) from a
+// wave-uniform base: a 16-byte load of records index and index + 1 at
+// byte offset soff (uniform) takes one vector register of address (the
+// record index, one v_mad_u32_u24 from the texel coordinates).
+typedef unsigned amx_u4 __attribute__((ext_vector_type(4)));
+__device__ amx_u4 amx_struct_load_v4(__amdgpu_buffer_rsrc_t r, int vindex, int voffset,
+                                     int soffset, int aux)
+    __asm("llvm.amdgcn.struct.ptr.buffer.load.v4i32");
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t pair_rsrc(const void *base) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(first_lane_ptr(base)), (short)8,
+                                             -1, 0x00020000);
+}
+
+__device__ __forceinline__ float4 ld4p(__amdgpu_buffer_rsrc_t r, unsigned index, unsigned soff) {
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    const f4 v = __builtin_bit_cast(f4, amx_struct_load_v4(r, (int)index, 0, (int)soff, 0));
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+
 // Four floats at base + byte offset, 4-byte aligned (one
 // global_load_dwordx4): two linear taps of one row when their texels lie
 // within four.
@@ -65,6 +95,11 @@ __device__ __forceinline__ float ld1b(const float *base, unsigned byte_off) {
 // converts to 0, which is in bounds.
 __device__ __forceinline__ unsigned pair_off(float p1, float p2, unsigned pitch) {
     return (__umul24((unsigned)p2, pitch) + (unsigned)p1) << 3;
+}
+
+// the same as a record index (8-byte records)
+__device__ __forceinline__ unsigned pair_idx(float p1, float p2, unsigned pitch) {
+    return __umul24((unsigned)p2, pitch) + (unsigned)p1;
 }
 
 // fractional part of a clamped (>= 0.5) padded coordinate: p - floor(p)

@@ -265,16 +265,22 @@ __device__ __forceinline__ void fp3d_ray(const FPArgs &A, const FP3DAngle &G, do
 // ray (c1, c2) = a s + c through the padded group copy.  Samples at even
 // offsets from slo go to acc0, odd ones to acc1; four loads are issued
 // before the first of them is used (the loop is bound by the loads'
-// latency).
+// latency).  The loads are structured buffer loads of the pair records:
+// the slice base and the slice offsets k * plane are scalar, the record
+// index (one v_mad_u32_u24) the only per-lane address term.
 __device__ __forceinline__ float march3d(const FPArgs &A, float a1f, float a2f, float c1f,
                                          float c2f, int slo, int shi) {
     const unsigned pitch = A.n1 + 2 * kPad;                        // pairs per row
     const size_t plane = (size_t)(A.n2 + 2 * kPad) * pitch * 2;     // floats per slice
+    // bytes per slice (< 1 GiB: host check), wave-uniform
+    const unsigned plane_b = __builtin_amdgcn_readfirstlane((unsigned)(plane * sizeof(float)));
     const float hi1 = (float)A.n1 + 2.5f, hi2 = (float)A.n2 + 2.5f;
     float acc0 = 0.f, acc1 = 0.f;
     int s = slo;
-    const float *pl = A.volp + (size_t)(slo > 0 ? slo : 0) * plane;
+    const char *pl =
+        first_lane_ptr(reinterpret_cast<const char *>(A.volp + (size_t)(slo > 0 ? slo : 0) * plane));
     for (; s + 3 <= shi; s += 4) {
+        const __amdgpu_buffer_rsrc_t rs = pair_rsrc(pl);
         float4 q[4];
         float w1[4], w2[4];
 #pragma unroll
@@ -284,32 +290,35 @@ __device__ __forceinline__ float march3d(const FPArgs &A, float a1f, float a2f, 
             const float p2 = clamp_pad(fmaf(a2f, sf, c2f), hi2);
             w1[k] = frac(p1);
             w2[k] = frac(p2);
-            q[k] = ld4b(pl + k * plane, pair_off(p1, p2, pitch));
+            q[k] = ld4p(rs, pair_idx(p1, p2, pitch), k * plane_b);
         }
         acc0 += bilerp_pairs(q[0], w1[0], w2[0]);
         acc1 += bilerp_pairs(q[1], w1[1], w2[1]);
         acc0 += bilerp_pairs(q[2], w1[2], w2[2]);
         acc1 += bilerp_pairs(q[3], w1[3], w2[3]);
-        pl += 4 * plane;
+        pl += 4 * (size_t)plane_b;
     }
+    // the last one to three slices
+    const __amdgpu_buffer_rsrc_t rs = pair_rsrc(pl);
+    unsigned so = 0;
     if (s + 1 <= shi) {
         const float sf = (float)s;
         float p1 = clamp_pad(fmaf(a1f, sf, c1f), hi1);
         float p2 = clamp_pad(fmaf(a2f, sf, c2f), hi2);
         float q1 = clamp_pad(fmaf(a1f, sf + 1.f, c1f), hi1);
         float q2 = clamp_pad(fmaf(a2f, sf + 1.f, c2f), hi2);
-        float4 r = ld4b(pl, pair_off(p1, p2, pitch));
-        float4 t = ld4b(pl + plane, pair_off(q1, q2, pitch));
+        float4 r = ld4p(rs, pair_idx(p1, p2, pitch), 0u);
+        float4 t = ld4p(rs, pair_idx(q1, q2, pitch), plane_b);
         acc0 += bilerp_pairs(r, frac(p1), frac(p2));
         acc1 += bilerp_pairs(t, frac(q1), frac(q2));
-        pl += 2 * plane;
+        so = 2 * plane_b;
         s += 2;
     }
     if (s <= shi) {
         const float sf = (float)s;
         float p1 = clamp_pad(fmaf(a1f, sf, c1f), hi1);
         float p2 = clamp_pad(fmaf(a2f, sf, c2f), hi2);
-        acc0 += bilerp_pairs(ld4b(pl, pair_off(p1, p2, pitch)), frac(p1), frac(p2));
+        acc0 += bilerp_pairs(ld4p(rs, pair_idx(p1, p2, pitch), so), frac(p1), frac(p2));
     }
     return acc0 + acc1;
 }
@@ -1587,6 +1596,10 @@ void fp3d_launch(const Geom3D &g, const float *vol, float *proj, float out_scale
         const int k = groups[i];
         make_group_copy(g, k, vol, copy.as<float2>(), s);
         FPArgs A = fp_args(g, k, out_scale);
+        // march3d's buffer loads address a slice at 32-bit scalar offsets
+        // of up to 3 slices
+        AMX_REQUIRE((size_t)(A.n1 + 2 * kPad) * (A.n2 + 2 * kPad) * sizeof(float2) <= (1ull << 30),
+                    "fp3d: volume slice too large (%u x %u)", A.n1, A.n2);
         A.volp = copy.as<float>();
         A.out = proj;
         A.rb = resid_b;
