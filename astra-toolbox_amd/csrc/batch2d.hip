@@ -99,8 +99,10 @@ __global__ __launch_bounds__(256) void batch_unpad(BatchArgs B, const float *__r
 // ------------------------------------------------------------------ FP ---
 // DB detectors x (256 / DB) angles per workgroup.  mode OUT_SET writes
 // osc * Ax; OUT_RESID writes (b - osc * Ax) * w.  Output: padded sinogram.
+// (8 waves per SIMD: the register allocation fits 64 VGPRs without spills)
 template <int DB>
-__global__ __launch_bounds__(256) void batch_fp(BatchArgs B, const float *__restrict__ rows,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void batch_fp(
+    BatchArgs B, const float *__restrict__ rows,
                                                 const float *__restrict__ cols,
                                                 const float *__restrict__ sinos,
                                                 const float *__restrict__ lw,

@@ -58,6 +58,33 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t pair_rsrc(const void *base) {
                                              -1, 0x00020000);
 }
 
+// Structured buffer of 4-byte records (a padded 2D slice copy) from a
+// wave-uniform base: 8- and 16-byte loads of consecutive texels from the
+// record index, the only per-lane address term.
+typedef unsigned amx_u2 __attribute__((ext_vector_type(2)));
+__device__ amx_u2 amx_struct_load_v2(__amdgpu_buffer_rsrc_t r, int vindex, int voffset,
+                                     int soffset, int aux)
+    __asm("llvm.amdgcn.struct.ptr.buffer.load.v2i32");
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t texel_rsrc(const float *base) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(first_lane_ptr(base)), (short)4,
+                                             -1, 0x00020000);
+}
+
+// texels index and index + 1
+__device__ __forceinline__ float2 ld2i(__amdgpu_buffer_rsrc_t r, unsigned index) {
+    typedef float f2 __attribute__((ext_vector_type(2)));
+    const f2 v = __builtin_bit_cast(f2, amx_struct_load_v2(r, (int)index, 0, 0, 0));
+    return make_float2(v.x, v.y);
+}
+
+// texels index .. index + 3
+__device__ __forceinline__ float4 ld4i(__amdgpu_buffer_rsrc_t r, unsigned index) {
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    const f4 v = __builtin_bit_cast(f4, amx_struct_load_v4(r, (int)index, 0, 0, 0));
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+
 __device__ __forceinline__ float4 ld4p(__amdgpu_buffer_rsrc_t r, unsigned index, unsigned soff) {
     typedef float f4 __attribute__((ext_vector_type(4)));
     const f4 v = __builtin_bit_cast(f4, amx_struct_load_v4(r, (int)index, 0, (int)soff, 0));

@@ -72,14 +72,16 @@ __device__ __forceinline__ Ray2D ray2d(const FP2DAngle &G, double fdet, bool ver
 }
 
 // Sum of linear samples along a ray over slices [lo, hi] of a padded
-// slice-major copy (slice pitch `pitch`, coordinate clamp `hi_t`).
-// Samples at even offsets from lo go to acc0, odd ones to acc1; four loads
-// are issued before the first of them is used (the loop is bound by the
-// loads' latency).
-__device__ __forceinline__ float march2d(const float *__restrict__ base, unsigned pitch,
-                                         float hi_t, const Ray2D &R) {
+// slice-major copy (slice pitch `pitch` texels, coordinate clamp `hi_t`),
+// read through a buffer resource over the copy (texel records: one
+// v_add3 of per-lane address per sample).  Samples at even offsets from lo
+// go to acc0, odd ones to acc1; four loads are issued before the first of
+// them is used (the loop is bound by the loads' latency).  The clamped
+// coordinate t >= 0.5 truncates to its floor, and frac(t) = t - floor(t).
+__device__ __forceinline__ float march2d(__amdgpu_buffer_rsrc_t rs, unsigned pitch, float hi_t,
+                                         const Ray2D &R) {
     float acc0 = 0.f, acc1 = 0.f;
-    const float *p = base + (size_t)(R.lo > 0 ? R.lo : 0) * pitch;
+    unsigned row = (unsigned)(R.lo > 0 ? R.lo : 0) * pitch;  // first texel of slice s
     int s = R.lo;
     for (; s + 3 <= R.hi; s += 4) {
         float2 q[4];
@@ -87,32 +89,29 @@ __device__ __forceinline__ float march2d(const float *__restrict__ base, unsigne
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const float t = clamp_pad(fmaf(R.a, (float)(s + k), R.c), hi_t);
-            const float f = floorf(t);
-            q[k] = ld2b(p + k * pitch, (unsigned)f << 2);
-            wt[k] = t - f;
+            q[k] = ld2i(rs, row + k * pitch + (unsigned)t);
+            wt[k] = frac(t);
         }
         acc0 += lerp(q[0].x, q[0].y, wt[0]);
         acc1 += lerp(q[1].x, q[1].y, wt[1]);
         acc0 += lerp(q[2].x, q[2].y, wt[2]);
         acc1 += lerp(q[3].x, q[3].y, wt[3]);
-        p += 4 * pitch;
+        row += 4 * pitch;
     }
     if (s + 1 <= R.hi) {
         const float sf = (float)s;
         float t0 = clamp_pad(fmaf(R.a, sf, R.c), hi_t);
         float t1 = clamp_pad(fmaf(R.a, sf + 1.f, R.c), hi_t);
-        float f0 = floorf(t0), f1 = floorf(t1);
-        float2 q0 = ld2b(p, (unsigned)f0 << 2), q1 = ld2b(p + pitch, (unsigned)f1 << 2);
-        acc0 += lerp(q0.x, q0.y, t0 - f0);
-        acc1 += lerp(q1.x, q1.y, t1 - f1);
-        p += 2 * pitch;
+        float2 q0 = ld2i(rs, row + (unsigned)t0), q1 = ld2i(rs, row + pitch + (unsigned)t1);
+        acc0 += lerp(q0.x, q0.y, frac(t0));
+        acc1 += lerp(q1.x, q1.y, frac(t1));
+        row += 2 * pitch;
         s += 2;
     }
     if (s <= R.hi) {
         float t0 = clamp_pad(fmaf(R.a, (float)s, R.c), hi_t);
-        float f0 = floorf(t0);
-        float2 q0 = ld2b(p, (unsigned)f0 << 2);
-        acc0 += lerp(q0.x, q0.y, t0 - f0);
+        float2 q0 = ld2i(rs, row + (unsigned)t0);
+        acc0 += lerp(q0.x, q0.y, frac(t0));
     }
     return acc0 + acc1;
 }
@@ -126,7 +125,7 @@ __device__ __forceinline__ float march2d(const float *__restrict__ base, unsigne
 // lower of the two left texels serves both rays when their left texels
 // are at most two apart (the load starts at most pitch - 4, so it stays
 // inside the row); otherwise the other ray takes its own 8-byte load.
-__device__ __forceinline__ float2 march2d_pair(const float *__restrict__ base, unsigned pitch,
+__device__ __forceinline__ float2 march2d_pair(__amdgpu_buffer_rsrc_t rs, unsigned pitch,
                                                float hi_t, const Ray2D &A, const Ray2D &B) {
     const bool ea = A.lo > A.hi, eb = B.lo > B.hi;
     if (ea && eb) return make_float2(0.f, 0.f);
@@ -134,18 +133,18 @@ __device__ __forceinline__ float2 march2d_pair(const float *__restrict__ base, u
     const int hi = ea ? B.hi : (eb ? A.hi : max(A.hi, B.hi));
     const float gmax = (float)pitch - 4.f;
     float a0 = 0.f, a1 = 0.f, b0 = 0.f, b1 = 0.f;
-    const float *p = base + (size_t)(lo > 0 ? lo : 0) * pitch;
-    for (int s = lo; s <= hi; ++s, p += pitch) {
+    unsigned row = (unsigned)(lo > 0 ? lo : 0) * pitch;
+    for (int s = lo; s <= hi; ++s, row += pitch) {
         const float sf = (float)s;
         const float ta = clamp_pad(fmaf(A.a, sf, A.c), hi_t);
         const float tb = clamp_pad(fmaf(B.a, sf, B.c), hi_t);
         const float fa = floorf(ta), fb = floorf(tb);
         const float g = fminf(fminf(fa, fb), gmax);
-        const float4 x = ld4u(p, (unsigned)g << 2);
+        const float4 x = ld4i(rs, row + (unsigned)g);
         const float da = fa - g, db = fb - g;
         float2 qa = pick2(x, da), qb = pick2(x, db);
-        if (da > 2.5f) qa = ld2b(p, (unsigned)fa << 2);
-        if (db > 2.5f) qb = ld2b(p, (unsigned)fb << 2);
+        if (da > 2.5f) qa = ld2i(rs, row + (unsigned)fa);
+        if (db > 2.5f) qb = ld2i(rs, row + (unsigned)fb);
         const float va = lerp(qa.x, qa.y, ta - fa), vb = lerp(qb.x, qb.y, tb - fb);
         if ((s - lo) & 1) {
             a1 += va;
@@ -164,13 +163,16 @@ template <bool FAN>
 __device__ __forceinline__ float fp2d_value(const FP2DAngle &G, int det, unsigned rays,
                                             unsigned w, unsigned h, const float *rows,
                                             const float *cols) {
+    // resources over the two copies, built from their (wave-uniform)
+    // bases before the per-ray choice of axis
+    const __amdgpu_buffer_rsrc_t rr = texel_rsrc(rows), rc = texel_rsrc(cols);
     const bool vertical = ray_vertical<FAN>(G, det);
     float sum = 0.f;
     for (unsigned k = 0; k < rays; ++k) {
         const double fdet = det + (0.5 + k) / (double)rays;
         Ray2D R = ray2d<FAN>(G, fdet, vertical, w, h);
-        float v = vertical ? march2d(rows, w + 2 * kPad, (float)w + 2.5f, R)
-                           : march2d(cols, h + 2 * kPad, (float)h + 2.5f, R);
+        float v = vertical ? march2d(rr, w + 2 * kPad, (float)w + 2.5f, R)
+                           : march2d(rc, h + 2 * kPad, (float)h + 2.5f, R);
         sum += v * R.scale;
     }
     return sum / (float)rays;
@@ -187,10 +189,11 @@ __device__ __forceinline__ float2 fp2d_value_pair(const FP2DAngle &G, int det, u
     if (va != vb)
         return make_float2(fp2d_value<FAN>(G, det, 1, w, h, rows, cols),
                            fp2d_value<FAN>(G, det + 1, 1, w, h, rows, cols));
+    const __amdgpu_buffer_rsrc_t rr = texel_rsrc(rows), rc = texel_rsrc(cols);
     const Ray2D A = ray2d<FAN>(G, det + 0.5, va, w, h);
     const Ray2D B = ray2d<FAN>(G, det + 1.5, va, w, h);
-    const float2 s = va ? march2d_pair(rows, w + 2 * kPad, (float)w + 2.5f, A, B)
-                        : march2d_pair(cols, h + 2 * kPad, (float)h + 2.5f, A, B);
+    const float2 s = va ? march2d_pair(rr, w + 2 * kPad, (float)w + 2.5f, A, B)
+                        : march2d_pair(rc, h + 2 * kPad, (float)h + 2.5f, A, B);
     // fp2d_value: (0 + sum * scale) / 1
     return make_float2(s.x * A.scale, s.y * B.scale);
 }
@@ -325,6 +328,7 @@ __device__ __forceinline__ void fp2d_tile_lds(int det, int a0, unsigned na, unsi
             horiz |= v ? 0 : 1;
         }
     }
+    const __amdgpu_buffer_rsrc_t rr = texel_rsrc(rows), rc = texel_rsrc(cols);
     const bool any_v = __syncthreads_or(vert), any_h = __syncthreads_or(horiz);
     if (any_v && any_h) {
         // rays along both axes: each ray from global memory
@@ -332,8 +336,8 @@ __device__ __forceinline__ void fp2d_tile_lds(int det, int a0, unsigned na, unsi
         for (int j = 0; j < kFp2Rays; ++j) {
             if (!on[j]) continue;
             const bool v = ray_vertical<FAN>(fpa[a0 + j], det);
-            const float s = v ? march2d(rows, w + 2 * kPad, (float)w + 2.5f, R[j])
-                              : march2d(cols, h + 2 * kPad, (float)h + 2.5f, R[j]);
+            const float s = v ? march2d(rr, w + 2 * kPad, (float)w + 2.5f, R[j])
+                              : march2d(rc, h + 2 * kPad, (float)h + 2.5f, R[j]);
             val[j] = s * R[j].scale;
         }
         return;
