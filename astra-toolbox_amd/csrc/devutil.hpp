@@ -44,10 +44,18 @@ __device__ __forceinline__ T *first_lane_ptr(T *p) {
     return (T *)(((unsigned long long)hi << 32) | lo);
 }
 
-// Structured buffer of 8-byte records (row-difference pairs) from a
-// wave-uniform base: a 16-byte load of records index and index + 1 at
-// byte offset soff (uniform) takes one vector register of address (the
-// record index, one v_mad_u32_u24 from the texel coordinates).
+// Buffer resources (V#) for loads whose base is wave-uniform: the base
+// goes into scalar registers and the texture unit adds the per-lane
+// record index x stride (and a scalar byte offset), so a load costs one
+// vector register of address and no 64-bit vector address arithmetic.
+// num_records = 0xffffffff (no range check: the callers' indices stay in
+// their arrays); word 3 = 32-bit data format, as gfx9 buffer loads expect.
+// The loads are the structured-buffer intrinsic (index, offset, soffset,
+// aux), declared by its LLVM name.
+//
+// Structured buffer of 8-byte records (row-difference pairs): a 16-byte
+// load of records index and index + 1 at byte offset soff; the index is
+// one v_mad_u32_u24 from the texel coordinates.
 typedef unsigned amx_u4 __attribute__((ext_vector_type(4)));
 __device__ amx_u4 amx_struct_load_v4(__amdgpu_buffer_rsrc_t r, int vindex, int voffset,
                                      int soffset, int aux)
@@ -152,9 +160,6 @@ __device__ __forceinline__ int wave_max(int v) {
     return v;
 }
 
-// Slice range [lo, hi] of s in [0, n0) for which -1 < a*s + c < n can hold
-// (widened by one slice on each side; exactness is not required because
-// samples outside the array read the zero border).
 // Tent weight (1 - |d|)+ of the linear / bilinear interpolation.  Written
 // as a clamp to [0, 1] (1 - |d| <= 1, so the upper bound changes nothing and
 // NaN still gives 0), which folds into the subtraction's clamp bit: one
@@ -163,6 +168,9 @@ __device__ __forceinline__ float tent(float d) {
     return fminf(fmaxf(1.f - fabsf(d), 0.f), 1.f);
 }
 
+// Slice range [lo, hi] of s in [0, n0) for which -1 < a*s + c < n can hold
+// (widened by one slice on each side; exactness is not required because
+// samples outside the array read the zero border).
 __device__ __forceinline__ void clip_range(double a, double c, double n, int n0, int &lo, int &hi) {
     if (fabs(a) < 1e-12) {
         if (!(c > -1.0 && c < n)) { lo = 1; hi = 0; }
