@@ -660,8 +660,10 @@ void launch_fp(const Batch2D &b, int cfg, const float *sinos, float *out, int mo
 
 // FP candidates: kFPDB tilings (0..2), the LDS-staged one (3) and the
 // kFPDB tilings with two detectors per thread (4..6).  The autotuner times
-// the first three; the others never won on the 4096-slice workload
-// (LDS: no faster; pairs: 160 vs 173 slices/s) and run by request only.
+// the first three; the others never won on the 4096-slice workload (LDS:
+// no faster; pairs: 197 vs 200 slices/s; workgroups of both kinds in one
+// launch, to keep the texture path and the VALU busy at once: 185) and run
+// by request only.
 constexpr int kFPCandidates = 7;
 constexpr int kFPTimed = 3;
 
@@ -772,11 +774,14 @@ void iterate(const Batch2D &b, const float *sinos, float *vols, int iters, float
 // samples + BP updates per iteration); per queue one iteration's items,
 // stage after stage, slices in that order.
 void build_persistent(Batch2D &b) {
-    AMX_REQUIRE(b.fp_cfg >= 0 && b.fp_cfg < 3 && b.bp_cfg >= 0 && b.bp_cfg < 6,
-                "batch2d_sirt: the persistent run supports FP tilings 0-2 and BP tilings 0-5 "
-                "(forced: FP %d, BP %d)", b.fp_cfg, b.bp_cfg);
-    if (b.p_items.bytes() && b.p_fp == b.fp_cfg && b.p_bp == b.bp_cfg) return;
-    const unsigned db = kFPDB[b.fp_cfg], py = kBPPY[b.bp_cfg % 3];
+    // the persistent kernel's tilings: the chosen ones, or for the
+    // candidates it has no form of (LDS FP, pair and mixed FP, row-pair BP)
+    // the plain tiling of the same detectors / rows per thread (the same
+    // floats in every case)
+    static const int fp_plain[kFPCandidates] = {0, 1, 2, 0, 0, 1, 2};
+    const int pf = fp_plain[b.fp_cfg], pb = b.bp_cfg < 6 ? b.bp_cfg : b.bp_cfg - 6;
+    if (b.p_items.bytes() && b.p_fp == pf && b.p_bp == pb) return;
+    const unsigned db = kFPDB[pf], py = kBPPY[pb % 3];
     std::vector<unsigned> order(b.n);
     std::vector<double> work(b.n);
     for (unsigned i = 0; i < b.n; ++i) {
@@ -810,8 +815,8 @@ void build_persistent(Batch2D &b) {
     if (!b.p_sync.bytes()) b.p_sync = DevArray(sizeof(unsigned) * (8 + 3 * (size_t)b.n + 1));
     if (!b.p_status_host)
         AMX_HIP(hipHostMalloc((void **)&b.p_status_host, sizeof(unsigned), hipHostMallocDefault));
-    b.p_fp = b.fp_cfg;
-    b.p_bp = b.bp_cfg;
+    b.p_fp = pf;
+    b.p_bp = pb;
 }
 
 // Most polls of a dependency wait (about 2 us each) before a persistent run

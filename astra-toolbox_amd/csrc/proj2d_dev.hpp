@@ -251,10 +251,10 @@ __device__ __forceinline__ void bp2d_column(int X, int Y0, unsigned w, unsigned 
         for (int i = 0; i < PY; ++i) {
             float fr = (KIND == 0 || KIND == 3) ? 1.f : __builtin_amdgcn_rcpf(den);
             float t = clamp_pad(num * fr, hi);
-            float f = floorf(t);
-            fl[i] = f;
-            if (!PR) q[i] = ld2b(row, (unsigned)f << 2);
-            wt[i] = t - f;
+            // t >= 0.5: truncation is the floor, frac(t) = t - floor(t)
+            if (PR) fl[i] = floorf(t);
+            if (!PR) q[i] = ld2b(row, (unsigned)t << 2);
+            wt[i] = frac(t);
             wg[i] = (KIND == 0) ? c.w : (KIND == 1 ? fr : (KIND == 2 ? fr * fr : 1.f));
             num -= c.num[2];  // next row: world y decreases by 1
             den -= c.den[2];
@@ -535,17 +535,17 @@ __device__ __forceinline__ void bp2d_tile_lds(int X0, int Yt0, unsigned w, unsig
             for (int i = 0; i < PY; ++i) {
                 float fr = (KIND == 0 || KIND == 3) ? 1.f : __builtin_amdgcn_rcpf(den);
                 float t = clamp_pad(num * fr, hi);
-                float fl = floorf(t);
+                const int ti = (int)t;  // t >= 0.5: truncation is the floor
                 float2 q;
                 if (f.y) {
                     // inside the staged interval by its margin; the clamp
                     // only keeps the LDS address inside the row
-                    const int o = min(max((int)fl - f.x, 0), kBp2Wcap - 2);
+                    const int o = min(max(ti - f.x, 0), kBp2Wcap - 2);
                     q = make_float2(lrow[o], lrow[o + 1]);
                 } else {
-                    q = ld2b(row, (unsigned)fl << 2);
+                    q = ld2b(row, (unsigned)ti << 2);
                 }
-                float val = lerp(q.x, q.y, t - fl);
+                float val = lerp(q.x, q.y, frac(t));
                 float wgt = (KIND == 0) ? c.w : (KIND == 1 ? fr : (KIND == 2 ? fr * fr : 1.f));
                 acc[i] = fmaf(wgt, val, acc[i]);
                 num -= c.num[2];

@@ -85,6 +85,24 @@ def test_persistent_every_tiling(fp_cfg, bp_cfg, monkeypatch):
     B.release()
 
 
+@pytest.mark.parametrize("fp_cfg,bp_cfg", [(4, 3), (5, 6), (3, 8), (6, 5)])
+def test_persistent_with_tilings_it_maps(fp_cfg, bp_cfg, monkeypatch):
+    """Chosen tilings the persistent kernel has no form of (LDS and pair
+    FP, row-pair BP) run with the plain tiling of the same detectors /
+    rows per thread: the same floats."""
+    import torch
+    monkeypatch.setenv("ASTRA_AMD_BATCH_FP_CFG", str(fp_cfg))
+    monkeypatch.setenv("ASTRA_AMD_BATCH_BP_CFG", str(bp_cfg))
+    B, sinos = _batch(SLICES[:8], seed0=9)
+    a = B.empty_volumes()
+    B.run(sinos, a, 3, min_constraint=0.0, persistent=True)
+    b = B.empty_volumes()
+    B.run(sinos, b, 3, min_constraint=0.0, use_graph=False)
+    torch.cuda.synchronize()
+    assert torch.equal(a, b)
+    B.release()
+
+
 def test_persistent_register_limited_variant(batch, monkeypatch):
     import torch
     B, sinos = batch
