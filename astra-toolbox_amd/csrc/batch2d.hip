@@ -484,7 +484,7 @@ struct Batch2D {
     // after / before the caller's stream
     hipStream_t cap = nullptr;
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
-    int g_mode = -1, g_wpe = 0;
+    int g_mode = -1, g_wpe = 0, g_chunk = 0;
     // persistent run: the queues (built for the tilings p_fp / p_bp), the
     // sync words (heads[8], done[3][n], status) and a pinned status copy
     DevArray p_items, p_sync;
@@ -883,23 +883,33 @@ void persist_dispatch(const Batch2D &b, const PersistArgs *P, int wpe, unsigned 
     }
 }
 
+// Iterations per persistent launch: as many as keep every queue's dequeue
+// counter below 2^31 (ASTRA_AMD_BATCH_PERSIST_ITERS caps it further; read
+// at every run).
+int persist_chunk(const Batch2D &b) {
+    size_t longest = 1;
+    for (int x = 0; x < 8; ++x) longest = std::max<size_t>(longest, b.p_qoff[x + 1] - b.p_qoff[x]);
+    long chunk = (long)(((1ull << 31) - 1 - (1u << 24)) / longest);
+    const char *e = std::getenv("ASTRA_AMD_BATCH_PERSIST_ITERS");
+    if (e && std::atol(e) > 0) chunk = std::min(chunk, std::atol(e));
+    AMX_REQUIRE(chunk >= 1, "batch2d_sirt: too many work items for a persistent launch");
+    return (int)std::min<long>(chunk, 1 << 30);
+}
+
 // One persistent run: clear the sync words (a fill kernel, so the run can
-// be captured), then the persistent kernel (b.p_wpe, b.p_grid).
-void iterate_persistent(const Batch2D &b, const float *sinos, float *vols, int iters, float lo,
-                        float hi, hipStream_t s) {
+// be captured), then the persistent kernel (b.p_wpe, b.p_grid) -- one
+// launch, or consecutive launches of `chunk` iterations each when the
+// dequeue counters would overflow (the stop flag is cleared only before
+// the first, so a stopped launch stops the ones after it).
+void iterate_persistent(const Batch2D &b, const float *sinos, float *vols, int iters, int chunk,
+                        float lo, float hi, hipStream_t s) {
     unsigned *sync = b.p_sync.as<unsigned>();
-    fill_launch(reinterpret_cast<float *>(sync), 8 + 3 * (size_t)b.n + 1, 0.f, s);
     PersistArgs P;
     P.desc = b.desc.as<SliceDesc>();
     P.fpa = b.fpa.as<FP2DAngle>();
     P.bpa = b.bpa.as<BP2DAngle>();
     P.items = b.p_items.as<uint2>();
     for (int x = 0; x <= 8; ++x) P.qoff[x] = b.p_qoff[x];
-    for (int x = 0; x < 8; ++x)
-        AMX_REQUIRE((size_t)(b.p_qoff[x + 1] - b.p_qoff[x]) * iters < (1ull << 31),
-                    "batch2d_sirt: too many work items for one persistent run (%d iterations)",
-                    iters);
-    P.iters = (unsigned)iters;
     P.n = b.n;
     P.spin_limit = persist_spin_limit();
     P.heads = sync;
@@ -914,8 +924,13 @@ void iterate_persistent(const Batch2D &b, const float *sinos, float *vols, int i
     P.cols = b.cols.as<float>();
     P.lo = lo;
     P.hi = hi;
-    unsigned grid = b.p_grid;
-    persist_dispatch(b, &P, b.p_wpe, &grid, s);
+    for (int done = 0; done < iters; done += chunk) {
+        const size_t words = 8 + 3 * (size_t)b.n + (done == 0 ? 1 : 0);
+        fill_launch(reinterpret_cast<float *>(sync), words, 0.f, s);
+        P.iters = (unsigned)std::min(chunk, iters - done);
+        unsigned grid = b.p_grid;
+        persist_dispatch(b, &P, b.p_wpe, &grid, s);
+    }
 }
 
 }  // namespace
@@ -936,10 +951,12 @@ void batch2d_sirt(Batch2D &b, const float *sinos, float *vols, int iters, double
     if (iters == 0) return;
     const bool persistent = run_mode == 2;
     const int wpe = persistent ? persist_wpe() : 1;
+    int chunk = 0;
     if (persistent) {
         build_persistent(b);
         persist_dispatch(b, nullptr, wpe, &b.p_grid, s);
         b.p_wpe = wpe;
+        chunk = persist_chunk(b);
     }
     if (run_mode == 0) {
         iterate(b, sinos, vols, iters, lo, hi, s);
@@ -954,7 +971,8 @@ void batch2d_sirt(Batch2D &b, const float *sinos, float *vols, int iters, double
     AMX_HIP(hipEventRecord(b.ev_in, s));
     AMX_HIP(hipStreamWaitEvent(b.cap, b.ev_in, 0));
     const bool same = b.exec && b.g_iters == iters && b.g_sinos == sinos && b.g_vols == vols &&
-                      b.g_lo == lo && b.g_hi == hi && b.g_mode == run_mode && b.g_wpe == wpe;
+                      b.g_lo == lo && b.g_hi == hi && b.g_mode == run_mode && b.g_wpe == wpe &&
+                      b.g_chunk == chunk;
     if (!same) {
         if (b.exec) {
             (void)hipGraphExecDestroy(b.exec);
@@ -964,7 +982,7 @@ void batch2d_sirt(Batch2D &b, const float *sinos, float *vols, int iters, double
         AMX_HIP(hipStreamBeginCapture(b.cap, hipStreamCaptureModeThreadLocal));
         try {
             if (persistent)
-                iterate_persistent(b, sinos, vols, iters, lo, hi, b.cap);
+                iterate_persistent(b, sinos, vols, iters, chunk, lo, hi, b.cap);
             else
                 iterate(b, sinos, vols, iters, lo, hi, b.cap);
         } catch (...) {
@@ -987,6 +1005,7 @@ void batch2d_sirt(Batch2D &b, const float *sinos, float *vols, int iters, double
         b.g_hi = hi;
         b.g_mode = run_mode;
         b.g_wpe = wpe;
+        b.g_chunk = chunk;
     }
     ProfToken pt = prof_begin(persistent ? "batch2d_sirt_persistent" : "batch2d_sirt",
                               (double)iters, b.cap);

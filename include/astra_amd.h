@@ -288,8 +288,9 @@ int astra_amd_batch2d_fp(astra_amd_batch2d b, const float *vols, float *sinos, v
  * whole run as ONE persistent kernel launch over device work queues
  * (captured in a graph with the reset of its queue counters); it returns
  * when the run has ended, with ASTRA_AMD_ERR_ARG if a work item waited past
- * the spin limit (ASTRA_AMD_BATCH_SPIN_LIMIT polls, default 2^23).  All
- * modes give the same floats. */
+ * the spin limit (ASTRA_AMD_BATCH_SPIN_LIMIT polls, default 2^23); a run
+ * whose work-item count would pass 2^31 is split into consecutive
+ * persistent launches.  All modes give the same floats. */
 int astra_amd_batch2d_sirt(astra_amd_batch2d b, const float *sinos, float *vols, int iterations,
                            float relaxation, float min_constraint, float max_constraint,
                            int use_graph, void *stream);

@@ -103,6 +103,21 @@ def test_persistent_with_tilings_it_maps(fp_cfg, bp_cfg, monkeypatch):
     B.release()
 
 
+@pytest.mark.parametrize("chunk", ["1", "2", "3"])
+def test_persistent_in_consecutive_launches(chunk, batch, monkeypatch):
+    """A run split into persistent launches of `chunk` iterations (as runs
+    whose dequeue counters would pass 2^31 are) equals one launch."""
+    import torch
+    B, sinos = batch
+    ref = B.empty_volumes()
+    B.run(sinos, ref, 7, min_constraint=0.0, persistent=True)
+    monkeypatch.setenv("ASTRA_AMD_BATCH_PERSIST_ITERS", chunk)
+    x = B.empty_volumes()
+    B.run(sinos, x, 7, min_constraint=0.0, persistent=True)
+    torch.cuda.synchronize()
+    assert torch.equal(x, ref)
+
+
 def test_persistent_register_limited_variant(batch, monkeypatch):
     import torch
     B, sinos = batch
