@@ -1525,21 +1525,27 @@ int bp_zs() {
 // steady state the resident workgroups sit at angles spread evenly over the
 // launch's range: with 1440 planes of 19 MB (1024^3 / 1536^2) two
 // neighbouring tiles almost never read the same plane at the same time and
-// the L2 hit rate of the BP falls to 8 %.  Launching the angles in chunks of
-// 40 keeps the resident workgroups within 40 planes (L2 hit rate 57 %; at
-// 1024^3 / 1440 x 1536^2 the BP is 1.17x faster with SOD 2000 / ODD 1000 and
-// 1.06x with SOD 1000 / ODD 500, whose wider cone spreads each footprint
-// more), at the price of one volume read-modify-write per chunk, which costs
-// more than it saves while the padded projection copy is small (5 % slower
-// at 512^3 / 720 x 768^2).
-// So chunking applies from 8 GiB of padded projections on, in every output
-// mode; ASTRA_AMD_BP3D_ACHUNK=<angles> forces a chunk size (0: default rule;
-// read at every call, so a process can change it between calls).
+// the L2 hit rate of the BP falls to 8 %.  Launching the angles in chunks
+// keeps the resident workgroups within fewer planes, at the price of one
+// volume read-modify-write per chunk.  Measured at 1024^3 / 1440 x 1536^2,
+// chunks of 20 / 40 / 80 / 160 / 1440 angles: 1034 / 1004 / 1035 / 1052 /
+// 1070 ms per BP with 8 z voxels per thread (L2 hit rate 57 % at 40), 992
+// / 1001 ms for 40 / 80 with 16; at 512^3 / 720 x 768^2 (3.4 GB of padded
+// planes, 16 z voxels per thread), chunks of 90 / 180 / 240 / 360 / 720:
+// 49.5 / 48.6 / 48.2 / 48.7 / 49.3 ms.  So: 40 angles from 8 GiB
+// of padded projections on; below that, equal chunks of at most 1.2 GB of
+// planes (a single launch below 1.2 GB); in every output mode.
+// ASTRA_AMD_BP3D_ACHUNK=<angles> forces a chunk size (0: default rule; read
+// at every call, so a process can change it between calls).
 unsigned bp_achunk(size_t pair_bytes, unsigned na) {
     const char *e = std::getenv("ASTRA_AMD_BP3D_ACHUNK");
     const int forced = e ? std::atoi(e) : 0;
     if (forced > 0) return (unsigned)forced;
-    return pair_bytes >= (8ull << 30) ? 40u : std::max(na, 1u);
+    if (pair_bytes >= (8ull << 30)) return 40u;
+    const size_t cap = 1200000000ull;
+    if (pair_bytes <= cap || na == 0) return std::max(na, 1u);
+    const size_t chunks = (pair_bytes + cap - 1) / cap;
+    return (unsigned)((na + chunks - 1) / chunks);
 }
 
 // Most angles per block of the FP workgroup map (fp_map): an XCD's range
