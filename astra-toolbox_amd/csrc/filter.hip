@@ -396,29 +396,43 @@ __global__ __launch_bounds__((FftPlan<LOGN, WGT>::WG)) void filter_rows_stockham
     const unsigned nd = A.n_det;
     float2 v[16];
     {
-        const float *row0 = A.rows + r0 * nd;
-        const float *row1 = row0 + nd;
-        const float *w0 = (A.weights && has0) ? A.weights + (r0 / A.rows_per_weight) * nd : nullptr;
-        const float *w1 = (A.weights && has1) ? A.weights + ((r0 + 1) / A.rows_per_weight) * nd : nullptr;
-        const float *p0 = (A.weights2 && has0) ? A.weights2 + (r0 % A.w2_rows) * nd : nullptr;
-        const float *p1 = (A.weights2 && has1) ? A.weights2 + ((r0 + 1) % A.w2_rows) * nd : nullptr;
+        // Every load is issued before any loaded value is used: the loads
+        // sit in wave-uniform branches (column block m inside the row) with
+        // clamped indices, and the products and the zero padding come after
+        // all of them.  (Multiplying inside per-element branches made the
+        // compiler wait for each weight before the next load.)  Rows past
+        // the end read a valid row and are zeroed by the selects.
+        const float *row0 = A.rows + (has0 ? r0 : 0) * nd;
+        const float *row1 = has1 ? row0 + nd : row0;
+        const bool wt = A.weights != nullptr, wt2 = A.weights2 != nullptr;
+        const float *w0 = wt ? A.weights + ((has0 ? r0 : 0) / A.rows_per_weight) * nd : nullptr;
+        const float *w1 = wt ? A.weights + ((has1 ? r0 + 1 : 0) / A.rows_per_weight) * nd : nullptr;
+        const float *p0 = wt2 ? A.weights2 + ((has0 ? r0 : 0) % A.w2_rows) * nd : nullptr;
+        const float *p1 = wt2 ? A.weights2 + ((has1 ? r0 + 1 : 0) % A.w2_rows) * nd : nullptr;
+        float2 wv[16], pv[16];
 #pragma unroll
         for (unsigned m = 0; m < 16; ++m) {
-            const unsigned i = t + T * m;
-            float a = 0.f, b = 0.f;
-            if (i < nd) {
-                if (has0) {
-                    a = row0[i];
-                    if (w0) a *= w0[i];
-                    if (p0) a *= p0[i];
-                }
-                if (has1) {
-                    b = row1[i];
-                    if (w1) b *= w1[i];
-                    if (p1) b *= p1[i];
-                }
+            v[m] = wv[m] = pv[m] = make_float2(0.f, 0.f);
+            if (T * m < nd) {
+                const unsigned i = min(t + T * m, nd - 1);
+                v[m] = make_float2(row0[i], row1[i]);
+                if (wt) wv[m] = make_float2(w0[i], w1[i]);
+                if (wt2) pv[m] = make_float2(p0[i], p1[i]);
             }
-            v[m] = make_float2(a, b);
+        }
+#pragma unroll
+        for (unsigned m = 0; m < 16; ++m) {
+            float a = v[m].x, b = v[m].y;
+            if (wt) {
+                a *= wv[m].x;
+                b *= wv[m].y;
+            }
+            if (wt2) {
+                a *= pv[m].x;
+                b *= pv[m].y;
+            }
+            const bool in = t + T * m < nd;
+            v[m] = make_float2(in && has0 ? a : 0.f, in && has1 ? b : 0.f);
         }
     }
     fft_passes<LOGN>(v, t, A.tw, lds);
