@@ -506,6 +506,7 @@ __device__ __forceinline__ void bp2d_tile_lds(int X0, int Yt0, unsigned w, unsig
     const float cy1 = -(min(Yt0 + 4 * PY - 1, (int)h - 1) - 0.5f * h + 0.5f);
     const unsigned pitch = nd + 2 * kPad;
     const float hi = (float)nd + 2.5f;
+    const __amdgpu_buffer_rsrc_t rs = texel_rsrc(sp);  // unstaged angles
 #pragma unroll
     for (int i = 0; i < PY; ++i) acc[i] = 0.f;
     for (unsigned a0 = 0; a0 < na; a0 += kBp2KA) {
@@ -526,24 +527,34 @@ __device__ __forceinline__ void bp2d_tile_lds(int X0, int Yt0, unsigned w, unsig
         const unsigned nk = min((unsigned)kBp2KA, na - a0);
         for (unsigned k = 0; k < nk; ++k) {
             const BP2DAngle c = ang[a0 + k];
-            const int2 f = L.fp[k];
+            // the footprint is the same for every lane: keep it in scalar
+            // registers (a scalar branch, and the LDS row offset folded
+            // into the tap's address computation)
+            const int fx0 = __builtin_amdgcn_readfirstlane(L.fp[k].x);
+            const int fwd = __builtin_amdgcn_readfirstlane(L.fp[k].y);
             float num = fmaf(fy0, c.num[2], fmaf(fx, c.num[1], c.num[0]));
             float den = fmaf(fy0, c.den[2], fmaf(fx, c.den[1], c.den[0]));
-            const float *row = sp + (size_t)(a0 + k) * pitch;
-            const float *lrow = L.buf + k * kBp2Wcap;
+            const unsigned rowi = (a0 + k) * pitch;  // first texel of the row
+            // LDS byte offset of texel 0 of the row: a tap's address is
+            // one shift-and-add of its texel index
+            const unsigned lb = __builtin_amdgcn_readfirstlane((unsigned)((int)k * kBp2Wcap - fx0) * 4u);
 #pragma unroll
             for (int i = 0; i < PY; ++i) {
                 float fr = (KIND == 0 || KIND == 3) ? 1.f : __builtin_amdgcn_rcpf(den);
                 float t = clamp_pad(num * fr, hi);
                 const int ti = (int)t;  // t >= 0.5: truncation is the floor
                 float2 q;
-                if (f.y) {
-                    // inside the staged interval by its margin; the clamp
-                    // only keeps the LDS address inside the row
-                    const int o = min(max(ti - f.x, 0), kBp2Wcap - 2);
-                    q = make_float2(lrow[o], lrow[o + 1]);
+                if (fwd) {
+                    // texels ti, ti + 1 lie inside the staged interval
+                    // [fx0, fx0 + fwd - 1]: the footprint widens the taps'
+                    // range by a texel on each side, far beyond the rounding
+                    // between its corner coordinates and the taps'
+                    // incremental ones (the tests replay both on the host)
+                    const float *l = reinterpret_cast<const float *>(
+                        reinterpret_cast<const char *>(L.buf) + (lb + ((unsigned)ti << 2)));
+                    q = make_float2(l[0], l[1]);
                 } else {
-                    q = ld2b(row, (unsigned)ti << 2);
+                    q = ld2i(rs, rowi + (unsigned)ti);
                 }
                 float val = lerp(q.x, q.y, frac(t));
                 float wgt = (KIND == 0) ? c.w : (KIND == 1 ? fr : (KIND == 2 ? fr * fr : 1.f));

@@ -199,3 +199,13 @@ def test_bp2d_taps_inside_staged_footprint(case, py):
 
 def test_bp2d_planted_first_row_footprint_is_caught():
     assert check2d("fan", 129, 97, 190, 60, 500.0, 300.0, 4, plant=True) > 0
+
+
+@pytest.mark.parametrize("case", [("par", 1024, 1024, 1536, 8, 0, 0), ("fan", 512, 512, 768, 12, 1024.0, 512.0)],
+                         ids=["par1024", "fan512"])
+def test_bp2d_taps_inside_staged_footprint_batched_sizes(case):
+    """Slices of the batched workload's sizes (fan: source-origin 2N,
+    origin-detector N), 8 rows per thread: the staged taps read LDS without
+    a clamp, so every tap must stay inside its footprint."""
+    kind, w, h, nd, na, sod, odd = case
+    assert check2d(kind, w, h, nd, na, sod, odd, 8) == 0
