@@ -141,20 +141,24 @@ __global__ __launch_bounds__(256) void filter_rows_kernel(FilterArgs A) {
     const float *v1 = (A.weights2 && has1) ? A.weights2 + ((r0 + 1) % A.w2_rows) * A.n_det : nullptr;
     const unsigned shift = 32 - A.log2n;
 
-    // load (bit-reversed positions), zero padding beyond n_det
+    // load (bit-reversed positions), zero padding beyond n_det; the loads
+    // of an element come before its products, so they are in flight
+    // together
     for (unsigned j = threadIdx.x; j < N; j += blockDim.x) {
-        float a = 0.f, b = 0.f;
+        float a = 0.f, b = 0.f, wa = 1.f, wb = 1.f, va = 1.f, vb = 1.f;
         if (j < A.n_det) {
             a = row0[j];
-            if (w0) a *= w0[j];
-            if (v0) a *= v0[j];
+            if (w0) wa = w0[j];
+            if (v0) va = v0[j];
             if (has1) {
                 b = row1[j];
-                if (w1) b *= w1[j];
-                if (v1) b *= v1[j];
+                if (w1) wb = w1[j];
+                if (v1) vb = v1[j];
             }
         }
-        X[__brev(j) >> shift] = make_float2(a, b);
+        // a product with 1 is exact: the same floats as multiplying only
+        // where a weight exists
+        X[__brev(j) >> shift] = make_float2(a * wa * va, b * wb * vb);
     }
     __syncthreads();
 
@@ -521,19 +525,20 @@ __global__ __launch_bounds__(256) void filter_row_real_kernel(FilterArgs A) {
 
     // load pairs (x[2m], x[2m+1]) at bit-reversed positions of the M-point transform
     for (unsigned m = threadIdx.x; m < M; m += blockDim.x) {
-        float a = 0.f, b = 0.f;
+        float a = 0.f, b = 0.f, wa = 1.f, wb = 1.f, va = 1.f, vb = 1.f;
         const unsigned j = 2 * m;
         if (j < A.n_det) {
             a = row[j];
-            if (w) a *= w[j];
-            if (v) a *= v[j];
+            if (w) wa = w[j];
+            if (v) va = v[j];
         }
         if (j + 1 < A.n_det) {
             b = row[j + 1];
-            if (w) b *= w[j + 1];
-            if (v) b *= v[j + 1];
+            if (w) wb = w[j + 1];
+            if (v) vb = v[j + 1];
         }
-        Z[__brev(m) >> shift] = make_float2(a, b);
+        // loads first, then the (exact where a weight is absent) products
+        Z[__brev(m) >> shift] = make_float2(a * wa * va, b * wb * vb);
     }
     __syncthreads();
 
@@ -602,10 +607,9 @@ __global__ void k_row_weights(float *rows, unsigned len, const float *w, size_t 
     const float *a = w ? w + (r / div) * len : nullptr;
     const float *b = w2 ? w2 + (r % mod) * len : nullptr;
     for (unsigned i = threadIdx.x; i < len; i += blockDim.x) {
-        float v = row[i];
-        if (a) v *= a[i];
-        if (b) v *= b[i];
-        row[i] = v;
+        // loads first, then the (exact where a weight is absent) products
+        const float v = row[i], wa = a ? a[i] : 1.f, wb = b ? b[i] : 1.f;
+        row[i] = v * wa * wb;
     }
 }
 

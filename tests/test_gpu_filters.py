@@ -277,6 +277,28 @@ def test_single_row_path_fdk_weights(single_row):
     test_fdk_short_scan()
 
 
+@pytest.fixture
+def lds_pair_kernel():
+    """Forces the LDS row-pair kernel for real spectra (the path of
+    per-row spectra and of transforms above 4096 points) for the calls
+    inside the test."""
+    import os
+    key = "ASTRA_AMD_FILTER_LDS_RADIX4"
+    old = os.environ.get(key)
+    os.environ[key] = "1"
+    yield
+    if old is None:
+        os.environ.pop(key, None)
+    else:
+        os.environ[key] = old
+
+
+def test_lds_pair_path_fdk_weights(lds_pair_kernel):
+    """The FDK pre-weights and Parker weights applied while loading a row
+    pair into the LDS kernel."""
+    test_fdk_short_scan()
+
+
 def test_fbp_detector_wider_than_a_row_pair():
     """10000 detector bins: the transform has 32768 points, whose row pair
     (256 KiB) exceeds the CU's LDS, so each row is filtered alone (128 KiB).
