@@ -224,12 +224,8 @@ __global__ __launch_bounds__(256) void batch_bp(BatchArgs B, const float *__rest
         bp2d_column<1, PY, PR>(X, Y0, d.w, d.h, d.na, d.nd, sp, ang, acc);
     else
         bp2d_column<0, PY, PR>(X, Y0, d.w, d.h, d.na, d.nd, sp, ang, acc);
-    for (int i = 0; i < PY; ++i) {
-        const int Y = Y0 + i;
-        if (Y >= (int)d.h) break;
-        const size_t idx = d.vol_off + (size_t)Y * d.w + X;
-        store_bp(vols, idx, acc[i] * d.osc, mode, pw, lo, hi);
-    }
+    store_bp_rows<PY>(vols, d.vol_off + (size_t)Y0 * d.w + X, d.w, min(PY, (int)d.h - Y0), acc, d.osc,
+                      mode, pw, lo, hi);
 }
 
 // The same with the sinogram footprints staged in LDS (bp2d_tile_lds).
@@ -256,12 +252,8 @@ __global__ __launch_bounds__(256) void batch_bp_lds(BatchArgs B, const float *__
     const int X = X0 + (threadIdx.x & 63);
     const int Y0 = Yt0 + (threadIdx.x >> 6) * PY;
     if (X >= (int)d.w || Y0 >= (int)d.h) return;
-    for (int i = 0; i < PY; ++i) {
-        const int Y = Y0 + i;
-        if (Y >= (int)d.h) break;
-        const size_t idx = d.vol_off + (size_t)Y * d.w + X;
-        store_bp(vols, idx, acc[i] * d.osc, mode, pw, lo, hi);
-    }
+    store_bp_rows<PY>(vols, d.vol_off + (size_t)Y0 * d.w + X, d.w, min(PY, (int)d.h - Y0), acc, d.osc,
+                      mode, pw, lo, hi);
 }
 
 // ---------------------------------------------------------- persistent ---
@@ -442,12 +434,9 @@ batch_sirt_persistent(PersistArgs P) {
                     else
                         bp2d_column<0, PY>(X, Y0, d.w, d.h, d.na, d.nd, sp, ang, acc);
                 }
-                for (int i = 0; i < PY; ++i) {
-                    const int Y = Y0 + i;
-                    if (Y >= (int)d.h) break;
-                    const size_t idx = d.vol_off + (size_t)Y * d.w + X;
-                    store_bp(P.vols, idx, acc[i] * d.osc, OUT_UPDATE, P.pw, P.lo, P.hi);
-                }
+                store_bp_rows<PY>(P.vols, d.vol_off + (size_t)Y0 * d.w + X, d.w,
+                                  min(PY, (int)d.h - Y0), acc, d.osc, OUT_UPDATE, P.pw, P.lo,
+                                  P.hi);
             }
         }
         // publish this thread's stores device-wide, then count the item

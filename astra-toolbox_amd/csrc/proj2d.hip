@@ -112,11 +112,8 @@ __global__ __launch_bounds__(256) void bp2d_kernel(BP2DArgs A) {
     if (X >= (int)A.w || Y0 >= (int)A.h) return;
     float acc[PY];
     bp2d_column<KIND, PY, PR>(X, Y0, A.w, A.h, A.na, A.nd, A.sp, A.ang, acc);
-    for (int i = 0; i < PY; ++i) {
-        const int Y = Y0 + i;
-        if (Y >= (int)A.h) break;
-        store_bp(A.vol, (size_t)Y * A.w + X, acc[i] * A.osc, A.mode, A.uw, A.lo, A.hi);
-    }
+    store_bp_rows<PY>(A.vol, (size_t)Y0 * A.w + X, A.w, min(PY, (int)A.h - Y0), acc, A.osc, A.mode,
+                      A.uw, A.lo, A.hi);
 }
 
 // The same with the sinogram footprints staged in LDS (bp2d_tile_lds):
@@ -131,11 +128,8 @@ __global__ __launch_bounds__(256) void bp2d_lds_kernel(BP2DArgs A) {
     const int X = X0 + (threadIdx.x & 63);
     const int Y0 = Yt0 + (threadIdx.x >> 6) * PY;
     if (X >= (int)A.w || Y0 >= (int)A.h) return;
-    for (int i = 0; i < PY; ++i) {
-        const int Y = Y0 + i;
-        if (Y >= (int)A.h) break;
-        store_bp(A.vol, (size_t)Y * A.w + X, acc[i] * A.osc, A.mode, A.uw, A.lo, A.hi);
-    }
+    store_bp_rows<PY>(A.vol, (size_t)Y0 * A.w + X, A.w, min(PY, (int)A.h - Y0), acc, A.osc, A.mode,
+                      A.uw, A.lo, A.hi);
 }
 
 // Supersampled BP (devBP_SS / devFanBP_SS, cuda/2d/par_bp.cu:90-137,

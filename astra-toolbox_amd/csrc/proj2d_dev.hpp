@@ -580,6 +580,41 @@ __device__ __forceinline__ void store_bp(float *vol, size_t idx, float val, int 
     }
 }
 
+// The PY outputs of a thread's pixel column: rows Y0 .. Y0 + nrows - 1
+// (nrows <= PY) at vol[idx0 + i * stride], val = acc[i] * scale, stored as
+// store_bp does.  Every load (old values, weights) is issued before the
+// first store, so the column costs one memory round trip, not one per row
+// (the compiler cannot move a load of vol above a store to vol by itself).
+template <int PY>
+__device__ __forceinline__ void store_bp_rows(float *vol, size_t idx0, size_t stride, int nrows,
+                                              const float (&acc)[PY], float scale, int mode,
+                                              const float *uw, float lo, float hi) {
+    const bool old = mode != OUT_SET;
+    const bool wts = (mode == OUT_EMUPD || mode == OUT_UPDATE) && uw;
+    float ov[PY], wv[PY];
+#pragma unroll
+    for (int i = 0; i < PY; ++i) {
+        ov[i] = 0.f;
+        wv[i] = 1.f;
+        if (i < nrows) {
+            if (old) ov[i] = vol[idx0 + i * stride];
+            if (wts) wv[i] = uw[idx0 + i * stride];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < PY; ++i) {
+        if (i < nrows) {
+            const float val = acc[i] * scale;
+            float out;
+            if (mode == OUT_ADD) out = ov[i] + val;
+            else if (mode == OUT_SET) out = val;
+            else if (mode == OUT_EMUPD) out = ov[i] * (val * wv[i]);
+            else out = fminf(fmaxf(fmaf(wv[i], val, ov[i]), lo), hi);  // OUT_UPDATE
+            vol[idx0 + i * stride] = out;
+        }
+    }
+}
+
 // One 64 x 64 tile of the padded slice copies of a [h][w] volume: rows copy
 // [h][w+4] and columns copy [w][h+4]; tile origin (c0, r0) in padded
 // coordinates.  Uses a 64 x 65 LDS tile; 256 threads.

@@ -762,24 +762,43 @@ struct BPArgs {
 
 // Epilogue of a column of ZS voxels: scale, add the earlier chunks, write
 // in the launch's mode.
+// Every load of the column (earlier chunks, old values, weights) is issued
+// before the first store, so the column's read-modify-write costs one
+// memory round trip, not one per voxel (the compiler cannot reorder a load
+// of vol after a store to vol by itself).  Z0 is the workgroup's, so the
+// bounds tests are uniform.
 template <int ZS>
 __device__ __forceinline__ void bp_column_store(const BPArgs &A, int X, int Y, int Z0,
                                                 const float (&acc)[ZS]) {
     const int nzr = min(ZS, (int)A.nz - Z0);
+    const size_t plane = (size_t)A.ny * A.nx;
+    const size_t base = ((size_t)Z0 * A.ny + Y) * A.nx + X;
+    const bool old = A.mode != OUT_SET;
+    const bool wts = (A.mode == OUT_EMUPD || A.mode == OUT_UPDATE) && A.uw;
+    float pv[ZS], ov[ZS], wv[ZS];
+#pragma unroll
     for (int i = 0; i < ZS; ++i) {
-        if (i >= nzr) break;
-        size_t idx = ((size_t)(Z0 + i) * A.ny + Y) * A.nx + X;
-        float val = acc[i] * A.osc;
-        if (A.pre) val += A.pre[idx];
-        if (A.mode == OUT_ADD) {
-            A.vol[idx] += val;
-        } else if (A.mode == OUT_SET) {
-            A.vol[idx] = val;
-        } else if (A.mode == OUT_EMUPD) {
-            A.vol[idx] *= val * A.uw[idx];
-        } else {  // OUT_UPDATE: clamp(x + w * val)
-            float w = A.uw ? A.uw[idx] : 1.f;
-            A.vol[idx] = fminf(fmaxf(fmaf(w, val, A.vol[idx]), A.lo), A.hi);
+        pv[i] = 0.f;
+        ov[i] = 0.f;
+        wv[i] = 1.f;
+        if (i < nzr) {
+            const size_t idx = base + i * plane;
+            if (A.pre) pv[i] = A.pre[idx];
+            if (old) ov[i] = A.vol[idx];
+            if (wts) wv[i] = A.uw[idx];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < ZS; ++i) {
+        if (i < nzr) {
+            float val = acc[i] * A.osc;
+            if (A.pre) val += pv[i];
+            float out;
+            if (A.mode == OUT_ADD) out = ov[i] + val;
+            else if (A.mode == OUT_SET) out = val;
+            else if (A.mode == OUT_EMUPD) out = ov[i] * (val * wv[i]);
+            else out = fminf(fmaxf(fmaf(wv[i], val, ov[i]), A.lo), A.hi);  // OUT_UPDATE
+            A.vol[base + i * plane] = out;
         }
     }
 }
