@@ -67,36 +67,47 @@ __global__ __launch_bounds__(256) void pad_copy_xfast(const float *__restrict__ 
     }
 }
 
-// x-group: dst[x][z + 2][y + 2] = (vol[z][y][x], vol[z+1][y][x] - vol[z][y][x]); per
-// padded z row an (x, y) transpose of planes z and z + 1 through two
-// 64 x 64 LDS tiles.  grid = (y tiles, x tiles, nz + 4).
+// x-group: dst[x][z + 2][y + 2] = (vol[z][y][x], vol[z+1][y][x] - vol[z][y][x]); an
+// (x, y) transpose of a 64 x 64 tile for kTzRows consecutive padded z rows,
+// each volume plane of the tile read once into one of two LDS tiles (row
+// zr combines planes zr - 2 and zr - 1).  grid = (y tiles, x tiles,
+// ceil((nz + 4) / kTzRows)).
+constexpr int kTzRows = 8;
 __global__ __launch_bounds__(256) void pad_transpose_x(const float *__restrict__ vol,
                                                        float2 *__restrict__ dst, unsigned nx,
                                                        unsigned ny, unsigned nz) {
     __shared__ float tile[2][64][65];
     const unsigned pyn = ny + 2 * kPad, pzn = nz + 2 * kPad;
-    const int zr = blockIdx.z;  // padded z row
+    const int zr0 = blockIdx.z * kTzRows;  // first padded z row
     const int x0 = blockIdx.y * 64;
     const int yp0 = blockIdx.x * 64;  // padded y tile origin
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-    for (int k = 0; k < 2; ++k) {
-        const int z = zr - kPad + k;
+    const int nrows = min(kTzRows, (int)pzn - zr0);
+    // plane k (k = 0 .. nrows) is z = zr0 - kPad + k, held in tile[k & 1]
+    for (int k = 0; k <= nrows; ++k) {
+        const int z = zr0 - kPad + k;
+        float *t = &tile[k & 1][0][0];
         for (int r = ty; r < 64; r += 4) {
-            int y = yp0 + r - kPad;
-            int x = x0 + tx;
+            const int y = yp0 + r - kPad, x = x0 + tx;
             float v = 0.f;
             if (z >= 0 && z < (int)nz && y >= 0 && y < (int)ny && x < (int)nx)
                 v = vol[((size_t)z * ny + y) * nx + x];
-            tile[k][r][tx] = v;
+            t[r * 65 + tx] = v;
         }
-    }
-    __syncthreads();
-    for (int r = ty; r < 64; r += 4) {
-        int x = x0 + r;
-        int yp = yp0 + tx;
-        if (x < (int)nx && yp < (int)pyn)
-            dst[((size_t)x * pzn + zr) * pyn + yp] =
-                make_float2(tile[0][tx][r], tile[1][tx][r] - tile[0][tx][r]);
+        __syncthreads();
+        if (k > 0) {
+            // padded row zr0 + k - 1 from planes k - 1 and k
+            const int zr = zr0 + k - 1;
+            const float *a = &tile[(k - 1) & 1][0][0], *b = &tile[k & 1][0][0];
+            for (int r = ty; r < 64; r += 4) {
+                const int x = x0 + r, yp = yp0 + tx;
+                if (x < (int)nx && yp < (int)pyn) {
+                    const float va = a[tx * 65 + r];
+                    dst[((size_t)x * pzn + zr) * pyn + yp] = make_float2(va, b[tx * 65 + r] - va);
+                }
+            }
+        }
+        __syncthreads();  // tile[(k + 1) & 1] is overwritten next
     }
 }
 
@@ -1466,7 +1477,8 @@ size_t group_elems(const Geom3D &g, int grp) {
 
 void make_group_copy(const Geom3D &g, int grp, const float *vol, float2 *dst, hipStream_t s) {
     if (grp == 0) {
-        dim3 grid((g.ny + 2 * kPad + 63) / 64, (g.nx + 63) / 64, g.nz + 2 * kPad);
+        dim3 grid((g.ny + 2 * kPad + 63) / 64, (g.nx + 63) / 64,
+                  (g.nz + 2 * kPad + kTzRows - 1) / kTzRows);
         pad_transpose_x<<<grid, 256, 0, s>>>(vol, dst, g.nx, g.ny, g.nz);
     } else {
         unsigned rows = (grp == 2) ? g.nz * (g.ny + 2 * kPad) : g.ny * (g.nz + 2 * kPad);
