@@ -132,22 +132,37 @@ __global__ __launch_bounds__(256) void proj_to_am(const float *__restrict__ proj
 // proj [v][a][u] -> padded angle-major difference pairs [a][v + 2][u + 2] =
 // (p[v][u], p[v+1][u] - p[v][u]) (borders read 0).  One workgroup per
 // destination row.
+// A workgroup writes kPairRows consecutive padded rows of one angle: each
+// thread walks a column strip down the rows, so every source row is read
+// once (plus one row per strip) and the row loads are independent.
+// grid = na * ceil((nv + 4) / kPairRows).
+constexpr int kPairRows = 8;
 __global__ __launch_bounds__(256) void proj_to_am_pairs(const float *__restrict__ proj,
                                                         float2 *__restrict__ pp, unsigned nu,
                                                         unsigned na, unsigned nv) {
     const unsigned pu = nu + 2 * kPad, pv = nv + 2 * kPad;
-    const unsigned row = blockIdx.x;  // a * pv + padded v
-    const unsigned a = row / pv;
-    const int v = (int)(row % pv) - kPad;
-    float2 *out = pp + (size_t)row * pu;
-    const bool in0 = v >= 0 && v < (int)nv, in1 = v + 1 >= 0 && v + 1 < (int)nv;
-    const float *s0 = proj + ((size_t)(in0 ? v : 0) * na + a) * nu;
-    const float *s1 = proj + ((size_t)(in1 ? v + 1 : 0) * na + a) * nu;
+    const unsigned blocks = (pv + kPairRows - 1) / kPairRows;
+    const unsigned a = blockIdx.x / blocks;
+    const int r0 = (int)(blockIdx.x % blocks) * kPairRows;  // first padded row
+    const int nrows = min(kPairRows, (int)pv - r0);
+    float2 *out = pp + ((size_t)a * pv + r0) * pu;
+    // source row of padded row r (zero outside the data)
+    auto src = [&](int r) -> const float * {
+        const int v = r - kPad;
+        return (v >= 0 && v < (int)nv) ? proj + ((size_t)v * na + a) * nu : nullptr;
+    };
     for (unsigned i = threadIdx.x; i < pu; i += blockDim.x) {
         const int u = (int)i - kPad;
         const bool uin = u >= 0 && u < (int)nu;
-        const float a = (in0 && uin) ? s0[u] : 0.f, b = (in1 && uin) ? s1[u] : 0.f;
-        out[i] = make_float2(a, b - a);
+        float cur[kPairRows + 1];
+#pragma unroll
+        for (int k = 0; k <= kPairRows; ++k) {
+            const float *s = k <= nrows ? src(r0 + k) : nullptr;
+            cur[k] = (s && uin) ? s[u] : 0.f;
+        }
+#pragma unroll
+        for (int k = 0; k < kPairRows; ++k)
+            if (k < nrows) out[(size_t)k * pu + i] = make_float2(cur[k], cur[k + 1] - cur[k]);
     }
 }
 
@@ -1930,8 +1945,8 @@ void bp3d_launch_padded(const Geom3D &g, const float *pp, float *vol, float out_
 void bp3d_launch(const Geom3D &g, const float *proj, float *vol, float out_scale, int kernel,
                  int mode, const float *upd_w, float lo, float hi, hipStream_t s, unsigned ss) {
     Scratch pairs(padded_am_elems(g) * sizeof(float2), s);
-    proj_to_am_pairs<<<g.na * (g.nv + 2 * kPad), 256, 0, s>>>(proj, pairs.as<float2>(), g.nu, g.na,
-                                                               g.nv);
+    proj_to_am_pairs<<<g.na * ((g.nv + 2 * kPad + kPairRows - 1) / kPairRows), 256, 0, s>>>(
+        proj, pairs.as<float2>(), g.nu, g.na, g.nv);
     AMX_LAUNCHED();
     bp3d_run_pairs(g, pairs.as<float2>(), vol, out_scale, kernel, mode, upd_w, lo, hi, s, ss);
 }
