@@ -166,18 +166,27 @@ __global__ __launch_bounds__(256) void proj_to_am_pairs(const float *__restrict_
     }
 }
 
-// padded single layout [a][v + 4][u + 4] -> difference pairs (same padding).
+// padded single layout [a][v + 4][u + 4] -> difference pairs (same padding);
+// kPairRows rows of one angle per workgroup, as proj_to_am_pairs.
+// grid = na * ceil((nv + 4) / kPairRows).
 __global__ __launch_bounds__(256) void am_to_pairs(const float *__restrict__ src,
                                                    float2 *__restrict__ pp, unsigned nu,
                                                    unsigned nv) {
     const unsigned pu = nu + 2 * kPad, pv = nv + 2 * kPad;
-    const unsigned row = blockIdx.x;  // a * pv + padded v
-    const bool last = (row % pv) == pv - 1;
-    const float *s0 = src + (size_t)row * pu;
-    float2 *out = pp + (size_t)row * pu;
+    const unsigned blocks = (pv + kPairRows - 1) / kPairRows;
+    const unsigned a = blockIdx.x / blocks;
+    const int r0 = (int)(blockIdx.x % blocks) * kPairRows;  // first padded row
+    const int nrows = min(kPairRows, (int)pv - r0);
+    const float *s0 = src + ((size_t)a * pv + r0) * pu;
+    float2 *out = pp + ((size_t)a * pv + r0) * pu;
     for (unsigned i = threadIdx.x; i < pu; i += blockDim.x) {
-        const float a = s0[i], b = last ? 0.f : s0[pu + i];
-        out[i] = make_float2(a, b - a);
+        float cur[kPairRows + 1];
+#pragma unroll
+        for (int k = 0; k <= kPairRows; ++k)  // row pv (past the last) reads 0
+            cur[k] = (k <= nrows && r0 + k < (int)pv) ? s0[(size_t)k * pu + i] : 0.f;
+#pragma unroll
+        for (int k = 0; k < kPairRows; ++k)
+            if (k < nrows) out[(size_t)k * pu + i] = make_float2(cur[k], cur[k + 1] - cur[k]);
     }
 }
 
@@ -1937,7 +1946,8 @@ void bp3d_launch_padded(const Geom3D &g, const float *pp, float *vol, float out_
                         int mode, const float *upd_w, float lo, float hi, hipStream_t s,
                         unsigned ss) {
     Scratch pairs(padded_am_elems(g) * sizeof(float2), s);
-    am_to_pairs<<<g.na * (g.nv + 2 * kPad), 256, 0, s>>>(pp, pairs.as<float2>(), g.nu, g.nv);
+    am_to_pairs<<<g.na * ((g.nv + 2 * kPad + kPairRows - 1) / kPairRows), 256, 0, s>>>(
+        pp, pairs.as<float2>(), g.nu, g.nv);
     AMX_LAUNCHED();
     bp3d_run_pairs(g, pairs.as<float2>(), vol, out_scale, kernel, mode, upd_w, lo, hi, s, ss);
 }
